@@ -1,0 +1,105 @@
+"""ctypes binding of libakua.so (include/akua.h).
+
+The library is loaded from this package directory only.  If it is missing the import of any
+solver class raises: there is no CPU fallback for the product path.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+from .config import CLambdaCorr, CPBFConfig
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libakua.so")
+
+AKUA_OK = 0
+STATUS_NAMES = {
+    0: "AKUA_OK", 1: "AKUA_ERR_INVALID_ARG", 2: "AKUA_ERR_OUT_OF_MEMORY", 3: "AKUA_ERR_HIP",
+    4: "AKUA_ERR_NON_FINITE", 5: "AKUA_ERR_STATE", 6: "AKUA_ERR_UNSUPPORTED", 7: "AKUA_ERR_NO_DEVICE",
+}
+
+# Every symbol include/akua.h declares (checked by tests/test_abi.py).
+EXPORTED = [
+    "akua_abi_version", "akua_last_error", "akua_default_config",
+    "akua_solver_create", "akua_solver_destroy", "akua_solver_set_particles",
+    "akua_solver_get_particles", "akua_solver_set_state", "akua_solver_get_state",
+    "akua_solver_step", "akua_solver_step_n", "akua_solver_step_device",
+    "akua_solver_set_gravity", "akua_solver_synchronize", "akua_solver_num_particles",
+    "akua_solver_set_graph", "akua_debug_hashes", "akua_debug_permutation",
+    "akua_debug_cell_ranges", "akua_debug_neighbours", "akua_solver_set_timing",
+    "akua_solver_reset_timing", "akua_solver_get_timing", "akua_solver_mean_neighbours",
+]
+
+
+class AkuaError(RuntimeError):
+    def __init__(self, status: int, message: str):
+        super().__init__(f"{STATUS_NAMES.get(status, status)}: {message}")
+        self.status = status
+
+
+class AkuaParticle(ctypes.Structure):
+    """``akua_particle``: the reference's 108-byte record (Particle.h:8-31)."""
+
+    _fields_ = [
+        ("position", ctypes.c_float * 3), ("velocity", ctypes.c_float * 3),
+        ("new_position", ctypes.c_float * 3), ("new_velocity", ctypes.c_float * 3),
+        ("position_delta", ctypes.c_float * 3), ("vorticity", ctypes.c_float * 3),
+        ("mass", ctypes.c_float), ("density", ctypes.c_float), ("lambda_", ctypes.c_float),
+        ("hash", ctypes.c_uint32), ("color", ctypes.c_float * 4), ("size", ctypes.c_float),
+    ]
+
+
+assert ctypes.sizeof(AkuaParticle) == 108
+
+_lib = None
+
+
+def load() -> ctypes.CDLL:
+    """Loads libakua.so (raises if it has not been built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise ImportError(f"{LIB_PATH} is missing: build it with `python -m akuaengine_amd.build`")
+    L = ctypes.CDLL(LIB_PATH)
+    P = ctypes.c_void_p
+    i32, f32 = ctypes.c_int32, ctypes.c_float
+    L.akua_abi_version.restype = i32
+    L.akua_last_error.restype = ctypes.c_char_p
+    L.akua_default_config.argtypes = [ctypes.POINTER(CPBFConfig), ctypes.POINTER(CLambdaCorr)]
+    sigs = {
+        "akua_solver_create": [i32, ctypes.POINTER(CPBFConfig), ctypes.POINTER(CLambdaCorr), ctypes.POINTER(P)],
+        "akua_solver_destroy": [P],
+        "akua_solver_set_particles": [P, P, i32],
+        "akua_solver_get_particles": [P, P, i32],
+        "akua_solver_set_state": [P, P, P, P, i32],
+        "akua_solver_get_state": [P, P, P, P, P, P, P, i32],
+        "akua_solver_step": [P, f32, P, P],
+        "akua_solver_step_n": [P, f32, P, P, i32],
+        "akua_solver_step_device": [P, P, i32, f32, P, P, P],
+        "akua_solver_set_gravity": [P, P],
+        "akua_solver_synchronize": [P],
+        "akua_solver_num_particles": [P, ctypes.POINTER(i32)],
+        "akua_solver_set_graph": [P, i32],
+        "akua_debug_hashes": [P, P, i32],
+        "akua_debug_permutation": [P, P, i32],
+        "akua_debug_cell_ranges": [P, P, P, P, i32, ctypes.POINTER(i32)],
+        "akua_debug_neighbours": [P, P, P, i32],
+        "akua_solver_set_timing": [P, i32],
+        "akua_solver_reset_timing": [P],
+        "akua_solver_get_timing": [P, P, i32, P, P, i32, ctypes.POINTER(i32)],
+        "akua_solver_mean_neighbours": [P, ctypes.POINTER(ctypes.c_double)],
+    }
+    for name, args in sigs.items():
+        fn = getattr(L, name)
+        fn.argtypes = args
+        fn.restype = i32
+    _lib = L
+    return L
+
+
+def check(status: int) -> None:
+    if status != AKUA_OK:
+        msg = load().akua_last_error()
+        raise AkuaError(status, msg.decode() if msg else "")

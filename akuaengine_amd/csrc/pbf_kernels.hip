@@ -1,0 +1,391 @@
+// pbf_kernels.hip -- the PBF step on gfx950, one thread per particle slot.
+//
+// Stage map (reference kernel -> kernel here):
+//   kernel_predict_position + kernel_compute_hashes     -> k_predict_hash
+//   thrust::sort of the records                         -> radix_sort.hip + k_gather
+//   table fill + kernel_build_hash_table                -> k_gather (cell-table inserts)
+//   kernel_find_neighbours                              -> k_neighbours
+//   kernel_calculate_densities + kernel_calculate_lambdas   -> k_density_lambda (one sweep)
+//   kernel_calculate_position_delta + kernel_correct_position -> k_delta_correct
+//   ... last iteration also kernel_update_position_and_velocity
+//       + kernel_apply_boundary_velocity_damping        -> k_delta_correct<final>
+//   kernel_compute_vorticities                          -> k_vorticity
+//   kernel_apply_vorticity_confinement                  -> k_confinement
+//   kernel_apply_xsph_viscosity (Jacobi)                -> k_xsph
+// Every kernel reads the previous kernel's complete output (Jacobi), like the reference's
+// launch-then-synchronise sequence, except XSPH, which reads a copy of the velocities instead of
+// racing on them.
+#include "pbf_kernels.h"
+#include "pbf_math.h"
+
+namespace akua {
+
+namespace {
+
+__device__ __forceinline__ const SceneDev& scene_of(const DeviceState& d, uint32_t i) {
+    return d.scenes[d.chunkScene[i >> 8]];
+}
+
+__device__ __forceinline__ size_t list_base(uint32_t i, int maxN) {
+    return (size_t)(i >> 6) * (size_t)maxN * 64u + (i & 63u);
+}
+
+__global__ void k_fill_u32(uint32_t* __restrict__ p, uint32_t n, uint32_t v) {
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+
+// kernel_predict_position (IntegrationCUDA.cu:27-36) + kernel_compute_hashes
+// (NeighbourSearchCUDA.cu:36-44) with smoothRadius as the cell size (NeighbourSearchCUDA.cu:163).
+__global__ __launch_bounds__(kBlock) void k_predict_hash(DeviceState d, SphParams P, StepParams S) {
+    uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+    if (i >= d.nslots) return;
+    const SceneDev& sc = scene_of(d, i);
+    if (i >= sc.end) return;
+    float4 x = d.X[i], v = d.V[i];
+    f3 xs = predict_x(make3(x), make3(v), S);
+    int cx, cy, cz;
+    discretize(xs, P.cellHash, cx, cy, cz);
+    d.keys[i] = cell_hash(cx, cy, cz, sc.tableSize);
+}
+
+__device__ __forceinline__ uint32_t table_insert(uint32_t* tkey, const SceneDev& sc, uint32_t h) {
+    const uint32_t mask = (1u << sc.tabLog2) - 1u;
+    uint32_t s = table_slot(h, sc.tabLog2);
+    // The table holds at least twice as many slots as the scene has particles, so an empty
+    // slot is always found; the probe count is bounded all the same.
+    for (uint32_t probe = 0; probe <= mask; ++probe) {
+        uint32_t prev = atomicCAS(&tkey[sc.tabOffset + s], kEmptyKey, h);
+        if (prev == kEmptyKey || prev == h) return sc.tabOffset + s;
+        s = (s + 1u) & mask;
+    }
+    return kEmptyKey;
+}
+
+// Moves each particle to its sorted slot (the record move of the reference's thrust::sort) and
+// registers every bucket's [start, end) in the scene's cell table (kernel_build_hash_table,
+// NeighbourSearchCUDA.cu:52-65, plus the bucket end the reference finds by scanning).
+// x* is recomputed from the gathered x and v with the same operations as k_predict_hash, so it is
+// bit-identical to the value that was hashed.
+__global__ __launch_bounds__(kBlock) void k_gather(DeviceState d, SphParams P, StepParams S) {
+    uint32_t k = blockIdx.x * kBlock + threadIdx.x;
+    if (k >= d.nslots) return;
+    const SceneDev& sc = scene_of(d, k);
+    if (k >= sc.end) return;
+    uint32_t src = d.perm[k];
+    if (src >= d.nslots) return;  // a consistent sort never produces this
+    float4 x = d.X[src], v = d.V[src];
+    f3 xs = predict_x(make3(x), make3(v), S);
+    d.X0[k] = x;
+    d.XsA[k] = make_float4(xs.x, xs.y, xs.z, x.w);
+    d.Tdens[k] = d.dens[src];
+    d.Tlam[k] = d.lam[src];
+    d.Tid[k] = d.id[src];
+    uint32_t h = d.skeys[k];
+    bool first = (k == sc.begin) || (d.skeys[k - 1] != h);
+    bool last = (k + 1 == sc.end) || (d.skeys[k + 1] != h);
+    if (first || last) {
+        uint32_t slot = table_insert(d.tkey, sc, h);
+        if (slot != kEmptyKey) {
+            if (first) d.trange[slot].x = k;
+            if (last) d.trange[slot].y = k + 1;
+        }
+    }
+}
+
+__device__ __forceinline__ bool table_find(const uint32_t* __restrict__ tkey,
+                                           const uint2* __restrict__ trange, const SceneDev& sc,
+                                           uint32_t h, uint2& range) {
+    const uint32_t mask = (1u << sc.tabLog2) - 1u;
+    uint32_t s = table_slot(h, sc.tabLog2);
+    for (uint32_t probe = 0; probe <= mask; ++probe) {
+        uint32_t k = tkey[sc.tabOffset + s];
+        if (k == h) {
+            range = trange[sc.tabOffset + s];
+            // a bucket lies inside its scene; clamp so a torn entry can never index outside
+            range.x = max(range.x, sc.begin);
+            range.y = min(range.y, sc.end);
+            return true;
+        }
+        if (k == kEmptyKey) return false;
+        s = (s + 1u) & mask;
+    }
+    return false;
+}
+
+// kernel_find_neighbours (NeighbourSearchCUDA.cu:72-130): stencil cells in (dx, dy, dz) order,
+// each bucket walked in sorted order, self skipped, r^2 < h^2 kept, at most maxNeighbours.  A
+// bucket reached from two stencil cells with equal hashes is walked twice, as in the reference.
+__global__ __launch_bounds__(kBlock) void k_neighbours(DeviceState d, SphParams P) {
+    uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+    uint32_t count = 0;
+    if (i < d.nslots) {
+        const SceneDev& sc = scene_of(d, i);
+        if (i < sc.end) {
+            const float4 xi4 = d.XsA[i];
+            const f3 xi = make3(xi4);
+            int cx, cy, cz;
+            discretize(xi, P.cellSearch, cx, cy, cz);
+            uint32_t* out = d.nbr + list_base(i, P.maxN);
+            const uint32_t maxN = (uint32_t)P.maxN;
+            for (int dx = -1; dx <= 1; ++dx)
+                for (int dy = -1; dy <= 1; ++dy)
+                    for (int dz = -1; dz <= 1; ++dz) {
+                        if (count >= maxN) continue;
+                        uint32_t h = cell_hash((int)((uint32_t)cx + (uint32_t)dx),
+                                               (int)((uint32_t)cy + (uint32_t)dy),
+                                               (int)((uint32_t)cz + (uint32_t)dz), sc.tableSize);
+                        uint2 r;
+                        if (!table_find(d.tkey, d.trange, sc, h, r)) continue;
+                        for (uint32_t k = r.x; k < r.y && count < maxN; ++k) {
+                            if (k == i) continue;
+                            f3 sep = xi - make3(d.XsA[k]);
+                            if (dot3(sep, sep) < P.h2) {
+                                out[(size_t)count * 64u] = k;
+                                ++count;
+                            }
+                        }
+                    }
+        }
+        d.cnt[i] = count;
+    }
+    // neighbour statistics: one atomic per wave
+    unsigned long long c = count;
+    for (int off = 32; off > 0; off >>= 1) c += __shfl_down(c, off, 64);
+    if (__lane_id() == 0 && c) atomicAdd(d.nbrSum, c);
+}
+
+// kernel_calculate_densities (ConstraintSolverCUDA.cu:16-42) and kernel_calculate_lambdas
+// (ConstraintSolverCUDA.cu:51-97) in one neighbour sweep.  Each sum keeps the reference's order
+// (the list order), so fusing the loops changes no rounding.
+__global__ __launch_bounds__(kBlock) void k_density_lambda(DeviceState d, SphParams P,
+                                                           const float4* __restrict__ Xs) {
+    uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+    if (i >= d.nslots) return;
+    if (i >= scene_of(d, i).end) return;
+    const float4 xi4 = Xs[i];
+    const f3 xi = make3(xi4);
+    const uint32_t n = d.cnt[i];
+    const uint32_t* lp = d.nbr + list_base(i, P.maxN);
+    float rho = xi4.w * P.w0;
+    f3 gi{0.0f, 0.0f, 0.0f};
+    float sum = 0.0f;
+    const float negInv = -P.invRest;
+    for (uint32_t k = 0; k < n; ++k) {
+        uint32_t j = lp[(size_t)k * 64u];
+        float4 xj4 = Xs[j];
+        f3 sep = xi - make3(xj4);
+        float d2 = dot3(sep, sep);
+        rho = rho + xj4.w * poly6(d2, P);
+        f3 g = grad_spiky(sep, P);
+        gi = gi + xj4.w * g;
+        f3 gj = (negInv * xj4.w) * g;
+        sum = sum + dot3(gj, gj);
+    }
+    gi = gi * P.invRest;
+    float C = rho * P.invRest - 1.0f;
+    d.dens[i] = rho;
+    d.lam[i] = -C / ((sum + dot3(gi, gi)) + P.relax);
+}
+
+// resolve_collision over the six walls in order (IntegrationCUDA.cu:51-102), restitution 0 and
+// friction 0.95 (PBFSolver.cpp:64); vector form as written, so every rounding matches.
+__device__ __forceinline__ f3 damp_walls(f3 x, f3 v, const SceneDev& sc) {
+    const float minDist = 0.025f, restitution = 0.0f, friction = 0.95f;
+#pragma unroll
+    for (int k = 0; k < 6; ++k) {
+        const int a = k >> 1;
+        const float sgn = (k & 1) ? -1.0f : 1.0f;
+        const float b = (k & 1) ? sc.bmax[a] : sc.bmin[a];
+        f3 pt{a == 0 ? b : 0.0f, a == 1 ? b : 0.0f, a == 2 ? b : 0.0f};
+        f3 nm{a == 0 ? sgn : 0.0f, a == 1 ? sgn : 0.0f, a == 2 ? sgn : 0.0f};
+        float distance = dot3(x - pt, nm);
+        float approaching = dot3(v, nm);
+        if (distance < minDist) {
+            f3 vn = approaching * nm;
+            f3 vt = v - vn;
+            if (approaching < 0.0f) {
+                v = (-restitution) * vn + (1.0f - friction) * vt;
+            } else if (fabsf(approaching) < 1e-5f) {
+                v = (1.0f - friction) * vt;
+            }
+        }
+    }
+    return v;
+}
+
+// kernel_calculate_position_delta (ConstraintSolverCUDA.cu:99-130), then kernel_correct_position
+// with handle_particle_collision (ConstraintSolverCUDA.cu:136-169).  x* is double-buffered: the
+// kernel reads XsIn and writes XsOut, so every particle sees the positions of the same iteration.
+// kFinal: also kernel_update_position_and_velocity (IntegrationCUDA.cu:38-49) and the wall
+// damping, writing the step's positions X, velocities V and the sorted ids.
+template <bool kFinal, bool kApplyDelta>
+__global__ __launch_bounds__(kBlock) void k_delta_correct(DeviceState d, SphParams P,
+                                                          StepParams S,
+                                                          const float4* __restrict__ XsIn,
+                                                          float4* __restrict__ XsOut) {
+    uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+    if (i >= d.nslots) return;
+    const SceneDev& sc = scene_of(d, i);
+    if (i >= sc.end) return;
+    const float4 xi4 = XsIn[i];
+    const f3 xi = make3(xi4);
+    f3 q = xi;
+    if (kApplyDelta) {
+        const float li = d.lam[i];
+        const uint32_t n = d.cnt[i];
+        const uint32_t* lp = d.nbr + list_base(i, P.maxN);
+        f3 dp{0.0f, 0.0f, 0.0f};
+        for (uint32_t k = 0; k < n; ++k) {
+            uint32_t j = lp[(size_t)k * 64u];
+            float4 xj4 = XsIn[j];
+            float lj = d.lam[j];
+            f3 sep = xi - make3(xj4);
+            float d2 = dot3(sep, sep);
+            float lc = P.negCorrK * corr_pow(poly6(d2, P) / P.wdq, P);
+            f3 g = grad_spiky(sep, P);
+            dp = dp + (((li + lj) + lc) * xj4.w) * g;
+        }
+        dp = dp * P.invRest;
+        q = xi + dp;
+        const float minDist = 0.025f, stiff = 0.5f;
+        float qa[3] = {q.x, q.y, q.z};
+        float c[3] = {0.0f, 0.0f, 0.0f};
+#pragma unroll
+        for (int a = 0; a < 3; ++a) {
+            if (qa[a] < sc.bmin[a] + minDist) c[a] = c[a] + stiff * (sc.bmin[a] + minDist - qa[a]);
+            if (qa[a] > sc.bmax[a] - minDist) c[a] = c[a] + stiff * (sc.bmax[a] - minDist - qa[a]);
+        }
+        q = q + f3{c[0], c[1], c[2]};
+    }
+    if (!kFinal) {
+        XsOut[i] = make_float4(q.x, q.y, q.z, xi4.w);
+        return;
+    }
+    const float4 x0 = d.X0[i];
+    f3 v = (q - make3(x0)) / S.dt;
+    v = damp_walls(q, v, sc);
+    d.X[i] = make_float4(q.x, q.y, q.z, xi4.w);
+    d.V[i] = make_float4(v.x, v.y, v.z, 0.0f);
+    d.id[i] = d.Tid[i];
+    if (!kApplyDelta) {  // no iteration ran: density and lambda carry over
+        d.dens[i] = d.Tdens[i];
+        d.lam[i] = d.Tlam[i];
+    }
+}
+
+// kernel_compute_vorticities (IntegrationCUDA.cu:104-128).  W.w keeps |omega| for the next pass.
+__global__ __launch_bounds__(kBlock) void k_vorticity(DeviceState d, SphParams P) {
+    uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+    if (i >= d.nslots) return;
+    if (i >= scene_of(d, i).end) return;
+    const f3 xi = make3(d.X[i]);
+    const f3 vi = make3(d.V[i]);
+    const uint32_t n = d.cnt[i];
+    const uint32_t* lp = d.nbr + list_base(i, P.maxN);
+    f3 w{0.0f, 0.0f, 0.0f};
+    for (uint32_t k = 0; k < n; ++k) {
+        uint32_t j = lp[(size_t)k * 64u];
+        float4 xj4 = d.X[j];
+        f3 vij = make3(d.V[j]) - vi;
+        f3 sep = xi - make3(xj4);
+        w = w + (-xj4.w) * cross3(vij, grad_spiky(sep, P));
+    }
+    d.W[i] = make_float4(w.x, w.y, w.z, length3(w));
+}
+
+// kernel_apply_vorticity_confinement (IntegrationCUDA.cu:130-165), writing Vc.
+__global__ __launch_bounds__(kBlock) void k_confinement(DeviceState d, SphParams P, StepParams S) {
+    uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+    if (i >= d.nslots) return;
+    if (i >= scene_of(d, i).end) return;
+    const f3 xi = make3(d.X[i]);
+    const float4 wi = d.W[i];
+    const float invD = 1.0f / d.dens[i];
+    const uint32_t n = d.cnt[i];
+    const uint32_t* lp = d.nbr + list_base(i, P.maxN);
+    f3 eta{0.0f, 0.0f, 0.0f};
+    for (uint32_t k = 0; k < n; ++k) {
+        uint32_t j = lp[(size_t)k * 64u];
+        float4 xj4 = d.X[j];
+        float diff = wi.w - d.W[j].w;
+        f3 sep = xi - make3(xj4);
+        eta = eta + (xj4.w * diff) * grad_spiky(sep, P);
+    }
+    eta = eta * invD;
+    float el = length3(eta);
+    float4 v = d.V[i];
+    if (!(el < 1e-5f)) {
+        f3 N = eta / el;
+        f3 f = P.vortEps * cross3(N, make3(wi));
+        f3 vn = make3(v) + S.dt * f;
+        v = make_float4(vn.x, vn.y, vn.z, 0.0f);
+    }
+    d.Vc[i] = v;
+}
+
+// kernel_apply_xsph_viscosity (IntegrationCUDA.cu:167-195), Jacobi: reads Vc, writes V.
+__global__ __launch_bounds__(kBlock) void k_xsph(DeviceState d, SphParams P) {
+    uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+    if (i >= d.nslots) return;
+    if (i >= scene_of(d, i).end) return;
+    const f3 xi = make3(d.X[i]);
+    const f3 vi = make3(d.Vc[i]);
+    const uint32_t n = d.cnt[i];
+    const uint32_t* lp = d.nbr + list_base(i, P.maxN);
+    f3 dv{0.0f, 0.0f, 0.0f};
+    for (uint32_t k = 0; k < n; ++k) {
+        uint32_t j = lp[(size_t)k * 64u];
+        float4 xj4 = d.X[j];
+        f3 vij = make3(d.Vc[j]) - vi;
+        f3 sep = xi - make3(xj4);
+        float w = poly6(dot3(sep, sep), P);
+        dv = dv + ((xj4.w / d.dens[j]) * vij) * w;
+    }
+    f3 vn = vi + P.visc * dv;
+    d.V[i] = make_float4(vn.x, vn.y, vn.z, 0.0f);
+}
+
+inline unsigned grid(uint32_t n) { return (n + kBlock - 1) / kBlock; }
+
+}  // namespace
+
+void launch_fill_u32(hipStream_t st, uint32_t* p, uint32_t n, uint32_t v) {
+    if (n) k_fill_u32<<<grid(n), kBlock, 0, st>>>(p, n, v);
+}
+void launch_predict_hash(hipStream_t st, const DeviceState& d, const SphParams& P,
+                         const StepParams& S) {
+    k_predict_hash<<<grid(d.nslots), kBlock, 0, st>>>(d, P, S);
+}
+void launch_gather(hipStream_t st, const DeviceState& d, const SphParams& P, const StepParams& S) {
+    k_gather<<<grid(d.nslots), kBlock, 0, st>>>(d, P, S);
+}
+void launch_neighbours(hipStream_t st, const DeviceState& d, const SphParams& P) {
+    k_neighbours<<<grid(d.nslots), kBlock, 0, st>>>(d, P);
+}
+void launch_density_lambda(hipStream_t st, const DeviceState& d, const SphParams& P,
+                           const float4* Xs) {
+    k_density_lambda<<<grid(d.nslots), kBlock, 0, st>>>(d, P, Xs);
+}
+void launch_delta_correct(hipStream_t st, const DeviceState& d, const SphParams& P,
+                          const StepParams& S, const float4* XsIn, float4* XsOut, bool final,
+                          bool applyDelta) {
+    if (final && applyDelta)
+        k_delta_correct<true, true><<<grid(d.nslots), kBlock, 0, st>>>(d, P, S, XsIn, XsOut);
+    else if (final)
+        k_delta_correct<true, false><<<grid(d.nslots), kBlock, 0, st>>>(d, P, S, XsIn, XsOut);
+    else
+        k_delta_correct<false, true><<<grid(d.nslots), kBlock, 0, st>>>(d, P, S, XsIn, XsOut);
+}
+void launch_vorticity(hipStream_t st, const DeviceState& d, const SphParams& P) {
+    k_vorticity<<<grid(d.nslots), kBlock, 0, st>>>(d, P);
+}
+void launch_confinement(hipStream_t st, const DeviceState& d, const SphParams& P,
+                        const StepParams& S) {
+    k_confinement<<<grid(d.nslots), kBlock, 0, st>>>(d, P, S);
+}
+void launch_xsph(hipStream_t st, const DeviceState& d, const SphParams& P) {
+    k_xsph<<<grid(d.nslots), kBlock, 0, st>>>(d, P);
+}
+
+}  // namespace akua

@@ -1,0 +1,840 @@
+// solver.cpp -- the C ABI (include/akua.h) over the gfx950 PBF kernels.
+//
+// akua_solver is the headless counterpart of AkuaEngine::PBFSolver (PBFSolver.h:12-31): it owns
+// every device buffer, sequences the stages of PBFSolver::step (PBFSolver.cpp:22-78) on one HIP
+// stream without host synchronisation, and replays the whole step as one hipGraph.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <new>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "../../include/akua.h"
+#include "aos.h"
+#include "pbf_kernels.h"
+#include "pbf_types.h"
+#include "radix_sort.h"
+
+using namespace akua;
+
+namespace {
+
+thread_local std::string g_lastError;
+
+struct AkuaError : std::runtime_error {
+    akua_status code;
+    AkuaError(akua_status c, const std::string& m) : std::runtime_error(m), code(c) {}
+};
+
+#define HIP_CHECK(expr)                                                                    \
+    do {                                                                                   \
+        hipError_t _e = (expr);                                                            \
+        if (_e != hipSuccess)                                                              \
+            throw AkuaError(_e == hipErrorOutOfMemory ? AKUA_ERR_OUT_OF_MEMORY : AKUA_ERR_HIP, \
+                            std::string(#expr) + ": " + hipGetErrorString(_e));            \
+    } while (0)
+
+void fail(akua_status c, const std::string& m) { throw AkuaError(c, m); }
+
+template <class F>
+akua_status guarded(F&& f) {
+    try {
+        f();
+        g_lastError.clear();
+        return AKUA_OK;
+    } catch (const AkuaError& e) {
+        g_lastError = e.what();
+        return e.code;
+    } catch (const std::bad_alloc&) {
+        g_lastError = "host allocation failed";
+        return AKUA_ERR_OUT_OF_MEMORY;
+    } catch (const std::exception& e) {
+        g_lastError = e.what();
+        return AKUA_ERR_HIP;
+    }
+}
+
+// The constants of the SPH kernels, computed with the float expressions of
+// SmoothingKernelsCUDA.h:16-28 (poly6 and spiky factors) and ConstraintSolverCUDA.cu:201, :123-124.
+float poly6_host(float d2, float h2, float cp) {
+    if (d2 > h2) return 0.0f;
+    float t = h2 - d2;
+    return cp * ((t * t) * t);
+}
+
+SphParams make_sph(const akua_pbf_config& c, const akua_lambda_corr& corr) {
+    SphParams P{};
+    P.h = c.smoothRadius;
+    P.h2 = P.h * P.h;
+    P.cp = 315.0f / (64.0f * 3.14f * powf(P.h, 9.0f));
+    P.cs = -45.0f / (3.14f * powf(P.h, 6.0f));
+    P.w0 = poly6_host(0.0f, P.h2, P.cp);
+    float dq2 = corr.delta_q * corr.delta_q;
+    P.wdq = poly6_host(dq2, P.h2, P.cp);
+    P.invRest = 1.0f / c.restDensity;
+    P.relax = c.relaxation;
+    P.negCorrK = -corr.k;
+    P.corrN = corr.n;
+    P.corrNInt = (corr.n >= 0.0f && corr.n <= 16.0f && corr.n == floorf(corr.n)) ? (int)corr.n : -1;
+    P.cellHash = c.smoothRadius;
+    P.cellSearch = c.spatialHashCellSize;
+    P.vortEps = c.vorticityEpsilon;
+    P.visc = c.viscosity;
+    P.maxN = c.maxNeighbours;
+    return P;
+}
+
+uint32_t ceil_log2(uint64_t v) {
+    uint32_t r = 0;
+    while ((1ull << r) < v) ++r;
+    return r;
+}
+
+template <class T>
+struct DevBuf {
+    T* p = nullptr;
+    size_t n = 0;
+    void alloc(size_t count) {
+        n = count;
+        if (count) HIP_CHECK(hipMalloc(&p, count * sizeof(T)));
+    }
+    ~DevBuf() {
+        if (p) (void)hipFree(p);
+    }
+};
+
+struct TimedLaunch {
+    const char* name;
+    hipEvent_t a, b;
+};
+
+}  // namespace
+
+struct akua_solver {
+    akua_pbf_config cfg{};
+    akua_lambda_corr corr{};
+    SphParams sph{};
+    int device = 0;
+    hipStream_t stream = nullptr;
+
+    // scenes
+    std::vector<SceneDev> scenes;
+    std::vector<uint32_t> chunkSceneHost;
+    uint32_t nslots = 0;
+    uint32_t nparticles = 0;
+    uint32_t tableTotal = 0;
+    int keyBits = 1;
+    SortPlan plan;
+    std::vector<SortTile> tilesHost;
+    uint32_t countsPerDigit = 0;
+    bool scenesDirty = true;
+
+    // device buffers
+    DevBuf<SceneDev> dScenes;
+    DevBuf<uint32_t> dChunkScene;
+    DevBuf<float4> X, V, X0, XsA, XsB, W, Vc, color;
+    DevBuf<float> dens, lam, Tdens, Tlam, size;
+    DevBuf<uint32_t> id, Tid, keysA, valsA, keysB, valsB, counts, tkey, nbr, cnt;
+    DevBuf<uint2> trange;
+    DevBuf<SortTile> dTiles;
+    DevBuf<unsigned long long> nbrSum;
+    bool hasState = false;
+    int sortResult = 0;  // 0: (keysA, valsA); 1: (keysB, valsB)
+
+    // step graph
+    bool useGraph = true;
+    hipGraphExec_t graphExec = nullptr;
+    StepParams graphParams{};
+    uint64_t stepsSinceStats = 0;
+
+    // timing
+    bool timing = false;
+    std::vector<TimedLaunch> pending;
+    std::vector<hipEvent_t> eventPool;
+    size_t eventNext = 0;
+    std::map<std::string, std::pair<double, int64_t>> timeAcc;
+    std::vector<std::string> timeOrder;
+
+    DeviceState devState() {
+        DeviceState d{};
+        d.nslots = nslots;
+        d.scenes = dScenes.p;
+        d.chunkScene = dChunkScene.p;
+        d.X = X.p; d.V = V.p; d.dens = dens.p; d.lam = lam.p; d.id = id.p;
+        d.X0 = X0.p; d.XsA = XsA.p; d.XsB = XsB.p; d.W = W.p; d.Vc = Vc.p;
+        d.Tdens = Tdens.p; d.Tlam = Tlam.p; d.Tid = Tid.p;
+        d.keys = keysA.p;
+        d.skeys = sortResult == 0 ? keysA.p : keysB.p;
+        d.perm = sortResult == 0 ? valsA.p : valsB.p;
+        d.tkey = tkey.p; d.trange = trange.p; d.tableTotal = tableTotal;
+        d.nbr = nbr.p; d.cnt = cnt.p; d.nbrSum = nbrSum.p;
+        return d;
+    }
+
+    ~akua_solver() {
+        if (graphExec) (void)hipGraphExecDestroy(graphExec);
+        for (auto e : eventPool) (void)hipEventDestroy(e);
+        if (stream) (void)hipStreamDestroy(stream);
+    }
+};
+
+namespace {
+
+void validate_config(const akua_pbf_config* cfg, const akua_lambda_corr* corr) {
+    if (!cfg || !corr) fail(AKUA_ERR_INVALID_ARG, "config pointers must not be NULL");
+    if (!(cfg->smoothRadius > 0.0f) || !std::isfinite(cfg->smoothRadius))
+        fail(AKUA_ERR_INVALID_ARG, "smoothRadius must be a positive finite number");
+    if (!(cfg->spatialHashCellSize > 0.0f) || !std::isfinite(cfg->spatialHashCellSize))
+        fail(AKUA_ERR_INVALID_ARG, "spatialHashCellSize must be a positive finite number");
+    if (!(cfg->restDensity > 0.0f) || !std::isfinite(cfg->restDensity))
+        fail(AKUA_ERR_INVALID_ARG, "restDensity must be a positive finite number");
+    if (cfg->maxNeighbours < 1) fail(AKUA_ERR_INVALID_ARG, "maxNeighbours must be >= 1");
+    if (cfg->solverIterations < 0) fail(AKUA_ERR_INVALID_ARG, "solverIterations must be >= 0");
+    if (!std::isfinite(cfg->relaxation) || !std::isfinite(cfg->viscosity) ||
+        !std::isfinite(cfg->vorticityEpsilon) || !std::isfinite(corr->k) ||
+        !std::isfinite(corr->n) || !std::isfinite(corr->delta_q))
+        fail(AKUA_ERR_INVALID_ARG, "non-finite solver parameter");
+    for (int k = 0; k < 3; ++k)
+        if (!std::isfinite(cfg->gravity[k])) fail(AKUA_ERR_INVALID_ARG, "non-finite gravity");
+}
+
+void check_box(const float* bmin, const float* bmax) {
+    if (!bmin || !bmax) fail(AKUA_ERR_INVALID_ARG, "box pointers must not be NULL");
+    for (int k = 0; k < 3; ++k)
+        if (!std::isfinite(bmin[k]) || !std::isfinite(bmax[k]))
+            fail(AKUA_ERR_INVALID_ARG, "non-finite box bound");
+}
+
+void setup(akua_solver* s, const std::vector<uint32_t>& counts) {
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1)
+        fail(AKUA_ERR_NO_DEVICE, "no HIP device is visible");
+    HIP_CHECK(hipGetDevice(&s->device));
+    HIP_CHECK(hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking));
+
+    // scene slot layout
+    uint32_t slot = 0, tab = 0;
+    uint32_t maxTable = 1;
+    std::vector<uint32_t> sb, se;
+    for (size_t i = 0; i < counts.size(); ++i) {
+        uint64_t n = counts[i];
+        uint64_t T = (uint64_t)s->cfg.maxNeighbours * n;
+        if (T > 0xFFFFFFFFull)
+            fail(AKUA_ERR_UNSUPPORTED, "maxNeighbours * numParticles must fit in 32 bits");
+        SceneDev sc{};
+        sc.begin = slot;
+        sc.end = slot + (uint32_t)n;
+        sc.tableSize = (uint32_t)T;
+        uint32_t lg = std::max<uint32_t>(6u, ceil_log2(2ull * n));
+        sc.tabOffset = tab;
+        sc.tabLog2 = lg;
+        tab += 1u << lg;
+        for (int k = 0; k < 4; ++k) { sc.bmin[k] = 0.0f; sc.bmax[k] = 1.0f; }
+        s->scenes.push_back(sc);
+        sb.push_back(sc.begin);
+        se.push_back(sc.end);
+        maxTable = std::max(maxTable, sc.tableSize);
+        uint64_t next = (uint64_t)slot + n;
+        next = (next + kSlotAlign - 1) / kSlotAlign * kSlotAlign;
+        if (counts.size() == 1) next = slot + n;  // a single scene needs no padding
+        if (next > 0x7FFFFFFFull) fail(AKUA_ERR_UNSUPPORTED, "too many particles");
+        slot = (uint32_t)next;
+    }
+    s->nslots = slot;
+    s->tableTotal = tab;
+    s->nparticles = 0;
+    for (auto c : counts) s->nparticles += c;
+    s->keyBits = std::max<int>(1, (int)ceil_log2((uint64_t)maxTable));
+    s->plan.bits = s->keyBits;
+    s->plan.digitBits = 8;
+    // Balance the passes: the same number of passes with the smallest digit that covers the bits.
+    {
+        int P = s->plan.passes();
+        s->plan.digitBits = std::max(4, (s->keyBits + P - 1) / P);
+    }
+    s->countsPerDigit = build_sort_tiles(sb, se, s->tilesHost);
+
+    const uint32_t nChunks = (s->nslots + 255) / 256;
+    s->chunkSceneHost.assign(nChunks, 0);
+    for (size_t i = 0; i < s->scenes.size(); ++i) {
+        const SceneDev& sc = s->scenes[i];
+        for (uint32_t c = sc.begin / 256; c < nChunks && c * 256 < std::max(sc.end, sc.begin + 1); ++c)
+            s->chunkSceneHost[c] = (uint32_t)i;
+    }
+
+    const size_t ns = s->nslots;
+    const size_t nsList = (ns + 63) / 64 * 64;
+    s->dScenes.alloc(s->scenes.size());
+    s->dChunkScene.alloc(nChunks);
+    s->X.alloc(ns); s->V.alloc(ns); s->X0.alloc(ns); s->XsA.alloc(ns); s->XsB.alloc(ns);
+    s->W.alloc(ns); s->Vc.alloc(ns); s->color.alloc(ns);
+    s->dens.alloc(ns); s->lam.alloc(ns); s->Tdens.alloc(ns); s->Tlam.alloc(ns); s->size.alloc(ns);
+    s->id.alloc(ns); s->Tid.alloc(ns);
+    s->keysA.alloc(ns); s->valsA.alloc(ns); s->keysB.alloc(ns); s->valsB.alloc(ns);
+    s->counts.alloc((size_t)s->countsPerDigit * 256u);
+    s->tkey.alloc(s->tableTotal);
+    s->trange.alloc(s->tableTotal);
+    s->nbr.alloc(nsList * (size_t)s->cfg.maxNeighbours);
+    s->cnt.alloc(ns);
+    s->dTiles.alloc(s->tilesHost.size());
+    s->nbrSum.alloc(1);
+    HIP_CHECK(hipMemcpy(s->dChunkScene.p, s->chunkSceneHost.data(), nChunks * 4, hipMemcpyHostToDevice));
+    HIP_CHECK(hipMemcpy(s->dTiles.p, s->tilesHost.data(), s->tilesHost.size() * sizeof(SortTile),
+                        hipMemcpyHostToDevice));
+    HIP_CHECK(hipMemset(s->nbrSum.p, 0, sizeof(unsigned long long)));
+    HIP_CHECK(hipMemset(s->W.p, 0, ns * sizeof(float4)));
+    HIP_CHECK(hipMemset(s->keysA.p, 0, ns * 4));
+    HIP_CHECK(hipMemset(s->keysB.p, 0, ns * 4));
+    HIP_CHECK(hipMemset(s->X.p, 0, ns * sizeof(float4)));
+    HIP_CHECK(hipMemset(s->V.p, 0, ns * sizeof(float4)));
+    HIP_CHECK(hipMemset(s->cnt.p, 0, ns * 4));
+    s->sortResult = s->plan.passes() % 2 == 0 ? 0 : 1;
+    s->sph = make_sph(s->cfg, s->corr);
+}
+
+void timing_hook(void* ctx, const char* name, int phase);
+
+struct Launcher {
+    akua_solver* s;
+    hipStream_t st;
+    bool timed;
+    void before(const char* name) {
+        if (timed) timing_hook(s, name, 0);
+    }
+    void after(const char* name) {
+        if (timed) timing_hook(s, name, 1);
+    }
+};
+
+hipEvent_t next_event(akua_solver* s) {
+    if (s->eventNext == s->eventPool.size()) {
+        hipEvent_t e;
+        HIP_CHECK(hipEventCreate(&e));
+        s->eventPool.push_back(e);
+    }
+    return s->eventPool[s->eventNext++];
+}
+
+void timing_hook(void* ctx, const char* name, int phase) {
+    auto* s = static_cast<akua_solver*>(ctx);
+    if (phase == 0) {
+        TimedLaunch t{name, next_event(s), nullptr};
+        HIP_CHECK(hipEventRecord(t.a, s->stream));
+        s->pending.push_back(t);
+    } else {
+        TimedLaunch& t = s->pending.back();
+        t.b = next_event(s);
+        HIP_CHECK(hipEventRecord(t.b, s->stream));
+    }
+}
+
+void collect_timing(akua_solver* s) {
+    if (s->pending.empty()) return;
+    HIP_CHECK(hipStreamSynchronize(s->stream));
+    for (auto& t : s->pending) {
+        float ms = 0.0f;
+        HIP_CHECK(hipEventElapsedTime(&ms, t.a, t.b));
+        auto it = s->timeAcc.find(t.name);
+        if (it == s->timeAcc.end()) {
+            s->timeOrder.push_back(t.name);
+            s->timeAcc[t.name] = {ms, 1};
+        } else {
+            it->second.first += ms;
+            it->second.second += 1;
+        }
+    }
+    s->pending.clear();
+    s->eventNext = 0;
+}
+
+// PBFSolver::step (PBFSolver.cpp:22-78) as one stream-ordered sequence of launches.
+void enqueue_step(akua_solver* s, hipStream_t st, const StepParams& S, bool timed) {
+    Launcher L{s, st, timed};
+    DeviceState d = s->devState();
+    const SphParams& P = s->sph;
+    L.before("fill_table");
+    launch_fill_u32(st, s->tkey.p, s->tableTotal, kEmptyKey);
+    L.after("fill_table");
+    L.before("predict_hash");
+    launch_predict_hash(st, d, P, S);
+    L.after("predict_hash");
+    radix_sort_pairs(st, s->plan, s->keysA.p, s->valsA.p, s->keysB.p, s->valsB.p, s->dTiles.p,
+                     (uint32_t)s->tilesHost.size(), s->countsPerDigit, s->counts.p, s->nslots,
+                     timed ? timing_hook : nullptr, s);
+    L.before("gather");
+    launch_gather(st, d, P, S);
+    L.after("gather");
+    L.before("neighbours");
+    launch_neighbours(st, d, P);
+    L.after("neighbours");
+    const int I = s->cfg.solverIterations;
+    float4* xs[2] = {s->XsA.p, s->XsB.p};
+    if (I == 0) {
+        L.before("update_damping");
+        launch_delta_correct(st, d, P, S, xs[0], nullptr, true, false);
+        L.after("update_damping");
+    }
+    for (int it = 0; it < I; ++it) {
+        const float4* in = xs[it & 1];
+        float4* out = xs[(it + 1) & 1];
+        L.before("density_lambda");
+        launch_density_lambda(st, d, P, in);
+        L.after("density_lambda");
+        const bool last = it + 1 == I;
+        L.before(last ? "delta_correct_update" : "delta_correct");
+        launch_delta_correct(st, d, P, S, in, out, last, true);
+        L.after(last ? "delta_correct_update" : "delta_correct");
+    }
+    L.before("vorticity");
+    launch_vorticity(st, d, P);
+    L.after("vorticity");
+    L.before("confinement");
+    launch_confinement(st, d, P, S);
+    L.after("confinement");
+    L.before("xsph");
+    launch_xsph(st, d, P);
+    L.after("xsph");
+}
+
+void upload_scenes(akua_solver* s) {
+    if (!s->scenesDirty) return;
+    HIP_CHECK(hipStreamSynchronize(s->stream));
+    HIP_CHECK(hipMemcpy(s->dScenes.p, s->scenes.data(), s->scenes.size() * sizeof(SceneDev),
+                        hipMemcpyHostToDevice));
+    s->scenesDirty = false;
+}
+
+void set_box(akua_solver* s, size_t scene, const float* bmin, const float* bmax) {
+    SceneDev& sc = s->scenes[scene];
+    for (int k = 0; k < 3; ++k) {
+        if (sc.bmin[k] != bmin[k] || sc.bmax[k] != bmax[k]) s->scenesDirty = true;
+        sc.bmin[k] = bmin[k];
+        sc.bmax[k] = bmax[k];
+    }
+}
+
+void run_steps(akua_solver* s, float dt, int nsteps) {
+    if (!s->hasState) fail(AKUA_ERR_STATE, "set the particles before stepping");
+    if (!(dt > 0.0f) || !std::isfinite(dt)) fail(AKUA_ERR_INVALID_ARG, "deltaTime must be > 0");
+    upload_scenes(s);
+    StepParams S{dt, {s->cfg.gravity[0], s->cfg.gravity[1], s->cfg.gravity[2]}};
+    for (int k = 0; k < nsteps; ++k) {
+        if (s->timing) {
+            enqueue_step(s, s->stream, S, true);
+            collect_timing(s);
+        } else if (s->useGraph) {
+            bool same = s->graphExec && std::memcmp(&S, &s->graphParams, sizeof(S)) == 0;
+            if (!same) {
+                if (s->graphExec) {
+                    HIP_CHECK(hipGraphExecDestroy(s->graphExec));
+                    s->graphExec = nullptr;
+                }
+                hipGraph_t g;
+                HIP_CHECK(hipStreamBeginCapture(s->stream, hipStreamCaptureModeThreadLocal));
+                enqueue_step(s, s->stream, S, false);
+                HIP_CHECK(hipStreamEndCapture(s->stream, &g));
+                hipError_t e = hipGraphInstantiate(&s->graphExec, g, nullptr, nullptr, 0);
+                (void)hipGraphDestroy(g);
+                HIP_CHECK(e);
+                s->graphParams = S;
+            }
+            HIP_CHECK(hipGraphLaunch(s->graphExec, s->stream));
+        } else {
+            enqueue_step(s, s->stream, S, false);
+        }
+        HIP_CHECK(hipGetLastError());
+        s->stepsSinceStats++;
+    }
+}
+
+void check_n(akua_solver* s, int32_t n) {
+    if (n != (int32_t)s->nparticles)
+        fail(AKUA_ERR_INVALID_ARG, "n does not match the solver's particle count");
+}
+
+void upload_soa(akua_solver* s, const float* pos, const float* vel, const float* mass,
+                const float* dens, const float* lam, const float* color, const float* size,
+                bool fromAos, const akua_particle* aos) {
+    const size_t n = s->nparticles;
+    std::vector<float4> X(s->nslots, make_float4(0, 0, 0, 0)), V(s->nslots, make_float4(0, 0, 0, 0));
+    std::vector<float4> C(s->nslots, make_float4(0, 0, 1, 1));
+    std::vector<float> D(s->nslots, 0.0f), L(s->nslots, 0.0f), Z(s->nslots, 0.0f);
+    std::vector<uint32_t> I(s->nslots, 0);
+    // particles fill the scenes' slot ranges in scene order
+    size_t k = 0;
+    for (const SceneDev& sc : s->scenes) {
+        for (uint32_t slot = sc.begin; slot < sc.end; ++slot, ++k) {
+            float p[3], v[3], m, dd, ll;
+            if (fromAos) {
+                const akua_particle& a = aos[k];
+                for (int c = 0; c < 3; ++c) { p[c] = a.position[c]; v[c] = a.velocity[c]; }
+                m = a.mass; dd = a.density; ll = a.lambda;
+                C[k] = make_float4(a.color[0], a.color[1], a.color[2], a.color[3]);
+                Z[k] = a.size;
+            } else {
+                for (int c = 0; c < 3; ++c) { p[c] = pos[3 * k + c]; v[c] = vel[3 * k + c]; }
+                m = mass ? mass[k] : 1.0f;
+                dd = dens ? dens[k] : 0.0f;
+                ll = lam ? lam[k] : 0.0f;
+                if (color) C[k] = make_float4(color[4 * k], color[4 * k + 1], color[4 * k + 2], color[4 * k + 3]);
+                if (size) Z[k] = size[k];
+            }
+            for (int c = 0; c < 3; ++c)
+                if (!std::isfinite(p[c]) || !std::isfinite(v[c]))
+                    fail(AKUA_ERR_NON_FINITE, "particle " + std::to_string(k) + " has a non-finite position or velocity");
+            if (!std::isfinite(m)) fail(AKUA_ERR_NON_FINITE, "non-finite mass");
+            X[slot] = make_float4(p[0], p[1], p[2], m);
+            V[slot] = make_float4(v[0], v[1], v[2], 0.0f);
+            D[slot] = dd;
+            L[slot] = ll;
+            I[slot] = (uint32_t)k;
+        }
+    }
+    (void)n;
+    HIP_CHECK(hipStreamSynchronize(s->stream));
+    HIP_CHECK(hipMemcpy(s->X.p, X.data(), X.size() * sizeof(float4), hipMemcpyHostToDevice));
+    HIP_CHECK(hipMemcpy(s->V.p, V.data(), V.size() * sizeof(float4), hipMemcpyHostToDevice));
+    HIP_CHECK(hipMemcpy(s->dens.p, D.data(), D.size() * 4, hipMemcpyHostToDevice));
+    HIP_CHECK(hipMemcpy(s->lam.p, L.data(), L.size() * 4, hipMemcpyHostToDevice));
+    HIP_CHECK(hipMemcpy(s->id.p, I.data(), I.size() * 4, hipMemcpyHostToDevice));
+    HIP_CHECK(hipMemcpy(s->color.p, C.data(), C.size() * sizeof(float4), hipMemcpyHostToDevice));
+    HIP_CHECK(hipMemcpy(s->size.p, Z.data(), Z.size() * 4, hipMemcpyHostToDevice));
+    s->hasState = true;
+}
+
+template <class T>
+std::vector<T> download(akua_solver* s, const T* p, size_t n) {
+    std::vector<T> h(n);
+    HIP_CHECK(hipStreamSynchronize(s->stream));
+    if (n) HIP_CHECK(hipMemcpy(h.data(), p, n * sizeof(T), hipMemcpyDeviceToHost));
+    return h;
+}
+
+}  // namespace
+
+extern "C" {
+
+int32_t akua_abi_version(void) { return AKUA_ABI_VERSION; }
+
+const char* akua_last_error(void) { return g_lastError.c_str(); }
+
+void akua_default_config(akua_pbf_config* c, akua_lambda_corr* k) {
+    if (c) {
+        c->restDensity = 7600.0f;
+        c->particle_spacing = 0.05f;
+        c->smoothRadius = 0.1f;
+        c->spatialHashCellSize = 0.1f;
+        c->relaxation = 600.0f;
+        c->vorticityEpsilon = 0.00001f;
+        c->viscosity = 0.01f;
+        c->maxNeighbours = 128;
+        c->solverIterations = 4;
+        c->gravity[0] = 0.0f;
+        c->gravity[1] = -9.8f;
+        c->gravity[2] = 0.0f;
+    }
+    if (k) {
+        k->enabled = 1;
+        k->k = 0.0001f;
+        k->n = 4.0f;
+        k->delta_q = 0.03f;
+    }
+}
+
+akua_status akua_solver_create(int32_t numParticles, const akua_pbf_config* cfg,
+                               const akua_lambda_corr* corr, akua_solver** out) {
+    return guarded([&] {
+        if (!out) fail(AKUA_ERR_INVALID_ARG, "out must not be NULL");
+        *out = nullptr;
+        validate_config(cfg, corr);
+        if (numParticles < 1) fail(AKUA_ERR_INVALID_ARG, "numParticles must be >= 1");
+        auto* s = new akua_solver();
+        s->cfg = *cfg;
+        s->corr = *corr;
+        try {
+            setup(s, std::vector<uint32_t>{(uint32_t)numParticles});
+        } catch (...) {
+            delete s;
+            throw;
+        }
+        *out = s;
+    });
+}
+
+akua_status akua_solver_destroy(akua_solver* s) {
+    return guarded([&] {
+        if (!s) return;
+        if (s->stream) (void)hipStreamSynchronize(s->stream);
+        delete s;
+    });
+}
+
+akua_status akua_solver_set_particles(akua_solver* s, const akua_particle* particles, int32_t n) {
+    return guarded([&] {
+        if (!s || !particles) fail(AKUA_ERR_INVALID_ARG, "NULL argument");
+        check_n(s, n);
+        upload_soa(s, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, true, particles);
+    });
+}
+
+akua_status akua_solver_set_state(akua_solver* s, const float* pos, const float* vel,
+                                  const float* mass, int32_t n) {
+    return guarded([&] {
+        if (!s || !pos || !vel) fail(AKUA_ERR_INVALID_ARG, "NULL argument");
+        check_n(s, n);
+        upload_soa(s, pos, vel, mass, nullptr, nullptr, nullptr, nullptr, false, nullptr);
+    });
+}
+
+akua_status akua_solver_get_state(akua_solver* s, float* pos, float* vel, float* mass,
+                                  float* density, float* lambda, uint32_t* ids, int32_t n) {
+    return guarded([&] {
+        if (!s) fail(AKUA_ERR_INVALID_ARG, "NULL solver");
+        check_n(s, n);
+        auto X = download(s, s->X.p, s->nslots);
+        auto V = download(s, s->V.p, s->nslots);
+        auto D = download(s, s->dens.p, s->nslots);
+        auto L = download(s, s->lam.p, s->nslots);
+        auto I = download(s, s->id.p, s->nslots);
+        size_t k = 0;
+        for (const SceneDev& sc : s->scenes)
+            for (uint32_t slot = sc.begin; slot < sc.end; ++slot, ++k) {
+                if (pos) { pos[3 * k] = X[slot].x; pos[3 * k + 1] = X[slot].y; pos[3 * k + 2] = X[slot].z; }
+                if (vel) { vel[3 * k] = V[slot].x; vel[3 * k + 1] = V[slot].y; vel[3 * k + 2] = V[slot].z; }
+                if (mass) mass[k] = X[slot].w;
+                if (density) density[k] = D[slot];
+                if (lambda) lambda[k] = L[slot];
+                if (ids) ids[k] = I[slot];
+            }
+    });
+}
+
+akua_status akua_solver_get_particles(akua_solver* s, akua_particle* out, int32_t n) {
+    return guarded([&] {
+        if (!s || !out) fail(AKUA_ERR_INVALID_ARG, "NULL argument");
+        check_n(s, n);
+        if (s->scenes.size() != 1) fail(AKUA_ERR_UNSUPPORTED, "record export is per single-scene solver");
+        AosParticle* d = nullptr;
+        HIP_CHECK(hipMalloc(&d, (size_t)n * sizeof(AosParticle)));
+        DeviceState ds = s->devState();
+        launch_export_aos(s->stream, d, (uint32_t)n, s->X.p, s->V.p, s->dens.p, s->lam.p, s->id.p,
+                          s->W.p, ds.skeys, s->color.p, s->size.p);
+        hipError_t e = hipStreamSynchronize(s->stream);
+        if (e == hipSuccess) e = hipMemcpy(out, d, (size_t)n * sizeof(AosParticle), hipMemcpyDeviceToHost);
+        (void)hipFree(d);
+        HIP_CHECK(e);
+    });
+}
+
+akua_status akua_solver_step(akua_solver* s, float dt, const float boxMin[3], const float boxMax[3]) {
+    return guarded([&] {
+        if (!s) fail(AKUA_ERR_INVALID_ARG, "NULL solver");
+        check_box(boxMin, boxMax);
+        set_box(s, 0, boxMin, boxMax);
+        run_steps(s, dt, 1);
+    });
+}
+
+akua_status akua_solver_step_n(akua_solver* s, float dt, const float boxMin[3],
+                               const float boxMax[3], int32_t nsteps) {
+    return guarded([&] {
+        if (!s) fail(AKUA_ERR_INVALID_ARG, "NULL solver");
+        if (nsteps < 0) fail(AKUA_ERR_INVALID_ARG, "nsteps must be >= 0");
+        check_box(boxMin, boxMax);
+        for (size_t i = 0; i < s->scenes.size(); ++i) set_box(s, i, boxMin, boxMax);
+        run_steps(s, dt, nsteps);
+    });
+}
+
+akua_status akua_solver_step_device(akua_solver* s, void* dParticles, int32_t n, float dt,
+                                    const float boxMin[3], const float boxMax[3], void* stream) {
+    return guarded([&] {
+        if (!s || !dParticles) fail(AKUA_ERR_INVALID_ARG, "NULL argument");
+        check_n(s, n);
+        if (s->scenes.size() != 1) fail(AKUA_ERR_UNSUPPORTED, "record stepping is per single-scene solver");
+        check_box(boxMin, boxMax);
+        set_box(s, 0, boxMin, boxMax);
+        hipStream_t user = static_cast<hipStream_t>(stream);
+        hipEvent_t ev = nullptr;
+        if (user) {
+            // the solver's stream waits for the caller's work on the buffer
+            HIP_CHECK(hipEventCreateWithFlags(&ev, hipEventDisableTiming));
+            HIP_CHECK(hipEventRecord(ev, user));
+            HIP_CHECK(hipStreamWaitEvent(s->stream, ev, 0));
+        }
+        auto* buf = static_cast<AosParticle*>(dParticles);
+        launch_import_aos(s->stream, buf, (uint32_t)n, s->X.p, s->V.p, s->dens.p, s->lam.p, s->id.p,
+                          s->color.p, s->size.p);
+        s->hasState = true;
+        run_steps(s, dt, 1);
+        DeviceState ds = s->devState();
+        launch_export_aos(s->stream, buf, (uint32_t)n, s->X.p, s->V.p, s->dens.p, s->lam.p, s->id.p,
+                          s->W.p, ds.skeys, s->color.p, s->size.p);
+        if (user) {
+            HIP_CHECK(hipEventRecord(ev, s->stream));
+            HIP_CHECK(hipStreamWaitEvent(user, ev, 0));
+            HIP_CHECK(hipEventDestroy(ev));
+        }
+        HIP_CHECK(hipGetLastError());
+    });
+}
+
+akua_status akua_solver_set_gravity(akua_solver* s, const float g[3]) {
+    return guarded([&] {
+        if (!s || !g) fail(AKUA_ERR_INVALID_ARG, "NULL argument");
+        for (int k = 0; k < 3; ++k)
+            if (!std::isfinite(g[k])) fail(AKUA_ERR_INVALID_ARG, "non-finite gravity");
+        for (int k = 0; k < 3; ++k) s->cfg.gravity[k] = g[k];
+    });
+}
+
+akua_status akua_solver_synchronize(akua_solver* s) {
+    return guarded([&] {
+        if (!s) fail(AKUA_ERR_INVALID_ARG, "NULL solver");
+        HIP_CHECK(hipStreamSynchronize(s->stream));
+    });
+}
+
+akua_status akua_solver_num_particles(const akua_solver* s, int32_t* n) {
+    return guarded([&] {
+        if (!s || !n) fail(AKUA_ERR_INVALID_ARG, "NULL argument");
+        *n = (int32_t)s->nparticles;
+    });
+}
+
+akua_status akua_solver_set_graph(akua_solver* s, int32_t enable) {
+    return guarded([&] {
+        if (!s) fail(AKUA_ERR_INVALID_ARG, "NULL solver");
+        s->useGraph = enable != 0;
+    });
+}
+
+akua_status akua_debug_hashes(akua_solver* s, uint32_t* out, int32_t n) {
+    return guarded([&] {
+        if (!s || !out) fail(AKUA_ERR_INVALID_ARG, "NULL argument");
+        check_n(s, n);
+        auto h = download(s, s->devState().skeys, s->nslots);
+        size_t k = 0;
+        for (const SceneDev& sc : s->scenes)
+            for (uint32_t slot = sc.begin; slot < sc.end; ++slot) out[k++] = h[slot];
+    });
+}
+
+akua_status akua_debug_permutation(akua_solver* s, uint32_t* out, int32_t n) {
+    return guarded([&] {
+        if (!s || !out) fail(AKUA_ERR_INVALID_ARG, "NULL argument");
+        check_n(s, n);
+        auto h = download(s, s->devState().perm, s->nslots);
+        size_t k = 0;
+        for (const SceneDev& sc : s->scenes)
+            for (uint32_t slot = sc.begin; slot < sc.end; ++slot) out[k++] = h[slot] - sc.begin;
+    });
+}
+
+akua_status akua_debug_cell_ranges(akua_solver* s, uint32_t* hash, uint32_t* start, uint32_t* end,
+                                   int32_t cap, int32_t* ncells) {
+    return guarded([&] {
+        if (!s || !ncells) fail(AKUA_ERR_INVALID_ARG, "NULL argument");
+        if (s->scenes.size() != 1) fail(AKUA_ERR_UNSUPPORTED, "cell ranges are per single-scene solver");
+        auto keys = download(s, s->devState().skeys, s->nslots);
+        auto tk = download(s, s->tkey.p, s->tableTotal);
+        auto tr = download(s, s->trange.p, s->tableTotal);
+        const SceneDev& sc = s->scenes[0];
+        std::vector<uint32_t> H, S, E;
+        for (uint32_t k = sc.begin; k < sc.end; ++k) {
+            if (k != sc.begin && keys[k] == keys[k - 1]) continue;
+            uint32_t h = keys[k];
+            // probe exactly as the device lookup does
+            uint32_t mask = (1u << sc.tabLog2) - 1u;
+            uint32_t slot = (h * 2654435761u) >> (32u - sc.tabLog2);
+            uint32_t st = 0xFFFFFFFFu, en = 0xFFFFFFFFu;
+            for (uint32_t probe = 0; probe <= mask; ++probe) {
+                uint32_t kk = tk[sc.tabOffset + slot];
+                if (kk == h) { st = tr[sc.tabOffset + slot].x; en = tr[sc.tabOffset + slot].y; break; }
+                if (kk == kEmptyKey) break;
+                slot = (slot + 1u) & mask;
+            }
+            H.push_back(h); S.push_back(st); E.push_back(en);
+        }
+        *ncells = (int32_t)H.size();
+        if ((int32_t)H.size() > cap) fail(AKUA_ERR_INVALID_ARG, "cap is smaller than the number of cells");
+        for (size_t c = 0; c < H.size(); ++c) {
+            if (hash) hash[c] = H[c];
+            if (start) start[c] = S[c] - sc.begin;
+            if (end) end[c] = E[c] - sc.begin;
+        }
+    });
+}
+
+akua_status akua_debug_neighbours(akua_solver* s, uint32_t* nbrOut, uint32_t* cntOut, int32_t n) {
+    return guarded([&] {
+        if (!s) fail(AKUA_ERR_INVALID_ARG, "NULL solver");
+        check_n(s, n);
+        const int maxN = s->cfg.maxNeighbours;
+        auto L = download(s, s->nbr.p, s->nbr.n);
+        auto C = download(s, s->cnt.p, s->nslots);
+        size_t k = 0;
+        for (const SceneDev& sc : s->scenes)
+            for (uint32_t i = sc.begin; i < sc.end; ++i, ++k) {
+                if (cntOut) cntOut[k] = C[i];
+                if (!nbrOut) continue;
+                size_t base = (size_t)(i >> 6) * maxN * 64 + (i & 63);
+                for (int j = 0; j < maxN; ++j)
+                    nbrOut[k * maxN + j] = (uint32_t)j < C[i] ? L[base + (size_t)j * 64] - sc.begin : 0u;
+            }
+    });
+}
+
+akua_status akua_solver_set_timing(akua_solver* s, int32_t enable) {
+    return guarded([&] {
+        if (!s) fail(AKUA_ERR_INVALID_ARG, "NULL solver");
+        s->timing = enable != 0;
+    });
+}
+
+akua_status akua_solver_reset_timing(akua_solver* s) {
+    return guarded([&] {
+        if (!s) fail(AKUA_ERR_INVALID_ARG, "NULL solver");
+        collect_timing(s);
+        s->timeAcc.clear();
+        s->timeOrder.clear();
+    });
+}
+
+akua_status akua_solver_get_timing(akua_solver* s, char* nameBuf, int32_t nameLen, double* ms,
+                                   int64_t* launches, int32_t cap, int32_t* count) {
+    return guarded([&] {
+        if (!s || !count) fail(AKUA_ERR_INVALID_ARG, "NULL argument");
+        collect_timing(s);
+        *count = (int32_t)s->timeOrder.size();
+        for (int32_t i = 0; i < cap && i < *count; ++i) {
+            const std::string& nm = s->timeOrder[i];
+            const auto& acc = s->timeAcc[nm];
+            if (nameBuf && nameLen > 0) {
+                std::strncpy(nameBuf + (size_t)i * nameLen, nm.c_str(), nameLen - 1);
+                nameBuf[(size_t)i * nameLen + nameLen - 1] = 0;
+            }
+            if (ms) ms[i] = acc.first;
+            if (launches) launches[i] = acc.second;
+        }
+    });
+}
+
+akua_status akua_solver_mean_neighbours(akua_solver* s, double* mean) {
+    return guarded([&] {
+        if (!s || !mean) fail(AKUA_ERR_INVALID_ARG, "NULL argument");
+        auto v = download(s, s->nbrSum.p, 1);
+        double steps = (double)s->stepsSinceStats;
+        *mean = steps > 0 ? (double)v[0] / (steps * (double)s->nparticles) : 0.0;
+        HIP_CHECK(hipMemset(s->nbrSum.p, 0, sizeof(unsigned long long)));
+        s->stepsSinceStats = 0;
+    });
+}
+
+}  // extern "C"

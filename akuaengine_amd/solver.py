@@ -1,0 +1,172 @@
+"""PBFSolver: the reference's Simulation-layer class over the C ABI.
+
+Mirrors AkuaEngine::PBFSolver (PBFSolver.h:12-31): constructed from a particle count, a
+``PBFConfig`` and ``LambdaCorrParams``; ``step(deltaTime, boxMin, boxMax)`` advances the particle
+buffer by one PBF step (PBFSolver.cpp:22-78); ``setGravity`` replaces the gravity vector.  The
+reference's step takes the OpenGL interop resource that holds the particles; here the solver owns
+the particle buffer (``set_particles`` / ``get_particles``), or ``step_device`` steps a caller-owned
+device buffer of 108-byte records in place.
+"""
+from __future__ import annotations
+
+import ctypes
+from typing import Dict, Optional, Sequence
+
+import numpy as np
+
+from . import _lib
+from .config import LambdaCorrParams, PBFConfig, to_c_config, to_c_corr
+
+
+def _p(a: Optional[np.ndarray]):
+    return None if a is None else a.ctypes.data_as(ctypes.c_void_p)
+
+
+def _f3(v: Sequence[float]) -> np.ndarray:
+    return np.ascontiguousarray(np.asarray(v, dtype=np.float32).reshape(3))
+
+
+class PBFSolver:
+    """One scene on one GPU.  Particle arrays are returned in the solver's current order, which
+    the spatial sort changes every step (as the reference's buffer is physically reordered);
+    ``ids`` maps each slot back to the particle's original index."""
+
+    def __init__(self, numParticles: int, config: PBFConfig, corrParams: LambdaCorrParams):
+        self._L = _lib.load()
+        self.numParticles = int(numParticles)
+        self.config = config
+        self.corrParams = corrParams
+        self._c_cfg = to_c_config(config)
+        self._c_corr = to_c_corr(corrParams)
+        h = ctypes.c_void_p()
+        _lib.check(self._L.akua_solver_create(self.numParticles, ctypes.byref(self._c_cfg),
+                                              ctypes.byref(self._c_corr), ctypes.byref(h)))
+        self._h = h
+
+    def close(self) -> None:
+        if getattr(self, "_h", None) and self._h.value:
+            self._L.akua_solver_destroy(self._h)
+            self._h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    # -- state --------------------------------------------------------------------------------
+    def set_state(self, pos: np.ndarray, vel: np.ndarray, mass: Optional[np.ndarray] = None) -> None:
+        n = self.numParticles
+        pos = np.ascontiguousarray(pos, np.float32).reshape(n, 3)
+        vel = np.ascontiguousarray(vel, np.float32).reshape(n, 3)
+        m = None if mass is None else np.ascontiguousarray(mass, np.float32).reshape(n)
+        _lib.check(self._L.akua_solver_set_state(self._h, _p(pos), _p(vel), _p(m), n))
+
+    def get_state(self) -> Dict[str, np.ndarray]:
+        n = self.numParticles
+        out = dict(pos=np.zeros((n, 3), np.float32), vel=np.zeros((n, 3), np.float32),
+                   mass=np.zeros(n, np.float32), density=np.zeros(n, np.float32),
+                   lam=np.zeros(n, np.float32), ids=np.zeros(n, np.uint32))
+        _lib.check(self._L.akua_solver_get_state(self._h, _p(out["pos"]), _p(out["vel"]),
+                                                 _p(out["mass"]), _p(out["density"]),
+                                                 _p(out["lam"]), _p(out["ids"]), n))
+        return out
+
+    def set_particles(self, particles) -> None:
+        """Uploads an array of ``_lib.AkuaParticle`` records (Renderer::uploadParticles)."""
+        _lib.check(self._L.akua_solver_set_particles(self._h, ctypes.cast(particles, ctypes.c_void_p),
+                                                     len(particles)))
+
+    def get_particles(self):
+        arr = (_lib.AkuaParticle * self.numParticles)()
+        _lib.check(self._L.akua_solver_get_particles(self._h, ctypes.cast(arr, ctypes.c_void_p),
+                                                     self.numParticles))
+        return arr
+
+    # -- stepping -----------------------------------------------------------------------------
+    def step(self, deltaTime: float, boxMin: Sequence[float], boxMax: Sequence[float]) -> None:
+        _lib.check(self._L.akua_solver_step(self._h, float(deltaTime), _p(_f3(boxMin)), _p(_f3(boxMax))))
+
+    def step_n(self, deltaTime: float, boxMin, boxMax, nsteps: int) -> None:
+        _lib.check(self._L.akua_solver_step_n(self._h, float(deltaTime), _p(_f3(boxMin)),
+                                              _p(_f3(boxMax)), int(nsteps)))
+
+    def step_device(self, d_particles: int, deltaTime: float, boxMin, boxMax, stream: int = 0) -> None:
+        """Steps a device buffer of numParticles 108-byte records (e.g. a torch uint8 tensor's
+        data_ptr()) on `stream` (0: the solver's own stream)."""
+        _lib.check(self._L.akua_solver_step_device(self._h, ctypes.c_void_p(d_particles),
+                                                   self.numParticles, float(deltaTime),
+                                                   _p(_f3(boxMin)), _p(_f3(boxMax)),
+                                                   ctypes.c_void_p(stream) if stream else None))
+
+    def setGravity(self, gravity: Sequence[float]) -> None:
+        _lib.check(self._L.akua_solver_set_gravity(self._h, _p(_f3(gravity))))
+        self.config.gravity = tuple(float(g) for g in gravity)
+
+    def synchronize(self) -> None:
+        _lib.check(self._L.akua_solver_synchronize(self._h))
+
+    def set_graph(self, enable: bool) -> None:
+        _lib.check(self._L.akua_solver_set_graph(self._h, 1 if enable else 0))
+
+    # -- parity taps --------------------------------------------------------------------------
+    def debug_hashes(self) -> np.ndarray:
+        out = np.zeros(self.numParticles, np.uint32)
+        _lib.check(self._L.akua_debug_hashes(self._h, _p(out), self.numParticles))
+        return out
+
+    def debug_permutation(self) -> np.ndarray:
+        out = np.zeros(self.numParticles, np.uint32)
+        _lib.check(self._L.akua_debug_permutation(self._h, _p(out), self.numParticles))
+        return out
+
+    def debug_cell_ranges(self):
+        n = ctypes.c_int32(0)
+        st = self._L.akua_debug_cell_ranges(self._h, None, None, None, 0, ctypes.byref(n))
+        if st not in (_lib.AKUA_OK, 1):
+            _lib.check(st)
+        cap = n.value
+        h = np.zeros(cap, np.uint32)
+        s = np.zeros(cap, np.uint32)
+        e = np.zeros(cap, np.uint32)
+        _lib.check(self._L.akua_debug_cell_ranges(self._h, _p(h), _p(s), _p(e), cap, ctypes.byref(n)))
+        return h, s, e
+
+    def debug_neighbours(self):
+        m = int(self.config.maxNeighbours)
+        nbr = np.zeros((self.numParticles, m), np.uint32)
+        cnt = np.zeros(self.numParticles, np.uint32)
+        _lib.check(self._L.akua_debug_neighbours(self._h, _p(nbr), _p(cnt), self.numParticles))
+        return nbr, cnt
+
+    # -- measurement --------------------------------------------------------------------------
+    def set_timing(self, enable: bool) -> None:
+        _lib.check(self._L.akua_solver_set_timing(self._h, 1 if enable else 0))
+
+    def reset_timing(self) -> None:
+        _lib.check(self._L.akua_solver_reset_timing(self._h))
+
+    def get_timing(self) -> Dict[str, tuple]:
+        cap, nl = 64, 64
+        names = ctypes.create_string_buffer(cap * nl)
+        ms = np.zeros(cap, np.float64)
+        launches = np.zeros(cap, np.int64)
+        count = ctypes.c_int32(0)
+        _lib.check(self._L.akua_solver_get_timing(self._h, names, nl, _p(ms), _p(launches), cap,
+                                                  ctypes.byref(count)))
+        out = {}
+        for i in range(min(cap, count.value)):
+            nm = names.raw[i * nl:(i + 1) * nl].split(b"\0", 1)[0].decode()
+            out[nm] = (float(ms[i]), int(launches[i]))
+        return out
+
+    def mean_neighbours(self) -> float:
+        v = ctypes.c_double(0.0)
+        _lib.check(self._L.akua_solver_mean_neighbours(self._h, ctypes.byref(v)))
+        return float(v.value)

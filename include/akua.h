@@ -1,0 +1,165 @@
+/* akua.h -- C ABI of the MI355X-native Position-Based Fluids solver (libakua.so).
+ *
+ * The ABI replaces the reference's Simulation-layer API and its CUDA op wrappers:
+ *   AkuaEngine::PBFSolver (PBFSolver.h:12-31, PBFSolver.cpp:13-78), whose step() maps an
+ *   OpenGL-owned particle buffer through the CUDA-GL interop (InteropResource.cpp:9-35) and calls
+ *   IntegrationCUDA / NeighbourSearchCUDA / ConstraintSolverCUDA (IntegrationCUDA.h:12-32,
+ *   NeighbourSearchCUDA.h:12-21, ConstraintSolverCUDA.h:14-27).
+ * Here the solver owns its device memory, runs headless, and every call returns an akua_status.
+ * No C++ exception crosses the ABI.  Signatures use only plain C types and pointers.
+ *
+ * Threading: one solver runs on one HIP stream.  Calls on one solver must be serialised by the
+ * caller; different solvers may be used from different threads.  akua_solver_step() is
+ * asynchronous with respect to the host; every get/debug call synchronises the solver's stream.
+ */
+#ifndef AKUA_H
+#define AKUA_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define AKUA_ABI_VERSION 1
+
+typedef enum akua_status {
+    AKUA_OK = 0,
+    AKUA_ERR_INVALID_ARG = 1,   /* null pointer, size mismatch, out-of-range parameter */
+    AKUA_ERR_OUT_OF_MEMORY = 2, /* device allocation failed */
+    AKUA_ERR_HIP = 3,           /* a HIP runtime call failed; akua_last_error() names it */
+    AKUA_ERR_NON_FINITE = 4,    /* NaN or Inf in caller-provided particle data */
+    AKUA_ERR_STATE = 5,         /* call out of order (e.g. step before set_particles) */
+    AKUA_ERR_UNSUPPORTED = 6,   /* configuration this build does not handle */
+    AKUA_ERR_NO_DEVICE = 7      /* no HIP device is visible */
+} akua_status;
+
+/* Mirrors AkuaEngine::PBFConfig field for field (PBFConfig.h:18-29), glm::vec3 -> float[3].
+ * Defaults (akua_default_config): restDensity 7600, particle_spacing 0.05, smoothRadius 0.1,
+ * spatialHashCellSize 0.1, relaxation 600, vorticityEpsilon 1e-5, viscosity 0.01,
+ * maxNeighbours 128, solverIterations 4, gravity (0, -9.8, 0). */
+typedef struct akua_pbf_config {
+    float restDensity;
+    float particle_spacing;
+    float smoothRadius;          /* h; also the hashing cell size (NeighbourSearchCUDA.cu:163) */
+    float spatialHashCellSize;   /* cell size of the neighbour search (NeighbourSearchCUDA.cu:178) */
+    float relaxation;            /* epsilon in the lambda denominator */
+    float vorticityEpsilon;
+    float viscosity;             /* XSPH coefficient c */
+    int32_t maxNeighbours;       /* neighbour-list cap; the hash table size is maxNeighbours * N */
+    int32_t solverIterations;    /* Jacobi iterations per step */
+    float gravity[3];
+} akua_pbf_config;
+
+/* Mirrors AkuaEngine::LambdaCorrParams (PBFConfig.h:10-15).  As in the reference, `enabled` does
+ * not switch the artificial-pressure term off (ConstraintSolverCUDA.cu:99-130 never reads it);
+ * set k = 0 for no correction.  Defaults: enabled 1, k 1e-4, n 4, delta_q 0.03. */
+typedef struct akua_lambda_corr {
+    int32_t enabled;
+    float k;
+    float n;
+    float delta_q;
+} akua_lambda_corr;
+
+/* The reference's 108-byte interleaved particle record (Particle.h:8-31), field order and offsets
+ * kept so a renderer's vertex buffer (position at offset 0, stride 108) can be filled directly. */
+typedef struct akua_particle {
+    float position[3];
+    float velocity[3];
+    float new_position[3];
+    float new_velocity[3];
+    float position_delta[3];
+    float vorticity[3];
+    float mass;
+    float density;
+    float lambda;
+    uint32_t hash;
+    float color[4];
+    float size;
+} akua_particle;
+
+typedef struct akua_solver akua_solver;
+
+/* ---- library ------------------------------------------------------------------------------ */
+int32_t akua_abi_version(void);
+/* Message of the last failing call on this thread ("" if none). */
+const char* akua_last_error(void);
+void akua_default_config(akua_pbf_config* cfg, akua_lambda_corr* corr);
+
+/* ---- single-scene solver (PBFSolver) ------------------------------------------------------ */
+/* PBFSolver::PBFSolver(int, const PBFConfig&, const LambdaCorrParams&) (PBFSolver.h:14,
+ * PBFSolver.cpp:13-20).  numParticles >= 1; maxNeighbours * numParticles must fit in uint32. */
+akua_status akua_solver_create(int32_t numParticles, const akua_pbf_config* cfg,
+                               const akua_lambda_corr* corr, akua_solver** out);
+akua_status akua_solver_destroy(akua_solver* s);
+
+/* Upload particles as the reference's AoS records (Renderer::uploadParticles,
+ * Renderer.cpp:185-219).  Reads position, velocity, mass, density, lambda, color and size. */
+akua_status akua_solver_set_particles(akua_solver* s, const akua_particle* particles, int32_t n);
+/* Download in the solver's current (sorted) order, as the reference's buffer is physically
+ * reordered by each step (NeighbourSearchCUDA.cu:166-170).  Fills every field from the last
+ * step: position, velocity, new_position (= position), new_velocity (= the velocity before
+ * vorticity and XSPH), position_delta (last iteration), vorticity, mass, density, lambda, hash. */
+akua_status akua_solver_get_particles(akua_solver* s, akua_particle* particles, int32_t n);
+
+/* Structure-of-arrays upload: pos/vel are 3*n floats (xyz interleaved per particle), mass n
+ * floats (NULL: all 1).  Particle ids are assigned 0..n-1 in the given order. */
+akua_status akua_solver_set_state(akua_solver* s, const float* pos, const float* vel,
+                                  const float* mass, int32_t n);
+/* Structure-of-arrays download in the current order; any pointer may be NULL.  ids[k] is the
+ * original index of the particle now in slot k. */
+akua_status akua_solver_get_state(akua_solver* s, float* pos, float* vel, float* mass,
+                                  float* density, float* lambda, uint32_t* ids, int32_t n);
+
+/* PBFSolver::step (PBFSolver.h:17, PBFSolver.cpp:22-78) on the solver-owned buffers. */
+akua_status akua_solver_step(akua_solver* s, float deltaTime, const float boxMin[3],
+                             const float boxMax[3]);
+/* Run nsteps steps with the same arguments (one graph launch per step). */
+akua_status akua_solver_step_n(akua_solver* s, float deltaTime, const float boxMin[3],
+                               const float boxMax[3], int32_t nsteps);
+/* The GL-interop replacement: the caller owns a device buffer of n akua_particle records (the
+ * role of the mapped VBO, InteropResource.cpp:9-35).  The call imports it, runs one step on
+ * `stream` (NULL: the solver's stream) and writes the records back in the new sorted order.
+ * The buffer must stay valid until the stream has completed the step. */
+akua_status akua_solver_step_device(akua_solver* s, void* d_particles, int32_t n, float deltaTime,
+                                    const float boxMin[3], const float boxMax[3], void* stream);
+/* PBFSolver::setGravity (PBFSolver.h:18, :29-31). */
+akua_status akua_solver_set_gravity(akua_solver* s, const float gravity[3]);
+akua_status akua_solver_synchronize(akua_solver* s);
+akua_status akua_solver_num_particles(const akua_solver* s, int32_t* n);
+
+/* Use a hipGraph per step (default 1).  0 launches the kernels directly each step. */
+akua_status akua_solver_set_graph(akua_solver* s, int32_t enable);
+
+/* ---- parity taps: the last step's intermediate results, copied to host ---------------------- */
+/* Sorted hash keys (sorted_hash[k] is the hash of the particle in slot k after the sort). */
+akua_status akua_debug_hashes(akua_solver* s, uint32_t* sorted_hash, int32_t n);
+/* perm[k] = slot, in the previous order, of the particle now in slot k. */
+akua_status akua_debug_permutation(akua_solver* s, uint32_t* perm, int32_t n);
+/* Occupied buckets in sorted order, looked up through the device cell table: for each bucket c
+ * (c < *ncells), its hash, first slot and one-past-last slot.  With cap too small, *ncells is
+ * still set and AKUA_ERR_INVALID_ARG returned. */
+akua_status akua_debug_cell_ranges(akua_solver* s, uint32_t* hash, uint32_t* start, uint32_t* end,
+                                   int32_t cap, int32_t* ncells);
+/* Neighbour lists in the reference's row-major layout nbr[i * maxNeighbours + k]
+ * (NeighbourSearchCUDA.cu:94, :119) and counts cnt[i]; entries past cnt[i] are 0. */
+akua_status akua_debug_neighbours(akua_solver* s, uint32_t* nbr, uint32_t* cnt, int32_t n);
+
+/* ---- timing ---------------------------------------------------------------------------------- */
+/* With timing on, the solver records HIP events on its stream around each kernel of a step
+ * (graphs are then not used).  names/ms: per-kernel accumulated device time and launch count. */
+akua_status akua_solver_set_timing(akua_solver* s, int32_t enable);
+akua_status akua_solver_reset_timing(akua_solver* s);
+/* Fills up to cap entries; *count gets the number of timed kernels.  name_buf receives cap names
+ * of name_len bytes each. */
+akua_status akua_solver_get_timing(akua_solver* s, char* name_buf, int32_t name_len, double* ms,
+                                   int64_t* launches, int32_t cap, int32_t* count);
+/* Mean neighbour count of the last step (device reduction), for the byte model. */
+akua_status akua_solver_mean_neighbours(akua_solver* s, double* mean);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* AKUA_H */

@@ -1,0 +1,235 @@
+// oracle_capi.cpp -- C ABI over pbf_oracle.hpp for the tests and the CPU baseline (ctypes).
+//
+// TEST INFRASTRUCTURE ONLY: the oracle is the checker.  Nothing in the product library links or
+// loads this file.  One handle holds one scene's state in the reference's physical (sorted) order
+// in FP32 (prec = 32, the parity oracle) or FP64 (prec = 64, the error reference).
+#include "pbf_oracle.hpp"
+
+#include <memory>
+
+using namespace pbf_oracle;
+
+namespace {
+
+struct Handle {
+    int prec;
+    Config cfg;
+    LambdaCorr corr;
+    uint32_t tableSize;
+    State<float> sf;
+    State<double> sd;
+    Taps taps;
+    std::vector<uint32_t> preSortHash;
+};
+
+template <class T>
+State<T>& st(Handle* h);
+template <>
+State<float>& st<float>(Handle* h) { return h->sf; }
+template <>
+State<double>& st<double>(Handle* h) { return h->sd; }
+
+template <class T>
+void set_state(Handle* h, long n, const float* x, const float* v, const float* m, const float* d,
+               const float* l, const uint32_t* id) {
+    State<T>& s = st<T>(h);
+    s.resize(n);
+    for (long i = 0; i < n; ++i) {
+        s.x[i] = {T(x[3 * i]), T(x[3 * i + 1]), T(x[3 * i + 2])};
+        s.v[i] = {T(v[3 * i]), T(v[3 * i + 1]), T(v[3 * i + 2])};
+        s.xs[i] = s.x[i];
+        s.vs[i] = s.v[i];
+        s.dp[i] = {T(0), T(0), T(0)};
+        s.w[i] = {T(0), T(0), T(0)};
+        s.mass[i] = m ? T(m[i]) : T(1);
+        s.density[i] = d ? T(d[i]) : T(0);
+        s.lambda[i] = l ? T(l[i]) : T(0);
+        s.id[i] = id ? id[i] : (uint32_t)i;
+        s.hash[i] = 0;
+    }
+}
+
+template <class T>
+void get_state(Handle* h, double* x, double* v, double* d, double* l, uint32_t* id, double* m) {
+    State<T>& s = st<T>(h);
+    const long n = (long)s.size();
+    for (long i = 0; i < n; ++i) {
+        if (x) { x[3 * i] = s.x[i].x; x[3 * i + 1] = s.x[i].y; x[3 * i + 2] = s.x[i].z; }
+        if (v) { v[3 * i] = s.v[i].x; v[3 * i + 1] = s.v[i].y; v[3 * i + 2] = s.v[i].z; }
+        if (d) d[i] = s.density[i];
+        if (l) l[i] = s.lambda[i];
+        if (id) id[i] = s.id[i];
+        if (m) m[i] = s.mass[i];
+    }
+}
+
+template <class T>
+void get_stage(Handle* h, double* xs, double* vs, double* dp, double* w) {
+    State<T>& s = st<T>(h);
+    const long n = (long)s.size();
+    for (long i = 0; i < n; ++i) {
+        if (xs) { xs[3 * i] = s.xs[i].x; xs[3 * i + 1] = s.xs[i].y; xs[3 * i + 2] = s.xs[i].z; }
+        if (vs) { vs[3 * i] = s.vs[i].x; vs[3 * i + 1] = s.vs[i].y; vs[3 * i + 2] = s.vs[i].z; }
+        if (dp) { dp[3 * i] = s.dp[i].x; dp[3 * i + 1] = s.dp[i].y; dp[3 * i + 2] = s.dp[i].z; }
+        if (w) { w[3 * i] = s.w[i].x; w[3 * i + 1] = s.w[i].y; w[3 * i + 2] = s.w[i].z; }
+    }
+}
+
+template <class T>
+void run_steps(Handle* h, float dt, const float* bmin, const float* bmax, int nsteps) {
+    Params<T> p = make_params<T>(h->cfg, h->corr, dt, bmin, bmax, h->tableSize);
+    for (int k = 0; k < nsteps; ++k) step(st<T>(h), p, &h->taps);
+}
+
+// Stage ids follow PBFSolver::step's order (PBFSolver.cpp:29-77).
+template <class T>
+void run_stage(Handle* h, int stage, float dt, const float* bmin, const float* bmax) {
+    Params<T> p = make_params<T>(h->cfg, h->corr, dt, bmin, bmax, h->tableSize);
+    State<T>& s = st<T>(h);
+    Taps& t = h->taps;
+    switch (stage) {
+        case 0: predict(s, p); break;
+        case 1: compute_hashes(s, p); h->preSortHash = s.hash; break;
+        case 2:
+            sort_by_hash(s, t.perm);
+            cell_ranges(s.hash, t.cell_hash, t.cell_start, t.cell_end);
+            break;
+        case 3: find_neighbours(s, p, t.cell_hash, t.cell_start, t.cell_end, t.nbr, t.cnt); break;
+        case 4: densities(s, p, t.nbr, t.cnt); break;
+        case 5: lambdas(s, p, t.nbr, t.cnt); break;
+        case 6: position_deltas(s, p, t.nbr, t.cnt); break;
+        case 7: correct_positions(s, p); break;
+        case 8: update_position_velocity(s, p); break;
+        case 9: boundary_damping(s, p, T(0.0f), T(0.95f)); break;
+        case 10: vorticities(s, p, t.nbr, t.cnt); break;
+        case 11: vorticity_confinement(s, p, t.nbr, t.cnt); break;
+        case 12: {
+            std::vector<V3<T>> vn;
+            xsph(s, p, t.nbr, t.cnt, vn);
+            break;
+        }
+        default: break;
+    }
+}
+
+}  // namespace
+
+extern "C" {
+
+void orc_set_threads(int n) {
+#ifdef _OPENMP
+    if (n > 0) omp_set_num_threads(n);
+#else
+    (void)n;
+#endif
+}
+
+int orc_max_threads(void) {
+#ifdef _OPENMP
+    return omp_get_max_threads();
+#else
+    return 1;
+#endif
+}
+
+void* orc_create(int prec, long n, const Config* cfg, const LambdaCorr* corr, uint32_t tableSize) {
+    auto* h = new Handle();
+    h->prec = prec;
+    h->cfg = *cfg;
+    h->corr = *corr;
+    h->tableSize = tableSize ? tableSize : (uint32_t)((uint64_t)cfg->maxNeighbours * (uint64_t)n);
+    if (prec == 64) h->sd.resize(n); else h->sf.resize(n);
+    return h;
+}
+
+void orc_destroy(void* hp) { delete static_cast<Handle*>(hp); }
+
+void orc_set_gravity(void* hp, const float g[3]) {
+    auto* h = static_cast<Handle*>(hp);
+    for (int k = 0; k < 3; ++k) h->cfg.gravity[k] = g[k];
+}
+
+void orc_set_state(void* hp, long n, const float* x, const float* v, const float* m, const float* d,
+                   const float* l, const uint32_t* id) {
+    auto* h = static_cast<Handle*>(hp);
+    if (h->prec == 64) set_state<double>(h, n, x, v, m, d, l, id);
+    else set_state<float>(h, n, x, v, m, d, l, id);
+}
+
+void orc_get_state(void* hp, double* x, double* v, double* d, double* l, uint32_t* id, double* m) {
+    auto* h = static_cast<Handle*>(hp);
+    if (h->prec == 64) get_state<double>(h, x, v, d, l, id, m);
+    else get_state<float>(h, x, v, d, l, id, m);
+}
+
+void orc_get_stage(void* hp, double* xs, double* vs, double* dp, double* w) {
+    auto* h = static_cast<Handle*>(hp);
+    if (h->prec == 64) get_stage<double>(h, xs, vs, dp, w);
+    else get_stage<float>(h, xs, vs, dp, w);
+}
+
+void orc_step(void* hp, float dt, const float bmin[3], const float bmax[3], int nsteps) {
+    auto* h = static_cast<Handle*>(hp);
+    if (h->prec == 64) run_steps<double>(h, dt, bmin, bmax, nsteps);
+    else run_steps<float>(h, dt, bmin, bmax, nsteps);
+}
+
+void orc_stage(void* hp, int stage, float dt, const float bmin[3], const float bmax[3]) {
+    auto* h = static_cast<Handle*>(hp);
+    if (h->prec == 64) run_stage<double>(h, stage, dt, bmin, bmax);
+    else run_stage<float>(h, stage, dt, bmin, bmax);
+}
+
+long orc_ncells(void* hp) { return (long)static_cast<Handle*>(hp)->taps.cell_hash.size(); }
+
+// Taps of the last step (or of the last stages run).  Any pointer may be NULL.
+void orc_get_taps(void* hp, uint32_t* sorted_hash, uint32_t* perm, uint32_t* nbr, uint32_t* cnt,
+                  uint32_t* cell_hash, uint32_t* cell_start, uint32_t* cell_end) {
+    auto* h = static_cast<Handle*>(hp);
+    const Taps& t = h->taps;
+    const std::vector<uint32_t>& hs = (h->prec == 64) ? h->sd.hash : h->sf.hash;
+    if (sorted_hash) std::copy(hs.begin(), hs.end(), sorted_hash);
+    if (perm) std::copy(t.perm.begin(), t.perm.end(), perm);
+    if (nbr) std::copy(t.nbr.begin(), t.nbr.end(), nbr);
+    if (cnt) std::copy(t.cnt.begin(), t.cnt.end(), cnt);
+    if (cell_hash) std::copy(t.cell_hash.begin(), t.cell_hash.end(), cell_hash);
+    if (cell_start) std::copy(t.cell_start.begin(), t.cell_start.end(), cell_start);
+    if (cell_end) std::copy(t.cell_end.begin(), t.cell_end.end(), cell_end);
+}
+
+void orc_density_at_positions(void* hp, float dt, const float bmin[3], const float bmax[3],
+                              double* rho) {
+    auto* h = static_cast<Handle*>(hp);
+    if (h->prec == 64) {
+        Params<double> p = make_params<double>(h->cfg, h->corr, dt, bmin, bmax, h->tableSize);
+        std::vector<double> r;
+        density_at_positions(h->sd, p, r);
+        std::copy(r.begin(), r.end(), rho);
+    } else {
+        Params<float> p = make_params<float>(h->cfg, h->corr, dt, bmin, bmax, h->tableSize);
+        std::vector<float> r;
+        density_at_positions(h->sf, p, r);
+        for (size_t i = 0; i < r.size(); ++i) rho[i] = r[i];
+    }
+}
+
+double orc_poly6(int prec, double d2, double h) {
+    if (prec == 64) return poly6<double>(d2, h);
+    return (double)poly6<float>((float)d2, (float)h);
+}
+
+void orc_grad_spiky(int prec, const double r[3], double h, double out[3]) {
+    if (prec == 64) {
+        V3<double> g = grad_spiky<double>({r[0], r[1], r[2]}, h);
+        out[0] = g.x; out[1] = g.y; out[2] = g.z;
+    } else {
+        V3<float> g = grad_spiky<float>({(float)r[0], (float)r[1], (float)r[2]}, (float)h);
+        out[0] = g.x; out[1] = g.y; out[2] = g.z;
+    }
+}
+
+uint32_t orc_cell_hash(int32_t cx, int32_t cy, int32_t cz, uint32_t tableSize) {
+    return cell_hash(cx, cy, cz, tableSize);
+}
+
+}  // extern "C"

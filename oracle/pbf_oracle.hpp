@@ -1,0 +1,587 @@
+// pbf_oracle.hpp -- CPU restatement of the AkuaEngine Position-Based Fluids step.
+//
+// TEST INFRASTRUCTURE ONLY.  This header is the correctness oracle: it is compiled into
+// oracle/liboracle.so and used by tests/, by __graft_entry__.smoke() and by bench.py's
+// cpu_baseline leg, always as the checker or the timed CPU baseline, never as a product path.
+//
+// It restates, per function, the reference's CUDA kernels (file:line citations below point into
+// the Kevin22888/AkuaEngine sources) as a plain serial/OpenMP C++ template on T = float
+// (parity) or T = double (error reference).  Semantics fixed here, where the reference leaves
+// them to the hardware or to a library:
+//   * stable sort by hash over the current order (thrust::sort's tie order is unspecified);
+//   * the neighbour walk follows the reference's stencil order (dx, dy, dz nested -1..1),
+//     walks a bucket once per stencil cell that hashes to it (duplicates kept), caps at
+//     maxNeighbours;
+//   * XSPH is Jacobi: it reads the velocities from before the pass (the reference races);
+//   * no FMA contraction anywhere (build with -ffp-contract=off);
+//   * pi is 3.14f, as in SmoothingKernelsCUDA.h;
+//   * powf(x, k) with a small integer exponent k inside the per-pair loops (the cubes and
+//     squares of the SPH kernels, and s_corr's exponent n when it is an integer) is the k-fold
+//     product x*x*...*x evaluated left to right; a non-integer n calls the libm powf.  The
+//     per-call constants powf(h, 9) and powf(h, 6) call the libm powf as written.
+// Parity status of each function is listed in the comment above it ("pinned by ...").
+#pragma once
+
+#include <algorithm>
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+#include <numeric>
+#include <vector>
+
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+namespace pbf_oracle {
+
+// Mirrors AkuaEngine::PBFConfig (PBFConfig.h:18-29).  Plain C layout, no glm.
+struct Config {
+    float restDensity;
+    float particle_spacing;
+    float smoothRadius;
+    float spatialHashCellSize;
+    float relaxation;
+    float vorticityEpsilon;
+    float viscosity;
+    int32_t maxNeighbours;
+    int32_t solverIterations;
+    float gravity[3];
+};
+
+// Mirrors AkuaEngine::LambdaCorrParams (PBFConfig.h:10-15).  `enabled` is carried but, as in the
+// reference (ConstraintSolverCUDA.cu:99-130 never reads it), the correction is always applied.
+struct LambdaCorr {
+    int32_t enabled;
+    float k;
+    float n;
+    float delta_q;
+};
+
+template <class T>
+struct V3 {
+    T x, y, z;
+};
+template <class T> inline V3<T> operator+(V3<T> a, V3<T> b) { return {a.x + b.x, a.y + b.y, a.z + b.z}; }
+template <class T> inline V3<T> operator-(V3<T> a, V3<T> b) { return {a.x - b.x, a.y - b.y, a.z - b.z}; }
+// scalar * vec and vec * scalar, each component rounded once (glm operator*, type_vec3.inl)
+template <class T> inline V3<T> operator*(T s, V3<T> a) { return {s * a.x, s * a.y, s * a.z}; }
+template <class T> inline V3<T> operator*(V3<T> a, T s) { return {a.x * s, a.y * s, a.z * s}; }
+template <class T> inline V3<T> operator/(V3<T> a, T s) { return {a.x / s, a.y / s, a.z / s}; }
+// glm dot for vec3: tmp = a*b; tmp.x + tmp.y + tmp.z (glm/detail/func_geometric.inl:48-54)
+template <class T> inline T dot(V3<T> a, V3<T> b) {
+    T tx = a.x * b.x, ty = a.y * b.y, tz = a.z * b.z;
+    return (tx + ty) + tz;
+}
+// glm length = sqrt(dot(v, v)) (glm/detail/func_geometric.inl:8-13)
+template <class T> inline T length(V3<T> a) { return std::sqrt(dot(a, a)); }
+// MathUtilsCUDA::cross (MathUtilsCUDA.h:12-18)
+template <class T> inline V3<T> cross(V3<T> a, V3<T> b) {
+    return {a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x};
+}
+
+template <class T> inline T powT(T x, T e) { return std::pow(x, e); }
+template <> inline float powT<float>(float x, float e) { return powf(x, e); }
+
+// powf with the exponent given as a float: a small integer exponent k (0..16) is the k-fold
+// product evaluated left to right, otherwise libm pow.
+template <class T>
+inline T ipow(T x, T e) {
+    if (e >= T(0) && e <= T(16) && e == std::floor(e)) {
+        int k = (int)e;
+        if (k == 0) return T(1);
+        T r = x;
+        for (int i = 1; i < k; ++i) r = r * x;
+        return r;
+    }
+    return powT<T>(x, e);
+}
+
+// The constant factors of the two SPH kernels, as written in SmoothingKernelsCUDA.h:20 and :27.
+template <class T>
+inline T poly6_coef(T h) { return T(315.0f) / (T(64.0f) * T(3.14f) * powT<T>(h, T(9.0f))); }
+template <class T>
+inline T spiky_coef(T h) { return -T(45.0f) / (T(3.14f) * powT<T>(h, T(6.0f))); }
+
+// SmoothingKernels::device_poly6 (SmoothingKernelsCUDA.h:16-21).  Pinned by the closed form
+// (known-answer tests in tests/test_oracle.py) and by the interior-lattice density.
+// Kern caches h, h*h and the two constant factors; the value of each is the one the reference
+// computes inline on every call.
+template <class T>
+struct Kern {
+    T h, h2, cp, cs;
+    explicit Kern(T hh) : h(hh), h2(hh * hh), cp(poly6_coef<T>(hh)), cs(spiky_coef<T>(hh)) {}
+};
+
+template <class T>
+inline T poly6(T d2, const Kern<T>& k) {
+    if (d2 > k.h2) return T(0);
+    return k.cp * ipow<T>(k.h2 - d2, T(3.0f));
+}
+template <class T>
+inline T poly6(T d2, T h) { return poly6<T>(d2, Kern<T>(h)); }
+
+// SmoothingKernels::device_gradient_spiky (SmoothingKernelsCUDA.h:23-28).  Pinned by a central
+// finite difference of the spiky kernel and by its closed form along an axis.
+template <class T>
+inline V3<T> grad_spiky(V3<T> r, const Kern<T>& k) {
+    T rm = length(r);
+    if (rm > k.h || rm < T(1e-5f)) return {T(0), T(0), T(0)};
+    T s = k.cs * ipow<T>(k.h - rm, T(2.0f));
+    return (s * r) / rm;
+}
+template <class T>
+inline V3<T> grad_spiky(V3<T> r, T h) { return grad_spiky<T>(r, Kern<T>(h)); }
+
+// device_discretize_position (NeighbourSearchCUDA.cu:15-21): floor(x / cellSize) per axis.
+template <class T>
+inline void discretize(V3<T> p, T cs, int32_t c[3]) {
+    c[0] = static_cast<int32_t>(std::floor(p.x / cs));
+    c[1] = static_cast<int32_t>(std::floor(p.y / cs));
+    c[2] = static_cast<int32_t>(std::floor(p.z / cs));
+}
+
+// device_get_hash (NeighbourSearchCUDA.cu:23-27).  The reference multiplies in int and casts to
+// uint32; the wrap is restated as a uint32 multiply (no signed-overflow UB).  `% tableSize`
+// converts the int table size to unsigned.  Pinned by hand-computed known answers.
+inline uint32_t cell_hash(int32_t cx, int32_t cy, int32_t cz, uint32_t tableSize) {
+    uint32_t a = static_cast<uint32_t>(cx) * 73856093u;
+    uint32_t b = static_cast<uint32_t>(cy) * 19349663u;
+    uint32_t c = static_cast<uint32_t>(cz) * 83492791u;
+    return (a ^ b ^ c) % tableSize;
+}
+
+// Per-particle state, in the reference's physical (sorted) order.  The reference keeps it in the
+// 108-byte Particle record (Particle.h:8-31); only the fields the step reads or writes are here.
+template <class T>
+struct State {
+    std::vector<V3<T>> x;     // position
+    std::vector<V3<T>> v;     // velocity
+    std::vector<V3<T>> xs;    // new_position (x*)
+    std::vector<V3<T>> vs;    // new_velocity
+    std::vector<V3<T>> dp;    // position_delta
+    std::vector<V3<T>> w;     // vorticity
+    std::vector<T> mass, density, lambda;
+    std::vector<uint32_t> hash, id;
+    size_t size() const { return x.size(); }
+    void resize(size_t n) {
+        x.resize(n); v.resize(n); xs.resize(n); vs.resize(n); dp.resize(n); w.resize(n);
+        mass.resize(n); density.resize(n); lambda.resize(n); hash.resize(n); id.resize(n);
+    }
+};
+
+// Everything one step leaves behind besides the state: the parity taps.
+struct Taps {
+    std::vector<uint32_t> perm;          // sorted slot k came from pre-sort slot perm[k]
+    std::vector<uint32_t> nbr;           // row-major [i * maxNeighbours + k] as the reference
+    std::vector<uint32_t> cnt;
+    std::vector<uint32_t> cell_hash;     // occupied buckets in sorted order
+    std::vector<uint32_t> cell_start;    // first sorted index of each bucket
+    std::vector<uint32_t> cell_end;      // one past the last sorted index
+};
+
+template <class T>
+struct Params {
+    Kern<T> K{T(0.1f)};
+    T h, h2, cellHash, cellSearch, invRest, relax, vortEps, visc, dt;
+    T g[3], bmin[3], bmax[3];
+    T corrK, corrN, corrDq;
+    uint32_t tableSize;
+    int32_t maxN, iters;
+};
+
+template <class T>
+Params<T> make_params(const Config& c, const LambdaCorr& lc, float dt, const float bmin[3],
+                      const float bmax[3], uint32_t tableSize) {
+    Params<T> p;
+    p.K = Kern<T>(T(c.smoothRadius));
+    p.h = T(c.smoothRadius);
+    p.h2 = p.h * p.h;
+    // NeighbourSearchCUDA.cu:163 passes smoothRadius as the hashing cell size; the search itself
+    // (NeighbourSearchCUDA.cu:177-178) uses spatialHashCellSize.
+    p.cellHash = T(c.smoothRadius);
+    p.cellSearch = T(c.spatialHashCellSize);
+    // ConstraintSolverCUDA.cu:201: inverseRestDensity = 1.0f / restDensity (in float).
+    p.invRest = T(1.0f / c.restDensity);
+    if (sizeof(T) == 8) p.invRest = T(1.0) / T(c.restDensity);
+    p.relax = T(c.relaxation);
+    p.vortEps = T(c.vorticityEpsilon);
+    p.visc = T(c.viscosity);
+    p.dt = T(dt);
+    for (int k = 0; k < 3; ++k) {
+        p.g[k] = T(c.gravity[k]);
+        p.bmin[k] = T(bmin[k]);
+        p.bmax[k] = T(bmax[k]);
+    }
+    p.corrK = T(lc.k);
+    p.corrN = T(lc.n);
+    p.corrDq = T(lc.delta_q);
+    p.tableSize = tableSize;
+    p.maxN = c.maxNeighbours;
+    p.iters = c.solverIterations;
+    return p;
+}
+
+// kernel_predict_position (IntegrationCUDA.cu:27-36): v* = v + g*dt; x* = x + v*·dt.
+template <class T>
+void predict(State<T>& s, const Params<T>& p) {
+    const V3<T> g{p.g[0], p.g[1], p.g[2]};
+    const long n = (long)s.size();
+#pragma omp parallel for schedule(static)
+    for (long i = 0; i < n; ++i) {
+        s.vs[i] = s.v[i] + g * p.dt;
+        s.xs[i] = s.x[i] + s.vs[i] * p.dt;
+    }
+}
+
+// kernel_compute_hashes (NeighbourSearchCUDA.cu:36-44), launched with smoothRadius as cellSize
+// (NeighbourSearchCUDA.cu:163).
+template <class T>
+void compute_hashes(State<T>& s, const Params<T>& p) {
+    const long n = (long)s.size();
+#pragma omp parallel for schedule(static)
+    for (long i = 0; i < n; ++i) {
+        int32_t c[3];
+        discretize(s.xs[i], p.cellHash, c);
+        s.hash[i] = cell_hash(c[0], c[1], c[2], p.tableSize);
+    }
+}
+
+template <class A>
+static void permute(std::vector<A>& a, const std::vector<uint32_t>& perm, std::vector<A>& tmp) {
+    tmp.resize(a.size());
+    for (size_t k = 0; k < a.size(); ++k) tmp[k] = a[perm[k]];
+    a.swap(tmp);
+}
+
+// thrust::sort by hash (NeighbourSearchCUDA.cu:166-170), restated as a STABLE sort over the
+// current order; the whole record moves, so the permutation accumulates across steps.
+// Pinned by std::stable_sort itself and by the sortedness/stability invariants in the tests.
+template <class T>
+void sort_by_hash(State<T>& s, std::vector<uint32_t>& perm) {
+    const size_t n = s.size();
+    perm.resize(n);
+    std::iota(perm.begin(), perm.end(), 0u);
+    std::stable_sort(perm.begin(), perm.end(),
+                     [&](uint32_t a, uint32_t b) { return s.hash[a] < s.hash[b]; });
+    std::vector<V3<T>> t3;
+    std::vector<T> t1;
+    std::vector<uint32_t> tu;
+    permute(s.x, perm, t3); permute(s.v, perm, t3); permute(s.xs, perm, t3);
+    permute(s.vs, perm, t3); permute(s.dp, perm, t3); permute(s.w, perm, t3);
+    permute(s.mass, perm, t1); permute(s.density, perm, t1); permute(s.lambda, perm, t1);
+    permute(s.hash, perm, tu); permute(s.id, perm, tu);
+}
+
+// kernel_build_hash_table (NeighbourSearchCUDA.cu:52-65) stores the first sorted index of each
+// hash; a bucket ends where the hash changes (the walk in NeighbourSearchCUDA.cu:103-124).  The
+// oracle keeps the occupied buckets as sorted (hash, start, end) triples and finds a bucket by
+// binary search, which yields the same first index the reference's table holds.
+inline void cell_ranges(const std::vector<uint32_t>& sortedHash, std::vector<uint32_t>& ch,
+                        std::vector<uint32_t>& cs, std::vector<uint32_t>& ce) {
+    ch.clear(); cs.clear(); ce.clear();
+    const size_t n = sortedHash.size();
+    for (size_t i = 0; i < n; ++i) {
+        if (i == 0 || sortedHash[i] != sortedHash[i - 1]) {
+            ch.push_back(sortedHash[i]);
+            cs.push_back((uint32_t)i);
+            if (i > 0) ce.push_back((uint32_t)i);
+        }
+    }
+    if (n > 0) ce.push_back((uint32_t)n);
+}
+
+// kernel_find_neighbours (NeighbourSearchCUDA.cu:72-130).  Pinned by O(N^2) brute force as
+// multisets (duplicates from stencil hash collisions computed from the hash) in the tests.
+template <class T>
+void find_neighbours(const State<T>& s, const Params<T>& p, const std::vector<uint32_t>& ch,
+                     const std::vector<uint32_t>& cs, const std::vector<uint32_t>& ce,
+                     std::vector<uint32_t>& nbr, std::vector<uint32_t>& cnt) {
+    const long n = (long)s.size();
+    const int maxN = p.maxN;
+    nbr.assign((size_t)n * (size_t)maxN, 0u);
+    cnt.assign((size_t)n, 0u);
+    // NeighbourSearchCUDA.cu:90: float smoothRadiusSquared = smoothRadius * smoothRadius
+    const T h2 = p.h * p.h;
+#pragma omp parallel for schedule(dynamic, 256)
+    for (long i = 0; i < n; ++i) {
+        int32_t c[3];
+        discretize(s.xs[i], p.cellSearch, c);
+        int count = 0;
+        const size_t start = (size_t)i * maxN;
+        for (int dx = -1; dx <= 1; ++dx)
+            for (int dy = -1; dy <= 1; ++dy)
+                for (int dz = -1; dz <= 1; ++dz) {
+                    // glm ivec3 addition wraps like int; restate in uint32 then back.
+                    int32_t qx = (int32_t)((uint32_t)c[0] + (uint32_t)dx);
+                    int32_t qy = (int32_t)((uint32_t)c[1] + (uint32_t)dy);
+                    int32_t qz = (int32_t)((uint32_t)c[2] + (uint32_t)dz);
+                    uint32_t hh = cell_hash(qx, qy, qz, p.tableSize);
+                    auto it = std::lower_bound(ch.begin(), ch.end(), hh);
+                    if (it == ch.end() || *it != hh) continue;  // table entry UINT32_MAX
+                    size_t b = (size_t)(it - ch.begin());
+                    long k = (long)cs[b];
+                    while (k < n && count < maxN) {
+                        if (k == i) { ++k; continue; }
+                        if (s.hash[k] != hh) break;
+                        V3<T> sep = s.xs[i] - s.xs[k];
+                        T d2 = dot(sep, sep);
+                        if (d2 < h2) nbr[start + count++] = (uint32_t)k;
+                        ++k;
+                    }
+                }
+        cnt[i] = (uint32_t)count;
+    }
+}
+
+// kernel_calculate_densities (ConstraintSolverCUDA.cu:16-42).  Pinned by the interior-lattice
+// known answer and by brute-force sums over all particles within h.
+template <class T>
+void densities(State<T>& s, const Params<T>& p, const std::vector<uint32_t>& nbr,
+               const std::vector<uint32_t>& cnt) {
+    const long n = (long)s.size();
+#pragma omp parallel for schedule(static)
+    for (long i = 0; i < n; ++i) {
+        T rho = s.mass[i] * poly6<T>(T(0), p.K);
+        const size_t st = (size_t)i * p.maxN;
+        for (uint32_t o = 0; o < cnt[i]; ++o) {
+            uint32_t j = nbr[st + o];
+            V3<T> sep = s.xs[i] - s.xs[j];
+            T d2 = dot(sep, sep);
+            rho = rho + s.mass[j] * poly6<T>(d2, p.K);
+        }
+        s.density[i] = rho;
+    }
+}
+
+// kernel_calculate_lambdas (ConstraintSolverCUDA.cu:51-97).  Pinned by the interior-lattice
+// known answer and the two-particle closed form.
+template <class T>
+void lambdas(State<T>& s, const Params<T>& p, const std::vector<uint32_t>& nbr,
+             const std::vector<uint32_t>& cnt) {
+    const long n = (long)s.size();
+#pragma omp parallel for schedule(static)
+    for (long i = 0; i < n; ++i) {
+        const size_t st = (size_t)i * p.maxN;
+        V3<T> gi{T(0), T(0), T(0)};
+        for (uint32_t o = 0; o < cnt[i]; ++o) {
+            uint32_t j = nbr[st + o];
+            V3<T> sep = s.xs[i] - s.xs[j];
+            gi = gi + s.mass[j] * grad_spiky<T>(sep, p.K);
+        }
+        gi = gi * p.invRest;  // grad_pi_Ci *= inverseRestDensity
+        T sum = T(0);
+        for (uint32_t o = 0; o < cnt[i]; ++o) {
+            uint32_t j = nbr[st + o];
+            V3<T> sep = s.xs[i] - s.xs[j];
+            V3<T> gj = (-p.invRest * s.mass[j]) * grad_spiky<T>(sep, p.K);
+            sum = sum + dot(gj, gj);
+        }
+        T C = s.density[i] * p.invRest - T(1);
+        s.lambda[i] = -C / (sum + dot(gi, gi) + p.relax);
+    }
+}
+
+// kernel_calculate_position_delta (ConstraintSolverCUDA.cu:99-130).  Pinned by
+// sum_i m_i * dp_i = 0 for symmetric neighbour sets and by the two-particle closed form.
+template <class T>
+void position_deltas(State<T>& s, const Params<T>& p, const std::vector<uint32_t>& nbr,
+                     const std::vector<uint32_t>& cnt) {
+    const long n = (long)s.size();
+    const T dq2 = p.corrDq * p.corrDq;
+    const T wdq = poly6<T>(dq2, p.K);
+#pragma omp parallel for schedule(static)
+    for (long i = 0; i < n; ++i) {
+        const size_t st = (size_t)i * p.maxN;
+        V3<T> d{T(0), T(0), T(0)};
+        for (uint32_t o = 0; o < cnt[i]; ++o) {
+            uint32_t j = nbr[st + o];
+            V3<T> sep = s.xs[i] - s.xs[j];
+            T d2 = dot(sep, sep);
+            T lc = -p.corrK * ipow<T>(poly6<T>(d2, p.K) / wdq, p.corrN);
+            d = d + ((s.lambda[i] + s.lambda[j] + lc) * s.mass[j]) * grad_spiky<T>(sep, p.K);
+        }
+        s.dp[i] = d * p.invRest;
+    }
+}
+
+// handle_particle_collision + kernel_correct_position (ConstraintSolverCUDA.cu:136-169).
+// Pinned by per-wall special cases.
+template <class T>
+void correct_positions(State<T>& s, const Params<T>& p) {
+    const long n = (long)s.size();
+    const T minDist = T(0.025f), stiff = T(0.5f);
+#pragma omp parallel for schedule(static)
+    for (long i = 0; i < n; ++i) {
+        V3<T> q = s.xs[i] + s.dp[i];
+        T c[3] = {T(0), T(0), T(0)};
+        T qa[3] = {q.x, q.y, q.z};
+        for (int a = 0; a < 3; ++a) {
+            if (qa[a] < p.bmin[a] + minDist) c[a] = c[a] + stiff * (p.bmin[a] + minDist - qa[a]);
+            if (qa[a] > p.bmax[a] - minDist) c[a] = c[a] + stiff * (p.bmax[a] - minDist - qa[a]);
+        }
+        s.xs[i] = q + V3<T>{c[0], c[1], c[2]};
+    }
+}
+
+// kernel_update_position_and_velocity (IntegrationCUDA.cu:38-49).  Pinned by the lone-particle
+// ballistic recursion.
+template <class T>
+void update_position_velocity(State<T>& s, const Params<T>& p) {
+    const long n = (long)s.size();
+#pragma omp parallel for schedule(static)
+    for (long i = 0; i < n; ++i) {
+        s.vs[i] = (s.xs[i] - s.x[i]) / p.dt;
+        s.x[i] = s.xs[i];
+        s.v[i] = s.vs[i];
+    }
+}
+
+// resolve_collision + kernel_apply_boundary_velocity_damping (IntegrationCUDA.cu:51-102), called
+// with restitution 0 and friction 0.95 (PBFSolver.cpp:64).  Planes are applied in order, each
+// seeing the velocity the previous one left.  Pinned by per-plane special cases.
+template <class T>
+void boundary_damping(State<T>& s, const Params<T>& p, T restitution, T friction) {
+    const long n = (long)s.size();
+    const T minDist = T(0.025f);
+    const V3<T> pt[6] = {{p.bmin[0], 0, 0}, {p.bmax[0], 0, 0}, {0, p.bmin[1], 0},
+                         {0, p.bmax[1], 0}, {0, 0, p.bmin[2]}, {0, 0, p.bmax[2]}};
+    const V3<T> nm[6] = {{1, 0, 0}, {-1, 0, 0}, {0, 1, 0}, {0, -1, 0}, {0, 0, 1}, {0, 0, -1}};
+#pragma omp parallel for schedule(static)
+    for (long i = 0; i < n; ++i) {
+        V3<T> x = s.x[i], v = s.v[i];
+        for (int k = 0; k < 6; ++k) {
+            T distance = dot(x - pt[k], nm[k]);
+            T approaching = dot(v, nm[k]);
+            if (distance < minDist) {
+                V3<T> vn = approaching * nm[k];
+                V3<T> vt = v - vn;
+                if (approaching < T(0)) {
+                    v = (-restitution) * vn + (T(1) - friction) * vt;
+                } else if (std::fabs(approaching) < T(1e-5f)) {
+                    v = (T(1) - friction) * vt;
+                }
+            }
+        }
+        s.v[i] = v;
+    }
+}
+
+// kernel_compute_vorticities (IntegrationCUDA.cu:104-128).  Pinned by zero vorticity under a
+// uniform velocity field and by rigid rotation giving a vorticity parallel to the axis.
+template <class T>
+void vorticities(State<T>& s, const Params<T>& p, const std::vector<uint32_t>& nbr,
+                 const std::vector<uint32_t>& cnt) {
+    const long n = (long)s.size();
+#pragma omp parallel for schedule(static)
+    for (long i = 0; i < n; ++i) {
+        const size_t st = (size_t)i * p.maxN;
+        V3<T> w{T(0), T(0), T(0)};
+        for (uint32_t o = 0; o < cnt[i]; ++o) {
+            uint32_t j = nbr[st + o];
+            V3<T> vij = s.v[j] - s.v[i];
+            V3<T> sep = s.xs[i] - s.xs[j];
+            w = w + (-s.mass[j]) * cross(vij, grad_spiky<T>(sep, p.K));
+        }
+        s.w[i] = w;
+    }
+}
+
+// kernel_apply_vorticity_confinement (IntegrationCUDA.cu:130-165).  Pinned by "no change when the
+// vorticity field is uniform" (eta = 0).
+template <class T>
+void vorticity_confinement(State<T>& s, const Params<T>& p, const std::vector<uint32_t>& nbr,
+                           const std::vector<uint32_t>& cnt) {
+    const long n = (long)s.size();
+#pragma omp parallel for schedule(static)
+    for (long i = 0; i < n; ++i) {
+        const size_t st = (size_t)i * p.maxN;
+        T invD = T(1) / s.density[i];
+        V3<T> eta{T(0), T(0), T(0)};
+        const T li = length(s.w[i]);
+        for (uint32_t o = 0; o < cnt[i]; ++o) {
+            uint32_t j = nbr[st + o];
+            T diff = li - length(s.w[j]);
+            V3<T> sep = s.xs[i] - s.xs[j];
+            eta = eta + (s.mass[j] * diff) * grad_spiky<T>(sep, p.K);
+        }
+        eta = eta * invD;
+        T el = length(eta);
+        if (el < T(1e-5f)) continue;
+        V3<T> N = eta / el;
+        V3<T> f = p.vortEps * cross(N, s.w[i]);
+        s.v[i] = s.v[i] + p.dt * f;
+    }
+}
+
+// kernel_apply_xsph_viscosity (IntegrationCUDA.cu:167-195), Jacobi: every particle reads the
+// velocities from before the pass.  Pinned by "no change under a uniform velocity field".
+template <class T>
+void xsph(State<T>& s, const Params<T>& p, const std::vector<uint32_t>& nbr,
+          const std::vector<uint32_t>& cnt, std::vector<V3<T>>& vnew) {
+    const long n = (long)s.size();
+    vnew.resize(n);
+#pragma omp parallel for schedule(static)
+    for (long i = 0; i < n; ++i) {
+        const size_t st = (size_t)i * p.maxN;
+        V3<T> dv{T(0), T(0), T(0)};
+        for (uint32_t o = 0; o < cnt[i]; ++o) {
+            uint32_t j = nbr[st + o];
+            V3<T> vij = s.v[j] - s.v[i];
+            V3<T> sep = s.xs[i] - s.xs[j];
+            T w = poly6<T>(dot(sep, sep), p.K);
+            dv = dv + ((s.mass[j] / s.density[j]) * vij) * w;
+        }
+        vnew[i] = s.v[i] + p.visc * dv;
+    }
+    s.v.swap(vnew);
+}
+
+// One iteration of runConstraintSolverCUDA's loop (ConstraintSolverCUDA.cu:206-218).
+template <class T>
+void constraint_iteration(State<T>& s, const Params<T>& p, const std::vector<uint32_t>& nbr,
+                          const std::vector<uint32_t>& cnt) {
+    densities(s, p, nbr, cnt);
+    lambdas(s, p, nbr, cnt);
+    position_deltas(s, p, nbr, cnt);
+    correct_positions(s, p);
+}
+
+// PBFSolver::step (PBFSolver.cpp:22-78): predict, hash, sort, neighbours, solverIterations x
+// constraint iteration, update, damping(0, 0.95), vorticity, confinement, XSPH.
+template <class T>
+void step(State<T>& s, const Params<T>& p, Taps* taps) {
+    Taps local;
+    Taps& t = taps ? *taps : local;
+    predict(s, p);
+    compute_hashes(s, p);
+    sort_by_hash(s, t.perm);
+    cell_ranges(s.hash, t.cell_hash, t.cell_start, t.cell_end);
+    find_neighbours(s, p, t.cell_hash, t.cell_start, t.cell_end, t.nbr, t.cnt);
+    for (int it = 0; it < p.iters; ++it) constraint_iteration(s, p, t.nbr, t.cnt);
+    update_position_velocity(s, p);
+    boundary_damping(s, p, T(0.0f), T(0.95f));
+    vorticities(s, p, t.nbr, t.cnt);
+    vorticity_confinement(s, p, t.nbr, t.cnt);
+    std::vector<V3<T>> vnew;
+    xsph(s, p, t.nbr, t.cnt, vnew);
+}
+
+// Density of every particle at the current positions (x), over all particles within h found by
+// a fresh bucket walk: the quantity the max-density-error metric reads.
+template <class T>
+void density_at_positions(State<T>& s, const Params<T>& p, std::vector<T>& rho) {
+    State<T> t = s;
+    t.xs = t.x;
+    compute_hashes(t, p);
+    std::vector<uint32_t> perm;
+    sort_by_hash(t, perm);
+    std::vector<uint32_t> ch, cs, ce, nbr, cnt;
+    cell_ranges(t.hash, ch, cs, ce);
+    find_neighbours(t, p, ch, cs, ce, nbr, cnt);
+    densities(t, p, nbr, cnt);
+    rho.assign(s.size(), T(0));
+    for (size_t k = 0; k < perm.size(); ++k) rho[perm[k]] = t.density[k];
+}
+
+}  // namespace pbf_oracle
