@@ -29,6 +29,7 @@ EXPORTED = [
     "akua_solver_set_graph", "akua_debug_hashes", "akua_debug_permutation",
     "akua_debug_cell_ranges", "akua_debug_neighbours", "akua_solver_set_timing",
     "akua_solver_reset_timing", "akua_solver_get_timing", "akua_solver_mean_neighbours",
+    "akua_solver_density_error", "akua_debug_probe_density", "akua_stream_copy_bandwidth",
 ]
 
 
@@ -73,7 +74,7 @@ def load() -> ctypes.CDLL:
         "akua_solver_destroy": [P],
         "akua_solver_set_particles": [P, P, i32],
         "akua_solver_get_particles": [P, P, i32],
-        "akua_solver_set_state": [P, P, P, P, i32],
+        "akua_solver_set_state": [P, P, P, P, P, P, i32],
         "akua_solver_get_state": [P, P, P, P, P, P, P, i32],
         "akua_solver_step": [P, f32, P, P],
         "akua_solver_step_n": [P, f32, P, P, i32],
@@ -90,6 +91,9 @@ def load() -> ctypes.CDLL:
         "akua_solver_reset_timing": [P],
         "akua_solver_get_timing": [P, P, i32, P, P, i32, ctypes.POINTER(i32)],
         "akua_solver_mean_neighbours": [P, ctypes.POINTER(ctypes.c_double)],
+        "akua_solver_density_error": [P, ctypes.POINTER(ctypes.c_float), ctypes.POINTER(ctypes.c_double)],
+        "akua_debug_probe_density": [P, P, i32],
+        "akua_stream_copy_bandwidth": [ctypes.c_int64, i32, ctypes.POINTER(ctypes.c_double)],
     }
     for name, args in sigs.items():
         fn = getattr(L, name)

@@ -65,5 +65,8 @@ void launch_vorticity(hipStream_t st, const DeviceState& d, const SphParams& P);
 void launch_confinement(hipStream_t st, const DeviceState& d, const SphParams& P,
                         const StepParams& S);
 void launch_xsph(hipStream_t st, const DeviceState& d, const SphParams& P);
+// Exact SPH density at the positions held in XsA (sorted order), per-scene error max and sum.
+void launch_density_probe(hipStream_t st, const DeviceState& d, const SphParams& P, float* rho,
+                          unsigned int* errMaxBits, double* errSum);
 
 }  // namespace akua

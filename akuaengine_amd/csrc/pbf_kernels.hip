@@ -346,6 +346,68 @@ __global__ __launch_bounds__(kBlock) void k_xsph(DeviceState d, SphParams P) {
     d.V[i] = make_float4(vn.x, vn.y, vn.z, 0.0f);
 }
 
+// Density at the current positions over every particle within h, each counted once: the walk of
+// k_neighbours with the hashing cell size, each distinct bucket of the stencil visited once (no
+// duplicates from stencil hash collisions) and no neighbour cap.  Accumulates, per scene, the
+// maximum and the sum of max(0, rho / rho0 - 1) (the density-constraint error).
+__global__ __launch_bounds__(kBlock) void k_density_probe(DeviceState d, SphParams P,
+                                                          float* __restrict__ rhoOut,
+                                                          unsigned int* __restrict__ errMaxBits,
+                                                          double* __restrict__ errSum) {
+    uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+    float err = 0.0f;
+    uint32_t scene = 0;
+    bool active = false;
+    if (i < d.nslots) {
+        scene = d.chunkScene[i >> 8];
+        const SceneDev& sc = d.scenes[scene];
+        if (i < sc.end) {
+            active = true;
+            const float4 xi4 = d.XsA[i];
+            const f3 xi = make3(xi4);
+            int cx, cy, cz;
+            discretize(xi, P.cellHash, cx, cy, cz);
+            float rho = xi4.w * P.w0;
+            uint32_t hs[27];
+#pragma unroll
+            for (int q = 0; q < 27; ++q)
+                hs[q] = cell_hash((int)((uint32_t)cx + (uint32_t)(q / 9 - 1)),
+                                  (int)((uint32_t)cy + (uint32_t)((q / 3) % 3 - 1)),
+                                  (int)((uint32_t)cz + (uint32_t)(q % 3 - 1)), sc.tableSize);
+#pragma unroll
+            for (int q = 0; q < 27; ++q) {
+                bool dup = false;
+#pragma unroll
+                for (int p = 0; p < q; ++p) dup |= hs[p] == hs[q];
+                uint2 r;
+                if (dup || !table_find(d.tkey, d.trange, sc, hs[q], r)) continue;
+                for (uint32_t k = r.x; k < r.y; ++k) {
+                    if (k == i) continue;
+                    float4 xk = d.XsA[k];
+                    f3 sep = xi - make3(xk);
+                    float d2 = dot3(sep, sep);
+                    if (d2 < P.h2) rho = rho + xk.w * poly6(d2, P);
+                }
+            }
+            if (rhoOut) rhoOut[i] = rho;
+            err = fmaxf(0.0f, rho * P.invRest - 1.0f);
+        }
+    }
+    // one atomic per wave (a wave never spans two scenes: scenes start 256-aligned)
+    float m = err;
+    double sum = err;
+    for (int off = 32; off > 0; off >>= 1) {
+        m = fmaxf(m, __shfl_down(m, off, 64));
+        sum += __shfl_down(sum, off, 64);
+    }
+    const uint64_t any = __ballot(active);
+    const uint32_t s0 = __shfl(scene, 0, 64);
+    if (__lane_id() == 0 && any) {
+        atomicMax(&errMaxBits[s0], __float_as_uint(m));  // non-negative floats order as integers
+        atomicAdd(&errSum[s0], sum);
+    }
+}
+
 inline unsigned grid(uint32_t n) { return (n + kBlock - 1) / kBlock; }
 
 }  // namespace
@@ -386,6 +448,10 @@ void launch_confinement(hipStream_t st, const DeviceState& d, const SphParams& P
 }
 void launch_xsph(hipStream_t st, const DeviceState& d, const SphParams& P) {
     k_xsph<<<grid(d.nslots), kBlock, 0, st>>>(d, P);
+}
+void launch_density_probe(hipStream_t st, const DeviceState& d, const SphParams& P, float* rho,
+                          unsigned int* errMaxBits, double* errSum) {
+    k_density_probe<<<grid(d.nslots), kBlock, 0, st>>>(d, P, rho, errMaxBits, errSum);
 }
 
 }  // namespace akua

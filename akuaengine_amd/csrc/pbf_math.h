@@ -62,11 +62,20 @@ __device__ __forceinline__ float corr_pow(float x, const SphParams& P) {
     return r;
 }
 
+// float -> int with NaN -> 0 and saturation, spelled out so the result never depends on what
+// the compiler assumes about an out-of-range conversion.
+__device__ __forceinline__ int to_int_sat(float x) {
+    if (!(x == x)) return 0;
+    if (x >= 2147483648.0f) return 2147483647;
+    if (x < -2147483648.0f) return (int)0x80000000u;
+    return (int)x;
+}
+
 // device_discretize_position (NeighbourSearchCUDA.cu:15-21)
 __device__ __forceinline__ void discretize(f3 p, float cs, int& cx, int& cy, int& cz) {
-    cx = (int)floorf(p.x / cs);
-    cy = (int)floorf(p.y / cs);
-    cz = (int)floorf(p.z / cs);
+    cx = to_int_sat(floorf(p.x / cs));
+    cy = to_int_sat(floorf(p.y / cs));
+    cz = to_int_sat(floorf(p.z / cs));
 }
 
 // device_get_hash (NeighbourSearchCUDA.cu:23-27), with the int multiply's wrap done in uint32.

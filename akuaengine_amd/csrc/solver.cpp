@@ -144,6 +144,9 @@ struct akua_solver {
     DevBuf<uint2> trange;
     DevBuf<SortTile> dTiles;
     DevBuf<unsigned long long> nbrSum;
+    DevBuf<float> probeRho;
+    DevBuf<unsigned int> probeMax;
+    DevBuf<double> probeSum;
     bool hasState = false;
     int sortResult = 0;  // 0: (keysA, valsA); 1: (keysB, valsB)
 
@@ -284,6 +287,9 @@ void setup(akua_solver* s, const std::vector<uint32_t>& counts) {
     s->cnt.alloc(ns);
     s->dTiles.alloc(s->tilesHost.size());
     s->nbrSum.alloc(1);
+    s->probeRho.alloc(ns);
+    s->probeMax.alloc(s->scenes.size());
+    s->probeSum.alloc(s->scenes.size());
     HIP_CHECK(hipMemcpy(s->dChunkScene.p, s->chunkSceneHost.data(), nChunks * 4, hipMemcpyHostToDevice));
     HIP_CHECK(hipMemcpy(s->dTiles.p, s->tilesHost.data(), s->tilesHost.size() * sizeof(SortTile),
                         hipMemcpyHostToDevice));
@@ -453,6 +459,27 @@ void run_steps(akua_solver* s, float dt, int nsteps) {
     }
 }
 
+// Exact density at the current positions (k_density_probe): hashes and sorts the positions as a
+// step with dt = 0 would, then walks each distinct stencil bucket once.  Touches only step scratch
+// (the sort buffers, X0, XsA, the cell table), never the particle state.
+void run_density_probe(akua_solver* s) {
+    if (!s->hasState) fail(AKUA_ERR_STATE, "set the particles first");
+    upload_scenes(s);
+    const StepParams S0{0.0f, {0.0f, 0.0f, 0.0f}};
+    hipStream_t st = s->stream;
+    DeviceState d = s->devState();
+    HIP_CHECK(hipMemsetAsync(s->probeMax.p, 0, s->scenes.size() * sizeof(unsigned int), st));
+    HIP_CHECK(hipMemsetAsync(s->probeSum.p, 0, s->scenes.size() * sizeof(double), st));
+    launch_fill_u32(st, s->tkey.p, s->tableTotal, kEmptyKey);
+    launch_predict_hash(st, d, s->sph, S0);
+    radix_sort_pairs(st, s->plan, s->keysA.p, s->valsA.p, s->keysB.p, s->valsB.p, s->dTiles.p,
+                     (uint32_t)s->tilesHost.size(), s->countsPerDigit, s->counts.p, s->nslots,
+                     nullptr, nullptr);
+    launch_gather(st, d, s->sph, S0);
+    launch_density_probe(st, d, s->sph, s->probeRho.p, s->probeMax.p, s->probeSum.p);
+    HIP_CHECK(hipGetLastError());
+}
+
 void check_n(akua_solver* s, int32_t n) {
     if (n != (int32_t)s->nparticles)
         fail(AKUA_ERR_INVALID_ARG, "n does not match the solver's particle count");
@@ -584,11 +611,12 @@ akua_status akua_solver_set_particles(akua_solver* s, const akua_particle* parti
 }
 
 akua_status akua_solver_set_state(akua_solver* s, const float* pos, const float* vel,
-                                  const float* mass, int32_t n) {
+                                  const float* mass, const float* density, const float* lambda,
+                                  int32_t n) {
     return guarded([&] {
         if (!s || !pos || !vel) fail(AKUA_ERR_INVALID_ARG, "NULL argument");
         check_n(s, n);
-        upload_soa(s, pos, vel, mass, nullptr, nullptr, nullptr, nullptr, false, nullptr);
+        upload_soa(s, pos, vel, mass, density, lambda, nullptr, nullptr, false, nullptr);
     });
 }
 
@@ -823,6 +851,36 @@ akua_status akua_solver_get_timing(akua_solver* s, char* nameBuf, int32_t nameLe
             if (ms) ms[i] = acc.first;
             if (launches) launches[i] = acc.second;
         }
+    });
+}
+
+akua_status akua_solver_density_error(akua_solver* s, float* maxError, double* meanError) {
+    return guarded([&] {
+        if (!s) fail(AKUA_ERR_INVALID_ARG, "NULL solver");
+        run_density_probe(s);
+        auto mx = download(s, s->probeMax.p, s->scenes.size());
+        auto sm = download(s, s->probeSum.p, s->scenes.size());
+        float m = 0.0f;
+        double tot = 0.0;
+        for (size_t i = 0; i < mx.size(); ++i) {
+            float v;
+            std::memcpy(&v, &mx[i], sizeof(v));
+            m = std::max(m, v);
+            tot += sm[i];
+        }
+        if (maxError) *maxError = m;
+        if (meanError) *meanError = tot / (double)s->nparticles;
+    });
+}
+
+akua_status akua_debug_probe_density(akua_solver* s, float* rho, int32_t n) {
+    return guarded([&] {
+        if (!s || !rho) fail(AKUA_ERR_INVALID_ARG, "NULL argument");
+        check_n(s, n);
+        auto r = download(s, s->probeRho.p, s->nslots);
+        size_t k = 0;
+        for (const SceneDev& sc : s->scenes)
+            for (uint32_t slot = sc.begin; slot < sc.end; ++slot) rho[k++] = r[slot];
     });
 }
 

@@ -61,12 +61,14 @@ class PBFSolver:
         self.close()
 
     # -- state --------------------------------------------------------------------------------
-    def set_state(self, pos: np.ndarray, vel: np.ndarray, mass: Optional[np.ndarray] = None) -> None:
+    def set_state(self, pos: np.ndarray, vel: np.ndarray, mass: Optional[np.ndarray] = None,
+                  density: Optional[np.ndarray] = None, lam: Optional[np.ndarray] = None) -> None:
         n = self.numParticles
         pos = np.ascontiguousarray(pos, np.float32).reshape(n, 3)
         vel = np.ascontiguousarray(vel, np.float32).reshape(n, 3)
-        m = None if mass is None else np.ascontiguousarray(mass, np.float32).reshape(n)
-        _lib.check(self._L.akua_solver_set_state(self._h, _p(pos), _p(vel), _p(m), n))
+        opt = [None if a is None else np.ascontiguousarray(a, np.float32).reshape(n)
+               for a in (mass, density, lam)]
+        _lib.check(self._L.akua_solver_set_state(self._h, _p(pos), _p(vel), *[_p(a) for a in opt], n))
 
     def get_state(self) -> Dict[str, np.ndarray]:
         n = self.numParticles
@@ -164,6 +166,18 @@ class PBFSolver:
         for i in range(min(cap, count.value)):
             nm = names.raw[i * nl:(i + 1) * nl].split(b"\0", 1)[0].decode()
             out[nm] = (float(ms[i]), int(launches[i]))
+        return out
+
+    def density_error(self):
+        """(max, mean) over particles of max(0, rho / restDensity - 1), rho the exact SPH density
+        at the current positions (computed on the device)."""
+        m, a = ctypes.c_float(0.0), ctypes.c_double(0.0)
+        _lib.check(self._L.akua_solver_density_error(self._h, ctypes.byref(m), ctypes.byref(a)))
+        return float(m.value), float(a.value)
+
+    def debug_probe_density(self) -> np.ndarray:
+        out = np.zeros(self.numParticles, np.float32)
+        _lib.check(self._L.akua_debug_probe_density(self._h, _p(out), self.numParticles))
         return out
 
     def mean_neighbours(self) -> float:

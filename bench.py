@@ -1,0 +1,201 @@
+#!/usr/bin/env python
+"""Throughput of the Position-Based Fluids step on one MI355X.
+
+Workload (the headline configuration): a 1,000,000-particle dam break -- a 100^3 lattice at
+spacing 0.05 with +-0.005 seeded jitter in a [0,10]x[0,6]x[0,5] box, rest density 6378, h = 0.1,
+relaxation 600, artificial pressure k = 0.1, n = 4, delta_q = 0.2 h, XSPH c = 0.01,
+dt = 0.0083, gravity -9.8 y, 4 Jacobi iterations -- stepped by the HIP solver (libakua.so)
+with its inputs resident in HBM.  A step is one PBFSolver::step (predict, hash, sort, neighbour
+lists, 4 x (density/lambda, position correction), update, wall damping, vorticity confinement,
+XSPH); the timed region launches one hipGraph per step.
+
+Printed: one JSON line with the throughput (particle-substeps/s), the roofline of the dominant
+kernel (algorithmic bytes per launch / mean launch time from HIP events on the solver's stream,
+against the 8 TB/s HBM peak), the maximum and mean density-constraint error after the timed
+steps, and the CPU oracle timed on the host's cores on a bounded sample of the same workload.
+
+Usage: python bench.py [--steps K] [--warmup W] [--gpus 1] [--cpu-seconds S] [--no-cpu-baseline]
+"""
+from __future__ import annotations
+
+import argparse
+import ctypes
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "particle-substeps/sec at 1M particles, learned vs plain, max density err ≤1%"
+HBM_PEAK_GBPS = 8000.0
+
+# Algorithmic bytes per particle of each kernel (FP32 structure of arrays, every array element a
+# pass must stream counted once; neighbour positions are gathered and counted as cache-resident;
+# nbar = measured mean neighbour count, the neighbour list is 4 bytes per entry).
+#   density/lambda : x* 12 + count 4 + density 4 + lambda 4 + list 4 nbar
+#   delta/correct  : x* 12 + lambda 4 + count 4 + x* out 12 + list 4 nbar
+#   last iteration : the above + update and damping (x 12 in, x 12 + v 12 out, ids 8) 48
+#   neighbours     : x* 12 + count 4 + list 4 nbar
+#   vorticity      : x 12 + v 12 + count 4 + omega 12 + list 4 nbar
+#   confinement    : x 12 + omega 12 + rho 4 + v 12 + count 4 + v out 12 + list 4 nbar
+#   xsph           : x 12 + v 12 + rho 4 + count 4 + v out 12 + list 4 nbar
+#   predict/hash 44, gather (reorder) 72, each sort kernel 16 per pass over 3 kernels, table 8
+BYTES = {
+    "density_lambda": lambda nb: 24 + 4 * nb,
+    "delta_correct": lambda nb: 32 + 4 * nb,
+    "delta_correct_update": lambda nb: 80 + 4 * nb,
+    "neighbours": lambda nb: 16 + 4 * nb,
+    "vorticity": lambda nb: 40 + 4 * nb,
+    "confinement": lambda nb: 56 + 4 * nb,
+    "xsph": lambda nb: 44 + 4 * nb,
+    "predict_hash": lambda nb: 44,
+    "gather": lambda nb: 72,
+    "rs_hist": lambda nb: 4,
+    "rs_scan": lambda nb: 0,
+    "rs_scatter": lambda nb: 16,
+    "fill_table": lambda nb: 8,
+}
+
+
+def log(msg: str) -> None:
+    print(msg, file=sys.stderr, flush=True)
+
+
+def cpu_baseline(scene, cfg, corr, dt, budget_s: float):
+    """The FP32 CPU oracle (OpenMP over particles) on the same 1M workload from its initial
+    state: steps until the time budget is spent (at least one)."""
+    from oracle import oracle as O
+    threads = O.max_threads()
+    o = O.Oracle(scene.n, cfg, corr, 32)
+    o.set_state(scene.pos, scene.vel, scene.mass)
+    steps, t0 = 0, time.perf_counter()
+    while True:
+        o.step(dt, scene.box_min, scene.box_max, 1)
+        steps += 1
+        el = time.perf_counter() - t0
+        log(f"[bench] cpu oracle step {steps}: {el:.1f} s")
+        if el >= budget_s or steps >= 50:
+            break
+    return {"value": scene.n * steps / el, "unit": "particle-substeps/s", "cores": threads,
+            "kind": "port",
+            "sample": f"{steps} steps of the 1M dam break from its initial state, FP32 oracle, "
+                      f"{threads} OpenMP threads, {el:.1f} s"}
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=100)
+    ap.add_argument("--warmup", type=int, default=10)
+    ap.add_argument("--timing-steps", type=int, default=10,
+                    help="steps with per-kernel HIP events after the timed region")
+    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU: every box in this setup has one MI355X, so --gpus "
+              f"{args.gpus} is not supported (the multi-GPU decomposition is a design note only).",
+              file=sys.stderr)
+        return 2
+
+    from akuaengine_amd import _lib, scenes
+    from akuaengine_amd.config import DAM_BREAK_DT, dam_break_params
+    from akuaengine_amd.solver import PBFSolver
+
+    cfg, corr = dam_break_params(4)
+    dt = DAM_BREAK_DT
+    sc = scenes.dam_break_1m()
+    log(f"[bench] workload dam_break_1m N={sc.n}")
+    s = PBFSolver(sc.n, cfg, corr)
+    s.set_state(sc.pos, sc.vel, sc.mass)
+
+    s.step_n(dt, sc.box_min, sc.box_max, args.warmup)
+    s.synchronize()
+    s.mean_neighbours()  # resets the neighbour statistics
+    log(f"[bench] warm-up {args.warmup} steps done; timing {args.steps} steps")
+    t0 = time.perf_counter()
+    s.step_n(dt, sc.box_min, sc.box_max, args.steps)
+    s.synchronize()
+    elapsed = time.perf_counter() - t0
+    nbar = s.mean_neighbours()
+    value = sc.n * args.steps / elapsed
+    log(f"[bench] {elapsed * 1e3 / args.steps:.3f} ms/step, {value:.4e} particle-substeps/s, nbar {nbar:.2f}")
+
+    # per-kernel device time: HIP events around every launch on the solver's stream
+    s.set_timing(True)
+    s.reset_timing()
+    s.step_n(dt, sc.box_min, sc.box_max, args.timing_steps)
+    times = s.get_timing()
+    s.set_timing(False)
+    nbar_t = s.mean_neighbours()
+    steps_t = args.timing_steps
+    kernels = {}
+    for name, (ms, launches) in times.items():
+        per_launch = ms / launches
+        b = BYTES.get(name, lambda nb: 0)(nbar_t) * sc.n
+        kernels[name] = {"ms_per_launch": round(per_launch, 5), "launches_per_step": launches // steps_t,
+                         "ms_per_step": round(ms / steps_t, 5),
+                         "achieved_gbps": round(b / (per_launch * 1e-3) / 1e9, 1) if b else None}
+    dominant = max(kernels, key=lambda k: kernels[k]["ms_per_step"])
+
+    def roof(name):
+        bpp = BYTES[name](nbar_t)
+        per_launch_s = kernels[name]["ms_per_launch"] * 1e-3
+        achieved = bpp * sc.n / per_launch_s / 1e9
+        return {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBPS, "unit": "GB/s",
+                "frac": round(achieved / HBM_PEAK_GBPS, 4), "traffic": None, "kernel": name,
+                "bytes_per_particle": round(bpp, 2), "particles_per_launch": sc.n}
+
+    roofline = roof(dominant)
+    roof_dl = roof("density_lambda")
+    step_bytes = sum(BYTES.get(k, lambda nb: 0)(nbar_t) * v["launches_per_step"] for k, v in kernels.items())
+    ms_step = elapsed * 1e3 / args.steps
+    roof_step = {"bytes_per_particle_step": round(step_bytes, 1),
+                 "achieved": round(step_bytes * sc.n / (ms_step * 1e-3) / 1e9, 1),
+                 "frac": round(step_bytes * sc.n / (ms_step * 1e-3) / 1e9 / HBM_PEAK_GBPS, 4)}
+    traffic_file = os.path.join(ROOT, "profiles", "traffic.json")
+    if os.path.exists(traffic_file):
+        with open(traffic_file) as f:
+            tr = json.load(f)
+        for r in (roofline, roof_dl):
+            t = tr.get(r["kernel"])
+            if t:
+                r["traffic"] = t["bytes_per_launch"]
+
+    gbps = ctypes.c_double(0.0)
+    _lib.check(_lib.load().akua_stream_copy_bandwidth(1 << 30, 20, ctypes.byref(gbps)))
+    emax, emean = s.density_error()
+    log(f"[bench] density error after the timed steps: max {emax:.4f} mean {emean:.5f}")
+
+    cpu = None
+    if not args.no_cpu_baseline:
+        cpu = cpu_baseline(sc, cfg, corr, dt, args.cpu_seconds)
+
+    out = {
+        "metric": METRIC, "value": round(value, 1), "unit": "particle-substeps/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ms_step, 4),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "fp32",
+        "data": "synthetic",
+        "config": {"workload": "dam_break_1m_plain", "particles": sc.n, "solver": "plain",
+                   "solverIterations": cfg.solverIterations, "dt": dt, "restDensity": cfg.restDensity,
+                   "corr_k": corr.k, "corr_n": corr.n, "corr_delta_q": corr.delta_q,
+                   "box_max": list(sc.box_max), "parallelism": "single-gpu"},
+        "roofline": roofline,
+        "roofline_density_lambda": roof_dl,
+        "roofline_step": roof_step,
+        "stream_copy_gbps": round(gbps.value, 1),
+        "mean_neighbours": round(nbar_t, 3),
+        "density_error": {"max": round(emax, 5), "mean": round(emean, 6),
+                          "definition": "max/mean over particles of max(0, rho/rho0 - 1), exact SPH "
+                                        "density at the positions after the timed steps"},
+        "kernels": kernels,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out))
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

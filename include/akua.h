@@ -103,10 +103,13 @@ akua_status akua_solver_set_particles(akua_solver* s, const akua_particle* parti
  * vorticity and XSPH), position_delta (last iteration), vorticity, mass, density, lambda, hash. */
 akua_status akua_solver_get_particles(akua_solver* s, akua_particle* particles, int32_t n);
 
-/* Structure-of-arrays upload: pos/vel are 3*n floats (xyz interleaved per particle), mass n
- * floats (NULL: all 1).  Particle ids are assigned 0..n-1 in the given order. */
+/* Structure-of-arrays upload: pos/vel are 3*n floats (xyz interleaved per particle); mass,
+ * density and lambda n floats each (NULL: mass 1, density 0, lambda 0).  Density and lambda only
+ * matter to a step with solverIterations == 0, which reuses them as the reference reuses the
+ * record's fields.  Particle ids are assigned 0..n-1 in the given order. */
 akua_status akua_solver_set_state(akua_solver* s, const float* pos, const float* vel,
-                                  const float* mass, int32_t n);
+                                  const float* mass, const float* density, const float* lambda,
+                                  int32_t n);
 /* Structure-of-arrays download in the current order; any pointer may be NULL.  ids[k] is the
  * original index of the particle now in slot k. */
 akua_status akua_solver_get_state(akua_solver* s, float* pos, float* vel, float* mass,
@@ -155,6 +158,20 @@ akua_status akua_solver_reset_timing(akua_solver* s);
  * of name_len bytes each. */
 akua_status akua_solver_get_timing(akua_solver* s, char* name_buf, int32_t name_len, double* ms,
                                    int64_t* launches, int32_t cap, int32_t* count);
+/* Density-constraint error at the current positions: the exact SPH density of every particle
+ * over all particles within h, each counted once (no neighbour cap, no duplicates from hash
+ * collisions), then max and mean over particles of max(0, density / restDensity - 1).  Runs on
+ * the device; overwrites the step's scratch (so the parity taps then describe this evaluation). */
+akua_status akua_solver_density_error(akua_solver* s, float* maxError, double* meanError);
+/* Densities of the last akua_solver_density_error evaluation, in the order it sorted the
+ * particles (hash order of the current positions). */
+akua_status akua_debug_probe_density(akua_solver* s, float* rho, int32_t n);
+
+/* Achievable device bandwidth: a grid-stride float4 copy of `bytes` bytes, `reps` times, timed
+ * with HIP events; *gbps counts bytes read plus bytes written.  Also the known-size kernel that
+ * calibrates hardware byte counters. */
+akua_status akua_stream_copy_bandwidth(int64_t bytes, int32_t reps, double* gbps);
+
 /* Mean neighbour count of the last step (device reduction), for the byte model. */
 akua_status akua_solver_mean_neighbours(akua_solver* s, double* mean);
 

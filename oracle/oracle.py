@@ -59,7 +59,7 @@ def lib():
         L.orc_ncells.restype = lng
         L.orc_ncells.argtypes = [vp]
         L.orc_get_taps.argtypes = [vp, P, P, P, P, P, P, P]
-        L.orc_density_at_positions.argtypes = [vp, f, P, P, P]
+        L.orc_true_density.argtypes = [vp, P, P]
         L.orc_poly6.restype = d
         L.orc_poly6.argtypes = [ctypes.c_int, d, d]
         L.orc_grad_spiky.argtypes = [ctypes.c_int, P, d, P]
@@ -178,7 +178,14 @@ class Oracle:
         out["nbr"] = out["nbr"].reshape(n, self.max_neighbours)
         return out
 
-    def density_at_positions(self, box_min=(0, 0, 0), box_max=(1, 1, 1)) -> np.ndarray:
-        rho = np.zeros(self.n)
-        lib().orc_density_at_positions(self._h, 0.0, _p(_f3(box_min)), _p(_f3(box_max)), _p(rho))
-        return rho
+    def true_density(self):
+        """Exact SPH density at the current positions: (by sorted slot, by particle id)."""
+        a, b = np.zeros(self.n), np.zeros(self.n)
+        lib().orc_true_density(self._h, _p(a), _p(b))
+        return a, b
+
+    def density_error(self, rest_density: float):
+        """(max, mean) over particles of max(0, rho / rest_density - 1) at the current positions."""
+        _, rho = self.true_density()
+        e = np.maximum(0.0, rho / rest_density - 1.0)
+        return float(e.max()), float(e.mean())

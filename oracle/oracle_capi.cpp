@@ -197,19 +197,23 @@ void orc_get_taps(void* hp, uint32_t* sorted_hash, uint32_t* perm, uint32_t* nbr
     if (cell_end) std::copy(t.cell_end.begin(), t.cell_end.end(), cell_end);
 }
 
-void orc_density_at_positions(void* hp, float dt, const float bmin[3], const float bmax[3],
-                              double* rho) {
+// Exact density at the current positions (see pbf_oracle::true_density).  rho_sorted[k] is in
+// the order a step would sort the particles; rho_by_id[id] by particle id.  Either may be NULL.
+void orc_true_density(void* hp, double* rho_sorted, double* rho_by_id) {
     auto* h = static_cast<Handle*>(hp);
+    const float z[3] = {0.0f, 0.0f, 0.0f};
     if (h->prec == 64) {
-        Params<double> p = make_params<double>(h->cfg, h->corr, dt, bmin, bmax, h->tableSize);
-        std::vector<double> r;
-        density_at_positions(h->sd, p, r);
-        std::copy(r.begin(), r.end(), rho);
+        Params<double> p = make_params<double>(h->cfg, h->corr, 0.0f, z, z, h->tableSize);
+        std::vector<double> a, b;
+        true_density(h->sd, p, a, b);
+        if (rho_sorted) std::copy(a.begin(), a.end(), rho_sorted);
+        if (rho_by_id) std::copy(b.begin(), b.end(), rho_by_id);
     } else {
-        Params<float> p = make_params<float>(h->cfg, h->corr, dt, bmin, bmax, h->tableSize);
-        std::vector<float> r;
-        density_at_positions(h->sf, p, r);
-        for (size_t i = 0; i < r.size(); ++i) rho[i] = r[i];
+        Params<float> p = make_params<float>(h->cfg, h->corr, 0.0f, z, z, h->tableSize);
+        std::vector<float> a, b;
+        true_density(h->sf, p, a, b);
+        if (rho_sorted) for (size_t i = 0; i < a.size(); ++i) rho_sorted[i] = a[i];
+        if (rho_by_id) for (size_t i = 0; i < b.size(); ++i) rho_by_id[i] = b[i];
     }
 }
 

@@ -133,12 +133,22 @@ inline V3<T> grad_spiky(V3<T> r, const Kern<T>& k) {
 template <class T>
 inline V3<T> grad_spiky(V3<T> r, T h) { return grad_spiky<T>(r, Kern<T>(h)); }
 
+// static_cast<int>(float) as the GPU executes it: NaN gives 0 and out-of-range values saturate
+// (C++ leaves both undefined; x86 would give INT_MIN).
+template <class T>
+inline int32_t to_int_sat(T x) {
+    if (!(x == x)) return 0;
+    if (x >= T(2147483648.0)) return INT32_MAX;
+    if (x < T(-2147483648.0)) return INT32_MIN;
+    return static_cast<int32_t>(x);
+}
+
 // device_discretize_position (NeighbourSearchCUDA.cu:15-21): floor(x / cellSize) per axis.
 template <class T>
 inline void discretize(V3<T> p, T cs, int32_t c[3]) {
-    c[0] = static_cast<int32_t>(std::floor(p.x / cs));
-    c[1] = static_cast<int32_t>(std::floor(p.y / cs));
-    c[2] = static_cast<int32_t>(std::floor(p.z / cs));
+    c[0] = to_int_sat<T>(std::floor(p.x / cs));
+    c[1] = to_int_sat<T>(std::floor(p.y / cs));
+    c[2] = to_int_sat<T>(std::floor(p.z / cs));
 }
 
 // device_get_hash (NeighbourSearchCUDA.cu:23-27).  The reference multiplies in int and casts to
@@ -567,21 +577,54 @@ void step(State<T>& s, const Params<T>& p, Taps* taps) {
     xsph(s, p, t.nbr, t.cnt, vnew);
 }
 
-// Density of every particle at the current positions (x), over all particles within h found by
-// a fresh bucket walk: the quantity the max-density-error metric reads.
+// Density of every particle at the current positions (x) over all particles within h, each
+// counted once: particles are hashed with the hashing cell size and stably sorted as a step would
+// sort them; each distinct stencil bucket is walked once (no duplicates, no cap).  rhoSorted[k] is
+// the density of the particle in sorted slot k; rhoById[id] the same value by particle id.  This
+// is the quantity the max-density-error metric reads.
 template <class T>
-void density_at_positions(State<T>& s, const Params<T>& p, std::vector<T>& rho) {
+void true_density(const State<T>& s, const Params<T>& p, std::vector<T>& rhoSorted,
+                  std::vector<T>& rhoById) {
     State<T> t = s;
     t.xs = t.x;
     compute_hashes(t, p);
     std::vector<uint32_t> perm;
     sort_by_hash(t, perm);
-    std::vector<uint32_t> ch, cs, ce, nbr, cnt;
+    std::vector<uint32_t> ch, cs, ce;
     cell_ranges(t.hash, ch, cs, ce);
-    find_neighbours(t, p, ch, cs, ce, nbr, cnt);
-    densities(t, p, nbr, cnt);
-    rho.assign(s.size(), T(0));
-    for (size_t k = 0; k < perm.size(); ++k) rho[perm[k]] = t.density[k];
+    const long n = (long)t.size();
+    rhoSorted.assign(n, T(0));
+    rhoById.assign(n, T(0));
+#pragma omp parallel for schedule(dynamic, 256)
+    for (long i = 0; i < n; ++i) {
+        int32_t c[3];
+        discretize(t.xs[i], p.cellHash, c);
+        T rho = t.mass[i] * poly6<T>(T(0), p.K);
+        uint32_t seen[27];
+        int nseen = 0;
+        for (int dx = -1; dx <= 1; ++dx)
+            for (int dy = -1; dy <= 1; ++dy)
+                for (int dz = -1; dz <= 1; ++dz) {
+                    uint32_t hh = cell_hash((int32_t)((uint32_t)c[0] + (uint32_t)dx),
+                                            (int32_t)((uint32_t)c[1] + (uint32_t)dy),
+                                            (int32_t)((uint32_t)c[2] + (uint32_t)dz), p.tableSize);
+                    bool dup = false;
+                    for (int q = 0; q < nseen; ++q) dup = dup || seen[q] == hh;
+                    if (dup) continue;
+                    seen[nseen++] = hh;
+                    auto it = std::lower_bound(ch.begin(), ch.end(), hh);
+                    if (it == ch.end() || *it != hh) continue;
+                    size_t b = (size_t)(it - ch.begin());
+                    for (long k = (long)cs[b]; k < (long)ce[b]; ++k) {
+                        if (k == i) continue;
+                        V3<T> sep = t.xs[i] - t.xs[k];
+                        T d2 = dot(sep, sep);
+                        if (d2 < p.h2) rho = rho + t.mass[k] * poly6<T>(d2, p.K);
+                    }
+                }
+        rhoSorted[i] = rho;
+        rhoById[t.id[i]] = rho;
+    }
 }
 
 }  // namespace pbf_oracle
