@@ -6,8 +6,10 @@ namespace akua {
 
 namespace {
 
+// Records in buffer order become both the logical and the physical order (lid = phys = identity).
 __global__ void k_import_aos(const AosParticle* __restrict__ in, uint32_t n, float4* X, float4* V,
-                             float* dens, float* lam, uint32_t* id, float4* color, float* size) {
+                             float* dens, float* lam, uint32_t* id, uint32_t* lid, uint32_t* phys,
+                             float4* color, float* size) {
     uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     const AosParticle& p = in[i];
@@ -16,14 +18,17 @@ __global__ void k_import_aos(const AosParticle* __restrict__ in, uint32_t n, flo
     dens[i] = p.density;
     lam[i] = p.lambda;
     id[i] = i;
+    lid[i] = i;
+    phys[i] = i;
     color[i] = make_float4(p.color[0], p.color[1], p.color[2], p.color[3]);
     size[i] = p.size;
 }
 
+// Physical slot i is written to its logical position, the order of the reference's buffer.
 __global__ void k_export_aos(AosParticle* __restrict__ out, uint32_t n, const float4* X,
                              const float4* V, const float* dens, const float* lam,
-                             const uint32_t* id, const float4* W, const uint32_t* hash,
-                             const float4* color, const float* size) {
+                             const uint32_t* id, const uint32_t* lid, const float4* W,
+                             const uint32_t* hash, const float4* color, const float* size) {
     uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     AosParticle p;
@@ -37,26 +42,30 @@ __global__ void k_export_aos(AosParticle* __restrict__ out, uint32_t n, const fl
     p.mass = x.w;
     p.density = dens[i];
     p.lambda = lam[i];
-    p.hash = hash[i];
+    const uint32_t k = lid[i];
+    p.hash = hash[k];
     p.color[0] = c.x; p.color[1] = c.y; p.color[2] = c.z; p.color[3] = c.w;
     p.size = size[id[i]];
-    out[i] = p;
+    out[k] = p;
 }
 
 }  // namespace
 
 void launch_import_aos(hipStream_t st, const AosParticle* in, uint32_t n, float4* X, float4* V,
-                       float* dens, float* lam, uint32_t* id, float4* color, float* size) {
-    if (n) k_import_aos<<<(n + 255) / 256, 256, 0, st>>>(in, n, X, V, dens, lam, id, color, size);
+                       float* dens, float* lam, uint32_t* id, uint32_t* lid, uint32_t* phys,
+                       float4* color, float* size) {
+    if (n)
+        k_import_aos<<<(n + 255) / 256, 256, 0, st>>>(in, n, X, V, dens, lam, id, lid, phys, color,
+                                                      size);
 }
 
 void launch_export_aos(hipStream_t st, AosParticle* out, uint32_t n, const float4* X,
                        const float4* V, const float* dens, const float* lam, const uint32_t* id,
-                       const float4* W, const uint32_t* hash, const float4* color,
-                       const float* size) {
+                       const uint32_t* lid, const float4* W, const uint32_t* hash,
+                       const float4* color, const float* size) {
     if (n)
-        k_export_aos<<<(n + 255) / 256, 256, 0, st>>>(out, n, X, V, dens, lam, id, W, hash, color,
-                                                      size);
+        k_export_aos<<<(n + 255) / 256, 256, 0, st>>>(out, n, X, V, dens, lam, id, lid, W, hash,
+                                                      color, size);
 }
 
 }  // namespace akua

@@ -1,15 +1,27 @@
 // pbf_kernels.h -- host launchers of the PBF step kernels (pbf_kernels.hip).
 //
-// Data layout in HBM (structure of arrays, slots in the current sorted order):
+// Two orders of the particles:
+//   logical  -- the reference's order: after every step, the stable sort by cell hash of the
+//               previous logical order (NeighbourSearchCUDA.cu:166-170).  Bucket walks, the
+//               neighbour-list order, the permutation tap and akua_solver_get_state use it.
+//   physical -- where the particle's data lives in HBM: sorted by the Morton code of its
+//               2x2x2 super-cell (ties in logical order), so the 64 particles of a wave are
+//               spatial neighbours and their neighbour gathers hit the same cache lines.
+// lid[p] is the logical index of physical slot p, phys[k] the physical slot of logical index k.
+// Every per-particle sum runs over the neighbour list in the reference's order, so the physical
+// layout changes no result, only memory locality.
+//
+// Data layout in HBM (physical order):
 //   X    float4 (x, y, z, mass)   positions after the last step        16 B
 //   V    float4 (vx, vy, vz, 0)   velocities after the last step       16 B
 //   dens, lam  float              density and lambda of the last step   4 B each
 //   id   uint32                   original particle index               4 B
-// Per-step scratch: X0 (old positions, new order), XsA/XsB (x* ping-pong), W (vorticity and its
-// length), Vc (velocity after vorticity confinement), the sort keys/values, the cell table, and
-// the neighbour list.  The list is wave-interleaved: entry k of slot i is at
-//   ((i / 64) * maxN + k) * 64 + i % 64
-// so the 64 lanes of a wave read entry k of their 64 lists as one 256-byte line.
+//   lid  uint32                   logical index                         4 B
+// Per-step scratch: X0 (old positions), XsA/XsB (x* ping-pong), W (vorticity and its length),
+// Vc (velocity after vorticity confinement), two sorts' keys/values, the cell table, and the
+// neighbour list.  The list holds physical indices and is wave-interleaved: entry k of slot p is
+// at ((p / 64) * maxN + k) * 64 + p % 64, so the 64 lanes of a wave read entry k of their 64
+// lists as one 256-byte line.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -27,6 +39,8 @@ struct DeviceState {
     float* dens;
     float* lam;
     uint32_t* id;
+    uint32_t* lid;             // logical index of each physical slot (persistent)
+    uint32_t* phys;            // physical slot of each logical index (persistent)
     float4* X0;
     float4* XsA;
     float4* XsB;
@@ -35,11 +49,18 @@ struct DeviceState {
     float* Tdens;
     float* Tlam;
     uint32_t* Tid;
-    uint32_t* keys;        // hashes in the pre-sort order
-    const uint32_t* skeys; // sorted hashes
-    const uint32_t* perm;  // sorted slot -> pre-sort slot
-    uint32_t* tkey;        // cell table keys (all scenes)
-    uint2* trange;         // cell table (start, end)
+    uint32_t* lidT;            // this step's logical index of each physical slot
+    uint32_t* physT;           // this step's physical slot of each logical index
+    float4* XsL;               // x* (and mass) by logical index, for the bucket walks
+    uint32_t* srcL;            // previous physical slot of each logical index
+    uint32_t* keysLay;         // layout keys at their previous logical index
+    uint32_t* keysL;           // hashes at their previous logical index (first sort's input)
+    const uint32_t* skeys;     // hashes in logical order (first sort's output)
+    const uint32_t* perm1;     // logical index -> previous logical index
+    uint32_t* keysM;           // super-cell Morton keys in logical order (second sort's input)
+    const uint32_t* perm2;     // physical slot -> logical index
+    uint32_t* tkey;            // cell table keys (all scenes)
+    uint2* trange;             // cell table (first, one past last) logical index of a bucket
     uint32_t tableTotal;
     uint32_t* nbr;
     uint32_t* cnt;
@@ -51,13 +72,15 @@ constexpr int kBlock = 256;
 void launch_fill_u32(hipStream_t st, uint32_t* p, uint32_t n, uint32_t v);
 void launch_predict_hash(hipStream_t st, const DeviceState& d, const SphParams& P,
                          const StepParams& S);
+void launch_cell_keys(hipStream_t st, const DeviceState& d, const SphParams& P,
+                      const StepParams& S);
 void launch_gather(hipStream_t st, const DeviceState& d, const SphParams& P, const StepParams& S);
 void launch_neighbours(hipStream_t st, const DeviceState& d, const SphParams& P);
 void launch_density_lambda(hipStream_t st, const DeviceState& d, const SphParams& P,
                            const float4* Xs);
 // Position correction with the wall clamp.  final == true also runs the velocity update and the
-// wall damping and writes X, V and id (the last iteration); applyDelta == false skips the
-// correction (a step with zero iterations).
+// wall damping and writes X, V, id, lid and phys (the last iteration); applyDelta == false skips
+// the correction (a step with zero iterations).
 void launch_delta_correct(hipStream_t st, const DeviceState& d, const SphParams& P,
                           const StepParams& S, const float4* XsIn, float4* XsOut, bool final,
                           bool applyDelta);
@@ -65,7 +88,8 @@ void launch_vorticity(hipStream_t st, const DeviceState& d, const SphParams& P);
 void launch_confinement(hipStream_t st, const DeviceState& d, const SphParams& P,
                         const StepParams& S);
 void launch_xsph(hipStream_t st, const DeviceState& d, const SphParams& P);
-// Exact SPH density at the positions held in XsA (sorted order), per-scene error max and sum.
+// Exact SPH density at the positions held in XsA; rho by logical index, per-scene error max and
+// sum.
 void launch_density_probe(hipStream_t st, const DeviceState& d, const SphParams& P, float* rho,
                           unsigned int* errMaxBits, double* errSum);
 

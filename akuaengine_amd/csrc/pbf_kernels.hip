@@ -2,8 +2,10 @@
 //
 // Stage map (reference kernel -> kernel here):
 //   kernel_predict_position + kernel_compute_hashes     -> k_predict_hash
-//   thrust::sort of the records                         -> radix_sort.hip + k_gather
-//   table fill + kernel_build_hash_table                -> k_gather (cell-table inserts)
+//   thrust::sort of the records                         -> radix sort by hash (logical order),
+//                                                          k_cell_keys, radix sort by super-cell
+//                                                          (physical order), k_gather
+//   table fill + kernel_build_hash_table                -> k_cell_keys (cell-table inserts)
 //   kernel_find_neighbours                              -> k_neighbours
 //   kernel_calculate_densities + kernel_calculate_lambdas   -> k_density_lambda (one sweep)
 //   kernel_calculate_position_delta + kernel_correct_position -> k_delta_correct
@@ -15,12 +17,18 @@
 // Every kernel reads the previous kernel's complete output (Jacobi), like the reference's
 // launch-then-synchronise sequence, except XSPH, which reads a copy of the velocities instead of
 // racing on them.
+//
+// Kernels that walk neighbours map workgroups to XCDs by contiguous ranges of physical slots:
+// the hardware places workgroup b on XCD b % 8, so workgroup b takes chunk (b % 8) * (nb / 8) +
+// b / 8 and each XCD's L2 serves one spatially coherent eighth of the particles.
 #include "pbf_kernels.h"
 #include "pbf_math.h"
 
 namespace akua {
 
 namespace {
+
+constexpr uint32_t kXcds = 8;
 
 __device__ __forceinline__ const SceneDev& scene_of(const DeviceState& d, uint32_t i) {
     return d.scenes[d.chunkScene[i >> 8]];
@@ -30,23 +38,92 @@ __device__ __forceinline__ size_t list_base(uint32_t i, int maxN) {
     return (size_t)(i >> 6) * (size_t)maxN * 64u + (i & 63u);
 }
 
+// Visits the n entries of a neighbour list in order.  Memory requests are issued kListBatch at a
+// time (list entries, then the neighbours' data through load(j)), then body(value) runs for each
+// entry in list order, so every per-particle sum keeps its reference order.
+constexpr int kListBatch = 8;
+
+template <class T, class Load, class Body>
+__device__ __forceinline__ void walk_list(const uint32_t* __restrict__ lp, uint32_t n, Load load,
+                                          Body body) {
+    for (uint32_t k0 = 0; k0 < n; k0 += kListBatch) {
+        uint32_t j[kListBatch];
+#pragma unroll
+        for (int u = 0; u < kListBatch; ++u) j[u] = lp[(size_t)min(k0 + u, n - 1u) * 64u];
+        T v[kListBatch];
+#pragma unroll
+        for (int u = 0; u < kListBatch; ++u) v[u] = load(j[u]);
+#pragma unroll
+        for (int u = 0; u < kListBatch; ++u)
+            if (k0 + u < n) body(v[u]);
+    }
+}
+
+struct XLam {
+    float4 x;
+    float lam;
+};
+struct XV {
+    float4 x;
+    float4 v;
+};
+struct XW {
+    float4 x;
+    float w;
+};
+struct XVRho {
+    float4 x;
+    float4 v;
+    float rho;
+};
+
+// Physical slot of this thread under the XCD-contiguous workgroup order (the grid is a multiple
+// of 8 workgroups).
+__device__ __forceinline__ uint32_t xcd_slot() {
+    const uint32_t nb = gridDim.x, b = blockIdx.x;
+    return ((b % kXcds) * (nb / kXcds) + b / kXcds) * kBlock + threadIdx.x;
+}
+
 __global__ void k_fill_u32(uint32_t* __restrict__ p, uint32_t n, uint32_t v) {
     uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) p[i] = v;
 }
 
+__device__ __forceinline__ uint32_t spread3(uint32_t x) {
+    x &= 0x3FFu;
+    x = (x | (x << 16)) & 0xFF0000FFu;
+    x = (x | (x << 8)) & 0x0300F00Fu;
+    x = (x | (x << 4)) & 0x030C30C3u;
+    x = (x | (x << 2)) & 0x09249249u;
+    return x;
+}
+
+// Morton code of the particle's 2x2x2 super-cell, relative to the scene's layout origin and
+// clamped to the key range.  It only decides where the particle's data is stored.
+__device__ __forceinline__ uint32_t layout_key(int cx, int cy, int cz, const SceneDev& sc) {
+    const int hi = (1 << sc.layoutBits) - 1;
+    int u = min(max((cx >> 1) - sc.layoutOrigin[0], 0), hi);
+    int v = min(max((cy >> 1) - sc.layoutOrigin[1], 0), hi);
+    int w = min(max((cz >> 1) - sc.layoutOrigin[2], 0), hi);
+    return spread3((uint32_t)u) | (spread3((uint32_t)v) << 1) | (spread3((uint32_t)w) << 2);
+}
+
 // kernel_predict_position (IntegrationCUDA.cu:27-36) + kernel_compute_hashes
 // (NeighbourSearchCUDA.cu:36-44) with smoothRadius as the cell size (NeighbourSearchCUDA.cu:163).
+// The hash and the layout key are stored at the particle's previous logical index, the first
+// sort's input order.
 __global__ __launch_bounds__(kBlock) void k_predict_hash(DeviceState d, SphParams P, StepParams S) {
-    uint32_t i = blockIdx.x * kBlock + threadIdx.x;
-    if (i >= d.nslots) return;
-    const SceneDev& sc = scene_of(d, i);
-    if (i >= sc.end) return;
-    float4 x = d.X[i], v = d.V[i];
+    uint32_t p = blockIdx.x * kBlock + threadIdx.x;
+    if (p >= d.nslots) return;
+    const SceneDev& sc = scene_of(d, p);
+    if (p >= sc.end) return;
+    float4 x = d.X[p], v = d.V[p];
     f3 xs = predict_x(make3(x), make3(v), S);
     int cx, cy, cz;
     discretize(xs, P.cellHash, cx, cy, cz);
-    d.keys[i] = cell_hash(cx, cy, cz, sc.tableSize);
+    const uint32_t k = d.lid[p];
+    d.keysL[k] = cell_hash(cx, cy, cz, sc);
+    d.keysLay[k] = layout_key(cx, cy, cz, sc);
 }
 
 __device__ __forceinline__ uint32_t table_insert(uint32_t* tkey, const SceneDev& sc, uint32_t h) {
@@ -62,28 +139,21 @@ __device__ __forceinline__ uint32_t table_insert(uint32_t* tkey, const SceneDev&
     return kEmptyKey;
 }
 
-// Moves each particle to its sorted slot (the record move of the reference's thrust::sort) and
-// registers every bucket's [start, end) in the scene's cell table (kernel_build_hash_table,
+// Per logical index k: the layout key (second sort's input), the previous physical slot of the
+// particle, and the cell-table entry of k's bucket (kernel_build_hash_table,
 // NeighbourSearchCUDA.cu:52-65, plus the bucket end the reference finds by scanning).
-// x* is recomputed from the gathered x and v with the same operations as k_predict_hash, so it is
-// bit-identical to the value that was hashed.
-__global__ __launch_bounds__(kBlock) void k_gather(DeviceState d, SphParams P, StepParams S) {
+__global__ __launch_bounds__(kBlock) void k_cell_keys(DeviceState d, SphParams P, StepParams S) {
     uint32_t k = blockIdx.x * kBlock + threadIdx.x;
     if (k >= d.nslots) return;
     const SceneDev& sc = scene_of(d, k);
     if (k >= sc.end) return;
-    uint32_t src = d.perm[k];
-    if (src >= d.nslots) return;  // a consistent sort never produces this
-    float4 x = d.X[src], v = d.V[src];
-    f3 xs = predict_x(make3(x), make3(v), S);
-    d.X0[k] = x;
-    d.XsA[k] = make_float4(xs.x, xs.y, xs.z, x.w);
-    d.Tdens[k] = d.dens[src];
-    d.Tlam[k] = d.lam[src];
-    d.Tid[k] = d.id[src];
-    uint32_t h = d.skeys[k];
-    bool first = (k == sc.begin) || (d.skeys[k - 1] != h);
-    bool last = (k + 1 == sc.end) || (d.skeys[k + 1] != h);
+    const uint32_t prevLogical = d.perm1[k];
+    if (prevLogical >= d.nslots) return;  // a consistent sort never produces this
+    d.keysM[k] = d.keysLay[prevLogical];
+    d.srcL[k] = d.phys[prevLogical];
+    const uint32_t h = d.skeys[k];
+    const bool first = (k == sc.begin) || (d.skeys[k - 1] != h);
+    const bool last = (k + 1 == sc.end) || (d.skeys[k + 1] != h);
     if (first || last) {
         uint32_t slot = table_insert(d.tkey, sc, h);
         if (slot != kEmptyKey) {
@@ -91,6 +161,31 @@ __global__ __launch_bounds__(kBlock) void k_gather(DeviceState d, SphParams P, S
             if (last) d.trange[slot].y = k + 1;
         }
     }
+}
+
+// Moves each particle to its new physical slot (the record move of the reference's thrust::sort).
+// x* is recomputed from the gathered x and v with the same operations as k_predict_hash, so it is
+// bit-identical to the value that was hashed.
+__global__ __launch_bounds__(kBlock) void k_gather(DeviceState d, SphParams P, StepParams S) {
+    uint32_t p = blockIdx.x * kBlock + threadIdx.x;
+    if (p >= d.nslots) return;
+    const SceneDev& sc = scene_of(d, p);
+    if (p >= sc.end) return;
+    const uint32_t k = d.perm2[p];
+    if (k >= d.nslots) return;  // consistent sorts never produce these
+    const uint32_t src = d.srcL[k];
+    if (src >= d.nslots) return;
+    float4 x = d.X[src], v = d.V[src];
+    f3 xs = predict_x(make3(x), make3(v), S);
+    const float4 xs4 = make_float4(xs.x, xs.y, xs.z, x.w);
+    d.X0[p] = x;
+    d.XsA[p] = xs4;
+    d.XsL[k] = xs4;
+    d.Tdens[p] = d.dens[src];
+    d.Tlam[p] = d.lam[src];
+    d.Tid[p] = d.id[src];
+    d.lidT[p] = k;
+    d.physT[k] = p;
 }
 
 __device__ __forceinline__ bool table_find(const uint32_t* __restrict__ tkey,
@@ -114,40 +209,78 @@ __device__ __forceinline__ bool table_find(const uint32_t* __restrict__ tkey,
 }
 
 // kernel_find_neighbours (NeighbourSearchCUDA.cu:72-130): stencil cells in (dx, dy, dz) order,
-// each bucket walked in sorted order, self skipped, r^2 < h^2 kept, at most maxNeighbours.  A
+// each bucket walked in logical order, self skipped, r^2 < h^2 kept, at most maxNeighbours.  A
 // bucket reached from two stencil cells with equal hashes is walked twice, as in the reference.
+// The list stores physical slots.
+//
+// The walk is latency-bound, so memory requests are issued in batches: the nine table probes of
+// one dx slab together, then each bucket's candidates eight at a time (position and physical
+// slot), after which the candidates are tested and appended strictly in order.
+constexpr int kWalkBatch = 8;
+
 __global__ __launch_bounds__(kBlock) void k_neighbours(DeviceState d, SphParams P) {
-    uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+    const uint32_t p = xcd_slot();
     uint32_t count = 0;
-    if (i < d.nslots) {
-        const SceneDev& sc = scene_of(d, i);
-        if (i < sc.end) {
-            const float4 xi4 = d.XsA[i];
-            const f3 xi = make3(xi4);
+    if (p < d.nslots) {
+        const SceneDev& sc = scene_of(d, p);
+        if (p < sc.end) {
+            const uint32_t self = d.lidT[p];
+            const f3 xi = make3(d.XsA[p]);
             int cx, cy, cz;
             discretize(xi, P.cellSearch, cx, cy, cz);
-            uint32_t* out = d.nbr + list_base(i, P.maxN);
+            uint32_t* out = d.nbr + list_base(p, P.maxN);
             const uint32_t maxN = (uint32_t)P.maxN;
-            for (int dx = -1; dx <= 1; ++dx)
-                for (int dy = -1; dy <= 1; ++dy)
-                    for (int dz = -1; dz <= 1; ++dz) {
-                        if (count >= maxN) continue;
-                        uint32_t h = cell_hash((int)((uint32_t)cx + (uint32_t)dx),
-                                               (int)((uint32_t)cy + (uint32_t)dy),
-                                               (int)((uint32_t)cz + (uint32_t)dz), sc.tableSize);
-                        uint2 r;
-                        if (!table_find(d.tkey, d.trange, sc, h, r)) continue;
-                        for (uint32_t k = r.x; k < r.y && count < maxN; ++k) {
-                            if (k == i) continue;
-                            f3 sep = xi - make3(d.XsA[k]);
-                            if (dot3(sep, sep) < P.h2) {
-                                out[(size_t)count * 64u] = k;
-                                ++count;
+            for (int dx = -1; dx <= 1 && count < maxN; ++dx) {
+                uint32_t hq[9], sl[9], kq[9], ra[9], rb[9];
+#pragma unroll
+                for (int q = 0; q < 9; ++q) {
+                    hq[q] = cell_hash((int)((uint32_t)cx + (uint32_t)dx),
+                                      (int)((uint32_t)cy + (uint32_t)(q / 3 - 1)),
+                                      (int)((uint32_t)cz + (uint32_t)(q % 3 - 1)), sc);
+                    sl[q] = sc.tabOffset + table_slot(hq[q], sc.tabLog2);
+                }
+#pragma unroll
+                for (int q = 0; q < 9; ++q) kq[q] = d.tkey[sl[q]];
+#pragma unroll
+                for (int q = 0; q < 9; ++q) {
+                    uint2 r = make_uint2(0u, 0u);
+                    if (kq[q] == hq[q]) {
+                        r = d.trange[sl[q]];
+                        r.x = max(r.x, sc.begin);
+                        r.y = min(r.y, sc.end);
+                    } else if (kq[q] != kEmptyKey) {
+                        if (!table_find(d.tkey, d.trange, sc, hq[q], r)) r = make_uint2(0u, 0u);
+                    }
+                    ra[q] = r.x;
+                    rb[q] = r.y;
+                }
+#pragma unroll
+                for (int q = 0; q < 9; ++q) {
+                    for (uint32_t m0 = ra[q]; m0 < rb[q] && count < maxN; m0 += kWalkBatch) {
+                        float4 c[kWalkBatch];
+                        uint32_t ph[kWalkBatch];
+#pragma unroll
+                        for (int u = 0; u < kWalkBatch; ++u) {
+                            const uint32_t m = min(m0 + u, rb[q] - 1u);
+                            c[u] = d.XsL[m];
+                            ph[u] = d.physT[m];
+                        }
+#pragma unroll
+                        for (int u = 0; u < kWalkBatch; ++u) {
+                            const uint32_t m = m0 + u;
+                            if (m < rb[q] && m != self && count < maxN) {
+                                f3 sep = xi - make3(c[u]);
+                                if (dot3(sep, sep) < P.h2) {
+                                    out[(size_t)count * 64u] = ph[u];
+                                    ++count;
+                                }
                             }
                         }
                     }
+                }
+            }
         }
-        d.cnt[i] = count;
+        d.cnt[p] = count;
     }
     // neighbour statistics: one atomic per wave
     unsigned long long c = count;
@@ -160,7 +293,7 @@ __global__ __launch_bounds__(kBlock) void k_neighbours(DeviceState d, SphParams 
 // (the list order), so fusing the loops changes no rounding.
 __global__ __launch_bounds__(kBlock) void k_density_lambda(DeviceState d, SphParams P,
                                                            const float4* __restrict__ Xs) {
-    uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+    const uint32_t i = xcd_slot();
     if (i >= d.nslots) return;
     if (i >= scene_of(d, i).end) return;
     const float4 xi4 = Xs[i];
@@ -171,17 +304,17 @@ __global__ __launch_bounds__(kBlock) void k_density_lambda(DeviceState d, SphPar
     f3 gi{0.0f, 0.0f, 0.0f};
     float sum = 0.0f;
     const float negInv = -P.invRest;
-    for (uint32_t k = 0; k < n; ++k) {
-        uint32_t j = lp[(size_t)k * 64u];
-        float4 xj4 = Xs[j];
-        f3 sep = xi - make3(xj4);
-        float d2 = dot3(sep, sep);
-        rho = rho + xj4.w * poly6(d2, P);
-        f3 g = grad_spiky(sep, P);
-        gi = gi + xj4.w * g;
-        f3 gj = (negInv * xj4.w) * g;
-        sum = sum + dot3(gj, gj);
-    }
+    walk_list<float4>(
+        lp, n, [&](uint32_t j) { return Xs[j]; },
+        [&](const float4& xj4) {
+            f3 sep = xi - make3(xj4);
+            float d2 = dot3(sep, sep);
+            rho = rho + xj4.w * poly6(d2, P);
+            f3 g = grad_spiky(sep, P);
+            gi = gi + xj4.w * g;
+            f3 gj = (negInv * xj4.w) * g;
+            sum = sum + dot3(gj, gj);
+        });
     gi = gi * P.invRest;
     float C = rho * P.invRest - 1.0f;
     d.dens[i] = rho;
@@ -218,13 +351,13 @@ __device__ __forceinline__ f3 damp_walls(f3 x, f3 v, const SceneDev& sc) {
 // with handle_particle_collision (ConstraintSolverCUDA.cu:136-169).  x* is double-buffered: the
 // kernel reads XsIn and writes XsOut, so every particle sees the positions of the same iteration.
 // kFinal: also kernel_update_position_and_velocity (IntegrationCUDA.cu:38-49) and the wall
-// damping, writing the step's positions X, velocities V and the sorted ids.
+// damping, writing the step's positions X, velocities V, ids and the two order maps.
 template <bool kFinal, bool kApplyDelta>
 __global__ __launch_bounds__(kBlock) void k_delta_correct(DeviceState d, SphParams P,
                                                           StepParams S,
                                                           const float4* __restrict__ XsIn,
                                                           float4* __restrict__ XsOut) {
-    uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+    const uint32_t i = xcd_slot();
     if (i >= d.nslots) return;
     const SceneDev& sc = scene_of(d, i);
     if (i >= sc.end) return;
@@ -236,16 +369,15 @@ __global__ __launch_bounds__(kBlock) void k_delta_correct(DeviceState d, SphPara
         const uint32_t n = d.cnt[i];
         const uint32_t* lp = d.nbr + list_base(i, P.maxN);
         f3 dp{0.0f, 0.0f, 0.0f};
-        for (uint32_t k = 0; k < n; ++k) {
-            uint32_t j = lp[(size_t)k * 64u];
-            float4 xj4 = XsIn[j];
-            float lj = d.lam[j];
-            f3 sep = xi - make3(xj4);
-            float d2 = dot3(sep, sep);
-            float lc = P.negCorrK * corr_pow(poly6(d2, P) / P.wdq, P);
-            f3 g = grad_spiky(sep, P);
-            dp = dp + (((li + lj) + lc) * xj4.w) * g;
-        }
+        walk_list<XLam>(
+            lp, n, [&](uint32_t j) { return XLam{XsIn[j], d.lam[j]}; },
+            [&](const XLam& nb) {
+                f3 sep = xi - make3(nb.x);
+                float d2 = dot3(sep, sep);
+                float lc = P.negCorrK * corr_pow(poly6(d2, P) / P.wdq, P);
+                f3 g = grad_spiky(sep, P);
+                dp = dp + (((li + nb.lam) + lc) * nb.x.w) * g;
+            });
         dp = dp * P.invRest;
         q = xi + dp;
         const float minDist = 0.025f, stiff = 0.5f;
@@ -268,6 +400,9 @@ __global__ __launch_bounds__(kBlock) void k_delta_correct(DeviceState d, SphPara
     d.X[i] = make_float4(q.x, q.y, q.z, xi4.w);
     d.V[i] = make_float4(v.x, v.y, v.z, 0.0f);
     d.id[i] = d.Tid[i];
+    const uint32_t k = d.lidT[i];
+    d.lid[i] = k;
+    d.phys[k] = i;
     if (!kApplyDelta) {  // no iteration ran: density and lambda carry over
         d.dens[i] = d.Tdens[i];
         d.lam[i] = d.Tlam[i];
@@ -276,7 +411,7 @@ __global__ __launch_bounds__(kBlock) void k_delta_correct(DeviceState d, SphPara
 
 // kernel_compute_vorticities (IntegrationCUDA.cu:104-128).  W.w keeps |omega| for the next pass.
 __global__ __launch_bounds__(kBlock) void k_vorticity(DeviceState d, SphParams P) {
-    uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+    const uint32_t i = xcd_slot();
     if (i >= d.nslots) return;
     if (i >= scene_of(d, i).end) return;
     const f3 xi = make3(d.X[i]);
@@ -284,19 +419,19 @@ __global__ __launch_bounds__(kBlock) void k_vorticity(DeviceState d, SphParams P
     const uint32_t n = d.cnt[i];
     const uint32_t* lp = d.nbr + list_base(i, P.maxN);
     f3 w{0.0f, 0.0f, 0.0f};
-    for (uint32_t k = 0; k < n; ++k) {
-        uint32_t j = lp[(size_t)k * 64u];
-        float4 xj4 = d.X[j];
-        f3 vij = make3(d.V[j]) - vi;
-        f3 sep = xi - make3(xj4);
-        w = w + (-xj4.w) * cross3(vij, grad_spiky(sep, P));
-    }
+    walk_list<XV>(
+        lp, n, [&](uint32_t j) { return XV{d.X[j], d.V[j]}; },
+        [&](const XV& nb) {
+            f3 vij = make3(nb.v) - vi;
+            f3 sep = xi - make3(nb.x);
+            w = w + (-nb.x.w) * cross3(vij, grad_spiky(sep, P));
+        });
     d.W[i] = make_float4(w.x, w.y, w.z, length3(w));
 }
 
 // kernel_apply_vorticity_confinement (IntegrationCUDA.cu:130-165), writing Vc.
 __global__ __launch_bounds__(kBlock) void k_confinement(DeviceState d, SphParams P, StepParams S) {
-    uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+    const uint32_t i = xcd_slot();
     if (i >= d.nslots) return;
     if (i >= scene_of(d, i).end) return;
     const f3 xi = make3(d.X[i]);
@@ -305,13 +440,13 @@ __global__ __launch_bounds__(kBlock) void k_confinement(DeviceState d, SphParams
     const uint32_t n = d.cnt[i];
     const uint32_t* lp = d.nbr + list_base(i, P.maxN);
     f3 eta{0.0f, 0.0f, 0.0f};
-    for (uint32_t k = 0; k < n; ++k) {
-        uint32_t j = lp[(size_t)k * 64u];
-        float4 xj4 = d.X[j];
-        float diff = wi.w - d.W[j].w;
-        f3 sep = xi - make3(xj4);
-        eta = eta + (xj4.w * diff) * grad_spiky(sep, P);
-    }
+    walk_list<XW>(
+        lp, n, [&](uint32_t j) { return XW{d.X[j], d.W[j].w}; },
+        [&](const XW& nb) {
+            float diff = wi.w - nb.w;
+            f3 sep = xi - make3(nb.x);
+            eta = eta + (nb.x.w * diff) * grad_spiky(sep, P);
+        });
     eta = eta * invD;
     float el = length3(eta);
     float4 v = d.V[i];
@@ -326,7 +461,7 @@ __global__ __launch_bounds__(kBlock) void k_confinement(DeviceState d, SphParams
 
 // kernel_apply_xsph_viscosity (IntegrationCUDA.cu:167-195), Jacobi: reads Vc, writes V.
 __global__ __launch_bounds__(kBlock) void k_xsph(DeviceState d, SphParams P) {
-    uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+    const uint32_t i = xcd_slot();
     if (i >= d.nslots) return;
     if (i >= scene_of(d, i).end) return;
     const f3 xi = make3(d.X[i]);
@@ -334,14 +469,14 @@ __global__ __launch_bounds__(kBlock) void k_xsph(DeviceState d, SphParams P) {
     const uint32_t n = d.cnt[i];
     const uint32_t* lp = d.nbr + list_base(i, P.maxN);
     f3 dv{0.0f, 0.0f, 0.0f};
-    for (uint32_t k = 0; k < n; ++k) {
-        uint32_t j = lp[(size_t)k * 64u];
-        float4 xj4 = d.X[j];
-        f3 vij = make3(d.Vc[j]) - vi;
-        f3 sep = xi - make3(xj4);
-        float w = poly6(dot3(sep, sep), P);
-        dv = dv + ((xj4.w / d.dens[j]) * vij) * w;
-    }
+    walk_list<XVRho>(
+        lp, n, [&](uint32_t j) { return XVRho{d.X[j], d.Vc[j], d.dens[j]}; },
+        [&](const XVRho& nb) {
+            f3 vij = make3(nb.v) - vi;
+            f3 sep = xi - make3(nb.x);
+            float w = poly6(dot3(sep, sep), P);
+            dv = dv + ((nb.x.w / nb.rho) * vij) * w;
+        });
     f3 vn = vi + P.visc * dv;
     d.V[i] = make_float4(vn.x, vn.y, vn.z, 0.0f);
 }
@@ -354,7 +489,7 @@ __global__ __launch_bounds__(kBlock) void k_density_probe(DeviceState d, SphPara
                                                           float* __restrict__ rhoOut,
                                                           unsigned int* __restrict__ errMaxBits,
                                                           double* __restrict__ errSum) {
-    uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+    const uint32_t i = xcd_slot();
     float err = 0.0f;
     uint32_t scene = 0;
     bool active = false;
@@ -363,6 +498,7 @@ __global__ __launch_bounds__(kBlock) void k_density_probe(DeviceState d, SphPara
         const SceneDev& sc = d.scenes[scene];
         if (i < sc.end) {
             active = true;
+            const uint32_t self = d.lidT[i];
             const float4 xi4 = d.XsA[i];
             const f3 xi = make3(xi4);
             int cx, cy, cz;
@@ -373,23 +509,23 @@ __global__ __launch_bounds__(kBlock) void k_density_probe(DeviceState d, SphPara
             for (int q = 0; q < 27; ++q)
                 hs[q] = cell_hash((int)((uint32_t)cx + (uint32_t)(q / 9 - 1)),
                                   (int)((uint32_t)cy + (uint32_t)((q / 3) % 3 - 1)),
-                                  (int)((uint32_t)cz + (uint32_t)(q % 3 - 1)), sc.tableSize);
+                                  (int)((uint32_t)cz + (uint32_t)(q % 3 - 1)), sc);
 #pragma unroll
             for (int q = 0; q < 27; ++q) {
                 bool dup = false;
 #pragma unroll
-                for (int p = 0; p < q; ++p) dup |= hs[p] == hs[q];
-                uint2 r;
-                if (dup || !table_find(d.tkey, d.trange, sc, hs[q], r)) continue;
-                for (uint32_t k = r.x; k < r.y; ++k) {
-                    if (k == i) continue;
-                    float4 xk = d.XsA[k];
+                for (int r = 0; r < q; ++r) dup |= hs[r] == hs[q];
+                uint2 range;
+                if (dup || !table_find(d.tkey, d.trange, sc, hs[q], range)) continue;
+                for (uint32_t m = range.x; m < range.y; ++m) {
+                    if (m == self) continue;
+                    float4 xk = d.XsL[m];
                     f3 sep = xi - make3(xk);
                     float d2 = dot3(sep, sep);
                     if (d2 < P.h2) rho = rho + xk.w * poly6(d2, P);
                 }
             }
-            if (rhoOut) rhoOut[i] = rho;
+            if (rhoOut) rhoOut[self] = rho;
             err = fmaxf(0.0f, rho * P.invRest - 1.0f);
         }
     }
@@ -409,6 +545,8 @@ __global__ __launch_bounds__(kBlock) void k_density_probe(DeviceState d, SphPara
 }
 
 inline unsigned grid(uint32_t n) { return (n + kBlock - 1) / kBlock; }
+// Neighbour kernels: a whole number of workgroups per XCD.
+inline unsigned grid_xcd(uint32_t n) { return (grid(n) + kXcds - 1) / kXcds * kXcds; }
 
 }  // namespace
 
@@ -419,39 +557,44 @@ void launch_predict_hash(hipStream_t st, const DeviceState& d, const SphParams& 
                          const StepParams& S) {
     k_predict_hash<<<grid(d.nslots), kBlock, 0, st>>>(d, P, S);
 }
+void launch_cell_keys(hipStream_t st, const DeviceState& d, const SphParams& P,
+                      const StepParams& S) {
+    k_cell_keys<<<grid(d.nslots), kBlock, 0, st>>>(d, P, S);
+}
 void launch_gather(hipStream_t st, const DeviceState& d, const SphParams& P, const StepParams& S) {
     k_gather<<<grid(d.nslots), kBlock, 0, st>>>(d, P, S);
 }
 void launch_neighbours(hipStream_t st, const DeviceState& d, const SphParams& P) {
-    k_neighbours<<<grid(d.nslots), kBlock, 0, st>>>(d, P);
+    k_neighbours<<<grid_xcd(d.nslots), kBlock, 0, st>>>(d, P);
 }
 void launch_density_lambda(hipStream_t st, const DeviceState& d, const SphParams& P,
                            const float4* Xs) {
-    k_density_lambda<<<grid(d.nslots), kBlock, 0, st>>>(d, P, Xs);
+    k_density_lambda<<<grid_xcd(d.nslots), kBlock, 0, st>>>(d, P, Xs);
 }
 void launch_delta_correct(hipStream_t st, const DeviceState& d, const SphParams& P,
                           const StepParams& S, const float4* XsIn, float4* XsOut, bool final,
                           bool applyDelta) {
+    const unsigned g = grid_xcd(d.nslots);
     if (final && applyDelta)
-        k_delta_correct<true, true><<<grid(d.nslots), kBlock, 0, st>>>(d, P, S, XsIn, XsOut);
+        k_delta_correct<true, true><<<g, kBlock, 0, st>>>(d, P, S, XsIn, XsOut);
     else if (final)
-        k_delta_correct<true, false><<<grid(d.nslots), kBlock, 0, st>>>(d, P, S, XsIn, XsOut);
+        k_delta_correct<true, false><<<g, kBlock, 0, st>>>(d, P, S, XsIn, XsOut);
     else
-        k_delta_correct<false, true><<<grid(d.nslots), kBlock, 0, st>>>(d, P, S, XsIn, XsOut);
+        k_delta_correct<false, true><<<g, kBlock, 0, st>>>(d, P, S, XsIn, XsOut);
 }
 void launch_vorticity(hipStream_t st, const DeviceState& d, const SphParams& P) {
-    k_vorticity<<<grid(d.nslots), kBlock, 0, st>>>(d, P);
+    k_vorticity<<<grid_xcd(d.nslots), kBlock, 0, st>>>(d, P);
 }
 void launch_confinement(hipStream_t st, const DeviceState& d, const SphParams& P,
                         const StepParams& S) {
-    k_confinement<<<grid(d.nslots), kBlock, 0, st>>>(d, P, S);
+    k_confinement<<<grid_xcd(d.nslots), kBlock, 0, st>>>(d, P, S);
 }
 void launch_xsph(hipStream_t st, const DeviceState& d, const SphParams& P) {
-    k_xsph<<<grid(d.nslots), kBlock, 0, st>>>(d, P);
+    k_xsph<<<grid_xcd(d.nslots), kBlock, 0, st>>>(d, P);
 }
 void launch_density_probe(hipStream_t st, const DeviceState& d, const SphParams& P, float* rho,
                           unsigned int* errMaxBits, double* errSum) {
-    k_density_probe<<<grid(d.nslots), kBlock, 0, st>>>(d, P, rho, errMaxBits, errSum);
+    k_density_probe<<<grid_xcd(d.nslots), kBlock, 0, st>>>(d, P, rho, errMaxBits, errSum);
 }
 
 }  // namespace akua

@@ -78,12 +78,13 @@ __device__ __forceinline__ void discretize(f3 p, float cs, int& cx, int& cy, int
     cz = to_int_sat(floorf(p.z / cs));
 }
 
-// device_get_hash (NeighbourSearchCUDA.cu:23-27), with the int multiply's wrap done in uint32.
-__device__ __forceinline__ uint32_t cell_hash(int cx, int cy, int cz, uint32_t tableSize) {
+// device_get_hash (NeighbourSearchCUDA.cu:23-27), with the int multiply's wrap done in uint32
+// and the modulo by the scene's table size done by multiply-high (exact for every input).
+__device__ __forceinline__ uint32_t cell_hash(int cx, int cy, int cz, const SceneDev& sc) {
     uint32_t a = (uint32_t)cx * 73856093u;
     uint32_t b = (uint32_t)cy * 19349663u;
     uint32_t c = (uint32_t)cz * 83492791u;
-    return (a ^ b ^ c) % tableSize;
+    return fast_mod(a ^ b ^ c, sc.tableSize, sc.modMagic, sc.modShift);
 }
 
 // kernel_predict_position (IntegrationCUDA.cu:27-36): v* = v + g dt; x* = x + v* dt.

@@ -1,6 +1,8 @@
 // pbf_types.h -- structures shared by the host solver and the gfx950 kernels.
 #pragma once
 
+#include <hip/hip_runtime.h>
+
 #include <cstdint>
 
 namespace akua {
@@ -14,7 +16,10 @@ struct SceneDev {
     uint32_t tableSize;    // maxNeighbours * N_s: the spatial-hash modulus (PBFSolver.cpp:15)
     uint32_t tabOffset;    // first slot of this scene's cell table
     uint32_t tabLog2;      // log2 of this scene's cell-table capacity
-    uint32_t pad[3];
+    uint32_t layoutBits;   // bits per axis of the super-cell Morton key (memory layout only)
+    uint32_t modMagic;     // x % tableSize by multiply-high (fast_mod below)
+    uint32_t modShift;
+    int32_t layoutOrigin[4];  // super-cell coordinates mapped to Morton coordinate 0
     float bmin[4];         // box (xyz) for the wall clamp and the velocity damping
     float bmax[4];
 };
@@ -46,6 +51,32 @@ struct StepParams {
     float dt;
     float g[3];
 };
+
+// x % d for 32-bit unsigned x and a divisor d fixed per scene, by the round-up multiply-high
+// method (Granlund & Montgomery 1994; Hacker's Delight, 2nd ed., 10-8): with l = ceil(log2 d),
+// magic = floor(2^32 (2^l - d) / d) + 1, t = mulhi(magic, x), the quotient is
+// (t + ((x - t) >> 1)) >> (l - 1) for every x (d >= 2); d == 1 uses shift 0 and magic 0 as a flag.
+struct FastMod {
+    uint32_t d, magic, shift;
+};
+
+inline FastMod make_fast_mod(uint32_t d) {
+    FastMod f{d, 0u, 0u};
+    if (d <= 1) return f;
+    uint32_t l = 0;
+    while ((1ull << l) < d) ++l;
+    f.magic = (uint32_t)((((1ull << l) - d) << 32) / d + 1);
+    f.shift = l - 1;
+    return f;
+}
+
+__host__ __device__ inline uint32_t fast_mod(uint32_t x, uint32_t d, uint32_t magic,
+                                             uint32_t shift) {
+    if (d <= 1) return 0u;
+    const uint32_t t = (uint32_t)(((uint64_t)magic * x) >> 32);
+    const uint32_t q = (t + ((x - t) >> 1)) >> shift;
+    return x - q * d;
+}
 
 constexpr uint32_t kSlotAlign = 256;
 constexpr uint32_t kEmptyKey = 0xFFFFFFFFu;  // never a hash: hashes are < tableSize <= 2^32 - 1

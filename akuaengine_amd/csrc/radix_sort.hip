@@ -1,13 +1,17 @@
 // radix_sort.hip -- stable LSD radix sort of (key, slot) pairs for gfx950.
 //
-// Per pass (digit of D bits, R = 2^D buckets), three kernels:
-//   rs_hist    : per tile, the digit histogram (wave64 match-by-ballot, one LDS add per group).
-//   rs_scan    : one workgroup turns the scene-major [scene][digit][tile] counts into exclusive
-//                offsets; the scan order makes every scene's keys land inside its own segment.
+// Per sort, one fill of the per-(pass, scene, digit) totals; then per pass (digit of D bits,
+// R = 2^D buckets) three kernels:
+//   rs_hist    : per tile, the digit histogram (wave64 match-by-ballot, one LDS add per group),
+//                stored as [scene][digit][tile] counts and added to the scene's digit totals.
+//   rs_scan    : one wave per (scene, digit): the digit's base slot (scene begin + the totals of
+//                the smaller digits) plus the exclusive scan of its tile counts.
 //   rs_scatter : per tile, stable ranks by wave64 ballots (wave w owns a contiguous quarter of
 //                the tile, in 16 rounds of 64 consecutive keys), a tile-local counting sort in
 //                LDS, then coalesced runs written to the global offsets.
 #include "radix_sort.h"
+
+#include <algorithm>
 
 namespace akua {
 
@@ -28,9 +32,8 @@ __device__ __forceinline__ uint64_t match_digit(uint32_t d, uint64_t valid) {
     return peers;
 }
 
-// Exclusive scan of one value per thread over a 256-thread block; also returns the total.
-__device__ __forceinline__ uint32_t block_exclusive_scan_256(uint32_t v, uint32_t* ldsWave,
-                                                             uint32_t& total) {
+// Exclusive scan of one value per thread over a 256-thread block.
+__device__ __forceinline__ uint32_t block_exclusive_scan_256(uint32_t v, uint32_t* ldsWave) {
     const uint32_t lane = lane_id();
     const uint32_t wave = threadIdx.x >> 6;
     uint32_t inc = v;
@@ -43,15 +46,20 @@ __device__ __forceinline__ uint32_t block_exclusive_scan_256(uint32_t v, uint32_
     __syncthreads();
     uint32_t base = 0;
     for (uint32_t w = 0; w < wave; ++w) base += ldsWave[w];
-    total = ldsWave[0] + ldsWave[1] + ldsWave[2] + ldsWave[3];
     __syncthreads();
     return base + inc - v;
+}
+
+__global__ void rs_zero(uint32_t* __restrict__ p, uint32_t n) {
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = 0u;
 }
 
 template <int D>
 __global__ __launch_bounds__(kSortBlock) void rs_hist(const uint32_t* __restrict__ keys,
                                                       const SortTile* __restrict__ tiles,
-                                                      uint32_t shift, uint32_t* __restrict__ counts) {
+                                                      uint32_t shift, uint32_t* __restrict__ counts,
+                                                      uint32_t* __restrict__ totals) {
     constexpr uint32_t R = 1u << D;
     __shared__ uint32_t hist[R];
     const SortTile t = tiles[blockIdx.x];
@@ -69,36 +77,41 @@ __global__ __launch_bounds__(kSortBlock) void rs_hist(const uint32_t* __restrict
         if (valid && (peers >> lane) == 1ull) atomicAdd(&hist[d], (uint32_t)__popcll(peers));
     }
     __syncthreads();
-    for (uint32_t d = threadIdx.x; d < R; d += kSortBlock)
-        counts[t.countBase * R + d * t.sceneTiles + t.localTile] = hist[d];
+    for (uint32_t d = threadIdx.x; d < R; d += kSortBlock) {
+        const uint32_t c = hist[d];
+        counts[t.countBase * R + d * t.sceneTiles + t.localTile] = c;
+        if (c) atomicAdd(&totals[t.scene * R + d], c);
+    }
 }
 
-// Exclusive scan of counts[0, n) in place, one 1024-thread workgroup, each thread a contiguous
-// chunk.
-__global__ __launch_bounds__(1024) void rs_scan(uint32_t* __restrict__ counts, uint32_t n) {
-    __shared__ uint32_t waveSums[16];
-    const uint32_t tid = threadIdx.x;
-    const uint32_t chunk = (n + 1023) / 1024;
-    const uint32_t b = min(n, tid * chunk), e = min(n, b + chunk);
-    uint32_t sum = 0;
-    for (uint32_t i = b; i < e; ++i) sum += counts[i];
-    // block exclusive scan of sum over 16 waves
-    const uint32_t lane = lane_id(), wave = tid >> 6;
-    uint32_t inc = sum;
+// One wave per (scene, digit): offsets[scene][digit][tile] = first output slot of the tile's keys
+// with that digit (the scene's first slot, plus the scene's keys with a smaller digit, plus the
+// keys of this digit in earlier tiles).
+template <int D>
+__global__ __launch_bounds__(64) void rs_scan(uint32_t* __restrict__ counts,
+                                              const uint32_t* __restrict__ totals,
+                                              const SortScene* __restrict__ scenes) {
+    constexpr uint32_t R = 1u << D;
+    const uint32_t s = blockIdx.x / R, d = blockIdx.x % R;
+    const SortScene sc = scenes[s];
+    const uint32_t lane = lane_id();
+    uint32_t below = 0;
+    for (uint32_t q = lane; q < d; q += 64) below += totals[s * R + q];
 #pragma unroll
-    for (int off = 1; off < 64; off <<= 1) {
-        uint32_t t = __shfl_up(inc, off, 64);
-        if (lane >= (uint32_t)off) inc += t;
-    }
-    if (lane == 63) waveSums[wave] = inc;
-    __syncthreads();
-    uint32_t base = 0;
-    for (uint32_t w = 0; w < wave; ++w) base += waveSums[w];
-    uint32_t run = base + inc - sum;
-    for (uint32_t i = b; i < e; ++i) {
-        uint32_t c = counts[i];
-        counts[i] = run;
-        run += c;
+    for (int off = 32; off > 0; off >>= 1) below += __shfl_xor(below, off, 64);
+    uint32_t carry = sc.begin + below;
+    uint32_t* c = counts + (size_t)sc.countBase * R + (size_t)d * sc.tiles;
+    for (uint32_t base = 0; base < sc.tiles; base += 64) {
+        const uint32_t t = base + lane;
+        const uint32_t v = t < sc.tiles ? c[t] : 0u;
+        uint32_t inc = v;
+#pragma unroll
+        for (int off = 1; off < 64; off <<= 1) {
+            uint32_t y = __shfl_up(inc, off, 64);
+            if (lane >= (uint32_t)off) inc += y;
+        }
+        if (t < sc.tiles) c[t] = carry + inc - v;
+        carry += __shfl(inc, 63, 64);
     }
 }
 
@@ -158,11 +171,10 @@ __global__ __launch_bounds__(kSortBlock) void rs_scatter(
                 tot += c;
             }
         }
-        uint32_t total;
-        uint32_t start = block_exclusive_scan_256(d < R ? tot : 0u, scanTmp, total);
+        uint32_t start = block_exclusive_scan_256(d < R ? tot : 0u, scanTmp);
         if (d < R) {
             tileStart[d] = start;
-            globalBase[d] = offsets[t.countBase * R + d * t.sceneTiles + t.localTile] + t.slotShift;
+            globalBase[d] = offsets[t.countBase * R + d * t.sceneTiles + t.localTile];
         }
     }
     __syncthreads();
@@ -196,14 +208,13 @@ __global__ __launch_bounds__(kSortBlock) void rs_scatter(
 template <int D>
 void launch_pass(hipStream_t st, bool first, const uint32_t* kin, const uint32_t* vin,
                  uint32_t* kout, uint32_t* vout, const SortTile* tiles, uint32_t ntiles,
-                 uint32_t shift, uint32_t countsPerDigit, uint32_t* counts, uint32_t limit,
-                 LaunchHook hook, void* ctx) {
+                 const SortScene* scenes, uint32_t nscenes, uint32_t shift, uint32_t* counts,
+                 uint32_t* totals, uint32_t limit, LaunchHook hook, void* ctx) {
     if (hook) hook(ctx, "rs_hist", 0);
-    rs_hist<D><<<ntiles, kSortBlock, 0, st>>>(kin, tiles, shift, counts);
+    rs_hist<D><<<ntiles, kSortBlock, 0, st>>>(kin, tiles, shift, counts, totals);
     if (hook) hook(ctx, "rs_hist", 1);
-    const uint32_t n = countsPerDigit * (1u << D);
     if (hook) hook(ctx, "rs_scan", 0);
-    rs_scan<<<1, 1024, 0, st>>>(counts, n);
+    rs_scan<D><<<nscenes * (1u << D), 64, 0, st>>>(counts, totals, scenes);
     if (hook) hook(ctx, "rs_scan", 1);
     if (hook) hook(ctx, "rs_scatter", 0);
     if (first)
@@ -215,54 +226,66 @@ void launch_pass(hipStream_t st, bool first, const uint32_t* kin, const uint32_t
 
 }  // namespace
 
+SortPlan SortPlan::balanced(int bits) {
+    SortPlan p;
+    p.bits = bits < 1 ? 1 : bits;
+    const int passes = (p.bits + kMaxDigitBits - 1) / kMaxDigitBits;
+    p.digitBits = std::max(4, (p.bits + passes - 1) / passes);
+    return p;
+}
+
 uint32_t build_sort_tiles(const std::vector<uint32_t>& sceneBegin,
-                          const std::vector<uint32_t>& sceneEnd, std::vector<SortTile>& tiles) {
+                          const std::vector<uint32_t>& sceneEnd, std::vector<SortTile>& tiles,
+                          std::vector<SortScene>& scenes) {
     tiles.clear();
-    // First pass: tiles per scene, to know each scene's count base.
-    std::vector<uint32_t> nt(sceneBegin.size());
-    uint32_t total = 0;
+    scenes.clear();
+    uint32_t tileBase = 0;
     for (size_t s = 0; s < sceneBegin.size(); ++s) {
-        uint32_t n = sceneEnd[s] - sceneBegin[s];
-        nt[s] = (n + kSortTile - 1) / kSortTile;
-        total += nt[s];
-    }
-    uint32_t tileBase = 0;  // tiles of all earlier scenes
-    uint32_t rankBase = 0;  // particles of all earlier scenes
-    for (size_t s = 0; s < sceneBegin.size(); ++s) {
-        for (uint32_t k = 0; k < nt[s]; ++k) {
+        const uint32_t n = sceneEnd[s] - sceneBegin[s];
+        const uint32_t nt = (n + kSortTile - 1) / kSortTile;
+        for (uint32_t k = 0; k < nt; ++k) {
             SortTile t{};
             t.begin = sceneBegin[s] + k * kSortTile;
             t.end = std::min(sceneEnd[s], t.begin + (uint32_t)kSortTile);
             t.countBase = tileBase;
-            t.sceneTiles = nt[s];
+            t.sceneTiles = nt;
             t.localTile = k;
-            t.slotShift = (int32_t)sceneBegin[s] - (int32_t)rankBase;
+            t.scene = (uint32_t)s;
             tiles.push_back(t);
         }
-        tileBase += nt[s];
-        rankBase += sceneEnd[s] - sceneBegin[s];
+        scenes.push_back(SortScene{sceneBegin[s], tileBase, nt, 0u});
+        tileBase += nt;
     }
-    return total;
+    return tileBase;
 }
 
-int radix_sort_pairs(hipStream_t stream, const SortPlan& plan, uint32_t* keysA, uint32_t* valsA,
-                     uint32_t* keysB, uint32_t* valsB, const SortTile* dTiles, uint32_t numTiles,
-                     uint32_t countsPerDigit, uint32_t* dCounts, uint32_t limit, LaunchHook hook,
-                     void* hookCtx) {
+int radix_sort_pairs(hipStream_t stream, const SortPlan& plan, const SortBuffers& buf,
+                     const SortTile* dTiles, uint32_t numTiles, const SortScene* dScenes,
+                     uint32_t numScenes, uint32_t limit, LaunchHook hook, void* hookCtx) {
     const int P = plan.passes();
-    uint32_t* kin = keysA;
-    uint32_t* vin = valsA;
-    uint32_t* kout = keysB;
-    uint32_t* vout = valsB;
+    const uint32_t R = 1u << plan.digitBits;
+    const uint32_t nTotals = (uint32_t)P * R * numScenes;
+    if (hook) hook(hookCtx, "rs_zero", 0);
+    rs_zero<<<(nTotals + 255) / 256, 256, 0, stream>>>(buf.totals, nTotals);
+    if (hook) hook(hookCtx, "rs_zero", 1);
+    uint32_t* kin = buf.keysA;
+    uint32_t* vin = buf.valsA;
+    uint32_t* kout = buf.keysB;
+    uint32_t* vout = buf.valsB;
     for (int p = 0; p < P; ++p) {
-        uint32_t shift = (uint32_t)(p * plan.digitBits);
+        const uint32_t shift = (uint32_t)(p * plan.digitBits);
+        uint32_t* tot = buf.totals + (size_t)p * R * numScenes;
+#define AKUA_PASS(DB)                                                                        \
+    launch_pass<DB>(stream, p == 0, kin, vin, kout, vout, dTiles, numTiles, dScenes, numScenes, \
+                    shift, buf.counts, tot, limit, hook, hookCtx)
         switch (plan.digitBits) {
-            case 4: launch_pass<4>(stream, p == 0, kin, vin, kout, vout, dTiles, numTiles, shift, countsPerDigit, dCounts, limit, hook, hookCtx); break;
-            case 5: launch_pass<5>(stream, p == 0, kin, vin, kout, vout, dTiles, numTiles, shift, countsPerDigit, dCounts, limit, hook, hookCtx); break;
-            case 6: launch_pass<6>(stream, p == 0, kin, vin, kout, vout, dTiles, numTiles, shift, countsPerDigit, dCounts, limit, hook, hookCtx); break;
-            case 7: launch_pass<7>(stream, p == 0, kin, vin, kout, vout, dTiles, numTiles, shift, countsPerDigit, dCounts, limit, hook, hookCtx); break;
-            default: launch_pass<8>(stream, p == 0, kin, vin, kout, vout, dTiles, numTiles, shift, countsPerDigit, dCounts, limit, hook, hookCtx); break;
+            case 4: AKUA_PASS(4); break;
+            case 5: AKUA_PASS(5); break;
+            case 6: AKUA_PASS(6); break;
+            case 7: AKUA_PASS(7); break;
+            default: AKUA_PASS(8); break;
         }
+#undef AKUA_PASS
         std::swap(kin, kout);
         std::swap(vin, vout);
     }

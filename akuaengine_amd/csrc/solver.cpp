@@ -129,31 +129,34 @@ struct akua_solver {
     uint32_t nslots = 0;
     uint32_t nparticles = 0;
     uint32_t tableTotal = 0;
-    int keyBits = 1;
-    SortPlan plan;
+    SortPlan plan1;          // hash sort (logical order)
+    SortPlan plan2;          // super-cell sort (physical layout)
     std::vector<SortTile> tilesHost;
-    uint32_t countsPerDigit = 0;
+    std::vector<SortScene> sortScenesHost;
+    uint32_t numTiles = 0;
     bool scenesDirty = true;
 
     // device buffers
     DevBuf<SceneDev> dScenes;
     DevBuf<uint32_t> dChunkScene;
-    DevBuf<float4> X, V, X0, XsA, XsB, W, Vc, color;
+    DevBuf<float4> X, V, X0, XsA, XsB, XsL, W, Vc, color;
     DevBuf<float> dens, lam, Tdens, Tlam, size;
-    DevBuf<uint32_t> id, Tid, keysA, valsA, keysB, valsB, counts, tkey, nbr, cnt;
+    DevBuf<uint32_t> id, Tid, lid, phys, lidT, physT, srcL, keysLay, tkey, nbr, cnt;
+    DevBuf<uint32_t> s1KA, s1VA, s1KB, s1VB, s2KA, s2VA, s2KB, s2VB, counts, totals;
     DevBuf<uint2> trange;
     DevBuf<SortTile> dTiles;
+    DevBuf<SortScene> dSortScenes;
     DevBuf<unsigned long long> nbrSum;
     DevBuf<float> probeRho;
     DevBuf<unsigned int> probeMax;
     DevBuf<double> probeSum;
     bool hasState = false;
-    int sortResult = 0;  // 0: (keysA, valsA); 1: (keysB, valsB)
 
     // step graph
     bool useGraph = true;
     hipGraphExec_t graphExec = nullptr;
     StepParams graphParams{};
+    int graphPlan2 = -1;
     uint64_t stepsSinceStats = 0;
 
     // timing
@@ -170,11 +173,17 @@ struct akua_solver {
         d.scenes = dScenes.p;
         d.chunkScene = dChunkScene.p;
         d.X = X.p; d.V = V.p; d.dens = dens.p; d.lam = lam.p; d.id = id.p;
+        d.lid = lid.p; d.phys = phys.p;
         d.X0 = X0.p; d.XsA = XsA.p; d.XsB = XsB.p; d.W = W.p; d.Vc = Vc.p;
         d.Tdens = Tdens.p; d.Tlam = Tlam.p; d.Tid = Tid.p;
-        d.keys = keysA.p;
-        d.skeys = sortResult == 0 ? keysA.p : keysB.p;
-        d.perm = sortResult == 0 ? valsA.p : valsB.p;
+        d.lidT = lidT.p; d.physT = physT.p;
+        d.XsL = XsL.p; d.srcL = srcL.p; d.keysLay = keysLay.p;
+        const bool r1 = plan1.passes() % 2 == 1, r2 = plan2.passes() % 2 == 1;
+        d.keysL = s1KA.p;
+        d.skeys = r1 ? s1KB.p : s1KA.p;
+        d.perm1 = r1 ? s1VB.p : s1VA.p;
+        d.keysM = s2KA.p;
+        d.perm2 = r2 ? s2VB.p : s2VA.p;
         d.tkey = tkey.p; d.trange = trange.p; d.tableTotal = tableTotal;
         d.nbr = nbr.p; d.cnt = cnt.p; d.nbrSum = nbrSum.p;
         return d;
@@ -188,6 +197,25 @@ struct akua_solver {
 };
 
 namespace {
+
+void update_layout(akua_solver* s);
+
+// The multiply-high modulo must agree with % for this divisor; checked on edge values and a
+// pseudo-random sample before any kernel uses it.
+void check_fast_mod(const FastMod& f) {
+    uint32_t xs[] = {0u, 1u, f.d - 1u, f.d, f.d + 1u, 2u * f.d - 1u, 2u * f.d, 0x7FFFFFFFu,
+                     0x80000000u, 0xFFFFFFFEu, 0xFFFFFFFFu};
+    uint64_t z = 0x9E3779B97F4A7C15ull ^ f.d;
+    for (uint32_t x : xs)
+        if (fast_mod(x, f.d, f.magic, f.shift) != x % f.d)
+            fail(AKUA_ERR_UNSUPPORTED, "fast modulo self-check failed for table size " + std::to_string(f.d));
+    for (int i = 0; i < 4096; ++i) {
+        z = z * 6364136223846793005ull + 1442695040888963407ull;
+        uint32_t x = (uint32_t)(z >> 32);
+        if (fast_mod(x, f.d, f.magic, f.shift) != x % f.d)
+            fail(AKUA_ERR_UNSUPPORTED, "fast modulo self-check failed for table size " + std::to_string(f.d));
+    }
+}
 
 void validate_config(const akua_pbf_config* cfg, const akua_lambda_corr* corr) {
     if (!cfg || !corr) fail(AKUA_ERR_INVALID_ARG, "config pointers must not be NULL");
@@ -234,6 +262,10 @@ void setup(akua_solver* s, const std::vector<uint32_t>& counts) {
         sc.begin = slot;
         sc.end = slot + (uint32_t)n;
         sc.tableSize = (uint32_t)T;
+        const FastMod fm = make_fast_mod(sc.tableSize);
+        sc.modMagic = fm.magic;
+        sc.modShift = fm.shift;
+        check_fast_mod(fm);
         uint32_t lg = std::max<uint32_t>(6u, ceil_log2(2ull * n));
         sc.tabOffset = tab;
         sc.tabLog2 = lg;
@@ -253,15 +285,9 @@ void setup(akua_solver* s, const std::vector<uint32_t>& counts) {
     s->tableTotal = tab;
     s->nparticles = 0;
     for (auto c : counts) s->nparticles += c;
-    s->keyBits = std::max<int>(1, (int)ceil_log2((uint64_t)maxTable));
-    s->plan.bits = s->keyBits;
-    s->plan.digitBits = 8;
-    // Balance the passes: the same number of passes with the smallest digit that covers the bits.
-    {
-        int P = s->plan.passes();
-        s->plan.digitBits = std::max(4, (s->keyBits + P - 1) / P);
-    }
-    s->countsPerDigit = build_sort_tiles(sb, se, s->tilesHost);
+    s->plan1 = SortPlan::balanced((int)ceil_log2((uint64_t)maxTable));
+    s->plan2 = SortPlan::balanced(3);
+    s->numTiles = build_sort_tiles(sb, se, s->tilesHost, s->sortScenesHost);
 
     const uint32_t nChunks = (s->nslots + 255) / 256;
     s->chunkSceneHost.assign(nChunks, 0);
@@ -279,13 +305,18 @@ void setup(akua_solver* s, const std::vector<uint32_t>& counts) {
     s->W.alloc(ns); s->Vc.alloc(ns); s->color.alloc(ns);
     s->dens.alloc(ns); s->lam.alloc(ns); s->Tdens.alloc(ns); s->Tlam.alloc(ns); s->size.alloc(ns);
     s->id.alloc(ns); s->Tid.alloc(ns);
-    s->keysA.alloc(ns); s->valsA.alloc(ns); s->keysB.alloc(ns); s->valsB.alloc(ns);
-    s->counts.alloc((size_t)s->countsPerDigit * 256u);
+    s->lid.alloc(ns); s->phys.alloc(ns); s->lidT.alloc(ns); s->physT.alloc(ns);
+    s->XsL.alloc(ns); s->srcL.alloc(ns); s->keysLay.alloc(ns);
+    s->s1KA.alloc(ns); s->s1VA.alloc(ns); s->s1KB.alloc(ns); s->s1VB.alloc(ns);
+    s->s2KA.alloc(ns); s->s2VA.alloc(ns); s->s2KB.alloc(ns); s->s2VB.alloc(ns);
+    s->counts.alloc((size_t)s->numTiles * (1u << kMaxDigitBits));
+    s->totals.alloc((size_t)kMaxPasses * (1u << kMaxDigitBits) * s->scenes.size());
     s->tkey.alloc(s->tableTotal);
     s->trange.alloc(s->tableTotal);
     s->nbr.alloc(nsList * (size_t)s->cfg.maxNeighbours);
     s->cnt.alloc(ns);
     s->dTiles.alloc(s->tilesHost.size());
+    s->dSortScenes.alloc(s->sortScenesHost.size());
     s->nbrSum.alloc(1);
     s->probeRho.alloc(ns);
     s->probeMax.alloc(s->scenes.size());
@@ -293,15 +324,24 @@ void setup(akua_solver* s, const std::vector<uint32_t>& counts) {
     HIP_CHECK(hipMemcpy(s->dChunkScene.p, s->chunkSceneHost.data(), nChunks * 4, hipMemcpyHostToDevice));
     HIP_CHECK(hipMemcpy(s->dTiles.p, s->tilesHost.data(), s->tilesHost.size() * sizeof(SortTile),
                         hipMemcpyHostToDevice));
+    HIP_CHECK(hipMemcpy(s->dSortScenes.p, s->sortScenesHost.data(),
+                        s->sortScenesHost.size() * sizeof(SortScene), hipMemcpyHostToDevice));
     HIP_CHECK(hipMemset(s->nbrSum.p, 0, sizeof(unsigned long long)));
     HIP_CHECK(hipMemset(s->W.p, 0, ns * sizeof(float4)));
-    HIP_CHECK(hipMemset(s->keysA.p, 0, ns * 4));
-    HIP_CHECK(hipMemset(s->keysB.p, 0, ns * 4));
+    HIP_CHECK(hipMemset(s->s1KA.p, 0, ns * 4));
+    HIP_CHECK(hipMemset(s->lidT.p, 0, ns * 4));
+    HIP_CHECK(hipMemset(s->srcL.p, 0, ns * 4));
+    HIP_CHECK(hipMemset(s->keysLay.p, 0, ns * 4));
+    HIP_CHECK(hipMemset(s->XsL.p, 0, ns * sizeof(float4)));
+    HIP_CHECK(hipMemset(s->physT.p, 0, ns * 4));
+    HIP_CHECK(hipMemset(s->lid.p, 0, ns * 4));
+    HIP_CHECK(hipMemset(s->phys.p, 0, ns * 4));
+    HIP_CHECK(hipMemset(s->s2KA.p, 0, ns * 4));
     HIP_CHECK(hipMemset(s->X.p, 0, ns * sizeof(float4)));
     HIP_CHECK(hipMemset(s->V.p, 0, ns * sizeof(float4)));
     HIP_CHECK(hipMemset(s->cnt.p, 0, ns * 4));
-    s->sortResult = s->plan.passes() % 2 == 0 ? 0 : 1;
     s->sph = make_sph(s->cfg, s->corr);
+    update_layout(s);
 }
 
 void timing_hook(void* ctx, const char* name, int phase);
@@ -359,6 +399,20 @@ void collect_timing(akua_solver* s) {
     s->eventNext = 0;
 }
 
+// The logical order: hashes stably sorted over the previous logical order.
+void sort_hash(akua_solver* s, hipStream_t st, bool timed) {
+    SortBuffers b{s->s1KA.p, s->s1VA.p, s->s1KB.p, s->s1VB.p, s->counts.p, s->totals.p};
+    radix_sort_pairs(st, s->plan1, b, s->dTiles.p, s->numTiles, s->dSortScenes.p,
+                     (uint32_t)s->scenes.size(), s->nslots, timed ? timing_hook : nullptr, s);
+}
+
+// The physical layout: super-cell Morton keys stably sorted over the new logical order.
+void sort_layout(akua_solver* s, hipStream_t st, bool timed) {
+    SortBuffers b{s->s2KA.p, s->s2VA.p, s->s2KB.p, s->s2VB.p, s->counts.p, s->totals.p};
+    radix_sort_pairs(st, s->plan2, b, s->dTiles.p, s->numTiles, s->dSortScenes.p,
+                     (uint32_t)s->scenes.size(), s->nslots, timed ? timing_hook : nullptr, s);
+}
+
 // PBFSolver::step (PBFSolver.cpp:22-78) as one stream-ordered sequence of launches.
 void enqueue_step(akua_solver* s, hipStream_t st, const StepParams& S, bool timed) {
     Launcher L{s, st, timed};
@@ -370,9 +424,11 @@ void enqueue_step(akua_solver* s, hipStream_t st, const StepParams& S, bool time
     L.before("predict_hash");
     launch_predict_hash(st, d, P, S);
     L.after("predict_hash");
-    radix_sort_pairs(st, s->plan, s->keysA.p, s->valsA.p, s->keysB.p, s->valsB.p, s->dTiles.p,
-                     (uint32_t)s->tilesHost.size(), s->countsPerDigit, s->counts.p, s->nslots,
-                     timed ? timing_hook : nullptr, s);
+    sort_hash(s, st, timed);
+    L.before("cell_keys");
+    launch_cell_keys(st, d, P, S);
+    L.after("cell_keys");
+    sort_layout(s, st, timed);
     L.before("gather");
     launch_gather(st, d, P, S);
     L.after("gather");
@@ -416,12 +472,37 @@ void upload_scenes(akua_solver* s) {
     s->scenesDirty = false;
 }
 
+// Super-cell layout of a scene from its box: Morton coordinates start one super-cell below the
+// box and cover it with one super-cell to spare; bits per axis (at most 10) cover the longest
+// axis.  Particles outside are clamped to the edge of the key range, which only costs locality.
+void update_layout(akua_solver* s) {
+    int bits = 1;
+    for (SceneDev& sc : s->scenes) {
+        int ext = 1;
+        for (int a = 0; a < 3; ++a) {
+            const int lo = (int)std::floor(std::floor(sc.bmin[a] / s->sph.cellHash) / 2.0f) - 1;
+            const int hi = (int)std::floor(std::floor(sc.bmax[a] / s->sph.cellHash) / 2.0f) + 1;
+            sc.layoutOrigin[a] = lo;
+            ext = std::max(ext, hi - lo + 1);
+        }
+        sc.layoutBits = std::min<uint32_t>(10u, std::max<uint32_t>(1u, ceil_log2((uint64_t)ext)));
+        bits = std::max(bits, (int)sc.layoutBits);
+    }
+    for (SceneDev& sc : s->scenes) sc.layoutBits = (uint32_t)bits;
+    s->plan2 = SortPlan::balanced(3 * bits);
+}
+
 void set_box(akua_solver* s, size_t scene, const float* bmin, const float* bmax) {
     SceneDev& sc = s->scenes[scene];
+    bool changed = false;
     for (int k = 0; k < 3; ++k) {
-        if (sc.bmin[k] != bmin[k] || sc.bmax[k] != bmax[k]) s->scenesDirty = true;
+        if (sc.bmin[k] != bmin[k] || sc.bmax[k] != bmax[k]) changed = true;
         sc.bmin[k] = bmin[k];
         sc.bmax[k] = bmax[k];
+    }
+    if (changed) {
+        s->scenesDirty = true;
+        update_layout(s);
     }
 }
 
@@ -435,7 +516,8 @@ void run_steps(akua_solver* s, float dt, int nsteps) {
             enqueue_step(s, s->stream, S, true);
             collect_timing(s);
         } else if (s->useGraph) {
-            bool same = s->graphExec && std::memcmp(&S, &s->graphParams, sizeof(S)) == 0;
+            bool same = s->graphExec && std::memcmp(&S, &s->graphParams, sizeof(S)) == 0 &&
+                        s->graphPlan2 == s->plan2.bits;
             if (!same) {
                 if (s->graphExec) {
                     HIP_CHECK(hipGraphExecDestroy(s->graphExec));
@@ -449,6 +531,7 @@ void run_steps(akua_solver* s, float dt, int nsteps) {
                 (void)hipGraphDestroy(g);
                 HIP_CHECK(e);
                 s->graphParams = S;
+                s->graphPlan2 = s->plan2.bits;
             }
             HIP_CHECK(hipGraphLaunch(s->graphExec, s->stream));
         } else {
@@ -472,9 +555,9 @@ void run_density_probe(akua_solver* s) {
     HIP_CHECK(hipMemsetAsync(s->probeSum.p, 0, s->scenes.size() * sizeof(double), st));
     launch_fill_u32(st, s->tkey.p, s->tableTotal, kEmptyKey);
     launch_predict_hash(st, d, s->sph, S0);
-    radix_sort_pairs(st, s->plan, s->keysA.p, s->valsA.p, s->keysB.p, s->valsB.p, s->dTiles.p,
-                     (uint32_t)s->tilesHost.size(), s->countsPerDigit, s->counts.p, s->nslots,
-                     nullptr, nullptr);
+    sort_hash(s, st, false);
+    launch_cell_keys(st, d, s->sph, S0);
+    sort_layout(s, st, false);
     launch_gather(st, d, s->sph, S0);
     launch_density_probe(st, d, s->sph, s->probeRho.p, s->probeMax.p, s->probeSum.p);
     HIP_CHECK(hipGetLastError());
@@ -530,6 +613,10 @@ void upload_soa(akua_solver* s, const float* pos, const float* vel, const float*
     HIP_CHECK(hipMemcpy(s->dens.p, D.data(), D.size() * 4, hipMemcpyHostToDevice));
     HIP_CHECK(hipMemcpy(s->lam.p, L.data(), L.size() * 4, hipMemcpyHostToDevice));
     HIP_CHECK(hipMemcpy(s->id.p, I.data(), I.size() * 4, hipMemcpyHostToDevice));
+    std::vector<uint32_t> ident(s->nslots);
+    for (uint32_t i = 0; i < s->nslots; ++i) ident[i] = i;
+    HIP_CHECK(hipMemcpy(s->lid.p, ident.data(), ident.size() * 4, hipMemcpyHostToDevice));
+    HIP_CHECK(hipMemcpy(s->phys.p, ident.data(), ident.size() * 4, hipMemcpyHostToDevice));
     HIP_CHECK(hipMemcpy(s->color.p, C.data(), C.size() * sizeof(float4), hipMemcpyHostToDevice));
     HIP_CHECK(hipMemcpy(s->size.p, Z.data(), Z.size() * 4, hipMemcpyHostToDevice));
     s->hasState = true;
@@ -630,9 +717,11 @@ akua_status akua_solver_get_state(akua_solver* s, float* pos, float* vel, float*
         auto D = download(s, s->dens.p, s->nslots);
         auto L = download(s, s->lam.p, s->nslots);
         auto I = download(s, s->id.p, s->nslots);
-        size_t k = 0;
-        for (const SceneDev& sc : s->scenes)
-            for (uint32_t slot = sc.begin; slot < sc.end; ++slot, ++k) {
+        auto LI = download(s, s->lid.p, s->nslots);
+        size_t base = 0;
+        for (const SceneDev& sc : s->scenes) {
+            for (uint32_t slot = sc.begin; slot < sc.end; ++slot) {
+                const size_t k = base + (LI[slot] - sc.begin);  // logical position
                 if (pos) { pos[3 * k] = X[slot].x; pos[3 * k + 1] = X[slot].y; pos[3 * k + 2] = X[slot].z; }
                 if (vel) { vel[3 * k] = V[slot].x; vel[3 * k + 1] = V[slot].y; vel[3 * k + 2] = V[slot].z; }
                 if (mass) mass[k] = X[slot].w;
@@ -640,6 +729,8 @@ akua_status akua_solver_get_state(akua_solver* s, float* pos, float* vel, float*
                 if (lambda) lambda[k] = L[slot];
                 if (ids) ids[k] = I[slot];
             }
+            base += sc.end - sc.begin;
+        }
     });
 }
 
@@ -652,7 +743,7 @@ akua_status akua_solver_get_particles(akua_solver* s, akua_particle* out, int32_
         HIP_CHECK(hipMalloc(&d, (size_t)n * sizeof(AosParticle)));
         DeviceState ds = s->devState();
         launch_export_aos(s->stream, d, (uint32_t)n, s->X.p, s->V.p, s->dens.p, s->lam.p, s->id.p,
-                          s->W.p, ds.skeys, s->color.p, s->size.p);
+                          s->lid.p, s->W.p, ds.skeys, s->color.p, s->size.p);
         hipError_t e = hipStreamSynchronize(s->stream);
         if (e == hipSuccess) e = hipMemcpy(out, d, (size_t)n * sizeof(AosParticle), hipMemcpyDeviceToHost);
         (void)hipFree(d);
@@ -698,12 +789,12 @@ akua_status akua_solver_step_device(akua_solver* s, void* dParticles, int32_t n,
         }
         auto* buf = static_cast<AosParticle*>(dParticles);
         launch_import_aos(s->stream, buf, (uint32_t)n, s->X.p, s->V.p, s->dens.p, s->lam.p, s->id.p,
-                          s->color.p, s->size.p);
+                          s->lid.p, s->phys.p, s->color.p, s->size.p);
         s->hasState = true;
         run_steps(s, dt, 1);
         DeviceState ds = s->devState();
         launch_export_aos(s->stream, buf, (uint32_t)n, s->X.p, s->V.p, s->dens.p, s->lam.p, s->id.p,
-                          s->W.p, ds.skeys, s->color.p, s->size.p);
+                          s->lid.p, s->W.p, ds.skeys, s->color.p, s->size.p);
         if (user) {
             HIP_CHECK(hipEventRecord(ev, s->stream));
             HIP_CHECK(hipStreamWaitEvent(user, ev, 0));
@@ -758,7 +849,7 @@ akua_status akua_debug_permutation(akua_solver* s, uint32_t* out, int32_t n) {
     return guarded([&] {
         if (!s || !out) fail(AKUA_ERR_INVALID_ARG, "NULL argument");
         check_n(s, n);
-        auto h = download(s, s->devState().perm, s->nslots);
+        auto h = download(s, s->devState().perm1, s->nslots);
         size_t k = 0;
         for (const SceneDev& sc : s->scenes)
             for (uint32_t slot = sc.begin; slot < sc.end; ++slot) out[k++] = h[slot] - sc.begin;
@@ -807,15 +898,20 @@ akua_status akua_debug_neighbours(akua_solver* s, uint32_t* nbrOut, uint32_t* cn
         const int maxN = s->cfg.maxNeighbours;
         auto L = download(s, s->nbr.p, s->nbr.n);
         auto C = download(s, s->cnt.p, s->nslots);
-        size_t k = 0;
-        for (const SceneDev& sc : s->scenes)
-            for (uint32_t i = sc.begin; i < sc.end; ++i, ++k) {
+        auto LI = download(s, s->lid.p, s->nslots);
+        size_t base0 = 0;
+        for (const SceneDev& sc : s->scenes) {
+            for (uint32_t i = sc.begin; i < sc.end; ++i) {
+                const size_t k = base0 + (LI[i] - sc.begin);  // the row of particle i's logical index
                 if (cntOut) cntOut[k] = C[i];
                 if (!nbrOut) continue;
                 size_t base = (size_t)(i >> 6) * maxN * 64 + (i & 63);
                 for (int j = 0; j < maxN; ++j)
-                    nbrOut[k * maxN + j] = (uint32_t)j < C[i] ? L[base + (size_t)j * 64] - sc.begin : 0u;
+                    nbrOut[k * maxN + j] =
+                        (uint32_t)j < C[i] ? LI[L[base + (size_t)j * 64]] - sc.begin : 0u;
             }
+            base0 += sc.end - sc.begin;
+        }
     });
 }
 
@@ -877,7 +973,7 @@ akua_status akua_debug_probe_density(akua_solver* s, float* rho, int32_t n) {
     return guarded([&] {
         if (!s || !rho) fail(AKUA_ERR_INVALID_ARG, "NULL argument");
         check_n(s, n);
-        auto r = download(s, s->probeRho.p, s->nslots);
+        auto r = download(s, s->probeRho.p, s->nslots);  // indexed by logical index
         size_t k = 0;
         for (const SceneDev& sc : s->scenes)
             for (uint32_t slot = sc.begin; slot < sc.end; ++slot) rho[k++] = r[slot];
