@@ -30,6 +30,7 @@ EXPORTED = [
     "akua_debug_cell_ranges", "akua_debug_neighbours", "akua_solver_set_timing",
     "akua_solver_reset_timing", "akua_solver_get_timing", "akua_solver_mean_neighbours",
     "akua_solver_density_error", "akua_debug_probe_density", "akua_stream_copy_bandwidth",
+    "akua_selftest_arith",
 ]
 
 
@@ -94,6 +95,7 @@ def load() -> ctypes.CDLL:
         "akua_solver_density_error": [P, ctypes.POINTER(ctypes.c_float), ctypes.POINTER(ctypes.c_double)],
         "akua_debug_probe_density": [P, P, i32],
         "akua_stream_copy_bandwidth": [ctypes.c_int64, i32, ctypes.POINTER(ctypes.c_double)],
+        "akua_selftest_arith": [ctypes.c_uint64, ctypes.c_uint64, P],
     }
     for name, args in sigs.items():
         fn = getattr(L, name)

@@ -21,7 +21,8 @@ LIB = os.path.join(HERE, "libakua.so")
 OBJ = os.path.join(HERE, "_build")
 
 ARCH = os.environ.get("AKUA_OFFLOAD_ARCH", "gfx950")
-SOURCES = ["radix_sort.hip", "pbf_kernels.hip", "aos.hip", "stream_copy.hip", "solver.cpp"]
+SOURCES = ["radix_sort.hip", "pbf_kernels.hip", "aos.hip", "stream_copy.hip", "selftest.hip",
+           "solver.cpp"]
 
 HIPFLAGS = [
     f"--offload-arch={ARCH}", "-O3", "-std=c++17", "-fPIC",

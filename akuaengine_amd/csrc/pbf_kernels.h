@@ -60,7 +60,10 @@ struct DeviceState {
     uint32_t* keysM;           // super-cell Morton keys in logical order (second sort's input)
     const uint32_t* perm2;     // physical slot -> logical index
     uint32_t* tkey;            // cell table keys (all scenes)
-    uint2* trange;             // cell table (first, one past last) logical index of a bucket
+    uint4* tinfo;              // cell table entries: (first logical index, one past the last,
+                               // first physical slot, 1 if the bucket's physical slots are
+                               // consecutive in logical order)
+    uint32_t* bslot;           // table slot of the bucket a logical index starts (first only)
     uint32_t tableTotal;
     uint32_t* nbr;
     uint32_t* cnt;
@@ -75,6 +78,8 @@ void launch_predict_hash(hipStream_t st, const DeviceState& d, const SphParams& 
 void launch_cell_keys(hipStream_t st, const DeviceState& d, const SphParams& P,
                       const StepParams& S);
 void launch_gather(hipStream_t st, const DeviceState& d, const SphParams& P, const StepParams& S);
+// Physical start and contiguity of every bucket (after the gather).
+void launch_bucket_phys(hipStream_t st, const DeviceState& d);
 void launch_neighbours(hipStream_t st, const DeviceState& d, const SphParams& P);
 void launch_density_lambda(hipStream_t st, const DeviceState& d, const SphParams& P,
                            const float4* Xs);

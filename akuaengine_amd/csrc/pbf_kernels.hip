@@ -157,9 +157,10 @@ __global__ __launch_bounds__(kBlock) void k_cell_keys(DeviceState d, SphParams P
     if (first || last) {
         uint32_t slot = table_insert(d.tkey, sc, h);
         if (slot != kEmptyKey) {
-            if (first) d.trange[slot].x = k;
-            if (last) d.trange[slot].y = k + 1;
+            if (first) d.tinfo[slot].x = k;
+            if (last) d.tinfo[slot].y = k + 1;
         }
+        if (first) d.bslot[k] = slot;
     }
 }
 
@@ -188,18 +189,44 @@ __global__ __launch_bounds__(kBlock) void k_gather(DeviceState d, SphParams P, S
     d.physT[k] = p;
 }
 
+// Per bucket (at its first logical index k): where the bucket starts in physical order and
+// whether its particles occupy consecutive physical slots in logical order (always true for a
+// bucket of one cell, which lies inside one super-cell).  Reads physT in logical order.
+__global__ __launch_bounds__(kBlock) void k_bucket_phys(DeviceState d) {
+    const uint32_t k = blockIdx.x * kBlock + threadIdx.x;
+    if (k >= d.nslots) return;
+    const SceneDev& sc = scene_of(d, k);
+    if (k >= sc.end) return;
+    if (!(k == sc.begin || d.skeys[k - 1] != d.skeys[k])) return;
+    const uint32_t slot = d.bslot[k];
+    if (slot >= d.tableTotal) return;
+    const uint32_t len = d.tinfo[slot].y - k;
+    const uint32_t p0 = d.physT[k];
+    bool run = len <= 256u && k + len <= sc.end;
+    for (uint32_t t = 1; run && t < len; ++t) run = d.physT[k + t] == p0 + t;
+    d.tinfo[slot].z = p0;
+    d.tinfo[slot].w = run ? 1u : 0u;
+}
+
+// A bucket's entry with its ranges clamped to the scene (a consistent table never needs it, but
+// no entry can then index outside the scene).
+__device__ __forceinline__ uint4 clamp_entry(uint4 e, const SceneDev& sc) {
+    e.x = max(e.x, sc.begin);
+    e.y = min(e.y, sc.end);
+    if (e.y < e.x) e.y = e.x;
+    if (e.w && (e.z < sc.begin || e.z + (e.y - e.x) > sc.end)) e.w = 0u;
+    return e;
+}
+
 __device__ __forceinline__ bool table_find(const uint32_t* __restrict__ tkey,
-                                           const uint2* __restrict__ trange, const SceneDev& sc,
-                                           uint32_t h, uint2& range) {
+                                           const uint4* __restrict__ tinfo, const SceneDev& sc,
+                                           uint32_t h, uint4& entry) {
     const uint32_t mask = (1u << sc.tabLog2) - 1u;
     uint32_t s = table_slot(h, sc.tabLog2);
     for (uint32_t probe = 0; probe <= mask; ++probe) {
         uint32_t k = tkey[sc.tabOffset + s];
         if (k == h) {
-            range = trange[sc.tabOffset + s];
-            // a bucket lies inside its scene; clamp so a torn entry can never index outside
-            range.x = max(range.x, sc.begin);
-            range.y = min(range.y, sc.end);
+            entry = clamp_entry(tinfo[sc.tabOffset + s], sc);
             return true;
         }
         if (k == kEmptyKey) return false;
@@ -224,14 +251,14 @@ __global__ __launch_bounds__(kBlock) void k_neighbours(DeviceState d, SphParams 
     if (p < d.nslots) {
         const SceneDev& sc = scene_of(d, p);
         if (p < sc.end) {
-            const uint32_t self = d.lidT[p];
             const f3 xi = make3(d.XsA[p]);
             int cx, cy, cz;
             discretize(xi, P.cellSearch, cx, cy, cz);
             uint32_t* out = d.nbr + list_base(p, P.maxN);
             const uint32_t maxN = (uint32_t)P.maxN;
             for (int dx = -1; dx <= 1 && count < maxN; ++dx) {
-                uint32_t hq[9], sl[9], kq[9], ra[9], rb[9];
+                uint32_t hq[9], sl[9], kq[9];
+                uint4 e[9];
 #pragma unroll
                 for (int q = 0; q < 9; ++q) {
                     hq[q] = cell_hash((int)((uint32_t)cx + (uint32_t)dx),
@@ -243,38 +270,54 @@ __global__ __launch_bounds__(kBlock) void k_neighbours(DeviceState d, SphParams 
                 for (int q = 0; q < 9; ++q) kq[q] = d.tkey[sl[q]];
 #pragma unroll
                 for (int q = 0; q < 9; ++q) {
-                    uint2 r = make_uint2(0u, 0u);
+                    e[q] = make_uint4(0u, 0u, 0u, 0u);
                     if (kq[q] == hq[q]) {
-                        r = d.trange[sl[q]];
-                        r.x = max(r.x, sc.begin);
-                        r.y = min(r.y, sc.end);
+                        e[q] = clamp_entry(d.tinfo[sl[q]], sc);
                     } else if (kq[q] != kEmptyKey) {
-                        if (!table_find(d.tkey, d.trange, sc, hq[q], r)) r = make_uint2(0u, 0u);
+                        if (!table_find(d.tkey, d.tinfo, sc, hq[q], e[q])) e[q] = make_uint4(0u, 0u, 0u, 0u);
                     }
-                    ra[q] = r.x;
-                    rb[q] = r.y;
                 }
 #pragma unroll
                 for (int q = 0; q < 9; ++q) {
-                    for (uint32_t m0 = ra[q]; m0 < rb[q] && count < maxN; m0 += kWalkBatch) {
+                    const uint32_t a = e[q].x, b = e[q].y;
+                    // bucket of consecutive physical slots: read the cell-ordered positions
+                    // directly; otherwise go through the logical-order copy
+                    const bool run = e[q].w != 0u;
+                    const int32_t shift = (int32_t)e[q].z - (int32_t)a;  // physical - logical
+                    for (uint32_t m0 = a; m0 < b && count < maxN; m0 += kWalkBatch) {
                         float4 c[kWalkBatch];
                         uint32_t ph[kWalkBatch];
 #pragma unroll
                         for (int u = 0; u < kWalkBatch; ++u) {
-                            const uint32_t m = min(m0 + u, rb[q] - 1u);
-                            c[u] = d.XsL[m];
-                            ph[u] = d.physT[m];
+                            const uint32_t m = min(m0 + u, b - 1u);
+                            if (run) {
+                                ph[u] = (uint32_t)((int32_t)m + shift);
+                                c[u] = d.XsA[ph[u]];
+                            } else {
+                                c[u] = d.XsL[m];
+                                ph[u] = d.physT[m];
+                            }
                         }
+                        // test the whole batch without branches, then append in order
+                        uint32_t acc = 0u;
 #pragma unroll
                         for (int u = 0; u < kWalkBatch; ++u) {
-                            const uint32_t m = m0 + u;
-                            if (m < rb[q] && m != self && count < maxN) {
-                                f3 sep = xi - make3(c[u]);
-                                if (dot3(sep, sep) < P.h2) {
-                                    out[(size_t)count * 64u] = ph[u];
-                                    ++count;
+                            const f3 sep = xi - make3(c[u]);
+                            const bool ok = (dot3(sep, sep) < P.h2) & (ph[u] != p) & (m0 + u < b);
+                            acc |= (uint32_t)ok << u;
+                        }
+                        const uint32_t room = maxN - count;
+                        while ((uint32_t)__popc(acc) > room) acc &= ~(1u << (31 - __clz(acc)));
+                        if (acc) {
+                            uint32_t* o = out + (size_t)count * 64u;
+#pragma unroll
+                            for (int u = 0; u < kWalkBatch; ++u) {
+                                if (acc & (1u << u)) {
+                                    *o = ph[u];
+                                    o += 64;
                                 }
                             }
+                            count += (uint32_t)__popc(acc);
                         }
                     }
                 }
@@ -369,12 +412,13 @@ __global__ __launch_bounds__(kBlock) void k_delta_correct(DeviceState d, SphPara
         const uint32_t n = d.cnt[i];
         const uint32_t* lp = d.nbr + list_base(i, P.maxN);
         f3 dp{0.0f, 0.0f, 0.0f};
+        const Recip Rdq = recip(P.wdq);
         walk_list<XLam>(
             lp, n, [&](uint32_t j) { return XLam{XsIn[j], d.lam[j]}; },
             [&](const XLam& nb) {
                 f3 sep = xi - make3(nb.x);
                 float d2 = dot3(sep, sep);
-                float lc = P.negCorrK * corr_pow(poly6(d2, P) / P.wdq, P);
+                float lc = P.negCorrK * corr_pow(div_rn(poly6(d2, P), Rdq), P);
                 f3 g = grad_spiky(sep, P);
                 dp = dp + (((li + nb.lam) + lc) * nb.x.w) * g;
             });
@@ -395,7 +439,7 @@ __global__ __launch_bounds__(kBlock) void k_delta_correct(DeviceState d, SphPara
         return;
     }
     const float4 x0 = d.X0[i];
-    f3 v = (q - make3(x0)) / S.dt;
+    f3 v = div3_rn(q - make3(x0), S.dt);
     v = damp_walls(q, v, sc);
     d.X[i] = make_float4(q.x, q.y, q.z, xi4.w);
     d.V[i] = make_float4(v.x, v.y, v.z, 0.0f);
@@ -451,7 +495,7 @@ __global__ __launch_bounds__(kBlock) void k_confinement(DeviceState d, SphParams
     float el = length3(eta);
     float4 v = d.V[i];
     if (!(el < 1e-5f)) {
-        f3 N = eta / el;
+        f3 N = div3_rn(eta, el);
         f3 f = P.vortEps * cross3(N, make3(wi));
         f3 vn = make3(v) + S.dt * f;
         v = make_float4(vn.x, vn.y, vn.z, 0.0f);
@@ -515,11 +559,12 @@ __global__ __launch_bounds__(kBlock) void k_density_probe(DeviceState d, SphPara
                 bool dup = false;
 #pragma unroll
                 for (int r = 0; r < q; ++r) dup |= hs[r] == hs[q];
-                uint2 range;
-                if (dup || !table_find(d.tkey, d.trange, sc, hs[q], range)) continue;
-                for (uint32_t m = range.x; m < range.y; ++m) {
+                uint4 e;
+                if (dup || !table_find(d.tkey, d.tinfo, sc, hs[q], e)) continue;
+                const int32_t shift = (int32_t)e.z - (int32_t)e.x;
+                for (uint32_t m = e.x; m < e.y; ++m) {
                     if (m == self) continue;
-                    float4 xk = d.XsL[m];
+                    float4 xk = e.w ? d.XsA[(uint32_t)((int32_t)m + shift)] : d.XsL[m];
                     f3 sep = xi - make3(xk);
                     float d2 = dot3(sep, sep);
                     if (d2 < P.h2) rho = rho + xk.w * poly6(d2, P);
@@ -563,6 +608,9 @@ void launch_cell_keys(hipStream_t st, const DeviceState& d, const SphParams& P,
 }
 void launch_gather(hipStream_t st, const DeviceState& d, const SphParams& P, const StepParams& S) {
     k_gather<<<grid(d.nslots), kBlock, 0, st>>>(d, P, S);
+}
+void launch_bucket_phys(hipStream_t st, const DeviceState& d) {
+    k_bucket_phys<<<grid(d.nslots), kBlock, 0, st>>>(d);
 }
 void launch_neighbours(hipStream_t st, const DeviceState& d, const SphParams& P) {
     k_neighbours<<<grid_xcd(d.nslots), kBlock, 0, st>>>(d, P);

@@ -42,13 +42,85 @@ __device__ __forceinline__ float poly6(float d2, const SphParams& P) {
     return P.cp * ((t * t) * t);
 }
 
-// device_gradient_spiky (SmoothingKernelsCUDA.h:23-28): cs * (h - |r|)^2 * r / |r|.
+// ---- correctly rounded division and square root, written out ------------------------------
+// The gfx950 sequence for a correctly rounded a / b is
+//   den = div_scale(b); num = div_scale(a); r = rcp(den); r = fma(fma(-den, r, 1), r, r);
+//   q = num * r; t = fma(-den, q, num); q = fma(t, r, q); t = fma(-den, q, num);
+//   q = div_fmas(t, r, q); q = div_fixup(q, b, a).
+// For a positive divisor and a numerator of magnitude in [2^-40, 2^40] neither div_scale scales
+// (exponents far from the limits, difference below 96) nor div_fixup changes anything, and
+// div_fmas is a plain fma, so the same operations without those three instructions give the
+// identical quotient -- and the reciprocal refinement can be shared by several numerators over one
+// divisor.  Outside that range the '/' operator is used.  akua_selftest_arith checks the
+// equality on the device.
+struct Recip {
+    float b, r;
+};
+
+__device__ __forceinline__ Recip recip(float b) {
+    float r = __builtin_amdgcn_rcpf(b);
+    const float e = fmaf(-b, r, 1.0f);
+    r = fmaf(e, r, r);
+    return {b, r};
+}
+
+// a / b by the shared reciprocal, valid (equal to '/') when in_div_range(a) and
+// in_div_range(b); zero numerators keep their sign.
+__device__ __forceinline__ float div_rn_unchecked(float a, const Recip& R) {
+    float q = a * R.r;
+    float t = fmaf(-R.b, q, a);
+    q = fmaf(t, R.r, q);
+    t = fmaf(-R.b, q, a);
+    q = fmaf(t, R.r, q);
+    return a == 0.0f ? a : q;
+}
+
+__device__ __forceinline__ bool in_div_range(float x) {
+    const float m = fabsf(x);
+    return (m >= 0x1p-40f && m <= 0x1p40f) || x == 0.0f;
+}
+
+__device__ __forceinline__ float div_rn(float a, const Recip& R) {
+    float q = div_rn_unchecked(a, R);
+    if (!(in_div_range(a) && in_div_range(R.b) && R.b > 0.0f)) q = a / R.b;
+    return q;
+}
+
+// (x, y, z) / b with one range check for the four operands.
+__device__ __forceinline__ f3 div3_rn(f3 v, float b) {
+    const Recip R = recip(b);
+    f3 q{div_rn_unchecked(v.x, R), div_rn_unchecked(v.y, R), div_rn_unchecked(v.z, R)};
+    const bool ok = b > 0.0f && in_div_range(b) && in_div_range(v.x) && in_div_range(v.y) &&
+                    in_div_range(v.z);
+    if (!ok) q = {v.x / b, v.y / b, v.z / b};
+    return q;
+}
+
+// Correctly rounded sqrt for x >= 2^-96 (no scaling needed there): the 1-ulp hardware root,
+// corrected by the residuals of its two neighbours, as the compiler's own lowering does.  Below
+// 2^-96 it returns a value below 2^-47 (not necessarily correctly rounded): callers use it only
+// where every such value leads to the same branch.
+__device__ __forceinline__ float sqrt_rn_unscaled(float x) {
+    float s = __builtin_amdgcn_sqrtf(x);
+    const float sm = __uint_as_float(__float_as_uint(s) - 1u);
+    const float sp = __uint_as_float(__float_as_uint(s) + 1u);
+    const float em = fmaf(-sm, s, x);
+    const float ep = fmaf(-sp, s, x);
+    s = (em <= 0.0f) ? sm : s;
+    s = (ep > 0.0f) ? sp : s;
+    return s;
+}
+
+// device_gradient_spiky (SmoothingKernelsCUDA.h:23-28): cs * (h - |r|)^2 * r / |r|.  |r| and
+// the three quotients are correctly rounded (the reference's length and '/'): a squared length
+// below 2^-96 gives |r| < 1e-5, the zero branch, whatever its last bits.
 __device__ __forceinline__ f3 grad_spiky(f3 r, const SphParams& P) {
-    float rm = length3(r);
+    const float d2 = dot3(r, r);
+    const float rm = sqrt_rn_unscaled(d2);
     if (rm > P.h || rm < 1e-5f) return {0.0f, 0.0f, 0.0f};
     float t = P.h - rm;
     float s = P.cs * (t * t);
-    return (s * r) / rm;
+    return div3_rn(s * r, rm);
 }
 
 // powf(x, n) of the artificial pressure (ConstraintSolverCUDA.cu:124): an integer n is the n-fold

@@ -141,9 +141,9 @@ struct akua_solver {
     DevBuf<uint32_t> dChunkScene;
     DevBuf<float4> X, V, X0, XsA, XsB, XsL, W, Vc, color;
     DevBuf<float> dens, lam, Tdens, Tlam, size;
-    DevBuf<uint32_t> id, Tid, lid, phys, lidT, physT, srcL, keysLay, tkey, nbr, cnt;
+    DevBuf<uint32_t> id, Tid, lid, phys, lidT, physT, srcL, keysLay, bslot, tkey, nbr, cnt;
     DevBuf<uint32_t> s1KA, s1VA, s1KB, s1VB, s2KA, s2VA, s2KB, s2VB, counts, totals;
-    DevBuf<uint2> trange;
+    DevBuf<uint4> tinfo;
     DevBuf<SortTile> dTiles;
     DevBuf<SortScene> dSortScenes;
     DevBuf<unsigned long long> nbrSum;
@@ -184,7 +184,7 @@ struct akua_solver {
         d.perm1 = r1 ? s1VB.p : s1VA.p;
         d.keysM = s2KA.p;
         d.perm2 = r2 ? s2VB.p : s2VA.p;
-        d.tkey = tkey.p; d.trange = trange.p; d.tableTotal = tableTotal;
+        d.tkey = tkey.p; d.tinfo = tinfo.p; d.bslot = bslot.p; d.tableTotal = tableTotal;
         d.nbr = nbr.p; d.cnt = cnt.p; d.nbrSum = nbrSum.p;
         return d;
     }
@@ -312,7 +312,8 @@ void setup(akua_solver* s, const std::vector<uint32_t>& counts) {
     s->counts.alloc((size_t)s->numTiles * (1u << kMaxDigitBits));
     s->totals.alloc((size_t)kMaxPasses * (1u << kMaxDigitBits) * s->scenes.size());
     s->tkey.alloc(s->tableTotal);
-    s->trange.alloc(s->tableTotal);
+    s->tinfo.alloc(s->tableTotal);
+    s->bslot.alloc(ns);
     s->nbr.alloc(nsList * (size_t)s->cfg.maxNeighbours);
     s->cnt.alloc(ns);
     s->dTiles.alloc(s->tilesHost.size());
@@ -331,6 +332,8 @@ void setup(akua_solver* s, const std::vector<uint32_t>& counts) {
     HIP_CHECK(hipMemset(s->s1KA.p, 0, ns * 4));
     HIP_CHECK(hipMemset(s->lidT.p, 0, ns * 4));
     HIP_CHECK(hipMemset(s->srcL.p, 0, ns * 4));
+    HIP_CHECK(hipMemset(s->bslot.p, 0xFF, ns * 4));
+    HIP_CHECK(hipMemset(s->tinfo.p, 0, (size_t)s->tableTotal * sizeof(uint4)));
     HIP_CHECK(hipMemset(s->keysLay.p, 0, ns * 4));
     HIP_CHECK(hipMemset(s->XsL.p, 0, ns * sizeof(float4)));
     HIP_CHECK(hipMemset(s->physT.p, 0, ns * 4));
@@ -432,6 +435,9 @@ void enqueue_step(akua_solver* s, hipStream_t st, const StepParams& S, bool time
     L.before("gather");
     launch_gather(st, d, P, S);
     L.after("gather");
+    L.before("bucket_phys");
+    launch_bucket_phys(st, d);
+    L.after("bucket_phys");
     L.before("neighbours");
     launch_neighbours(st, d, P);
     L.after("neighbours");
@@ -559,6 +565,7 @@ void run_density_probe(akua_solver* s) {
     launch_cell_keys(st, d, s->sph, S0);
     sort_layout(s, st, false);
     launch_gather(st, d, s->sph, S0);
+    launch_bucket_phys(st, d);
     launch_density_probe(st, d, s->sph, s->probeRho.p, s->probeMax.p, s->probeSum.p);
     HIP_CHECK(hipGetLastError());
 }
@@ -863,7 +870,7 @@ akua_status akua_debug_cell_ranges(akua_solver* s, uint32_t* hash, uint32_t* sta
         if (s->scenes.size() != 1) fail(AKUA_ERR_UNSUPPORTED, "cell ranges are per single-scene solver");
         auto keys = download(s, s->devState().skeys, s->nslots);
         auto tk = download(s, s->tkey.p, s->tableTotal);
-        auto tr = download(s, s->trange.p, s->tableTotal);
+        auto tr = download(s, s->tinfo.p, s->tableTotal);
         const SceneDev& sc = s->scenes[0];
         std::vector<uint32_t> H, S, E;
         for (uint32_t k = sc.begin; k < sc.end; ++k) {

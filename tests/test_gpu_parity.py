@@ -347,3 +347,4 @@ def test_density_probe_matches_oracle(gpu, oracle_mod, dam_params):
     assert gmax == pytest.approx(emax, rel=1e-6)  # same densities; the ratio is taken in FP32 vs FP64
     assert gmean == pytest.approx(emean, rel=1e-6)
     _assert_state_equal(s, o)  # the probe leaves the particle state alone
+
