@@ -30,7 +30,8 @@ EXPORTED = [
     "akua_debug_cell_ranges", "akua_debug_neighbours", "akua_solver_set_timing",
     "akua_solver_reset_timing", "akua_solver_get_timing", "akua_solver_mean_neighbours",
     "akua_solver_density_error", "akua_debug_probe_density", "akua_stream_copy_bandwidth",
-    "akua_selftest_arith",
+    "akua_selftest_arith", "akua_batch_create", "akua_batch_num_scenes", "akua_batch_set_box",
+    "akua_batch_step", "akua_batch_density_error",
 ]
 
 
@@ -96,6 +97,11 @@ def load() -> ctypes.CDLL:
         "akua_debug_probe_density": [P, P, i32],
         "akua_stream_copy_bandwidth": [ctypes.c_int64, i32, ctypes.POINTER(ctypes.c_double)],
         "akua_selftest_arith": [ctypes.c_uint64, ctypes.c_uint64, P],
+        "akua_batch_create": [i32, P, ctypes.POINTER(CPBFConfig), ctypes.POINTER(CLambdaCorr), ctypes.POINTER(P)],
+        "akua_batch_num_scenes": [P, ctypes.POINTER(i32)],
+        "akua_batch_set_box": [P, i32, P, P],
+        "akua_batch_step": [P, f32, i32],
+        "akua_batch_density_error": [P, P, P],
     }
     for name, args in sigs.items():
         fn = getattr(L, name)

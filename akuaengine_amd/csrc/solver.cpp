@@ -688,6 +688,74 @@ akua_status akua_solver_create(int32_t numParticles, const akua_pbf_config* cfg,
     });
 }
 
+akua_status akua_batch_create(int32_t numScenes, const int32_t* counts, const akua_pbf_config* cfg,
+                              const akua_lambda_corr* corr, akua_solver** out) {
+    return guarded([&] {
+        if (!out || !counts) fail(AKUA_ERR_INVALID_ARG, "NULL argument");
+        *out = nullptr;
+        validate_config(cfg, corr);
+        if (numScenes < 1) fail(AKUA_ERR_INVALID_ARG, "numScenes must be >= 1");
+        std::vector<uint32_t> c((size_t)numScenes);
+        uint64_t total = 0;
+        for (int32_t i = 0; i < numScenes; ++i) {
+            if (counts[i] < 1) fail(AKUA_ERR_INVALID_ARG, "every scene needs at least one particle");
+            c[(size_t)i] = (uint32_t)counts[i];
+            total += (uint64_t)counts[i];
+        }
+        if (total > 0x7FFFFFFFull) fail(AKUA_ERR_UNSUPPORTED, "too many particles");
+        auto* s = new akua_solver();
+        s->cfg = *cfg;
+        s->corr = *corr;
+        try {
+            setup(s, c);
+        } catch (...) {
+            delete s;
+            throw;
+        }
+        *out = s;
+    });
+}
+
+akua_status akua_batch_num_scenes(const akua_solver* s, int32_t* n) {
+    return guarded([&] {
+        if (!s || !n) fail(AKUA_ERR_INVALID_ARG, "NULL argument");
+        *n = (int32_t)s->scenes.size();
+    });
+}
+
+akua_status akua_batch_set_box(akua_solver* s, int32_t scene, const float boxMin[3],
+                               const float boxMax[3]) {
+    return guarded([&] {
+        if (!s) fail(AKUA_ERR_INVALID_ARG, "NULL solver");
+        if (scene < 0 || (size_t)scene >= s->scenes.size()) fail(AKUA_ERR_INVALID_ARG, "scene out of range");
+        check_box(boxMin, boxMax);
+        set_box(s, (size_t)scene, boxMin, boxMax);
+    });
+}
+
+akua_status akua_batch_step(akua_solver* s, float dt, int32_t nsteps) {
+    return guarded([&] {
+        if (!s) fail(AKUA_ERR_INVALID_ARG, "NULL solver");
+        if (nsteps < 0) fail(AKUA_ERR_INVALID_ARG, "nsteps must be >= 0");
+        run_steps(s, dt, nsteps);
+    });
+}
+
+akua_status akua_batch_density_error(akua_solver* s, float* maxError, double* meanError) {
+    return guarded([&] {
+        if (!s) fail(AKUA_ERR_INVALID_ARG, "NULL solver");
+        run_density_probe(s);
+        auto mx = download(s, s->probeMax.p, s->scenes.size());
+        auto sm = download(s, s->probeSum.p, s->scenes.size());
+        for (size_t i = 0; i < mx.size(); ++i) {
+            float v;
+            std::memcpy(&v, &mx[i], sizeof(v));
+            if (maxError) maxError[i] = v;
+            if (meanError) meanError[i] = sm[i] / (double)(s->scenes[i].end - s->scenes[i].begin);
+        }
+    });
+}
+
 akua_status akua_solver_destroy(akua_solver* s) {
     return guarded([&] {
         if (!s) return;

@@ -184,3 +184,39 @@ class PBFSolver:
         v = ctypes.c_double(0.0)
         _lib.check(self._L.akua_solver_mean_neighbours(self._h, ctypes.byref(v)))
         return float(v.value)
+
+
+class BatchSolver(PBFSolver):
+    """Many independent scenes stepped together: one kernel launch per stage covers every scene
+    and each step is one hipGraph.  Particles are passed scene after scene (concatenated)."""
+
+    def __init__(self, counts: Sequence[int], config: PBFConfig, corrParams: LambdaCorrParams):
+        self._L = _lib.load()
+        self.counts = [int(c) for c in counts]
+        self.numParticles = int(sum(self.counts))
+        self.config = config
+        self.corrParams = corrParams
+        self._c_cfg = to_c_config(config)
+        self._c_corr = to_c_corr(corrParams)
+        arr = np.ascontiguousarray(self.counts, np.int32)
+        h = ctypes.c_void_p()
+        _lib.check(self._L.akua_batch_create(len(self.counts), _p(arr), ctypes.byref(self._c_cfg),
+                                             ctypes.byref(self._c_corr), ctypes.byref(h)))
+        self._h = h
+
+    @property
+    def num_scenes(self) -> int:
+        return len(self.counts)
+
+    def set_box(self, scene: int, boxMin, boxMax) -> None:
+        _lib.check(self._L.akua_batch_set_box(self._h, int(scene), _p(_f3(boxMin)), _p(_f3(boxMax))))
+
+    def step_batch(self, deltaTime: float, nsteps: int = 1) -> None:
+        _lib.check(self._L.akua_batch_step(self._h, float(deltaTime), int(nsteps)))
+
+    def density_errors(self):
+        """Per-scene (max, mean) density-constraint error at the current positions."""
+        mx = np.zeros(self.num_scenes, np.float32)
+        mean = np.zeros(self.num_scenes, np.float64)
+        _lib.check(self._L.akua_batch_density_error(self._h, _p(mx), _p(mean)))
+        return mx, mean

@@ -135,6 +135,23 @@ akua_status akua_solver_num_particles(const akua_solver* s, int32_t* n);
 /* Use a hipGraph per step (default 1).  0 launches the kernels directly each step. */
 akua_status akua_solver_set_graph(akua_solver* s, int32_t enable);
 
+/* ---- batched scenes ---------------------------------------------------------------------------- */
+/* Many independent scenes in one solver: every kernel of a step covers all scenes in one launch
+ * (each 256-slot chunk of particle slots belongs to one scene), and the whole batched step is one
+ * hipGraph.  Scene s has counts[s] particles, its own hash table of maxNeighbours * counts[s]
+ * entries and its own box; particles never interact across scenes.  No reference counterpart
+ * (the reference steps one scene).  set_state/get_state take and return the scenes' particles
+ * concatenated in scene order; akua_solver_step_n applies one box to all scenes. */
+akua_status akua_batch_create(int32_t numScenes, const int32_t* counts, const akua_pbf_config* cfg,
+                              const akua_lambda_corr* corr, akua_solver** out);
+akua_status akua_batch_num_scenes(const akua_solver* s, int32_t* numScenes);
+akua_status akua_batch_set_box(akua_solver* s, int32_t scene, const float boxMin[3],
+                               const float boxMax[3]);
+/* nsteps steps of every scene, each with its own box. */
+akua_status akua_batch_step(akua_solver* s, float deltaTime, int32_t nsteps);
+/* Per-scene max and mean density-constraint error (as akua_solver_density_error). */
+akua_status akua_batch_density_error(akua_solver* s, float* maxError, double* meanError);
+
 /* ---- parity taps: the last step's intermediate results, copied to host ---------------------- */
 /* Sorted hash keys (sorted_hash[k] is the hash of the particle in slot k after the sort). */
 akua_status akua_debug_hashes(akua_solver* s, uint32_t* sorted_hash, int32_t n);
