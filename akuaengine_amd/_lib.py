@@ -31,7 +31,8 @@ EXPORTED = [
     "akua_solver_reset_timing", "akua_solver_get_timing", "akua_solver_mean_neighbours",
     "akua_solver_density_error", "akua_debug_probe_density", "akua_stream_copy_bandwidth",
     "akua_selftest_arith", "akua_batch_create", "akua_batch_num_scenes", "akua_batch_set_box",
-    "akua_batch_step", "akua_batch_density_error",
+    "akua_batch_step", "akua_batch_density_error", "akua_solver_set_tuning",
+    "akua_solver_get_tuning", "akua_solver_autotune",
 ]
 
 
@@ -54,6 +55,16 @@ class AkuaParticle(ctypes.Structure):
 
 
 assert ctypes.sizeof(AkuaParticle) == 108
+
+
+class AkuaTuning(ctypes.Structure):
+    """``akua_tuning``: launch and layout choices that never change a result."""
+
+    _fields_ = [("block", ctypes.c_int32), ("layoutLevel", ctypes.c_int32),
+                ("hashDigitBits", ctypes.c_int32), ("layoutDigitBits", ctypes.c_int32)]
+
+    def as_dict(self):
+        return {k: int(getattr(self, k)) for k, _ in self._fields_}
 
 _lib = None
 
@@ -102,6 +113,9 @@ def load() -> ctypes.CDLL:
         "akua_batch_set_box": [P, i32, P, P],
         "akua_batch_step": [P, f32, i32],
         "akua_batch_density_error": [P, P, P],
+        "akua_solver_set_tuning": [P, ctypes.POINTER(AkuaTuning)],
+        "akua_solver_get_tuning": [P, ctypes.POINTER(AkuaTuning)],
+        "akua_solver_autotune": [P, f32, P, P, i32, ctypes.c_char_p, ctypes.POINTER(AkuaTuning)],
     }
     for name, args in sigs.items():
         fn = getattr(L, name)

@@ -80,19 +80,20 @@ void launch_cell_keys(hipStream_t st, const DeviceState& d, const SphParams& P,
 void launch_gather(hipStream_t st, const DeviceState& d, const SphParams& P, const StepParams& S);
 // Physical start and contiguity of every bucket (after the gather).
 void launch_bucket_phys(hipStream_t st, const DeviceState& d);
-void launch_neighbours(hipStream_t st, const DeviceState& d, const SphParams& P);
+// bs: workgroup size of the neighbour kernels (128, 256 or 512; results do not depend on it).
+void launch_neighbours(hipStream_t st, const DeviceState& d, const SphParams& P, int bs);
 void launch_density_lambda(hipStream_t st, const DeviceState& d, const SphParams& P,
-                           const float4* Xs);
+                           const float4* Xs, int bs);
 // Position correction with the wall clamp.  final == true also runs the velocity update and the
 // wall damping and writes X, V, id, lid and phys (the last iteration); applyDelta == false skips
 // the correction (a step with zero iterations).
 void launch_delta_correct(hipStream_t st, const DeviceState& d, const SphParams& P,
                           const StepParams& S, const float4* XsIn, float4* XsOut, bool final,
-                          bool applyDelta);
-void launch_vorticity(hipStream_t st, const DeviceState& d, const SphParams& P);
+                          bool applyDelta, int bs);
+void launch_vorticity(hipStream_t st, const DeviceState& d, const SphParams& P, int bs);
 void launch_confinement(hipStream_t st, const DeviceState& d, const SphParams& P,
-                        const StepParams& S);
-void launch_xsph(hipStream_t st, const DeviceState& d, const SphParams& P);
+                        const StepParams& S, int bs);
+void launch_xsph(hipStream_t st, const DeviceState& d, const SphParams& P, int bs);
 // Exact SPH density at the positions held in XsA; rho by logical index, per-scene error max and
 // sum.
 void launch_density_probe(hipStream_t st, const DeviceState& d, const SphParams& P, float* rho,

@@ -78,10 +78,11 @@ struct XVRho {
 };
 
 // Physical slot of this thread under the XCD-contiguous workgroup order (the grid is a multiple
-// of 8 workgroups).
+// of 8 workgroups of BS threads).
+template <int BS>
 __device__ __forceinline__ uint32_t xcd_slot() {
     const uint32_t nb = gridDim.x, b = blockIdx.x;
-    return ((b % kXcds) * (nb / kXcds) + b / kXcds) * kBlock + threadIdx.x;
+    return ((b % kXcds) * (nb / kXcds) + b / kXcds) * BS + threadIdx.x;
 }
 
 __global__ void k_fill_u32(uint32_t* __restrict__ p, uint32_t n, uint32_t v) {
@@ -98,13 +99,15 @@ __device__ __forceinline__ uint32_t spread3(uint32_t x) {
     return x;
 }
 
-// Morton code of the particle's 2x2x2 super-cell, relative to the scene's layout origin and
-// clamped to the key range.  It only decides where the particle's data is stored.
+// Morton code of the particle's super-cell (2^L cells per axis, L = layoutOrigin[3]), relative to
+// the scene's layout origin and clamped to the key range.  It only decides where the particle's
+// data is stored.
 __device__ __forceinline__ uint32_t layout_key(int cx, int cy, int cz, const SceneDev& sc) {
     const int hi = (1 << sc.layoutBits) - 1;
-    int u = min(max((cx >> 1) - sc.layoutOrigin[0], 0), hi);
-    int v = min(max((cy >> 1) - sc.layoutOrigin[1], 0), hi);
-    int w = min(max((cz >> 1) - sc.layoutOrigin[2], 0), hi);
+    const int L = sc.layoutOrigin[3];
+    int u = min(max((cx >> L) - sc.layoutOrigin[0], 0), hi);
+    int v = min(max((cy >> L) - sc.layoutOrigin[1], 0), hi);
+    int w = min(max((cz >> L) - sc.layoutOrigin[2], 0), hi);
     return spread3((uint32_t)u) | (spread3((uint32_t)v) << 1) | (spread3((uint32_t)w) << 2);
 }
 
@@ -245,8 +248,9 @@ __device__ __forceinline__ bool table_find(const uint32_t* __restrict__ tkey,
 // slot), after which the candidates are tested and appended strictly in order.
 constexpr int kWalkBatch = 8;
 
-__global__ __launch_bounds__(kBlock) void k_neighbours(DeviceState d, SphParams P) {
-    const uint32_t p = xcd_slot();
+template <int BS>
+__global__ __launch_bounds__(BS) void k_neighbours(DeviceState d, SphParams P) {
+    const uint32_t p = xcd_slot<BS>();
     uint32_t count = 0;
     if (p < d.nslots) {
         const SceneDev& sc = scene_of(d, p);
@@ -334,9 +338,10 @@ __global__ __launch_bounds__(kBlock) void k_neighbours(DeviceState d, SphParams 
 // kernel_calculate_densities (ConstraintSolverCUDA.cu:16-42) and kernel_calculate_lambdas
 // (ConstraintSolverCUDA.cu:51-97) in one neighbour sweep.  Each sum keeps the reference's order
 // (the list order), so fusing the loops changes no rounding.
-__global__ __launch_bounds__(kBlock) void k_density_lambda(DeviceState d, SphParams P,
+template <int BS>
+__global__ __launch_bounds__(BS) void k_density_lambda(DeviceState d, SphParams P,
                                                            const float4* __restrict__ Xs) {
-    const uint32_t i = xcd_slot();
+    const uint32_t i = xcd_slot<BS>();
     if (i >= d.nslots) return;
     if (i >= scene_of(d, i).end) return;
     const float4 xi4 = Xs[i];
@@ -395,12 +400,12 @@ __device__ __forceinline__ f3 damp_walls(f3 x, f3 v, const SceneDev& sc) {
 // kernel reads XsIn and writes XsOut, so every particle sees the positions of the same iteration.
 // kFinal: also kernel_update_position_and_velocity (IntegrationCUDA.cu:38-49) and the wall
 // damping, writing the step's positions X, velocities V, ids and the two order maps.
-template <bool kFinal, bool kApplyDelta>
-__global__ __launch_bounds__(kBlock) void k_delta_correct(DeviceState d, SphParams P,
+template <int BS, bool kFinal, bool kApplyDelta>
+__global__ __launch_bounds__(BS) void k_delta_correct(DeviceState d, SphParams P,
                                                           StepParams S,
                                                           const float4* __restrict__ XsIn,
                                                           float4* __restrict__ XsOut) {
-    const uint32_t i = xcd_slot();
+    const uint32_t i = xcd_slot<BS>();
     if (i >= d.nslots) return;
     const SceneDev& sc = scene_of(d, i);
     if (i >= sc.end) return;
@@ -454,8 +459,9 @@ __global__ __launch_bounds__(kBlock) void k_delta_correct(DeviceState d, SphPara
 }
 
 // kernel_compute_vorticities (IntegrationCUDA.cu:104-128).  W.w keeps |omega| for the next pass.
-__global__ __launch_bounds__(kBlock) void k_vorticity(DeviceState d, SphParams P) {
-    const uint32_t i = xcd_slot();
+template <int BS>
+__global__ __launch_bounds__(BS) void k_vorticity(DeviceState d, SphParams P) {
+    const uint32_t i = xcd_slot<BS>();
     if (i >= d.nslots) return;
     if (i >= scene_of(d, i).end) return;
     const f3 xi = make3(d.X[i]);
@@ -474,8 +480,9 @@ __global__ __launch_bounds__(kBlock) void k_vorticity(DeviceState d, SphParams P
 }
 
 // kernel_apply_vorticity_confinement (IntegrationCUDA.cu:130-165), writing Vc.
-__global__ __launch_bounds__(kBlock) void k_confinement(DeviceState d, SphParams P, StepParams S) {
-    const uint32_t i = xcd_slot();
+template <int BS>
+__global__ __launch_bounds__(BS) void k_confinement(DeviceState d, SphParams P, StepParams S) {
+    const uint32_t i = xcd_slot<BS>();
     if (i >= d.nslots) return;
     if (i >= scene_of(d, i).end) return;
     const f3 xi = make3(d.X[i]);
@@ -504,8 +511,9 @@ __global__ __launch_bounds__(kBlock) void k_confinement(DeviceState d, SphParams
 }
 
 // kernel_apply_xsph_viscosity (IntegrationCUDA.cu:167-195), Jacobi: reads Vc, writes V.
-__global__ __launch_bounds__(kBlock) void k_xsph(DeviceState d, SphParams P) {
-    const uint32_t i = xcd_slot();
+template <int BS>
+__global__ __launch_bounds__(BS) void k_xsph(DeviceState d, SphParams P) {
+    const uint32_t i = xcd_slot<BS>();
     if (i >= d.nslots) return;
     if (i >= scene_of(d, i).end) return;
     const f3 xi = make3(d.X[i]);
@@ -529,11 +537,12 @@ __global__ __launch_bounds__(kBlock) void k_xsph(DeviceState d, SphParams P) {
 // k_neighbours with the hashing cell size, each distinct bucket of the stencil visited once (no
 // duplicates from stencil hash collisions) and no neighbour cap.  Accumulates, per scene, the
 // maximum and the sum of max(0, rho / rho0 - 1) (the density-constraint error).
-__global__ __launch_bounds__(kBlock) void k_density_probe(DeviceState d, SphParams P,
+template <int BS>
+__global__ __launch_bounds__(BS) void k_density_probe(DeviceState d, SphParams P,
                                                           float* __restrict__ rhoOut,
                                                           unsigned int* __restrict__ errMaxBits,
                                                           double* __restrict__ errSum) {
-    const uint32_t i = xcd_slot();
+    const uint32_t i = xcd_slot<BS>();
     float err = 0.0f;
     uint32_t scene = 0;
     bool active = false;
@@ -590,8 +599,25 @@ __global__ __launch_bounds__(kBlock) void k_density_probe(DeviceState d, SphPara
 }
 
 inline unsigned grid(uint32_t n) { return (n + kBlock - 1) / kBlock; }
-// Neighbour kernels: a whole number of workgroups per XCD.
-inline unsigned grid_xcd(uint32_t n) { return (grid(n) + kXcds - 1) / kXcds * kXcds; }
+// Neighbour kernels: a whole number of workgroups of bs threads per XCD.
+inline unsigned grid_xcd(uint32_t n, int bs) {
+    const unsigned g = (n + (unsigned)bs - 1) / (unsigned)bs;
+    return (g + kXcds - 1) / kXcds * kXcds;
+}
+
+#define AKUA_BY_BLOCK(bs, CALL)         \
+    do {                                \
+        if ((bs) == 128) {              \
+            constexpr int BS = 128;     \
+            CALL;                       \
+        } else if ((bs) == 512) {       \
+            constexpr int BS = 512;     \
+            CALL;                       \
+        } else {                        \
+            constexpr int BS = 256;     \
+            CALL;                       \
+        }                               \
+    } while (0)
 
 }  // namespace
 
@@ -612,37 +638,43 @@ void launch_gather(hipStream_t st, const DeviceState& d, const SphParams& P, con
 void launch_bucket_phys(hipStream_t st, const DeviceState& d) {
     k_bucket_phys<<<grid(d.nslots), kBlock, 0, st>>>(d);
 }
-void launch_neighbours(hipStream_t st, const DeviceState& d, const SphParams& P) {
-    k_neighbours<<<grid_xcd(d.nslots), kBlock, 0, st>>>(d, P);
+void launch_neighbours(hipStream_t st, const DeviceState& d, const SphParams& P, int bs) {
+    AKUA_BY_BLOCK(bs, (k_neighbours<BS><<<grid_xcd(d.nslots, BS), BS, 0, st>>>(d, P)));
 }
 void launch_density_lambda(hipStream_t st, const DeviceState& d, const SphParams& P,
-                           const float4* Xs) {
-    k_density_lambda<<<grid_xcd(d.nslots), kBlock, 0, st>>>(d, P, Xs);
+                           const float4* Xs, int bs) {
+    AKUA_BY_BLOCK(bs, (k_density_lambda<BS><<<grid_xcd(d.nslots, BS), BS, 0, st>>>(d, P, Xs)));
+}
+template <int BS>
+void delta_correct_bs(hipStream_t st, const DeviceState& d, const SphParams& P,
+                      const StepParams& S, const float4* XsIn, float4* XsOut, bool final,
+                      bool applyDelta) {
+    const unsigned g = grid_xcd(d.nslots, BS);
+    if (final && applyDelta)
+        k_delta_correct<BS, true, true><<<g, BS, 0, st>>>(d, P, S, XsIn, XsOut);
+    else if (final)
+        k_delta_correct<BS, true, false><<<g, BS, 0, st>>>(d, P, S, XsIn, XsOut);
+    else
+        k_delta_correct<BS, false, true><<<g, BS, 0, st>>>(d, P, S, XsIn, XsOut);
 }
 void launch_delta_correct(hipStream_t st, const DeviceState& d, const SphParams& P,
                           const StepParams& S, const float4* XsIn, float4* XsOut, bool final,
-                          bool applyDelta) {
-    const unsigned g = grid_xcd(d.nslots);
-    if (final && applyDelta)
-        k_delta_correct<true, true><<<g, kBlock, 0, st>>>(d, P, S, XsIn, XsOut);
-    else if (final)
-        k_delta_correct<true, false><<<g, kBlock, 0, st>>>(d, P, S, XsIn, XsOut);
-    else
-        k_delta_correct<false, true><<<g, kBlock, 0, st>>>(d, P, S, XsIn, XsOut);
+                          bool applyDelta, int bs) {
+    AKUA_BY_BLOCK(bs, (delta_correct_bs<BS>(st, d, P, S, XsIn, XsOut, final, applyDelta)));
 }
-void launch_vorticity(hipStream_t st, const DeviceState& d, const SphParams& P) {
-    k_vorticity<<<grid_xcd(d.nslots), kBlock, 0, st>>>(d, P);
+void launch_vorticity(hipStream_t st, const DeviceState& d, const SphParams& P, int bs) {
+    AKUA_BY_BLOCK(bs, (k_vorticity<BS><<<grid_xcd(d.nslots, BS), BS, 0, st>>>(d, P)));
 }
 void launch_confinement(hipStream_t st, const DeviceState& d, const SphParams& P,
-                        const StepParams& S) {
-    k_confinement<<<grid_xcd(d.nslots), kBlock, 0, st>>>(d, P, S);
+                        const StepParams& S, int bs) {
+    AKUA_BY_BLOCK(bs, (k_confinement<BS><<<grid_xcd(d.nslots, BS), BS, 0, st>>>(d, P, S)));
 }
-void launch_xsph(hipStream_t st, const DeviceState& d, const SphParams& P) {
-    k_xsph<<<grid_xcd(d.nslots), kBlock, 0, st>>>(d, P);
+void launch_xsph(hipStream_t st, const DeviceState& d, const SphParams& P, int bs) {
+    AKUA_BY_BLOCK(bs, (k_xsph<BS><<<grid_xcd(d.nslots, BS), BS, 0, st>>>(d, P)));
 }
 void launch_density_probe(hipStream_t st, const DeviceState& d, const SphParams& P, float* rho,
                           unsigned int* errMaxBits, double* errSum) {
-    k_density_probe<<<grid_xcd(d.nslots), kBlock, 0, st>>>(d, P, rho, errMaxBits, errSum);
+    k_density_probe<256><<<grid_xcd(d.nslots, 256), 256, 0, st>>>(d, P, rho, errMaxBits, errSum);
 }
 
 }  // namespace akua

@@ -9,8 +9,10 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <fstream>
 #include <map>
 #include <new>
+#include <sstream>
 #include <stdexcept>
 #include <string>
 #include <vector>
@@ -131,6 +133,7 @@ struct akua_solver {
     uint32_t tableTotal = 0;
     SortPlan plan1;          // hash sort (logical order)
     SortPlan plan2;          // super-cell sort (physical layout)
+    akua_tuning tuning{256, 1, 0, 0};  // block, layout level, digit bits (0: balanced)
     std::vector<SortTile> tilesHost;
     std::vector<SortScene> sortScenesHost;
     uint32_t numTiles = 0;
@@ -157,6 +160,7 @@ struct akua_solver {
     hipGraphExec_t graphExec = nullptr;
     StepParams graphParams{};
     int graphPlan2 = -1;
+    akua_tuning graphTuning{0, 0, 0, 0};
     uint64_t stepsSinceStats = 0;
 
     // timing
@@ -438,35 +442,36 @@ void enqueue_step(akua_solver* s, hipStream_t st, const StepParams& S, bool time
     L.before("bucket_phys");
     launch_bucket_phys(st, d);
     L.after("bucket_phys");
+    const int bs = s->tuning.block;
     L.before("neighbours");
-    launch_neighbours(st, d, P);
+    launch_neighbours(st, d, P, bs);
     L.after("neighbours");
     const int I = s->cfg.solverIterations;
     float4* xs[2] = {s->XsA.p, s->XsB.p};
     if (I == 0) {
         L.before("update_damping");
-        launch_delta_correct(st, d, P, S, xs[0], nullptr, true, false);
+        launch_delta_correct(st, d, P, S, xs[0], nullptr, true, false, bs);
         L.after("update_damping");
     }
     for (int it = 0; it < I; ++it) {
         const float4* in = xs[it & 1];
         float4* out = xs[(it + 1) & 1];
         L.before("density_lambda");
-        launch_density_lambda(st, d, P, in);
+        launch_density_lambda(st, d, P, in, bs);
         L.after("density_lambda");
         const bool last = it + 1 == I;
         L.before(last ? "delta_correct_update" : "delta_correct");
-        launch_delta_correct(st, d, P, S, in, out, last, true);
+        launch_delta_correct(st, d, P, S, in, out, last, true, bs);
         L.after(last ? "delta_correct_update" : "delta_correct");
     }
     L.before("vorticity");
-    launch_vorticity(st, d, P);
+    launch_vorticity(st, d, P, bs);
     L.after("vorticity");
     L.before("confinement");
-    launch_confinement(st, d, P, S);
+    launch_confinement(st, d, P, S, bs);
     L.after("confinement");
     L.before("xsph");
-    launch_xsph(st, d, P);
+    launch_xsph(st, d, P, bs);
     L.after("xsph");
 }
 
@@ -481,21 +486,32 @@ void upload_scenes(akua_solver* s) {
 // Super-cell layout of a scene from its box: Morton coordinates start one super-cell below the
 // box and cover it with one super-cell to spare; bits per axis (at most 10) cover the longest
 // axis.  Particles outside are clamped to the edge of the key range, which only costs locality.
+SortPlan make_plan(int bits, int digitBits) {
+    SortPlan p = SortPlan::balanced(bits);
+    if (digitBits >= 4 && digitBits <= kMaxDigitBits) p.digitBits = digitBits;
+    return p;
+}
+
 void update_layout(akua_solver* s) {
     int bits = 1;
+    const int L = s->tuning.layoutLevel;
+    const float span = (float)(1 << L);
     for (SceneDev& sc : s->scenes) {
         int ext = 1;
         for (int a = 0; a < 3; ++a) {
-            const int lo = (int)std::floor(std::floor(sc.bmin[a] / s->sph.cellHash) / 2.0f) - 1;
-            const int hi = (int)std::floor(std::floor(sc.bmax[a] / s->sph.cellHash) / 2.0f) + 1;
+            const int lo = (int)std::floor(std::floor(sc.bmin[a] / s->sph.cellHash) / span) - 1;
+            const int hi = (int)std::floor(std::floor(sc.bmax[a] / s->sph.cellHash) / span) + 1;
             sc.layoutOrigin[a] = lo;
             ext = std::max(ext, hi - lo + 1);
         }
+        sc.layoutOrigin[3] = L;
         sc.layoutBits = std::min<uint32_t>(10u, std::max<uint32_t>(1u, ceil_log2((uint64_t)ext)));
         bits = std::max(bits, (int)sc.layoutBits);
     }
     for (SceneDev& sc : s->scenes) sc.layoutBits = (uint32_t)bits;
-    s->plan2 = SortPlan::balanced(3 * bits);
+    s->plan2 = make_plan(3 * bits, s->tuning.layoutDigitBits);
+    s->plan1 = make_plan(s->plan1.bits, s->tuning.hashDigitBits);
+    s->scenesDirty = true;
 }
 
 void set_box(akua_solver* s, size_t scene, const float* bmin, const float* bmax) {
@@ -523,7 +539,8 @@ void run_steps(akua_solver* s, float dt, int nsteps) {
             collect_timing(s);
         } else if (s->useGraph) {
             bool same = s->graphExec && std::memcmp(&S, &s->graphParams, sizeof(S)) == 0 &&
-                        s->graphPlan2 == s->plan2.bits;
+                        s->graphPlan2 == s->plan2.bits &&
+                        std::memcmp(&s->graphTuning, &s->tuning, sizeof(akua_tuning)) == 0;
             if (!same) {
                 if (s->graphExec) {
                     HIP_CHECK(hipGraphExecDestroy(s->graphExec));
@@ -538,6 +555,7 @@ void run_steps(akua_solver* s, float dt, int nsteps) {
                 HIP_CHECK(e);
                 s->graphParams = S;
                 s->graphPlan2 = s->plan2.bits;
+                s->graphTuning = s->tuning;
             }
             HIP_CHECK(hipGraphLaunch(s->graphExec, s->stream));
         } else {
@@ -635,6 +653,172 @@ std::vector<T> download(akua_solver* s, const T* p, size_t n) {
     HIP_CHECK(hipStreamSynchronize(s->stream));
     if (n) HIP_CHECK(hipMemcpy(h.data(), p, n * sizeof(T), hipMemcpyDeviceToHost));
     return h;
+}
+
+// ---- tuning ---------------------------------------------------------------------------------
+
+void validate_tuning(const akua_tuning& t) {
+    if (t.block != 128 && t.block != 256 && t.block != 512)
+        fail(AKUA_ERR_INVALID_ARG, "tuning.block must be 128, 256 or 512");
+    if (t.layoutLevel < 0 || t.layoutLevel > 3)
+        fail(AKUA_ERR_INVALID_ARG, "tuning.layoutLevel must be in [0, 3]");
+    for (int b : {t.hashDigitBits, t.layoutDigitBits})
+        if (b != 0 && (b < 5 || b > kMaxDigitBits))
+            fail(AKUA_ERR_INVALID_ARG, "tuning digit widths must be 0 or 5..8");
+}
+
+void apply_tuning(akua_solver* s, const akua_tuning& t) {
+    validate_tuning(t);
+    s->tuning = t;
+    update_layout(s);
+}
+
+// The per-slot state a step reads: a trial restarts from it and the tuner restores it.
+struct Snapshot {
+    DevBuf<float4> X, V, W;
+    DevBuf<float> dens, lam;
+    DevBuf<uint32_t> id, lid, phys;
+};
+
+template <class T>
+void copy_dev(T* dst, const T* src, size_t n, hipStream_t st) {
+    HIP_CHECK(hipMemcpyAsync(dst, src, n * sizeof(T), hipMemcpyDeviceToDevice, st));
+}
+
+void take_snapshot(akua_solver* s, Snapshot& q) {
+    const size_t n = s->nslots;
+    q.X.alloc(n); q.V.alloc(n); q.W.alloc(n); q.dens.alloc(n); q.lam.alloc(n);
+    q.id.alloc(n); q.lid.alloc(n); q.phys.alloc(n);
+    hipStream_t st = s->stream;
+    copy_dev(q.X.p, s->X.p, n, st); copy_dev(q.V.p, s->V.p, n, st); copy_dev(q.W.p, s->W.p, n, st);
+    copy_dev(q.dens.p, s->dens.p, n, st); copy_dev(q.lam.p, s->lam.p, n, st);
+    copy_dev(q.id.p, s->id.p, n, st); copy_dev(q.lid.p, s->lid.p, n, st);
+    copy_dev(q.phys.p, s->phys.p, n, st);
+}
+
+void restore_snapshot(akua_solver* s, const Snapshot& q) {
+    const size_t n = s->nslots;
+    hipStream_t st = s->stream;
+    copy_dev(s->X.p, q.X.p, n, st); copy_dev(s->V.p, q.V.p, n, st); copy_dev(s->W.p, q.W.p, n, st);
+    copy_dev(s->dens.p, q.dens.p, n, st); copy_dev(s->lam.p, q.lam.p, n, st);
+    copy_dev(s->id.p, q.id.p, n, st); copy_dev(s->lid.p, q.lid.p, n, st);
+    copy_dev(s->phys.p, q.phys.p, n, st);
+}
+
+// Device milliseconds per step of `steps` graph-launched steps from the snapshot, after one
+// untimed step that instantiates the graph for the setting.
+double time_trial(akua_solver* s, const Snapshot& q, float dt, int steps) {
+    restore_snapshot(s, q);
+    run_steps(s, dt, 1);
+    hipEvent_t a, b;
+    HIP_CHECK(hipEventCreate(&a));
+    HIP_CHECK(hipEventCreate(&b));
+    HIP_CHECK(hipEventRecord(a, s->stream));
+    run_steps(s, dt, steps);
+    HIP_CHECK(hipEventRecord(b, s->stream));
+    HIP_CHECK(hipEventSynchronize(b));
+    float ms = 0.0f;
+    hipError_t e = hipEventElapsedTime(&ms, a, b);
+    (void)hipEventDestroy(a);
+    (void)hipEventDestroy(b);
+    HIP_CHECK(e);
+    return (double)ms / steps;
+}
+
+std::string tuning_key(akua_solver* s) {
+    hipDeviceProp_t prop{};
+    HIP_CHECK(hipGetDeviceProperties(&prop, s->device));
+    std::string arch = prop.gcnArchName;
+    const size_t colon = arch.find(':');
+    if (colon != std::string::npos) arch.resize(colon);
+    std::ostringstream k;
+    k << arch << " n2^" << ceil_log2(std::max<uint64_t>(1, s->nparticles)) << " scenes2^"
+      << ceil_log2(std::max<uint64_t>(1, s->scenes.size())) << " cap" << s->cfg.maxNeighbours
+      << " it" << s->cfg.solverIterations;
+    return k.str();
+}
+
+// Cache lines: "<key> | block level hashBits layoutBits".  The last line for a key wins.
+bool cache_lookup(const char* path, const std::string& key, akua_tuning& t) {
+    std::ifstream f(path);
+    if (!f) return false;
+    std::string line;
+    bool found = false;
+    while (std::getline(f, line)) {
+        const size_t bar = line.find(" | ");
+        if (bar == std::string::npos || line.compare(0, bar, key) != 0 || bar != key.size()) continue;
+        std::istringstream v(line.substr(bar + 3));
+        akua_tuning c{};
+        if (!(v >> c.block >> c.layoutLevel >> c.hashDigitBits >> c.layoutDigitBits)) continue;
+        try {
+            validate_tuning(c);
+        } catch (const AkuaError&) {
+            continue;
+        }
+        t = c;
+        found = true;
+    }
+    return found;
+}
+
+void cache_store(const char* path, const std::string& key, const akua_tuning& t) {
+    std::ofstream f(path, std::ios::app);
+    if (!f) fail(AKUA_ERR_INVALID_ARG, std::string("cannot append to the tuning cache ") + path);
+    f << key << " | " << t.block << ' ' << t.layoutLevel << ' ' << t.hashDigitBits << ' '
+      << t.layoutDigitBits << '\n';
+}
+
+void autotune(akua_solver* s, float dt, const float* bmin, const float* bmax, int trialSteps,
+              const char* cachePath) {
+    if (!s->hasState) fail(AKUA_ERR_STATE, "set the particles before tuning");
+    if (!(dt > 0.0f) || !std::isfinite(dt)) fail(AKUA_ERR_INVALID_ARG, "deltaTime must be > 0");
+    if (trialSteps < 1 || trialSteps > 1000) fail(AKUA_ERR_INVALID_ARG, "trialSteps must be in [1, 1000]");
+    if ((bmin == nullptr) != (bmax == nullptr)) fail(AKUA_ERR_INVALID_ARG, "give both box corners or neither");
+    if (bmin) {
+        check_box(bmin, bmax);
+        for (size_t i = 0; i < s->scenes.size(); ++i) set_box(s, i, bmin, bmax);
+    }
+    std::string key;
+    if (cachePath) {
+        key = tuning_key(s);
+        akua_tuning c{};
+        if (cache_lookup(cachePath, key, c)) {
+            apply_tuning(s, c);
+            return;
+        }
+    }
+    const bool timing = s->timing;
+    s->timing = false;
+    const bool graph = s->useGraph;
+    s->useGraph = true;
+    Snapshot q;
+    take_snapshot(s, q);
+    akua_tuning best = s->tuning;
+    apply_tuning(s, best);
+    double bestMs = time_trial(s, q, dt, trialSteps);
+    const auto sweep = [&](int akua_tuning::*field, std::initializer_list<int> values) {
+        for (int v : values) {
+            if (best.*field == v) continue;
+            akua_tuning t = best;
+            t.*field = v;
+            apply_tuning(s, t);
+            const double ms = time_trial(s, q, dt, trialSteps);
+            if (ms < bestMs) {
+                bestMs = ms;
+                best = t;
+            }
+        }
+    };
+    sweep(&akua_tuning::block, {128, 256, 512});
+    sweep(&akua_tuning::layoutLevel, {0, 1, 2});
+    sweep(&akua_tuning::hashDigitBits, {0, 5, 6, 8});
+    sweep(&akua_tuning::layoutDigitBits, {0, 5, 8});
+    apply_tuning(s, best);
+    restore_snapshot(s, q);
+    HIP_CHECK(hipStreamSynchronize(s->stream));
+    s->timing = timing;
+    s->useGraph = graph;
+    if (cachePath) cache_store(cachePath, key, best);
 }
 
 }  // namespace
@@ -906,6 +1090,30 @@ akua_status akua_solver_set_graph(akua_solver* s, int32_t enable) {
     return guarded([&] {
         if (!s) fail(AKUA_ERR_INVALID_ARG, "NULL solver");
         s->useGraph = enable != 0;
+    });
+}
+
+akua_status akua_solver_set_tuning(akua_solver* s, const akua_tuning* t) {
+    return guarded([&] {
+        if (!s || !t) fail(AKUA_ERR_INVALID_ARG, "NULL argument");
+        apply_tuning(s, *t);
+    });
+}
+
+akua_status akua_solver_get_tuning(const akua_solver* s, akua_tuning* t) {
+    return guarded([&] {
+        if (!s || !t) fail(AKUA_ERR_INVALID_ARG, "NULL argument");
+        *t = s->tuning;
+    });
+}
+
+akua_status akua_solver_autotune(akua_solver* s, float dt, const float boxMin[3],
+                                 const float boxMax[3], int32_t trialSteps, const char* cachePath,
+                                 akua_tuning* best) {
+    return guarded([&] {
+        if (!s) fail(AKUA_ERR_INVALID_ARG, "NULL solver");
+        autotune(s, dt, boxMin, boxMax, trialSteps, cachePath);
+        if (best) *best = s->tuning;
     });
 }
 

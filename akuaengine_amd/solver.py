@@ -117,6 +117,28 @@ class PBFSolver:
     def set_graph(self, enable: bool) -> None:
         _lib.check(self._L.akua_solver_set_graph(self._h, 1 if enable else 0))
 
+    # -- tuning -------------------------------------------------------------------------------
+    def set_tuning(self, block: int = 256, layoutLevel: int = 1, hashDigitBits: int = 0,
+                   layoutDigitBits: int = 0) -> None:
+        t = _lib.AkuaTuning(block, layoutLevel, hashDigitBits, layoutDigitBits)
+        _lib.check(self._L.akua_solver_set_tuning(self._h, ctypes.byref(t)))
+
+    def get_tuning(self) -> Dict[str, int]:
+        t = _lib.AkuaTuning()
+        _lib.check(self._L.akua_solver_get_tuning(self._h, ctypes.byref(t)))
+        return t.as_dict()
+
+    def autotune(self, deltaTime: float, boxMin=None, boxMax=None, trialSteps: int = 5,
+                 cachePath: Optional[str] = None) -> Dict[str, int]:
+        """Times candidate settings on the current state (restored afterwards), keeps the fastest."""
+        t = _lib.AkuaTuning()
+        bmin = _f3(boxMin) if boxMin is not None else None
+        bmax = _f3(boxMax) if boxMax is not None else None
+        path = cachePath.encode() if cachePath else None
+        _lib.check(self._L.akua_solver_autotune(self._h, float(deltaTime), _p(bmin), _p(bmax),
+                                                int(trialSteps), path, ctypes.byref(t)))
+        return t.as_dict()
+
     # -- parity taps --------------------------------------------------------------------------
     def debug_hashes(self) -> np.ndarray:
         out = np.zeros(self.numParticles, np.uint32)

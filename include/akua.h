@@ -135,6 +135,31 @@ akua_status akua_solver_num_particles(const akua_solver* s, int32_t* n);
 /* Use a hipGraph per step (default 1).  0 launches the kernels directly each step. */
 akua_status akua_solver_set_graph(akua_solver* s, int32_t enable);
 
+/* ---- tuning ---------------------------------------------------------------------------------- */
+/* Launch and layout choices that never change a result: every setting gives bit-identical
+ * particle states.  No reference counterpart (the reference fixes 256-thread blocks and a
+ * thrust sort, NeighbourSearchCUDA.cu:150-172). */
+typedef struct akua_tuning {
+    int32_t block;            /* workgroup size of the neighbour-walking kernels: 128, 256 or 512 */
+    int32_t layoutLevel;      /* physical layout: super-cells of 2^level hash cells per axis, 0..3 */
+    int32_t hashDigitBits;    /* radix digit width of the hash sort: 0 (balanced) or 5..8 */
+    int32_t layoutDigitBits;  /* radix digit width of the layout sort: 0 (balanced) or 5..8 */
+} akua_tuning;
+
+akua_status akua_solver_set_tuning(akua_solver* s, const akua_tuning* t);
+akua_status akua_solver_get_tuning(const akua_solver* s, akua_tuning* t);
+/* Times short trials of candidate settings by coordinate descent (block, then layout level, then
+ * the two digit widths) and keeps the fastest.  Each trial restarts from a copy of the current
+ * particle state, which is restored afterwards, so the call leaves the state as it found it.
+ * boxMin/boxMax: the box the trials step in (NULL: each scene's current box).  trialSteps: timed
+ * steps per candidate (1..1000).  cachePath (NULL: no cache): a text file of earlier results keyed
+ * by device architecture, log2 of the particle count, log2 of the scene count, neighbour cap and
+ * iteration count; a hit applies the cached setting without trials, a miss appends the result.
+ * best (may be NULL) receives the setting now in use. */
+akua_status akua_solver_autotune(akua_solver* s, float deltaTime, const float boxMin[3],
+                                 const float boxMax[3], int32_t trialSteps, const char* cachePath,
+                                 akua_tuning* best);
+
 /* ---- batched scenes ---------------------------------------------------------------------------- */
 /* Many independent scenes in one solver: every kernel of a step covers all scenes in one launch
  * (each 256-slot chunk of particle slots belongs to one scene), and the whole batched step is one

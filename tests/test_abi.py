@@ -91,6 +91,15 @@ def test_null_pointers_are_rejected(lib):
     assert lib.akua_solver_destroy(None) == 0
 
 
+def test_tuning_struct_and_null_arguments(lib):
+    from akuaengine_amd import _lib
+    assert ctypes.sizeof(_lib.AkuaTuning) == 16
+    t = _lib.AkuaTuning(256, 1, 0, 0)
+    assert lib.akua_solver_set_tuning(None, ctypes.byref(t)) == 1
+    assert lib.akua_solver_get_tuning(None, ctypes.byref(t)) == 1
+    assert lib.akua_solver_autotune(None, ctypes.c_float(0.01), None, None, 5, None, None) == 1
+
+
 def test_no_device_is_reported(lib):
     import torch
     if torch.cuda.device_count() > 0:

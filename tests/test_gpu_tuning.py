@@ -1,0 +1,118 @@
+"""Tuning choices (workgroup size of the neighbour kernels, physical super-cell size, radix digit
+widths) change only speed: every setting yields bit-identical trajectories, and the autotuner
+leaves the particle state as it found it."""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+DT = 0.0083
+SETTINGS = [
+    dict(block=256, layoutLevel=1, hashDigitBits=0, layoutDigitBits=0),
+    dict(block=128, layoutLevel=0, hashDigitBits=5, layoutDigitBits=5),
+    dict(block=512, layoutLevel=2, hashDigitBits=8, layoutDigitBits=8),
+    dict(block=256, layoutLevel=3, hashDigitBits=6, layoutDigitBits=0),
+]
+
+
+@pytest.fixture(scope="module")
+def classes():
+    import torch
+    if torch.cuda.device_count() < 1:
+        pytest.fail("no GPU visible: the gpu tests need an MI355X")
+    from akuaengine_amd.solver import BatchSolver, PBFSolver
+    return BatchSolver, PBFSolver
+
+
+def _run(PBFSolver, sc, cfg, corr, steps, tuning=None):
+    s = PBFSolver(sc.n, cfg, corr)
+    if tuning:
+        s.set_tuning(**tuning)
+    s.set_state(sc.pos, sc.vel)
+    s.step_n(DT, sc.box_min, sc.box_max, steps)
+    return s
+
+
+@pytest.mark.parametrize("tuning", SETTINGS[1:], ids=lambda t: "b{block}-L{layoutLevel}-h{hashDigitBits}-l{layoutDigitBits}".format(**t))
+def test_every_setting_gives_identical_trajectories(classes, dam_params, tuning):
+    _, PBFSolver = classes
+    from akuaengine_amd import scenes
+    cfg, corr = dam_params
+    sc = scenes.dam_break_27k()
+    want = _run(PBFSolver, sc, cfg, corr, 20).get_state()
+    s = _run(PBFSolver, sc, cfg, corr, 20, tuning)
+    assert s.get_tuning() == tuning
+    got = s.get_state()
+    for k in ("pos", "vel", "density", "lam", "ids"):
+        np.testing.assert_array_equal(got[k], want[k], err_msg=k)
+
+
+def test_setting_changes_between_steps(classes, dam_params):
+    _, PBFSolver = classes
+    from akuaengine_amd import scenes
+    cfg, corr = dam_params
+    sc = scenes.dam_break_scaled(12)
+    want = _run(PBFSolver, sc, cfg, corr, 6).get_state()
+    s = PBFSolver(sc.n, cfg, corr)
+    s.set_state(sc.pos, sc.vel)
+    for t in SETTINGS + SETTINGS[:2]:
+        s.set_tuning(**t)
+        s.step(DT, sc.box_min, sc.box_max)
+    got = s.get_state()
+    for k in ("pos", "vel", "ids"):
+        np.testing.assert_array_equal(got[k], want[k], err_msg=k)
+
+
+def test_invalid_settings_are_rejected(classes, dam_params):
+    _, PBFSolver = classes
+    from akuaengine_amd._lib import AkuaError
+    cfg, corr = dam_params
+    s = PBFSolver(1000, cfg, corr)
+    for bad in (dict(block=64), dict(layoutLevel=4), dict(hashDigitBits=4), dict(layoutDigitBits=9)):
+        with pytest.raises(AkuaError):
+            s.set_tuning(**bad)
+    assert s.get_tuning() == SETTINGS[0]
+
+
+def test_autotune_restores_state_and_uses_cache(classes, dam_params, tmp_path):
+    _, PBFSolver = classes
+    from akuaengine_amd import scenes
+    cfg, corr = dam_params
+    sc = scenes.dam_break_27k()
+    s = _run(PBFSolver, sc, cfg, corr, 3)
+    before = s.get_state()
+    cache = str(tmp_path / "tuning.txt")
+    best = s.autotune(DT, sc.box_min, sc.box_max, trialSteps=2, cachePath=cache)
+    assert s.get_tuning() == best
+    after = s.get_state()
+    for k in ("pos", "vel", "density", "lam", "ids"):
+        np.testing.assert_array_equal(after[k], before[k], err_msg=k)
+    s.step_n(DT, sc.box_min, sc.box_max, 4)
+    want = _run(PBFSolver, sc, cfg, corr, 7).get_state()
+    got = s.get_state()
+    for k in ("pos", "vel", "ids"):
+        np.testing.assert_array_equal(got[k], want[k], err_msg=k)
+    lines = open(cache).read().splitlines()
+    assert len(lines) == 1 and " | " in lines[0]
+    s2 = _run(PBFSolver, sc, cfg, corr, 1)
+    assert s2.autotune(DT, trialSteps=2, cachePath=cache) == best
+    assert open(cache).read().splitlines() == lines  # a hit appends nothing
+
+
+def test_batch_settings_are_invariant(classes, dam_params):
+    BatchSolver, _ = classes
+    from akuaengine_amd import scenes
+    cfg, corr = dam_params
+    counts = [300, 4097, 2000, 9000]
+    scs = [scenes.batched_scene(n, i, seed=9) for i, n in enumerate(counts)]
+    out = []
+    for t in (SETTINGS[0], SETTINGS[2], SETTINGS[1]):
+        b = BatchSolver(counts, cfg, corr)
+        b.set_tuning(**t)
+        b.set_state(np.concatenate([s.pos for s in scs]), np.concatenate([s.vel for s in scs]))
+        for i, s in enumerate(scs):
+            b.set_box(i, s.box_min, s.box_max)
+        b.step_batch(DT, 4)
+        out.append(b.get_state())
+    for o in out[1:]:
+        for k in ("pos", "vel", "ids"):
+            np.testing.assert_array_equal(o[k], out[0][k], err_msg=k)
