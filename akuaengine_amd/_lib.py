@@ -32,8 +32,14 @@ EXPORTED = [
     "akua_solver_density_error", "akua_debug_probe_density", "akua_stream_copy_bandwidth",
     "akua_selftest_arith", "akua_batch_create", "akua_batch_num_scenes", "akua_batch_set_box",
     "akua_batch_step", "akua_batch_density_error", "akua_solver_set_tuning",
-    "akua_solver_get_tuning", "akua_solver_autotune",
+    "akua_solver_get_tuning", "akua_solver_autotune", "akua_solver_set_network",
+    "akua_solver_get_network", "akua_solver_set_learned", "akua_solver_learned_stats",
+    "akua_solver_train", "akua_debug_features", "akua_debug_warm_start", "akua_mlp_forward",
+    "akua_mlp_gradient", "akua_mlp_adam_step",
 ]
+
+MLP_PARAMS = 5544
+MLP_TRAINABLE = 5508
 
 
 class AkuaError(RuntimeError):
@@ -65,6 +71,14 @@ class AkuaTuning(ctypes.Structure):
 
     def as_dict(self):
         return {k: int(getattr(self, k)) for k, _ in self._fields_}
+
+
+class AkuaTrainConfig(ctypes.Structure):
+    """``akua_train_config``: online training of the warm-start network."""
+
+    _fields_ = [("samplesPerStep", ctypes.c_int32), ("minibatch", ctypes.c_int32),
+                ("epochs", ctypes.c_int32), ("learningRate", ctypes.c_float),
+                ("seed", ctypes.c_uint64), ("lossWeights", ctypes.c_float * 4)]
 
 _lib = None
 
@@ -116,6 +130,16 @@ def load() -> ctypes.CDLL:
         "akua_solver_set_tuning": [P, ctypes.POINTER(AkuaTuning)],
         "akua_solver_get_tuning": [P, ctypes.POINTER(AkuaTuning)],
         "akua_solver_autotune": [P, f32, P, P, i32, ctypes.c_char_p, ctypes.POINTER(AkuaTuning)],
+        "akua_solver_set_network": [P, P, i32],
+        "akua_solver_get_network": [P, P, i32],
+        "akua_solver_set_learned": [P, i32, f32, f32],
+        "akua_solver_learned_stats": [P, ctypes.POINTER(ctypes.c_int64), ctypes.POINTER(ctypes.c_int64), P],
+        "akua_solver_train": [P, f32, P, P, i32, ctypes.POINTER(AkuaTrainConfig), P, P],
+        "akua_debug_features": [P, P, P, i32],
+        "akua_debug_warm_start": [P, P, P, P, i32],
+        "akua_mlp_forward": [P, P, i32, P],
+        "akua_mlp_gradient": [P, P, P, i32, P, P, P],
+        "akua_mlp_adam_step": [P, P, P, ctypes.c_int64, f32, P, P, i32, P, P],
     }
     for name, args in sigs.items():
         fn = getattr(L, name)

@@ -21,7 +21,7 @@ LIB = os.path.join(HERE, "libakua.so")
 OBJ = os.path.join(HERE, "_build")
 
 ARCH = os.environ.get("AKUA_OFFLOAD_ARCH", "gfx950")
-SOURCES = ["radix_sort.hip", "pbf_kernels.hip", "aos.hip", "stream_copy.hip", "selftest.hip",
+SOURCES = ["radix_sort.hip", "pbf_kernels.hip", "mlp.hip", "aos.hip", "stream_copy.hip", "selftest.hip",
            "solver.cpp"]
 
 HIPFLAGS = [

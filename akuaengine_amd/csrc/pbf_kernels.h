@@ -68,7 +68,19 @@ struct DeviceState {
     uint32_t* nbr;
     uint32_t* cnt;
     unsigned long long* nbrSum;  // running sum of neighbour counts (statistics)
+    // learned warm start (null when unused)
+    float* feat;               // 16 features per slot at x*_pred
+    float* targ;               // training targets per slot: (x*_final - x*_pred) / h, lambda
+    float4* XsP;               // copy of x*_pred (training)
+    float4* XsW;               // warm-started x*
+    float* lamHat;             // the network's lambda output
+    unsigned int* resid;       // per scene, 4 slots of max(0, rho / rho0 - 1) as float bits:
+                               // 0 at x*_pred, 1 at x*_w, 2 at the last plain density pass
+    uint32_t* gate;            // per scene: 1 when this step falls back to the plain solve
+    uint32_t* fallbacks;       // per scene: running count of fallback steps
 };
+
+constexpr int kMlpFeatures = 16;
 
 constexpr int kBlock = 256;
 
@@ -82,14 +94,29 @@ void launch_gather(hipStream_t st, const DeviceState& d, const SphParams& P, con
 void launch_bucket_phys(hipStream_t st, const DeviceState& d);
 // bs: workgroup size of the neighbour kernels (128, 256 or 512; results do not depend on it).
 void launch_neighbours(hipStream_t st, const DeviceState& d, const SphParams& P, int bs);
+// residSlot >= 0: also the per-scene max of max(0, C) into d.resid[scene * 4 + residSlot].
+// gated: only scenes whose d.gate is set run (the fallback's iterations).
 void launch_density_lambda(hipStream_t st, const DeviceState& d, const SphParams& P,
-                           const float4* Xs, int bs);
+                           const float4* Xs, int bs, int residSlot = -1, bool gated = false);
 // Position correction with the wall clamp.  final == true also runs the velocity update and the
 // wall damping and writes X, V, id, lid and phys (the last iteration); applyDelta == false skips
 // the correction (a step with zero iterations).
+// gate == 1: only scenes whose d.gate is set run.  gate == 2 (final only): scenes whose d.gate
+// is clear read XsAlt instead of XsIn (the learned step's final correction).
 void launch_delta_correct(hipStream_t st, const DeviceState& d, const SphParams& P,
                           const StepParams& S, const float4* XsIn, float4* XsOut, bool final,
-                          bool applyDelta, int bs);
+                          bool applyDelta, int bs, int gate = 0, const float4* XsAlt = nullptr);
+// Features of the learned warm start at x*_pred (XsA) into d.feat, the per-scene residual at
+// x*_pred into resid slot 0, and with record also x*_pred into d.XsP.
+void launch_features(hipStream_t st, const DeviceState& d, const SphParams& P, bool record, int bs);
+// Fallback decision per scene: gate = max C at x*_w > ratio * max C at x*_pred + tolerance.
+void launch_gate(hipStream_t st, const DeviceState& d, uint32_t nscenes, float ratio,
+                 float tolerance);
+// Training targets of every slot after a recorded plain step.
+void launch_targets(hipStream_t st, const DeviceState& d, const SphParams& P);
+// Gathers count random samples (slots of scene 0) of features and targets.
+void launch_sample(hipStream_t st, const DeviceState& d, uint64_t seed, uint32_t count,
+                   float* featOut, float* targOut);
 void launch_vorticity(hipStream_t st, const DeviceState& d, const SphParams& P, int bs);
 void launch_confinement(hipStream_t st, const DeviceState& d, const SphParams& P,
                         const StepParams& S, int bs);

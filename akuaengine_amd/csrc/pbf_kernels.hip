@@ -338,35 +338,157 @@ __global__ __launch_bounds__(BS) void k_neighbours(DeviceState d, SphParams P) {
 // kernel_calculate_densities (ConstraintSolverCUDA.cu:16-42) and kernel_calculate_lambdas
 // (ConstraintSolverCUDA.cu:51-97) in one neighbour sweep.  Each sum keeps the reference's order
 // (the list order), so fusing the loops changes no rounding.
+// Residual contribution of one particle: max(0, C), with a non-finite C counted as infinite.
+__device__ __forceinline__ float positive_part(float C) {
+    return C > 0.0f ? C : (C == C ? 0.0f : __builtin_inff());
+}
+
+// Wave-wide max of non-negative values, one atomic per wave (float bits order as unsigned).
+__device__ __forceinline__ void wave_max_atomic(unsigned int* dst, float v) {
+    for (int off = 32; off > 0; off >>= 1) v = fmaxf(v, __shfl_xor(v, off, 64));
+    if (__lane_id() == 0 && v > 0.0f) atomicMax(dst, __float_as_uint(v));
+}
+
 template <int BS>
 __global__ __launch_bounds__(BS) void k_density_lambda(DeviceState d, SphParams P,
-                                                           const float4* __restrict__ Xs) {
+                                                           const float4* __restrict__ Xs,
+                                                           int residSlot, int gated) {
     const uint32_t i = xcd_slot<BS>();
+    const uint32_t i0 = i & ~63u;
+    if (i0 >= d.nslots) return;  // the whole wave is past the last slot
+    const uint32_t sIdx = d.chunkScene[i0 >> 8];  // a wave never spans two scenes
+    if (gated && !d.gate[sIdx]) return;
+    const bool valid = i < d.scenes[sIdx].end;
+    float C = 0.0f;
+    if (valid) {
+        const float4 xi4 = Xs[i];
+        const f3 xi = make3(xi4);
+        const uint32_t n = d.cnt[i];
+        const uint32_t* lp = d.nbr + list_base(i, P.maxN);
+        float rho = xi4.w * P.w0;
+        f3 gi{0.0f, 0.0f, 0.0f};
+        float sum = 0.0f;
+        const float negInv = -P.invRest;
+        walk_list<float4>(
+            lp, n, [&](uint32_t j) { return Xs[j]; },
+            [&](const float4& xj4) {
+                f3 sep = xi - make3(xj4);
+                float d2 = dot3(sep, sep);
+                rho = rho + xj4.w * poly6(d2, P);
+                f3 g = grad_spiky(sep, P);
+                gi = gi + xj4.w * g;
+                f3 gj = (negInv * xj4.w) * g;
+                sum = sum + dot3(gj, gj);
+            });
+        gi = gi * P.invRest;
+        C = rho * P.invRest - 1.0f;
+        d.dens[i] = rho;
+        d.lam[i] = -C / ((sum + dot3(gi, gi)) + P.relax);
+    }
+    if (residSlot >= 0) wave_max_atomic(&d.resid[sIdx * 4 + residSlot], valid ? positive_part(C) : 0.0f);
+}
+
+// Features of the learned warm start at x*_pred, one neighbour sweep (list order):
+//   0 C = rho / rho0 - 1 and 1 lambda, as the first Jacobi iteration computes them
+//   2 neighbour count, 3 lambda of the previous step
+//   4-6 (x*_pred - x) / h
+//   7-9 2 lambda (1/rho0) sum_j m_j grad W_ij / h: the correction if every neighbour had this
+//       particle's lambda
+//   10-12 (1/rho0) sum_j m_j s_corr,ij grad W_ij / h: the artificial-pressure correction
+//   13-15 wall proximity per axis, clamp((x - lo) / h, 0, 2) - clamp((hi - x) / h, 0, 2)
+template <int BS>
+__global__ __launch_bounds__(BS) void k_features(DeviceState d, SphParams P, int record) {
+    const uint32_t i = xcd_slot<BS>();
+    const uint32_t i0 = i & ~63u;
+    if (i0 >= d.nslots) return;
+    const uint32_t sIdx = d.chunkScene[i0 >> 8];
+    const SceneDev& sc = d.scenes[sIdx];
+    const bool valid = i < sc.end;
+    float C = 0.0f;
+    if (valid) {
+        const float4 xi4 = d.XsA[i];
+        const f3 xi = make3(xi4);
+        const uint32_t n = d.cnt[i];
+        const uint32_t* lp = d.nbr + list_base(i, P.maxN);
+        float rho = xi4.w * P.w0;
+        f3 gi{0.0f, 0.0f, 0.0f}, ac{0.0f, 0.0f, 0.0f};
+        float sum = 0.0f;
+        const float negInv = -P.invRest;
+        const Recip Rdq = recip(P.wdq);
+        walk_list<float4>(
+            lp, n, [&](uint32_t j) { return d.XsA[j]; },
+            [&](const float4& xj4) {
+                f3 sep = xi - make3(xj4);
+                float d2 = dot3(sep, sep);
+                const float w = poly6(d2, P);
+                rho = rho + xj4.w * w;
+                f3 g = grad_spiky(sep, P);
+                gi = gi + xj4.w * g;
+                f3 gj = (negInv * xj4.w) * g;
+                sum = sum + dot3(gj, gj);
+                const float lc = P.negCorrK * corr_pow(div_rn(w, Rdq), P);
+                ac = ac + (lc * xj4.w) * g;
+            });
+        gi = gi * P.invRest;
+        C = rho * P.invRest - 1.0f;
+        const float lam1 = -C / ((sum + dot3(gi, gi)) + P.relax);
+        const float invH = 1.0f / P.h;
+        const float4 x0 = d.X0[i];
+        const f3 sg = gi * (2.0f * lam1 * invH);
+        const f3 sa = ac * (P.invRest * invH);
+        float wall[3];
+        const float xa[3] = {xi.x, xi.y, xi.z};
+#pragma unroll
+        for (int a = 0; a < 3; ++a) {
+            const float lo = fminf(fmaxf((xa[a] - sc.bmin[a]) * invH, 0.0f), 2.0f);
+            const float hi = fminf(fmaxf((sc.bmax[a] - xa[a]) * invH, 0.0f), 2.0f);
+            wall[a] = lo - hi;
+        }
+        float4* f = reinterpret_cast<float4*>(d.feat + (size_t)i * kMlpFeatures);
+        f[0] = make_float4(C, lam1, (float)n, d.Tlam[i]);
+        f[1] = make_float4((xi.x - x0.x) * invH, (xi.y - x0.y) * invH, (xi.z - x0.z) * invH, sg.x);
+        f[2] = make_float4(sg.y, sg.z, sa.x, sa.y);
+        f[3] = make_float4(sa.z, wall[0], wall[1], wall[2]);
+        if (record) d.XsP[i] = xi4;
+    }
+    wave_max_atomic(&d.resid[sIdx * 4], valid ? positive_part(C) : 0.0f);
+}
+
+__global__ void k_gate(DeviceState d, uint32_t nscenes, float ratio, float tolerance) {
+    const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= nscenes) return;
+    const float rp = __uint_as_float(d.resid[s * 4]);
+    const float rw = __uint_as_float(d.resid[s * 4 + 1]);
+    const uint32_t g = (rw <= ratio * rp + tolerance) ? 0u : 1u;
+    d.gate[s] = g;
+    d.fallbacks[s] += g;
+}
+
+__global__ void k_targets(DeviceState d, SphParams P) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= d.nslots) return;
     if (i >= scene_of(d, i).end) return;
-    const float4 xi4 = Xs[i];
-    const f3 xi = make3(xi4);
-    const uint32_t n = d.cnt[i];
-    const uint32_t* lp = d.nbr + list_base(i, P.maxN);
-    float rho = xi4.w * P.w0;
-    f3 gi{0.0f, 0.0f, 0.0f};
-    float sum = 0.0f;
-    const float negInv = -P.invRest;
-    walk_list<float4>(
-        lp, n, [&](uint32_t j) { return Xs[j]; },
-        [&](const float4& xj4) {
-            f3 sep = xi - make3(xj4);
-            float d2 = dot3(sep, sep);
-            rho = rho + xj4.w * poly6(d2, P);
-            f3 g = grad_spiky(sep, P);
-            gi = gi + xj4.w * g;
-            f3 gj = (negInv * xj4.w) * g;
-            sum = sum + dot3(gj, gj);
-        });
-    gi = gi * P.invRest;
-    float C = rho * P.invRest - 1.0f;
-    d.dens[i] = rho;
-    d.lam[i] = -C / ((sum + dot3(gi, gi)) + P.relax);
+    const float invH = 1.0f / P.h;
+    const float4 x = d.X[i], p = d.XsP[i];
+    reinterpret_cast<float4*>(d.targ)[i] =
+        make_float4((x.x - p.x) * invH, (x.y - p.y) * invH, (x.z - p.z) * invH, d.lam[i]);
+}
+
+__global__ void k_sample(DeviceState d, uint64_t seed, uint32_t count, float* __restrict__ fo,
+                         float* __restrict__ to) {
+    const uint32_t b = blockIdx.x * blockDim.x + threadIdx.x;
+    if (b >= count) return;
+    const SceneDev& sc = d.scenes[0];
+    uint64_t z = seed + 0x9E3779B97F4A7C15ull * (uint64_t)(b + 1);
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    z ^= z >> 31;
+    const uint32_t slot = sc.begin + (uint32_t)(z % (uint64_t)(sc.end - sc.begin));
+    const float4* f = reinterpret_cast<const float4*>(d.feat + (size_t)slot * kMlpFeatures);
+    float4* g = reinterpret_cast<float4*>(fo + (size_t)b * kMlpFeatures);
+#pragma unroll
+    for (int k = 0; k < 4; ++k) g[k] = f[k];
+    reinterpret_cast<float4*>(to)[b] = reinterpret_cast<const float4*>(d.targ)[slot];
 }
 
 // resolve_collision over the six walls in order (IntegrationCUDA.cu:51-102), restitution 0 and
@@ -400,15 +522,21 @@ __device__ __forceinline__ f3 damp_walls(f3 x, f3 v, const SceneDev& sc) {
 // kernel reads XsIn and writes XsOut, so every particle sees the positions of the same iteration.
 // kFinal: also kernel_update_position_and_velocity (IntegrationCUDA.cu:38-49) and the wall
 // damping, writing the step's positions X, velocities V, ids and the two order maps.
+// gateMode 1: only scenes whose gate is set run; 2: scenes whose gate is clear read XsAlt.
 template <int BS, bool kFinal, bool kApplyDelta>
 __global__ __launch_bounds__(BS) void k_delta_correct(DeviceState d, SphParams P,
                                                           StepParams S,
-                                                          const float4* __restrict__ XsIn,
-                                                          float4* __restrict__ XsOut) {
+                                                          const float4* __restrict__ XsSet,
+                                                          float4* __restrict__ XsOut,
+                                                          const float4* __restrict__ XsAlt,
+                                                          int gateMode) {
     const uint32_t i = xcd_slot<BS>();
     if (i >= d.nslots) return;
-    const SceneDev& sc = scene_of(d, i);
+    const uint32_t sIdx = d.chunkScene[i >> 8];
+    const SceneDev& sc = d.scenes[sIdx];
     if (i >= sc.end) return;
+    if (gateMode == 1 && !d.gate[sIdx]) return;
+    const float4* __restrict__ XsIn = (gateMode == 2 && !d.gate[sIdx]) ? XsAlt : XsSet;
     const float4 xi4 = XsIn[i];
     const f3 xi = make3(xi4);
     f3 q = xi;
@@ -642,25 +770,41 @@ void launch_neighbours(hipStream_t st, const DeviceState& d, const SphParams& P,
     AKUA_BY_BLOCK(bs, (k_neighbours<BS><<<grid_xcd(d.nslots, BS), BS, 0, st>>>(d, P)));
 }
 void launch_density_lambda(hipStream_t st, const DeviceState& d, const SphParams& P,
-                           const float4* Xs, int bs) {
-    AKUA_BY_BLOCK(bs, (k_density_lambda<BS><<<grid_xcd(d.nslots, BS), BS, 0, st>>>(d, P, Xs)));
+                           const float4* Xs, int bs, int residSlot, bool gated) {
+    AKUA_BY_BLOCK(bs, (k_density_lambda<BS><<<grid_xcd(d.nslots, BS), BS, 0, st>>>(
+                          d, P, Xs, residSlot, gated ? 1 : 0)));
+}
+void launch_features(hipStream_t st, const DeviceState& d, const SphParams& P, bool record, int bs) {
+    AKUA_BY_BLOCK(bs, (k_features<BS><<<grid_xcd(d.nslots, BS), BS, 0, st>>>(d, P, record ? 1 : 0)));
+}
+void launch_gate(hipStream_t st, const DeviceState& d, uint32_t nscenes, float ratio,
+                 float tolerance) {
+    k_gate<<<(nscenes + 255) / 256, 256, 0, st>>>(d, nscenes, ratio, tolerance);
+}
+void launch_targets(hipStream_t st, const DeviceState& d, const SphParams& P) {
+    k_targets<<<grid(d.nslots), kBlock, 0, st>>>(d, P);
+}
+void launch_sample(hipStream_t st, const DeviceState& d, uint64_t seed, uint32_t count,
+                   float* featOut, float* targOut) {
+    if (count) k_sample<<<(count + 255) / 256, 256, 0, st>>>(d, seed, count, featOut, targOut);
 }
 template <int BS>
 void delta_correct_bs(hipStream_t st, const DeviceState& d, const SphParams& P,
                       const StepParams& S, const float4* XsIn, float4* XsOut, bool final,
-                      bool applyDelta) {
+                      bool applyDelta, int gate, const float4* XsAlt) {
     const unsigned g = grid_xcd(d.nslots, BS);
     if (final && applyDelta)
-        k_delta_correct<BS, true, true><<<g, BS, 0, st>>>(d, P, S, XsIn, XsOut);
+        k_delta_correct<BS, true, true><<<g, BS, 0, st>>>(d, P, S, XsIn, XsOut, XsAlt, gate);
     else if (final)
-        k_delta_correct<BS, true, false><<<g, BS, 0, st>>>(d, P, S, XsIn, XsOut);
+        k_delta_correct<BS, true, false><<<g, BS, 0, st>>>(d, P, S, XsIn, XsOut, XsAlt, gate);
     else
-        k_delta_correct<BS, false, true><<<g, BS, 0, st>>>(d, P, S, XsIn, XsOut);
+        k_delta_correct<BS, false, true><<<g, BS, 0, st>>>(d, P, S, XsIn, XsOut, XsAlt, gate);
 }
 void launch_delta_correct(hipStream_t st, const DeviceState& d, const SphParams& P,
                           const StepParams& S, const float4* XsIn, float4* XsOut, bool final,
-                          bool applyDelta, int bs) {
-    AKUA_BY_BLOCK(bs, (delta_correct_bs<BS>(st, d, P, S, XsIn, XsOut, final, applyDelta)));
+                          bool applyDelta, int bs, int gate, const float4* XsAlt) {
+    AKUA_BY_BLOCK(bs, (delta_correct_bs<BS>(st, d, P, S, XsIn, XsOut, final, applyDelta, gate,
+                                            XsAlt)));
 }
 void launch_vorticity(hipStream_t st, const DeviceState& d, const SphParams& P, int bs) {
     AKUA_BY_BLOCK(bs, (k_vorticity<BS><<<grid_xcd(d.nslots, BS), BS, 0, st>>>(d, P)));

@@ -19,6 +19,7 @@
 
 #include "../../include/akua.h"
 #include "aos.h"
+#include "mlp.h"
 #include "pbf_kernels.h"
 #include "pbf_types.h"
 #include "radix_sort.h"
@@ -106,9 +107,12 @@ struct DevBuf {
         n = count;
         if (count) HIP_CHECK(hipMalloc(&p, count * sizeof(T)));
     }
-    ~DevBuf() {
+    void release() {
         if (p) (void)hipFree(p);
+        p = nullptr;
+        n = 0;
     }
+    ~DevBuf() { release(); }
 };
 
 struct TimedLaunch {
@@ -155,12 +159,27 @@ struct akua_solver {
     DevBuf<double> probeSum;
     bool hasState = false;
 
+    // learned warm start
+    int mode = 0;                  // 0: plain solver, 1: learned warm start with fallback
+    float gateRatio = 1.0f, gateTol = 0.01f;
+    bool haveNet = false;
+    DevBuf<float> net, feat, targ, lamHat;
+    DevBuf<float4> XsP, XsW;
+    DevBuf<unsigned int> resid;
+    DevBuf<uint32_t> gate, fallbacks;
+    int64_t learnedSteps = 0;
+    // training
+    DevBuf<float> adamM, adamV, part, batchF, batchT, lossBuf;
+    uint32_t batchCap = 0, lossCap = 0;
+    int64_t adamT = 0;
+
     // step graph
     bool useGraph = true;
     hipGraphExec_t graphExec = nullptr;
     StepParams graphParams{};
     int graphPlan2 = -1;
     akua_tuning graphTuning{0, 0, 0, 0};
+    bool graphDirty = false;
     uint64_t stepsSinceStats = 0;
 
     // timing
@@ -190,6 +209,8 @@ struct akua_solver {
         d.perm2 = r2 ? s2VB.p : s2VA.p;
         d.tkey = tkey.p; d.tinfo = tinfo.p; d.bslot = bslot.p; d.tableTotal = tableTotal;
         d.nbr = nbr.p; d.cnt = cnt.p; d.nbrSum = nbrSum.p;
+        d.feat = feat.p; d.targ = targ.p; d.XsP = XsP.p; d.XsW = XsW.p; d.lamHat = lamHat.p;
+        d.resid = resid.p; d.gate = gate.p; d.fallbacks = fallbacks.p;
         return d;
     }
 
@@ -421,7 +442,11 @@ void sort_layout(akua_solver* s, hipStream_t st, bool timed) {
 }
 
 // PBFSolver::step (PBFSolver.cpp:22-78) as one stream-ordered sequence of launches.
-void enqueue_step(akua_solver* s, hipStream_t st, const StepParams& S, bool timed) {
+// record: also the features and training targets of the step (a plain step).  In learned mode
+// the step is the warm start, one Jacobi iteration and, per scene, the plain iterations again
+// from x*_pred when the residual check fails.
+void enqueue_step(akua_solver* s, hipStream_t st, const StepParams& S, bool timed,
+                  bool record = false) {
     Launcher L{s, st, timed};
     DeviceState d = s->devState();
     const SphParams& P = s->sph;
@@ -447,7 +472,23 @@ void enqueue_step(akua_solver* s, hipStream_t st, const StepParams& S, bool time
     launch_neighbours(st, d, P, bs);
     L.after("neighbours");
     const int I = s->cfg.solverIterations;
+    const bool learned = s->mode == 1 && !record;
     float4* xs[2] = {s->XsA.p, s->XsB.p};
+    if (learned || record) {
+        launch_fill_u32(st, s->resid.p, (uint32_t)(4 * s->scenes.size()), 0u);
+        L.before("features");
+        launch_features(st, d, P, record, bs);
+        L.after("features");
+    }
+    if (learned) {
+        L.before("mlp_apply");
+        launch_mlp_apply(st, d, s->net.p, P.h, s->XsA.p, s->XsW.p, s->lamHat.p);
+        L.after("mlp_apply");
+        L.before("density_lambda_warm");
+        launch_density_lambda(st, d, P, s->XsW.p, bs, 1, false);
+        L.after("density_lambda_warm");
+        launch_gate(st, d, (uint32_t)s->scenes.size(), s->gateRatio, s->gateTol);
+    }
     if (I == 0) {
         L.before("update_damping");
         launch_delta_correct(st, d, P, S, xs[0], nullptr, true, false, bs);
@@ -456,13 +497,22 @@ void enqueue_step(akua_solver* s, hipStream_t st, const StepParams& S, bool time
     for (int it = 0; it < I; ++it) {
         const float4* in = xs[it & 1];
         float4* out = xs[(it + 1) & 1];
-        L.before("density_lambda");
-        launch_density_lambda(st, d, P, in, bs);
-        L.after("density_lambda");
         const bool last = it + 1 == I;
-        L.before(last ? "delta_correct_update" : "delta_correct");
-        launch_delta_correct(st, d, P, S, in, out, last, true, bs);
-        L.after(last ? "delta_correct_update" : "delta_correct");
+        L.before(learned ? "density_lambda_fallback" : "density_lambda");
+        launch_density_lambda(st, d, P, in, bs, record && last ? 2 : -1, learned);
+        L.after(learned ? "density_lambda_fallback" : "density_lambda");
+        const char* name = last ? "delta_correct_update" : learned ? "delta_correct_fallback" : "delta_correct";
+        L.before(name);
+        // learned: the fallback's iterations run only where the gate is set; the final
+        // correction reads x*_w where it is clear
+        launch_delta_correct(st, d, P, S, in, out, last, true, bs, learned ? (last ? 2 : 1) : 0,
+                             s->XsW.p);
+        L.after(name);
+    }
+    if (record) {
+        L.before("targets");
+        launch_targets(st, d, P);
+        L.after("targets");
     }
     L.before("vorticity");
     launch_vorticity(st, d, P, bs);
@@ -538,7 +588,8 @@ void run_steps(akua_solver* s, float dt, int nsteps) {
             enqueue_step(s, s->stream, S, true);
             collect_timing(s);
         } else if (s->useGraph) {
-            bool same = s->graphExec && std::memcmp(&S, &s->graphParams, sizeof(S)) == 0 &&
+            bool same = s->graphExec && !s->graphDirty &&
+                        std::memcmp(&S, &s->graphParams, sizeof(S)) == 0 &&
                         s->graphPlan2 == s->plan2.bits &&
                         std::memcmp(&s->graphTuning, &s->tuning, sizeof(akua_tuning)) == 0;
             if (!same) {
@@ -556,6 +607,7 @@ void run_steps(akua_solver* s, float dt, int nsteps) {
                 s->graphParams = S;
                 s->graphPlan2 = s->plan2.bits;
                 s->graphTuning = s->tuning;
+                s->graphDirty = false;
             }
             HIP_CHECK(hipGraphLaunch(s->graphExec, s->stream));
         } else {
@@ -563,6 +615,7 @@ void run_steps(akua_solver* s, float dt, int nsteps) {
         }
         HIP_CHECK(hipGetLastError());
         s->stepsSinceStats++;
+        if (s->mode == 1) s->learnedSteps++;
     }
 }
 
@@ -766,6 +819,216 @@ void cache_store(const char* path, const std::string& key, const akua_tuning& t)
     if (!f) fail(AKUA_ERR_INVALID_ARG, std::string("cannot append to the tuning cache ") + path);
     f << key << " | " << t.block << ' ' << t.layoutLevel << ' ' << t.hashDigitBits << ' '
       << t.layoutDigitBits << '\n';
+}
+
+// ---- learned warm start ----------------------------------------------------------------------
+
+void ensure_learned(akua_solver* s) {
+    if (s->net.p) return;
+    const size_t ns = s->nslots, nsc = s->scenes.size();
+    s->net.alloc(kMlpParams);
+    s->feat.alloc(ns * kMlpFeatures);
+    s->lamHat.alloc(ns);
+    s->XsW.alloc(ns);
+    s->resid.alloc(4 * nsc);
+    s->gate.alloc(nsc);
+    s->fallbacks.alloc(nsc);
+    HIP_CHECK(hipMemset(s->net.p, 0, kMlpParams * sizeof(float)));
+    HIP_CHECK(hipMemset(s->feat.p, 0, ns * kMlpFeatures * sizeof(float)));
+    HIP_CHECK(hipMemset(s->lamHat.p, 0, ns * sizeof(float)));
+    HIP_CHECK(hipMemset(s->XsW.p, 0, ns * sizeof(float4)));
+    HIP_CHECK(hipMemset(s->resid.p, 0, 4 * nsc * sizeof(unsigned int)));
+    HIP_CHECK(hipMemset(s->gate.p, 0, nsc * sizeof(uint32_t)));
+    HIP_CHECK(hipMemset(s->fallbacks.p, 0, nsc * sizeof(uint32_t)));
+    s->graphDirty = true;
+}
+
+void reset_adam(akua_solver* s) {
+    s->adamT = 0;
+    if (s->adamM.p) {
+        HIP_CHECK(hipMemset(s->adamM.p, 0, kMlpTrainable * sizeof(float)));
+        HIP_CHECK(hipMemset(s->adamV.p, 0, kMlpTrainable * sizeof(float)));
+    }
+}
+
+void ensure_train(akua_solver* s, uint32_t batch, uint32_t losses) {
+    ensure_learned(s);
+    const size_t ns = s->nslots;
+    if (!s->XsP.p) {
+        s->XsP.alloc(ns);
+        s->targ.alloc(ns * 4);
+        s->adamM.alloc(kMlpTrainable);
+        s->adamV.alloc(kMlpTrainable);
+        s->part.alloc((size_t)kMlpPartRows * kMlpPartRow);
+        HIP_CHECK(hipMemset(s->XsP.p, 0, ns * sizeof(float4)));
+        HIP_CHECK(hipMemset(s->targ.p, 0, ns * 4 * sizeof(float)));
+        HIP_CHECK(hipMemset(s->part.p, 0, (size_t)kMlpPartRows * kMlpPartRow * sizeof(float)));
+        reset_adam(s);
+    }
+    if (batch > s->batchCap) {
+        s->batchF.release();
+        s->batchT.release();
+        s->batchF.alloc((size_t)batch * kMlpFeatures);
+        s->batchT.alloc((size_t)batch * 4);
+        s->batchCap = batch;
+    }
+    if (losses > s->lossCap) {
+        s->lossBuf.release();
+        s->lossBuf.alloc(losses);
+        s->lossCap = losses;
+    }
+}
+
+void check_network(const float* p) {
+    for (int k = 0; k < kMlpParams; ++k)
+        if (!std::isfinite(p[k])) fail(AKUA_ERR_NON_FINITE, "network parameter " + std::to_string(k) + " is not finite");
+    for (int k = 0; k < kMlpOut; ++k)
+        if (p[kOffYmul + k] == 0.0f) fail(AKUA_ERR_INVALID_ARG, "network output scale is zero");
+}
+
+void upload_network(akua_solver* s, const float* p) {
+    check_network(p);
+    ensure_learned(s);
+    HIP_CHECK(hipStreamSynchronize(s->stream));
+    HIP_CHECK(hipMemcpy(s->net.p, p, kMlpParams * sizeof(float), hipMemcpyHostToDevice));
+    s->haveNet = true;
+    reset_adam(s);
+}
+
+// splitmix64 stream and a normal deviate by Box-Muller.
+struct HostRng {
+    uint64_t z;
+    uint64_t next() {
+        uint64_t x = (z += 0x9E3779B97F4A7C15ull);
+        x = (x ^ (x >> 30)) * 0xBF58476D1CE4E5B9ull;
+        x = (x ^ (x >> 27)) * 0x94D049BB133111EBull;
+        return x ^ (x >> 31);
+    }
+    double uniform() { return ((next() >> 11) + 0.5) * (1.0 / 9007199254740992.0); }
+    double normal() { return std::sqrt(-2.0 * std::log(uniform())) * std::cos(6.283185307179586 * uniform()); }
+};
+
+// He-normal weights, zero biases, the last layer scaled down so that the untrained network's
+// warm start is close to none; normalisation from the first batch of samples.
+std::vector<float> initial_network(uint64_t seed, const std::vector<float>& F,
+                                   const std::vector<float>& T, uint32_t count) {
+    std::vector<float> p(kMlpParams, 0.0f);
+    HostRng rng{seed ^ 0x6A09E667F3BCC909ull};
+    for (int k = 0; k < kMlpIn * kMlpHid; ++k) p[kOffW1 + k] = (float)(rng.normal() * std::sqrt(2.0 / kMlpIn));
+    for (int k = 0; k < kMlpHid * kMlpHid; ++k) p[kOffW2 + k] = (float)(rng.normal() * std::sqrt(2.0 / kMlpHid));
+    for (int k = 0; k < kMlpHid * kMlpOut; ++k) p[kOffW3 + k] = (float)(rng.normal() * 0.1 * std::sqrt(1.0 / kMlpHid));
+    for (int f = 0; f < kMlpIn; ++f) {
+        double m = 0.0, q = 0.0;
+        for (uint32_t b = 0; b < count; ++b) m += F[(size_t)b * kMlpIn + f];
+        m /= count;
+        for (uint32_t b = 0; b < count; ++b) {
+            const double x = F[(size_t)b * kMlpIn + f] - m;
+            q += x * x;
+        }
+        const double sd = std::sqrt(q / count);
+        p[kOffMean + f] = (float)m;
+        p[kOffInv + f] = sd > 1e-30 && std::isfinite(sd) ? (float)(1.0 / sd) : 1.0f;
+    }
+    for (int o = 0; o < kMlpOut; ++o) {
+        double q = 0.0;
+        for (uint32_t b = 0; b < count; ++b) q += (double)T[(size_t)b * 4 + o] * T[(size_t)b * 4 + o];
+        const double rms = std::sqrt(q / count);
+        p[kOffYmul + o] = rms > 1e-30 && std::isfinite(rms) ? (float)rms : 1.0f;
+    }
+    return p;
+}
+
+void validate_train_config(const akua_train_config& c) {
+    if (c.samplesPerStep < 16 || c.samplesPerStep > (1 << 22))
+        fail(AKUA_ERR_INVALID_ARG, "samplesPerStep must be in [16, 2^22]");
+    if (c.minibatch < 16 || c.minibatch > c.samplesPerStep)
+        fail(AKUA_ERR_INVALID_ARG, "minibatch must be in [16, samplesPerStep]");
+    if (c.epochs < 1 || c.epochs > 64) fail(AKUA_ERR_INVALID_ARG, "epochs must be in [1, 64]");
+    if (!(c.learningRate > 0.0f) || !std::isfinite(c.learningRate))
+        fail(AKUA_ERR_INVALID_ARG, "learningRate must be > 0");
+    for (float w : c.lossWeights)
+        if (!(w >= 0.0f) || !std::isfinite(w)) fail(AKUA_ERR_INVALID_ARG, "loss weights must be >= 0");
+}
+
+// Online training on the solver's own plain trajectory: each step is a plain step that also
+// records features and targets; samplesPerStep random particles of it feed epochs passes of
+// Adam updates over minibatches.  ratio[k] = (max C at the last plain density pass) / (max C at
+// x*_pred) of step k, the plain solver's own residual reduction.
+void train(akua_solver* s, float dt, const float* bmin, const float* bmax, int steps,
+           const akua_train_config& c, float* lossOut, float* ratioOut) {
+    if (s->scenes.size() != 1) fail(AKUA_ERR_UNSUPPORTED, "training runs on a single-scene solver");
+    if (!s->hasState) fail(AKUA_ERR_STATE, "set the particles before training");
+    if (!(dt > 0.0f) || !std::isfinite(dt)) fail(AKUA_ERR_INVALID_ARG, "deltaTime must be > 0");
+    if (steps < 1 || steps > 1000000) fail(AKUA_ERR_INVALID_ARG, "steps must be in [1, 10^6]");
+    if (s->cfg.solverIterations < 1) fail(AKUA_ERR_UNSUPPORTED, "training needs solverIterations >= 1");
+    validate_train_config(c);
+    check_box(bmin, bmax);
+    set_box(s, 0, bmin, bmax);
+    upload_scenes(s);
+    const uint32_t B = (uint32_t)c.samplesPerStep, M = (uint32_t)c.minibatch;
+    const uint32_t nb = B / M;
+    const uint32_t perStep = nb * (uint32_t)c.epochs;
+    ensure_train(s, B, perStep * (uint32_t)steps);
+    hipStream_t st = s->stream;
+    const StepParams S{dt, {s->cfg.gravity[0], s->cfg.gravity[1], s->cfg.gravity[2]}};
+    MlpLossWeights lw{{c.lossWeights[0], c.lossWeights[1], c.lossWeights[2], c.lossWeights[3]}};
+    const float b1 = 0.9f, b2 = 0.999f, eps = 1e-8f;
+    std::vector<unsigned int> res(4);
+    for (int k = 0; k < steps; ++k) {
+        enqueue_step(s, st, S, false, true);
+        s->stepsSinceStats++;
+        DeviceState d = s->devState();
+        launch_sample(st, d, c.seed + 0x2545F4914F6CDD1Dull * (uint64_t)(s->adamT + k + 1), B,
+                      s->batchF.p, s->batchT.p);
+        HIP_CHECK(hipGetLastError());
+        if (!s->haveNet) {
+            std::vector<float> F((size_t)B * kMlpFeatures), T((size_t)B * 4);
+            HIP_CHECK(hipStreamSynchronize(st));
+            HIP_CHECK(hipMemcpy(F.data(), s->batchF.p, F.size() * 4, hipMemcpyDeviceToHost));
+            HIP_CHECK(hipMemcpy(T.data(), s->batchT.p, T.size() * 4, hipMemcpyDeviceToHost));
+            const std::vector<float> p = initial_network(c.seed, F, T, B);
+            upload_network(s, p.data());
+        }
+        for (int e = 0; e < c.epochs; ++e) {
+            for (uint32_t mb = 0; mb < nb; ++mb) {
+                launch_mlp_grad(st, s->net.p, s->batchF.p + (size_t)mb * M * kMlpFeatures,
+                                s->batchT.p + (size_t)mb * M * 4, M, lw, s->part.p);
+                s->adamT++;
+                const float c1 = (float)(1.0 / (1.0 - std::pow(0.9, (double)s->adamT)));
+                const float c2 = (float)(1.0 / (1.0 - std::pow(0.999, (double)s->adamT)));
+                launch_mlp_adam(st, s->part.p, s->net.p, s->adamM.p, s->adamV.p, c.learningRate,
+                                b1, b2, eps, c1, c2,
+                                s->lossBuf.p + (size_t)k * perStep + (size_t)e * nb + mb);
+            }
+        }
+        HIP_CHECK(hipGetLastError());
+        HIP_CHECK(hipStreamSynchronize(st));
+        HIP_CHECK(hipMemcpy(res.data(), s->resid.p, 4 * sizeof(unsigned int), hipMemcpyDeviceToHost));
+        float rp, r3;
+        std::memcpy(&rp, &res[0], 4);
+        std::memcpy(&r3, &res[2], 4);
+        if (ratioOut) ratioOut[k] = rp > 0.0f ? r3 / rp : 0.0f;
+    }
+    std::vector<float> L((size_t)perStep * steps);
+    HIP_CHECK(hipMemcpy(L.data(), s->lossBuf.p, L.size() * 4, hipMemcpyDeviceToHost));
+    if (lossOut)
+        for (int k = 0; k < steps; ++k) {
+            double a = 0.0;
+            for (uint32_t u = 0; u < perStep; ++u) a += L[(size_t)k * perStep + u];
+            lossOut[k] = (float)(a / perStep);
+        }
+}
+
+void need_device() {
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1)
+        fail(AKUA_ERR_NO_DEVICE, "no HIP device is visible");
+}
+
+template <class T>
+void to_device(DevBuf<T>& b, const T* h, size_t n) {
+    b.alloc(n);
+    if (n) HIP_CHECK(hipMemcpy(b.p, h, n * sizeof(T), hipMemcpyHostToDevice));
 }
 
 void autotune(akua_solver* s, float dt, const float* bmin, const float* bmax, int trialSteps,
@@ -1114,6 +1377,174 @@ akua_status akua_solver_autotune(akua_solver* s, float dt, const float boxMin[3]
         if (!s) fail(AKUA_ERR_INVALID_ARG, "NULL solver");
         autotune(s, dt, boxMin, boxMax, trialSteps, cachePath);
         if (best) *best = s->tuning;
+    });
+}
+
+akua_status akua_solver_set_network(akua_solver* s, const float* params, int32_t count) {
+    return guarded([&] {
+        if (!s || !params) fail(AKUA_ERR_INVALID_ARG, "NULL argument");
+        if (count != kMlpParams) fail(AKUA_ERR_INVALID_ARG, "count must be AKUA_MLP_PARAMS");
+        upload_network(s, params);
+    });
+}
+
+akua_status akua_solver_get_network(akua_solver* s, float* params, int32_t count) {
+    return guarded([&] {
+        if (!s || !params) fail(AKUA_ERR_INVALID_ARG, "NULL argument");
+        if (count != kMlpParams) fail(AKUA_ERR_INVALID_ARG, "count must be AKUA_MLP_PARAMS");
+        if (!s->haveNet) fail(AKUA_ERR_STATE, "the solver has no network");
+        auto h = download(s, s->net.p, kMlpParams);
+        std::memcpy(params, h.data(), kMlpParams * sizeof(float));
+    });
+}
+
+akua_status akua_solver_set_learned(akua_solver* s, int32_t enable, float ratio, float tolerance) {
+    return guarded([&] {
+        if (!s) fail(AKUA_ERR_INVALID_ARG, "NULL solver");
+        if (enable) {
+            if (!s->haveNet) fail(AKUA_ERR_STATE, "set or train a network first");
+            if (s->cfg.solverIterations < 1)
+                fail(AKUA_ERR_UNSUPPORTED, "the learned step needs solverIterations >= 1");
+            if (!std::isfinite(ratio) || !std::isfinite(tolerance) || ratio < 0.0f)
+                fail(AKUA_ERR_INVALID_ARG, "ratio must be >= 0 and tolerance finite");
+        }
+        s->mode = enable ? 1 : 0;
+        s->gateRatio = ratio;
+        s->gateTol = tolerance;
+        s->graphDirty = true;
+    });
+}
+
+akua_status akua_solver_learned_stats(akua_solver* s, int64_t* learnedSteps, int64_t* fallbacks,
+                                      float* residual) {
+    return guarded([&] {
+        if (!s) fail(AKUA_ERR_INVALID_ARG, "NULL solver");
+        int64_t fb = 0;
+        float r[4] = {0.0f, 0.0f, 0.0f, 0.0f};
+        if (s->fallbacks.p) {
+            auto f = download(s, s->fallbacks.p, s->scenes.size());
+            for (auto v : f) fb += v;
+            auto q = download(s, s->resid.p, 4);
+            std::memcpy(r, q.data(), sizeof(r));
+        }
+        if (learnedSteps) *learnedSteps = s->learnedSteps;
+        if (fallbacks) *fallbacks = fb;
+        if (residual) std::memcpy(residual, r, 3 * sizeof(float));
+    });
+}
+
+akua_status akua_solver_train(akua_solver* s, float dt, const float boxMin[3], const float boxMax[3],
+                              int32_t steps, const akua_train_config* config, float* loss,
+                              float* residualRatio) {
+    return guarded([&] {
+        if (!s || !config || !boxMin || !boxMax) fail(AKUA_ERR_INVALID_ARG, "NULL argument");
+        train(s, dt, boxMin, boxMax, steps, *config, loss, residualRatio);
+    });
+}
+
+akua_status akua_debug_features(akua_solver* s, float* features, float* targets, int32_t n) {
+    return guarded([&] {
+        if (!s) fail(AKUA_ERR_INVALID_ARG, "NULL solver");
+        check_n(s, n);
+        if (!s->feat.p) fail(AKUA_ERR_STATE, "no learned or recorded step has run");
+        if (targets && !s->targ.p) fail(AKUA_ERR_STATE, "no recorded step has run");
+        auto lid = download(s, s->lid.p, s->nslots);
+        auto F = download(s, s->feat.p, (size_t)s->nslots * kMlpFeatures);
+        std::vector<float> T;
+        if (targets) T = download(s, s->targ.p, (size_t)s->nslots * 4);
+        for (const SceneDev& sc : s->scenes)
+            for (uint32_t p = sc.begin; p < sc.end; ++p) {
+                const uint32_t k = lid[p];
+                if (features) std::memcpy(features + (size_t)k * kMlpFeatures, &F[(size_t)p * kMlpFeatures], kMlpFeatures * 4);
+                if (targets) std::memcpy(targets + (size_t)k * 4, &T[(size_t)p * 4], 16);
+            }
+    });
+}
+
+akua_status akua_debug_warm_start(akua_solver* s, float* xPred, float* xWarm, float* lambdaHat,
+                                  int32_t n) {
+    return guarded([&] {
+        if (!s) fail(AKUA_ERR_INVALID_ARG, "NULL solver");
+        check_n(s, n);
+        if (!s->XsW.p) fail(AKUA_ERR_STATE, "no learned step has run");
+        auto lid = download(s, s->lid.p, s->nslots);
+        auto A = download(s, s->XsA.p, s->nslots);
+        auto Wm = download(s, s->XsW.p, s->nslots);
+        auto L = download(s, s->lamHat.p, s->nslots);
+        for (const SceneDev& sc : s->scenes)
+            for (uint32_t p = sc.begin; p < sc.end; ++p) {
+                const uint32_t k = lid[p];
+                if (xPred) { xPred[3 * k] = A[p].x; xPred[3 * k + 1] = A[p].y; xPred[3 * k + 2] = A[p].z; }
+                if (xWarm) { xWarm[3 * k] = Wm[p].x; xWarm[3 * k + 1] = Wm[p].y; xWarm[3 * k + 2] = Wm[p].z; }
+                if (lambdaHat) lambdaHat[k] = L[p];
+            }
+    });
+}
+
+akua_status akua_mlp_forward(const float* params, const float* features, int32_t n, float* outputs) {
+    return guarded([&] {
+        if (!params || !features || !outputs || n < 1) fail(AKUA_ERR_INVALID_ARG, "NULL argument or n < 1");
+        need_device();
+        DevBuf<float> P, F, Y;
+        to_device(P, params, kMlpParams);
+        to_device(F, features, (size_t)n * kMlpFeatures);
+        Y.alloc((size_t)n * 4);
+        launch_mlp_eval(nullptr, P.p, F.p, (uint32_t)n, Y.p);
+        HIP_CHECK(hipGetLastError());
+        HIP_CHECK(hipDeviceSynchronize());
+        HIP_CHECK(hipMemcpy(outputs, Y.p, (size_t)n * 16, hipMemcpyDeviceToHost));
+    });
+}
+
+akua_status akua_mlp_gradient(const float* params, const float* features, const float* targets,
+                              int32_t n, const float lossWeights[4], float* grad, float* loss) {
+    return guarded([&] {
+        if (!params || !features || !targets || !lossWeights || !grad || !loss || n < 1)
+            fail(AKUA_ERR_INVALID_ARG, "NULL argument or n < 1");
+        need_device();
+        DevBuf<float> P, F, T, part, G, L;
+        to_device(P, params, kMlpParams);
+        to_device(F, features, (size_t)n * kMlpFeatures);
+        to_device(T, targets, (size_t)n * 4);
+        part.alloc((size_t)kMlpPartRows * kMlpPartRow);
+        G.alloc(kMlpTrainable);
+        L.alloc(1);
+        MlpLossWeights lw{{lossWeights[0], lossWeights[1], lossWeights[2], lossWeights[3]}};
+        launch_mlp_grad(nullptr, P.p, F.p, T.p, (uint32_t)n, lw, part.p);
+        launch_mlp_reduce(nullptr, part.p, G.p, L.p);
+        HIP_CHECK(hipGetLastError());
+        HIP_CHECK(hipDeviceSynchronize());
+        HIP_CHECK(hipMemcpy(grad, G.p, kMlpTrainable * sizeof(float), hipMemcpyDeviceToHost));
+        HIP_CHECK(hipMemcpy(loss, L.p, sizeof(float), hipMemcpyDeviceToHost));
+    });
+}
+
+akua_status akua_mlp_adam_step(float* params, float* m, float* v, int64_t t, float lr,
+                               const float* features, const float* targets, int32_t n,
+                               const float lossWeights[4], float* loss) {
+    return guarded([&] {
+        if (!params || !m || !v || !features || !targets || !lossWeights || !loss || n < 1 || t < 1)
+            fail(AKUA_ERR_INVALID_ARG, "NULL argument, n < 1 or t < 1");
+        need_device();
+        DevBuf<float> P, M, V, F, T, part, L;
+        to_device(P, params, kMlpParams);
+        to_device(M, m, kMlpTrainable);
+        to_device(V, v, kMlpTrainable);
+        to_device(F, features, (size_t)n * kMlpFeatures);
+        to_device(T, targets, (size_t)n * 4);
+        part.alloc((size_t)kMlpPartRows * kMlpPartRow);
+        L.alloc(1);
+        MlpLossWeights lw{{lossWeights[0], lossWeights[1], lossWeights[2], lossWeights[3]}};
+        const float c1 = (float)(1.0 / (1.0 - std::pow(0.9, (double)t)));
+        const float c2 = (float)(1.0 / (1.0 - std::pow(0.999, (double)t)));
+        launch_mlp_grad(nullptr, P.p, F.p, T.p, (uint32_t)n, lw, part.p);
+        launch_mlp_adam(nullptr, part.p, P.p, M.p, V.p, lr, 0.9f, 0.999f, 1e-8f, c1, c2, L.p);
+        HIP_CHECK(hipGetLastError());
+        HIP_CHECK(hipDeviceSynchronize());
+        HIP_CHECK(hipMemcpy(params, P.p, kMlpParams * sizeof(float), hipMemcpyDeviceToHost));
+        HIP_CHECK(hipMemcpy(m, M.p, kMlpTrainable * sizeof(float), hipMemcpyDeviceToHost));
+        HIP_CHECK(hipMemcpy(v, V.p, kMlpTrainable * sizeof(float), hipMemcpyDeviceToHost));
+        HIP_CHECK(hipMemcpy(loss, L.p, sizeof(float), hipMemcpyDeviceToHost));
     });
 }
 

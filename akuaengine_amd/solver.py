@@ -139,6 +139,57 @@ class PBFSolver:
                                                 int(trialSteps), path, ctypes.byref(t)))
         return t.as_dict()
 
+    # -- learned warm start -------------------------------------------------------------------
+    def set_network(self, params: np.ndarray) -> None:
+        p = np.ascontiguousarray(params, np.float32).reshape(_lib.MLP_PARAMS)
+        _lib.check(self._L.akua_solver_set_network(self._h, _p(p), _lib.MLP_PARAMS))
+
+    def get_network(self) -> np.ndarray:
+        p = np.zeros(_lib.MLP_PARAMS, np.float32)
+        _lib.check(self._L.akua_solver_get_network(self._h, _p(p), _lib.MLP_PARAMS))
+        return p
+
+    def set_learned(self, enable: bool, ratio: float = 1.0, tolerance: float = 0.01) -> None:
+        """Learned steps: warm start + one iteration, per-scene fallback to the plain solve when
+        max C(x*_w) > ratio * max C(x*_pred) + tolerance."""
+        _lib.check(self._L.akua_solver_set_learned(self._h, 1 if enable else 0, float(ratio),
+                                                   float(tolerance)))
+
+    def learned_stats(self) -> Dict[str, object]:
+        steps, fb = ctypes.c_int64(), ctypes.c_int64()
+        r = np.zeros(3, np.float32)
+        _lib.check(self._L.akua_solver_learned_stats(self._h, ctypes.byref(steps), ctypes.byref(fb), _p(r)))
+        return dict(steps=int(steps.value), fallbacks=int(fb.value),
+                    resid_pred=float(r[0]), resid_warm=float(r[1]), resid_plain=float(r[2]))
+
+    def train(self, deltaTime: float, boxMin, boxMax, steps: int, samplesPerStep: int = 65536,
+              minibatch: int = 8192, epochs: int = 1, learningRate: float = 1e-3, seed: int = 1,
+              lossWeights=(1.0, 1.0, 1.0, 0.25)):
+        """Online training on this scene's plain trajectory; returns (loss per step, residual
+        ratio per step)."""
+        c = _lib.AkuaTrainConfig()
+        c.samplesPerStep, c.minibatch, c.epochs = int(samplesPerStep), int(minibatch), int(epochs)
+        c.learningRate, c.seed = float(learningRate), int(seed)
+        for k in range(4):
+            c.lossWeights[k] = float(lossWeights[k])
+        loss = np.zeros(int(steps), np.float32)
+        ratio = np.zeros(int(steps), np.float32)
+        _lib.check(self._L.akua_solver_train(self._h, float(deltaTime), _p(_f3(boxMin)), _p(_f3(boxMax)),
+                                             int(steps), ctypes.byref(c), _p(loss), _p(ratio)))
+        return loss, ratio
+
+    def debug_features(self, targets: bool = False):
+        F = np.zeros((self.numParticles, 16), np.float32)
+        T = np.zeros((self.numParticles, 4), np.float32) if targets else None
+        _lib.check(self._L.akua_debug_features(self._h, _p(F), _p(T), self.numParticles))
+        return (F, T) if targets else F
+
+    def debug_warm_start(self):
+        n = self.numParticles
+        xp, xw, lh = np.zeros((n, 3), np.float32), np.zeros((n, 3), np.float32), np.zeros(n, np.float32)
+        _lib.check(self._L.akua_debug_warm_start(self._h, _p(xp), _p(xw), _p(lh), n))
+        return xp, xw, lh
+
     # -- parity taps --------------------------------------------------------------------------
     def debug_hashes(self) -> np.ndarray:
         out = np.zeros(self.numParticles, np.uint32)

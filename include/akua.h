@@ -160,6 +160,73 @@ akua_status akua_solver_autotune(akua_solver* s, float deltaTime, const float bo
                                  const float boxMax[3], int32_t trialSteps, const char* cachePath,
                                  akua_tuning* best);
 
+/* ---- learned warm start ------------------------------------------------------------------------ */
+/* A per-particle network (16 features -> 64 -> 64 -> 4, ReLU, BF16 weights and activations on the
+ * matrix cores, FP32 accumulation) predicts the position correction of the plain solve from
+ * features at the predicted positions x*_pred.  A learned step moves x* to the prediction x*_w,
+ * runs one Jacobi iteration from there, and per scene falls back to the plain solve from x*_pred
+ * (all solverIterations iterations, bit-identical to a plain step) when
+ *     max_i max(0, rho_i(x*_w) / rho0 - 1)  >  ratio * max_i max(0, rho_i(x*_pred) / rho0 - 1) + tolerance.
+ * No reference counterpart (the reference always runs solverIterations iterations,
+ * ConstraintSolverCUDA.cu:173-222).
+ *
+ * Parameter blob (AKUA_MLP_PARAMS floats): W1[16][64] b1[64] W2[64][64] b2[64] W3[64][4] b3[4]
+ * (weights input-major), then fmean[16] finv[16] (features enter as (f - fmean) * finv) and
+ * ymul[4] (outputs leave multiplied by ymul).  Features, per particle at x*_pred:
+ *   0 rho / rho0 - 1, 1 the first iteration's lambda, 2 neighbour count, 3 previous lambda,
+ *   4-6 (x*_pred - x) / h, 7-9 2 lambda (1/rho0) sum_j m_j grad W_ij / h,
+ *   10-12 (1/rho0) sum_j m_j s_corr,ij grad W_ij / h,
+ *   13-15 per axis clamp((x - lo) / h, 0, 2) - clamp((hi - x) / h, 0, 2).
+ * Outputs: (x*_final - x*_pred) / h of the plain solve (3), and its last lambda (1). */
+#define AKUA_MLP_PARAMS 5544
+#define AKUA_MLP_TRAINABLE 5508
+
+typedef struct akua_train_config {
+    int32_t samplesPerStep;   /* random particles recorded per plain step */
+    int32_t minibatch;        /* samples per Adam update */
+    int32_t epochs;           /* passes over each step's samples */
+    float learningRate;       /* Adam, beta1 0.9, beta2 0.999, eps 1e-8 */
+    uint64_t seed;            /* sampling and (for a solver without a network) initial weights */
+    float lossWeights[4];     /* per output, on outputs normalised by ymul */
+} akua_train_config;
+
+akua_status akua_solver_set_network(akua_solver* s, const float* params, int32_t count);
+akua_status akua_solver_get_network(akua_solver* s, float* params, int32_t count);
+/* enable 1: learned steps with the fallback rule above; 0: plain steps. */
+akua_status akua_solver_set_learned(akua_solver* s, int32_t enable, float ratio, float tolerance);
+/* Steps run in learned mode, scene-steps that fell back, and the last step's residuals of scene 0
+ * (residual[0] at x*_pred, [1] at x*_w, [2] at the last plain density pass of a recorded step). */
+akua_status akua_solver_learned_stats(akua_solver* s, int64_t* learnedSteps, int64_t* fallbacks,
+                                      float residual[3]);
+/* Trains the solver's network on its own plain trajectory (single-scene solvers): `steps` plain
+ * steps (the state advances exactly as akua_solver_step would), each recording features and
+ * targets of config->samplesPerStep random particles and running config->epochs passes of Adam
+ * over them in minibatches.  A solver without a network starts from He-normal weights and takes
+ * the feature and target scales from the first step's samples.  loss[k] (may be NULL): mean
+ * minibatch loss of step k; residualRatio[k] (may be NULL): residual at the last plain density
+ * pass over residual at x*_pred. */
+akua_status akua_solver_train(akua_solver* s, float deltaTime, const float boxMin[3],
+                              const float boxMax[3], int32_t steps, const akua_train_config* config,
+                              float* loss, float* residualRatio);
+/* Features (16 per particle) and, after a training step, targets (4 per particle) of the last
+ * learned or recorded step, by logical index (the order of akua_solver_get_state). */
+akua_status akua_debug_features(akua_solver* s, float* features, float* targets, int32_t n);
+/* x*_pred, x*_w (3 floats per particle) and the network's lambda output of the last learned step,
+ * by logical index.  x*_pred is only valid when the step did not fall back. */
+akua_status akua_debug_warm_start(akua_solver* s, float* xPred, float* xWarm, float* lambdaHat,
+                                  int32_t n);
+/* Device evaluation of the network on n feature rows (outputs: 4 per row, denormalised). */
+akua_status akua_mlp_forward(const float* params, const float* features, int32_t n, float* outputs);
+/* Gradient (AKUA_MLP_TRAINABLE floats) and value of the training loss
+ * (1/n) sum_rows sum_o lossWeights[o] (y_o - targets_o / ymul_o)^2, y the normalised outputs. */
+akua_status akua_mlp_gradient(const float* params, const float* features, const float* targets,
+                              int32_t n, const float lossWeights[4], float* grad, float* loss);
+/* One Adam step (t >= 1 is the step number for the bias corrections) on the loss over all n rows;
+ * params (AKUA_MLP_PARAMS), m and v (AKUA_MLP_TRAINABLE each) are updated in place. */
+akua_status akua_mlp_adam_step(float* params, float* m, float* v, int64_t t, float learningRate,
+                               const float* features, const float* targets, int32_t n,
+                               const float lossWeights[4], float* loss);
+
 /* ---- batched scenes ---------------------------------------------------------------------------- */
 /* Many independent scenes in one solver: every kernel of a step covers all scenes in one launch
  * (each 256-slot chunk of particle slots belongs to one scene), and the whole batched step is one

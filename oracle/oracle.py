@@ -60,6 +60,8 @@ def lib():
         L.orc_ncells.argtypes = [vp]
         L.orc_get_taps.argtypes = [vp, P, P, P, P, P, P, P]
         L.orc_true_density.argtypes = [vp, P, P]
+        L.orc_get_features.argtypes = [vp, P]
+        L.orc_set_xs.argtypes = [vp, P]
         L.orc_poly6.restype = d
         L.orc_poly6.argtypes = [ctypes.c_int, d, d]
         L.orc_grad_spiky.argtypes = [ctypes.c_int, P, d, P]
@@ -104,7 +106,7 @@ def cell_hash(cx: int, cy: int, cz: int, table_size: int) -> int:
 
 STAGES = {"predict": 0, "hashes": 1, "sort": 2, "neighbours": 3, "densities": 4, "lambdas": 5,
           "deltas": 6, "correct": 7, "update": 8, "damping": 9, "vorticity": 10,
-          "confinement": 11, "xsph": 12}
+          "confinement": 11, "xsph": 12, "features": 13}
 
 
 class Oracle:
@@ -177,6 +179,17 @@ class Oracle:
                                                            "cell_hash", "cell_start", "cell_end")])
         out["nbr"] = out["nbr"].reshape(n, self.max_neighbours)
         return out
+
+    def features(self) -> np.ndarray:
+        """Features of the last "features" stage: (n, 16), sorted order."""
+        out = np.zeros((self.n, 16))
+        lib().orc_get_features(self._h, _p(out))
+        return out
+
+    def set_xs(self, xs) -> None:
+        """Replaces x* (sorted order): continues a step from a warm start."""
+        a = np.ascontiguousarray(xs, np.float32).reshape(self.n, 3)
+        lib().orc_set_xs(self._h, _p(a))
 
     def true_density(self):
         """Exact SPH density at the current positions: (by sorted slot, by particle id)."""

@@ -20,6 +20,8 @@ struct Handle {
     State<double> sd;
     Taps taps;
     std::vector<uint32_t> preSortHash;
+    std::vector<float> featF;
+    std::vector<double> featD;
 };
 
 template <class T>
@@ -28,6 +30,13 @@ template <>
 State<float>& st<float>(Handle* h) { return h->sf; }
 template <>
 State<double>& st<double>(Handle* h) { return h->sd; }
+
+template <class T>
+std::vector<T>& feat(Handle* h);
+template <>
+std::vector<float>& feat<float>(Handle* h) { return h->featF; }
+template <>
+std::vector<double>& feat<double>(Handle* h) { return h->featD; }
 
 template <class T>
 void set_state(Handle* h, long n, const float* x, const float* v, const float* m, const float* d,
@@ -108,6 +117,7 @@ void run_stage(Handle* h, int stage, float dt, const float* bmin, const float* b
             xsph(s, p, t.nbr, t.cnt, vn);
             break;
         }
+        case 13: features(s, p, t.nbr, t.cnt, feat<T>(h)); break;
         default: break;
     }
 }
@@ -178,6 +188,23 @@ void orc_stage(void* hp, int stage, float dt, const float bmin[3], const float b
     auto* h = static_cast<Handle*>(hp);
     if (h->prec == 64) run_stage<double>(h, stage, dt, bmin, bmax);
     else run_stage<float>(h, stage, dt, bmin, bmax);
+}
+
+// Features of the last "features" stage (16 per particle, sorted order).
+void orc_get_features(void* hp, double* out) {
+    auto* h = static_cast<Handle*>(hp);
+    if (h->prec == 64) std::copy(h->featD.begin(), h->featD.end(), out);
+    else for (size_t i = 0; i < h->featF.size(); ++i) out[i] = h->featF[i];
+}
+
+// Replaces x* (sorted order, 3 floats per particle): the learned step's warm start.
+void orc_set_xs(void* hp, const float* xs) {
+    auto* h = static_cast<Handle*>(hp);
+    const long n = (long)(h->prec == 64 ? h->sd.size() : h->sf.size());
+    for (long i = 0; i < n; ++i) {
+        if (h->prec == 64) h->sd.xs[i] = {xs[3 * i], xs[3 * i + 1], xs[3 * i + 2]};
+        else h->sf.xs[i] = {xs[3 * i], xs[3 * i + 1], xs[3 * i + 2]};
+    }
 }
 
 long orc_ncells(void* hp) { return (long)static_cast<Handle*>(hp)->taps.cell_hash.size(); }

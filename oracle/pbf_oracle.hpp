@@ -547,6 +547,64 @@ void xsph(State<T>& s, const Params<T>& p, const std::vector<uint32_t>& nbr,
     s.v.swap(vnew);
 }
 
+// Features of the learned warm start (no reference counterpart), per particle at x* = x*_pred
+// right after the neighbour search, in list order:
+//   0 C = rho / rho0 - 1, 1 lambda (both as densities() + lambdas() compute them), 2 neighbour
+//   count, 3 the lambda carried from the previous step, 4-6 (x* - x) / h,
+//   7-9 2 lambda (1/rho0) sum_j m_j grad W_ij / h, 10-12 (1/rho0) sum_j m_j s_corr,ij grad W_ij / h,
+//   13-15 per axis clamp((x* - lo) / h, 0, 2) - clamp((hi - x*) / h, 0, 2).
+// Pinned by its first two entries equalling densities()/lambdas(), by the closed forms of the
+// others on hand-placed particles, and by the device kernel's independent implementation.
+template <class T>
+void features(const State<T>& s, const Params<T>& p, const std::vector<uint32_t>& nbr,
+              const std::vector<uint32_t>& cnt, std::vector<T>& F) {
+    const long n = (long)s.size();
+    F.assign((size_t)n * 16, T(0));
+    const T dq2 = p.corrDq * p.corrDq;
+    const T wdq = poly6<T>(dq2, p.K);
+#pragma omp parallel for schedule(static)
+    for (long i = 0; i < n; ++i) {
+        const size_t st = (size_t)i * p.maxN;
+        T rho = s.mass[i] * poly6<T>(T(0), p.K);
+        V3<T> gi{T(0), T(0), T(0)}, ac{T(0), T(0), T(0)};
+        T sum = T(0);
+        for (uint32_t o = 0; o < cnt[i]; ++o) {
+            const uint32_t j = nbr[st + o];
+            const V3<T> sep = s.xs[i] - s.xs[j];
+            const T d2 = dot(sep, sep);
+            const T w = poly6<T>(d2, p.K);
+            rho = rho + s.mass[j] * w;
+            const V3<T> g = grad_spiky<T>(sep, p.K);
+            gi = gi + s.mass[j] * g;
+            const V3<T> gj = (-p.invRest * s.mass[j]) * g;
+            sum = sum + dot(gj, gj);
+            const T lc = -p.corrK * ipow<T>(w / wdq, p.corrN);
+            ac = ac + (lc * s.mass[j]) * g;
+        }
+        gi = gi * p.invRest;
+        const T C = rho * p.invRest - T(1);
+        const T lam1 = -C / (sum + dot(gi, gi) + p.relax);
+        const T invH = T(1) / p.h;
+        const V3<T> sg = gi * (T(2) * lam1 * invH);
+        const V3<T> sa = ac * (p.invRest * invH);
+        const T xa[3] = {s.xs[i].x, s.xs[i].y, s.xs[i].z};
+        const T x0[3] = {s.x[i].x, s.x[i].y, s.x[i].z};
+        T* f = &F[(size_t)i * 16];
+        f[0] = C;
+        f[1] = lam1;
+        f[2] = T(cnt[i]);
+        f[3] = s.lambda[i];
+        for (int a = 0; a < 3; ++a) f[4 + a] = (xa[a] - x0[a]) * invH;
+        f[7] = sg.x; f[8] = sg.y; f[9] = sg.z;
+        f[10] = sa.x; f[11] = sa.y; f[12] = sa.z;
+        for (int a = 0; a < 3; ++a) {
+            const T lo = std::min(std::max((xa[a] - p.bmin[a]) * invH, T(0)), T(2));
+            const T hi = std::min(std::max((p.bmax[a] - xa[a]) * invH, T(0)), T(2));
+            f[13 + a] = lo - hi;
+        }
+    }
+}
+
 // One iteration of runConstraintSolverCUDA's loop (ConstraintSolverCUDA.cu:206-218).
 template <class T>
 void constraint_iteration(State<T>& s, const Params<T>& p, const std::vector<uint32_t>& nbr,
