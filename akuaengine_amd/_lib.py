@@ -77,7 +77,9 @@ class AkuaTrainConfig(ctypes.Structure):
     """``akua_train_config``: online training of the warm-start network."""
 
     _fields_ = [("samplesPerStep", ctypes.c_int32), ("minibatch", ctypes.c_int32),
-                ("epochs", ctypes.c_int32), ("learningRate", ctypes.c_float),
+                ("epochs", ctypes.c_int32), ("normSteps", ctypes.c_int32),
+                ("targetIterations", ctypes.c_int32),
+                ("learningRate", ctypes.c_float),
                 ("seed", ctypes.c_uint64), ("lossWeights", ctypes.c_float * 4)]
 
 _lib = None
@@ -132,7 +134,7 @@ def load() -> ctypes.CDLL:
         "akua_solver_autotune": [P, f32, P, P, i32, ctypes.c_char_p, ctypes.POINTER(AkuaTuning)],
         "akua_solver_set_network": [P, P, i32],
         "akua_solver_get_network": [P, P, i32],
-        "akua_solver_set_learned": [P, i32, f32, f32],
+        "akua_solver_set_learned": [P, i32, i32, i32, f32, f32],
         "akua_solver_learned_stats": [P, ctypes.POINTER(ctypes.c_int64), ctypes.POINTER(ctypes.c_int64), P],
         "akua_solver_train": [P, f32, P, P, i32, ctypes.POINTER(AkuaTrainConfig), P, P],
         "akua_debug_features": [P, P, P, i32],

@@ -27,6 +27,10 @@ constexpr int kOffYmul = kOffInv + kMlpIn;           // 5540
 constexpr int kMlpParams = kOffYmul + kMlpOut;       // 5544
 constexpr int kMlpPartRow = kMlpTrainable + 1;       // gradient partial row: params, then loss
 
+// Normalised targets are clipped to [-kMlpTargetClip, kMlpTargetClip] in the loss, so that the
+// rare particles of a splash do not dominate a minibatch.
+constexpr float kMlpTargetClip = 8.0f;
+
 // Loss weights per output in normalised units (three position components, lambda).
 struct MlpLossWeights {
     float w[4];

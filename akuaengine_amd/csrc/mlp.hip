@@ -89,16 +89,6 @@ __device__ void load_weights(const float* __restrict__ P, WeightsLds& W, Weights
     }
 }
 
-// Per-wave activations of inference.
-struct alignas(16) InferLds {
-    uint16_t x[16 * 16];
-    uint16_t h1[16 * kMlpHid], h2[16 * kMlpHid];
-    float y[16 * 16];
-    __device__ uint16_t* xt() { return nullptr; }
-    __device__ uint16_t* h1t() { return nullptr; }
-    __device__ uint16_t* h2t() { return nullptr; }
-};
-
 // Per-wave activations and gradients of training.
 struct alignas(16) ActLds {
     uint16_t x[16 * 16], xt_[16 * 16];
@@ -114,7 +104,7 @@ struct alignas(16) ActLds {
 };
 
 // Normalised BF16 features of samples row0..row0+15 (rows >= n are zero) into A.x (and, for
-// training, its transpose).
+// training, its transpose).  Training path only; inference keeps activations in registers.
 template <bool kTrain, class Act>
 __device__ __forceinline__ void load_features(const float* __restrict__ feat, uint32_t row0,
                                               uint32_t n, const WeightsLds& W, Act& A) {
@@ -189,34 +179,102 @@ __device__ __forceinline__ void forward_tile(const WeightsLds& W, Act& A, uint32
 constexpr int kInferBlock = 256;
 constexpr int kInferWaves = kInferBlock / 64;
 
+__device__ __forceinline__ bf16x4 pack_bf16(float a, float b, float c, float d) {
+    typedef __bf16 bf16x2v __attribute__((ext_vector_type(2)));
+    const bf16x2v lo = {(__bf16)a, (__bf16)b};  // v_cvt_pk_bf16_f32: round to nearest even
+    const bf16x2v hi = {(__bf16)c, (__bf16)d};
+    const uint32_t u0 = __builtin_bit_cast(uint32_t, lo), u1 = __builtin_bit_cast(uint32_t, hi);
+    bf16x4 r;
+    r[0] = (short)(u0 & 0xFFFFu);
+    r[1] = (short)(u0 >> 16);
+    r[2] = (short)(u1 & 0xFFFFu);
+    r[3] = (short)(u1 >> 16);
+    return r;
+}
+
+// Inference in the transposed orientation C[unit][sample] = W^T H^T: lane l = 16 g + r owns
+// sample r; the accumulator of output tile m0 holds units 16 m0 + 4 g + i of that sample, which
+// is exactly the B operand of the next layer's k-step m0, so activations stay in registers.  The
+// weight fragments (A operands) are loaded from LDS into registers once per wave.
+struct InferRegs {
+    bf16x4 w1[4], w2[4][4], w3[4];
+};
+
+__device__ __forceinline__ void load_infer_regs(const WeightsLds& W, InferRegs& R) {
+    const int l = __lane_id();
+    const int r = l & 15, g = l >> 4;
+#pragma unroll
+    for (int m0 = 0; m0 < 4; ++m0) {
+        R.w1[m0] = *reinterpret_cast<const bf16x4*>(W.w1t + (16 * m0 + r) * kMlpIn + 4 * g);
+#pragma unroll
+        for (int kk = 0; kk < 4; ++kk)
+            R.w2[m0][kk] = *reinterpret_cast<const bf16x4*>(W.w2t + (16 * m0 + r) * kMlpHid + 16 * kk + 4 * g);
+        R.w3[m0] = *reinterpret_cast<const bf16x4*>(W.w3t + r * kMlpHid + 16 * m0 + 4 * g);
+    }
+}
+
+// Normalised outputs y[i] = output 4g + i of sample row0 + r (meaningful for g == 0).
+__device__ __forceinline__ f32x4 infer_tile(const WeightsLds& W, const InferRegs& R,
+                                            const float* __restrict__ feat, uint32_t row0,
+                                            uint32_t n) {
+    const int l = __lane_id();
+    const int r = l & 15, g = l >> 4;
+    const uint32_t row = row0 + (uint32_t)r;
+    bf16x4 xb;
+    if (row < n) {
+        const float4 v = *reinterpret_cast<const float4*>(feat + (size_t)row * kMlpIn + 4 * g);
+        xb = pack_bf16((v.x - W.mean[4 * g + 0]) * W.inv[4 * g + 0],
+                       (v.y - W.mean[4 * g + 1]) * W.inv[4 * g + 1],
+                       (v.z - W.mean[4 * g + 2]) * W.inv[4 * g + 2],
+                       (v.w - W.mean[4 * g + 3]) * W.inv[4 * g + 3]);
+    } else {
+        xb = bf16x4{0, 0, 0, 0};
+    }
+    bf16x4 h1[4], h2[4];
+#pragma unroll
+    for (int m0 = 0; m0 < 4; ++m0) {
+        f32x4 c = {0.0f, 0.0f, 0.0f, 0.0f};
+        c = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(R.w1[m0], xb, c, 0, 0, 0);
+        const float* b = W.b1 + 16 * m0 + 4 * g;
+        h1[m0] = pack_bf16(fmaxf(c[0] + b[0], 0.0f), fmaxf(c[1] + b[1], 0.0f),
+                           fmaxf(c[2] + b[2], 0.0f), fmaxf(c[3] + b[3], 0.0f));
+    }
+#pragma unroll
+    for (int m0 = 0; m0 < 4; ++m0) {
+        f32x4 c = {0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+        for (int kk = 0; kk < 4; ++kk)
+            c = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(R.w2[m0][kk], h1[kk], c, 0, 0, 0);
+        const float* b = W.b2 + 16 * m0 + 4 * g;
+        h2[m0] = pack_bf16(fmaxf(c[0] + b[0], 0.0f), fmaxf(c[1] + b[1], 0.0f),
+                           fmaxf(c[2] + b[2], 0.0f), fmaxf(c[3] + b[3], 0.0f));
+    }
+    f32x4 c = {0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+    for (int kk = 0; kk < 4; ++kk)
+        c = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(R.w3[kk], h2[kk], c, 0, 0, 0);
+    const float* b = W.b3 + 4 * g;
+    return f32x4{c[0] + b[0], c[1] + b[1], c[2] + b[2], c[3] + b[3]};
+}
+
 __global__ __launch_bounds__(kInferBlock) void k_mlp_eval(const float* __restrict__ P,
                                                           const float* __restrict__ feat,
                                                           uint32_t n, float* __restrict__ y) {
     __shared__ WeightsLds W;
-    __shared__ InferLds act[kInferWaves];
     load_weights(P, W, nullptr);
     __syncthreads();
-    InferLds& A = act[threadIdx.x / 64];
+    InferRegs R;
+    load_infer_regs(W, R);
     const int l = __lane_id();
     const uint32_t tiles = (n + 15) / 16;
     const uint32_t wave = blockIdx.x * kInferWaves + threadIdx.x / 64;
     const uint32_t nwaves = gridDim.x * kInferWaves;
     for (uint32_t t = wave; t < tiles; t += nwaves) {
-        load_features<false>(feat, t * 16, n, W, A);
-        uint32_t m1, m2;
-        forward_tile<false>(W, A, m1, m2);
-        if (l < 16) {
-            const uint32_t row = t * 16 + (uint32_t)l;
-            if (row < n) {
-                float4 o;
-                o.x = A.y[l * 16 + 0] * W.ymul[0];
-                o.y = A.y[l * 16 + 1] * W.ymul[1];
-                o.z = A.y[l * 16 + 2] * W.ymul[2];
-                o.w = A.y[l * 16 + 3] * W.ymul[3];
-                *reinterpret_cast<float4*>(y + (size_t)row * 4) = o;
-            }
-        }
-        wave_sync();
+        const f32x4 o = infer_tile(W, R, feat, t * 16, n);
+        const uint32_t row = t * 16 + (uint32_t)l;
+        if (l < 16 && row < n)
+            *reinterpret_cast<float4*>(y + (size_t)row * 4) =
+                make_float4(o[0] * W.ymul[0], o[1] * W.ymul[1], o[2] * W.ymul[2], o[3] * W.ymul[3]);
     }
 }
 
@@ -227,31 +285,24 @@ __global__ __launch_bounds__(kInferBlock) void k_mlp_apply(DeviceState d, const 
                                                            float4* __restrict__ XsWarm,
                                                            float* __restrict__ lamHat) {
     __shared__ WeightsLds W;
-    __shared__ InferLds act[kInferWaves];
     load_weights(P, W, nullptr);
     __syncthreads();
-    InferLds& A = act[threadIdx.x / 64];
+    InferRegs R;
+    load_infer_regs(W, R);
     const int l = __lane_id();
     const uint32_t n = d.nslots;
     const uint32_t tiles = (n + 15) / 16;
     const uint32_t wave = blockIdx.x * kInferWaves + threadIdx.x / 64;
     const uint32_t nwaves = gridDim.x * kInferWaves;
     for (uint32_t t = wave; t < tiles; t += nwaves) {
-        load_features<false>(d.feat, t * 16, n, W, A);
-        uint32_t m1, m2;
-        forward_tile<false>(W, A, m1, m2);
-        if (l < 16) {
-            const uint32_t p = t * 16 + (uint32_t)l;
-            if (p < n && p < d.scenes[d.chunkScene[p >> 8]].end) {
-                const float4 x = XsPred[p];
-                const float dx = A.y[l * 16 + 0] * W.ymul[0];
-                const float dy = A.y[l * 16 + 1] * W.ymul[1];
-                const float dz = A.y[l * 16 + 2] * W.ymul[2];
-                XsWarm[p] = make_float4(x.x + h * dx, x.y + h * dy, x.z + h * dz, x.w);
-                lamHat[p] = A.y[l * 16 + 3] * W.ymul[3];
-            }
+        const f32x4 o = infer_tile(W, R, d.feat, t * 16, n);
+        const uint32_t p = t * 16 + (uint32_t)l;
+        if (l < 16 && p < n && p < d.scenes[d.chunkScene[p >> 8]].end) {
+            const float4 x = XsPred[p];
+            const float dx = o[0] * W.ymul[0], dy = o[1] * W.ymul[1], dz = o[2] * W.ymul[2];
+            XsWarm[p] = make_float4(x.x + h * dx, x.y + h * dy, x.z + h * dz, x.w);
+            lamHat[p] = o[3] * W.ymul[3];
         }
-        wave_sync();
     }
 }
 
@@ -296,7 +347,8 @@ __global__ __launch_bounds__(64 * kMlpGradWaves) void k_mlp_grad(const float* __
             const uint32_t row = t * 16 + (uint32_t)s;
             float G = 0.0f;
             if (r < kMlpOut && row < n) {
-                const float tn = targ[(size_t)row * kMlpOut + r] / W.ymul[r];
+                const float tn = fminf(fmaxf(targ[(size_t)row * kMlpOut + r] / W.ymul[r],
+                                             -kMlpTargetClip), kMlpTargetClip);
                 const float diff = A.y[s * 16 + r] - tn;
                 loss += lw.w[r] * diff * diff;
                 G = 2.0f * lw.w[r] * diff * invN;

@@ -74,13 +74,19 @@ struct DeviceState {
     float4* XsP;               // copy of x*_pred (training)
     float4* XsW;               // warm-started x*
     float* lamHat;             // the network's lambda output
-    unsigned int* resid;       // per scene, 4 slots of max(0, rho / rho0 - 1) as float bits:
-                               // 0 at x*_pred, 1 at x*_w, 2 at the last plain density pass
+    unsigned int* resid;       // per scene, kResidStride words: max over particles of
+                               // max(0, rho / rho0 - 1) as float bits at [k], and its sum as a
+                               // float at [kResidSum + k], for k = 0 at x*_pred, 1 at x*_w, 2 at
+                               // the last plain density pass
+    float2* residPart;         // per check (3) and wave: (max, sum) of max(0, C) over its lanes
+    uint32_t nWaves;           // waves of 64 slots: ceil(nslots / 64)
     uint32_t* gate;            // per scene: 1 when this step falls back to the plain solve
     uint32_t* fallbacks;       // per scene: running count of fallback steps
 };
 
 constexpr int kMlpFeatures = 16;
+constexpr int kResidStride = 8;
+constexpr int kResidSum = 4;
 
 constexpr int kBlock = 256;
 
@@ -94,7 +100,7 @@ void launch_gather(hipStream_t st, const DeviceState& d, const SphParams& P, con
 void launch_bucket_phys(hipStream_t st, const DeviceState& d);
 // bs: workgroup size of the neighbour kernels (128, 256 or 512; results do not depend on it).
 void launch_neighbours(hipStream_t st, const DeviceState& d, const SphParams& P, int bs);
-// residSlot >= 0: also the per-scene max of max(0, C) into d.resid[scene * 4 + residSlot].
+// residSlot >= 0: also the per-scene max and sum of max(0, C) into check residSlot of d.resid.
 // gated: only scenes whose d.gate is set run (the fallback's iterations).
 void launch_density_lambda(hipStream_t st, const DeviceState& d, const SphParams& P,
                            const float4* Xs, int bs, int residSlot = -1, bool gated = false);
@@ -109,11 +115,15 @@ void launch_delta_correct(hipStream_t st, const DeviceState& d, const SphParams&
 // Features of the learned warm start at x*_pred (XsA) into d.feat, the per-scene residual at
 // x*_pred into resid slot 0, and with record also x*_pred into d.XsP.
 void launch_features(hipStream_t st, const DeviceState& d, const SphParams& P, bool record, int bs);
-// Fallback decision per scene: gate = max C at x*_w > ratio * max C at x*_pred + tolerance.
-void launch_gate(hipStream_t st, const DeviceState& d, uint32_t nscenes, float ratio,
-                 float tolerance);
-// Training targets of every slot after a recorded plain step.
-void launch_targets(hipStream_t st, const DeviceState& d, const SphParams& P);
+// Fallback decision per scene: gate = r(x*_w) > ratio * r(x*_pred) + tolerance, r the max
+// (statistic 0) or the mean (statistic 1) over particles of max(0, rho / rho0 - 1).
+void launch_gate(hipStream_t st, const DeviceState& d, uint32_t nscenes, int statistic,
+                 float ratio, float tolerance);
+// Per-scene max and sum of the residual checks in slotMask (bit k: check k) into d.resid.
+void launch_resid_reduce(hipStream_t st, const DeviceState& d, uint32_t nscenes, int slotMask);
+// Training targets of every slot after a recorded plain step; XsLast holds x* before the last
+// iteration.
+void launch_targets(hipStream_t st, const DeviceState& d, const SphParams& P, const float4* XsLast);
 // Gathers count random samples (slots of scene 0) of features and targets.
 void launch_sample(hipStream_t st, const DeviceState& d, uint64_t seed, uint32_t count,
                    float* featOut, float* targOut);

@@ -343,10 +343,17 @@ __device__ __forceinline__ float positive_part(float C) {
     return C > 0.0f ? C : (C == C ? 0.0f : __builtin_inff());
 }
 
-// Wave-wide max of non-negative values, one atomic per wave (float bits order as unsigned).
-__device__ __forceinline__ void wave_max_atomic(unsigned int* dst, float v) {
-    for (int off = 32; off > 0; off >>= 1) v = fmaxf(v, __shfl_xor(v, off, 64));
-    if (__lane_id() == 0 && v > 0.0f) atomicMax(dst, __float_as_uint(v));
+// Residual statistics of one check: the wave's max and sum of non-negative values, written as
+// the wave's partial (no atomics; k_resid_reduce combines the partials of each scene in a fixed
+// order).  Every wave with a slot below nslots writes its partial.
+__device__ __forceinline__ void wave_resid_partial(const DeviceState& d, int slot, uint32_t i,
+                                                   float v) {
+    float m = v, t = v;
+    for (int off = 32; off > 0; off >>= 1) {
+        m = fmaxf(m, __shfl_xor(m, off, 64));
+        t += __shfl_xor(t, off, 64);
+    }
+    if (__lane_id() == 0) d.residPart[(size_t)slot * d.nWaves + (i >> 6)] = make_float2(m, t);
 }
 
 template <int BS>
@@ -385,7 +392,7 @@ __global__ __launch_bounds__(BS) void k_density_lambda(DeviceState d, SphParams 
         d.dens[i] = rho;
         d.lam[i] = -C / ((sum + dot3(gi, gi)) + P.relax);
     }
-    if (residSlot >= 0) wave_max_atomic(&d.resid[sIdx * 4 + residSlot], valid ? positive_part(C) : 0.0f);
+    if (residSlot >= 0) wave_resid_partial(d, residSlot, i, valid ? positive_part(C) : 0.0f);
 }
 
 // Features of the learned warm start at x*_pred, one neighbour sweep (list order):
@@ -451,25 +458,69 @@ __global__ __launch_bounds__(BS) void k_features(DeviceState d, SphParams P, int
         f[3] = make_float4(sa.z, wall[0], wall[1], wall[2]);
         if (record) d.XsP[i] = xi4;
     }
-    wave_max_atomic(&d.resid[sIdx * 4], valid ? positive_part(C) : 0.0f);
+    wave_resid_partial(d, 0, i, valid ? positive_part(C) : 0.0f);
 }
 
-__global__ void k_gate(DeviceState d, uint32_t nscenes, float ratio, float tolerance) {
-    const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
-    if (s >= nscenes) return;
-    const float rp = __uint_as_float(d.resid[s * 4]);
-    const float rw = __uint_as_float(d.resid[s * 4 + 1]);
-    const uint32_t g = (rw <= ratio * rp + tolerance) ? 0u : 1u;
-    d.gate[s] = g;
-    d.fallbacks[s] += g;
+// One workgroup per scene: max and sum over the scene's wave partials of each check in
+// slotMask (fixed reduction order), into the scene's record; with gate, also the fallback
+// decision from checks 0 (x*_pred) and 1 (x*_w).
+constexpr int kReduceBlock = 256;
+__global__ __launch_bounds__(kReduceBlock) void k_resid_reduce(DeviceState d, int slotMask,
+                                                               int gate, int statistic, float ratio,
+                                                               float tolerance) {
+    __shared__ float smax[kReduceBlock], ssum[kReduceBlock];
+    const uint32_t s = blockIdx.x;
+    const SceneDev& sc = d.scenes[s];
+    const uint32_t w0 = sc.begin >> 6, w1 = (sc.end + 63) >> 6;
+    unsigned int* rec = d.resid + s * kResidStride;
+    for (int k = 0; k < 3; ++k) {
+        if (!((slotMask >> k) & 1)) continue;
+        float m = 0.0f, t = 0.0f;
+        for (uint32_t w = w0 + threadIdx.x; w < w1; w += kReduceBlock) {
+            const float2 p = d.residPart[(size_t)k * d.nWaves + w];
+            m = fmaxf(m, p.x);
+            t += p.y;
+        }
+        smax[threadIdx.x] = m;
+        ssum[threadIdx.x] = t;
+        __syncthreads();
+        for (int off = kReduceBlock / 2; off > 0; off >>= 1) {
+            if ((int)threadIdx.x < off) {
+                smax[threadIdx.x] = fmaxf(smax[threadIdx.x], smax[threadIdx.x + off]);
+                ssum[threadIdx.x] += ssum[threadIdx.x + off];
+            }
+            __syncthreads();
+        }
+        if (threadIdx.x == 0) {
+            rec[k] = __float_as_uint(smax[0]);
+            rec[kResidSum + k] = __float_as_uint(ssum[0]);
+        }
+        __syncthreads();
+    }
+    if (gate && threadIdx.x == 0) {
+        float rp, rw;
+        if (statistic == 0) {
+            rp = __uint_as_float(rec[0]);
+            rw = __uint_as_float(rec[1]);
+        } else {
+            const float inv = 1.0f / (float)max(1u, sc.end - sc.begin);
+            rp = __uint_as_float(rec[kResidSum]) * inv;
+            rw = __uint_as_float(rec[kResidSum + 1]) * inv;
+        }
+        const uint32_t g = (rw <= ratio * rp + tolerance) ? 0u : 1u;
+        d.gate[s] = g;
+        d.fallbacks[s] += g;
+    }
 }
 
-__global__ void k_targets(DeviceState d, SphParams P) {
+// Targets: x* before the last Jacobi iteration (XsLast) relative to x*_pred, in units of h, and
+// the last iteration's lambda.
+__global__ void k_targets(DeviceState d, SphParams P, const float4* __restrict__ XsLast) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= d.nslots) return;
     if (i >= scene_of(d, i).end) return;
     const float invH = 1.0f / P.h;
-    const float4 x = d.X[i], p = d.XsP[i];
+    const float4 x = XsLast[i], p = d.XsP[i];
     reinterpret_cast<float4*>(d.targ)[i] =
         make_float4((x.x - p.x) * invH, (x.y - p.y) * invH, (x.z - p.z) * invH, d.lam[i]);
 }
@@ -777,12 +828,15 @@ void launch_density_lambda(hipStream_t st, const DeviceState& d, const SphParams
 void launch_features(hipStream_t st, const DeviceState& d, const SphParams& P, bool record, int bs) {
     AKUA_BY_BLOCK(bs, (k_features<BS><<<grid_xcd(d.nslots, BS), BS, 0, st>>>(d, P, record ? 1 : 0)));
 }
-void launch_gate(hipStream_t st, const DeviceState& d, uint32_t nscenes, float ratio,
-                 float tolerance) {
-    k_gate<<<(nscenes + 255) / 256, 256, 0, st>>>(d, nscenes, ratio, tolerance);
+void launch_gate(hipStream_t st, const DeviceState& d, uint32_t nscenes, int statistic,
+                 float ratio, float tolerance) {
+    k_resid_reduce<<<nscenes, kReduceBlock, 0, st>>>(d, 3, 1, statistic, ratio, tolerance);
 }
-void launch_targets(hipStream_t st, const DeviceState& d, const SphParams& P) {
-    k_targets<<<grid(d.nslots), kBlock, 0, st>>>(d, P);
+void launch_resid_reduce(hipStream_t st, const DeviceState& d, uint32_t nscenes, int slotMask) {
+    k_resid_reduce<<<nscenes, kReduceBlock, 0, st>>>(d, slotMask, 0, 0, 0.0f, 0.0f);
+}
+void launch_targets(hipStream_t st, const DeviceState& d, const SphParams& P, const float4* XsLast) {
+    k_targets<<<grid(d.nslots), kBlock, 0, st>>>(d, P, XsLast);
 }
 void launch_sample(hipStream_t st, const DeviceState& d, uint64_t seed, uint32_t count,
                    float* featOut, float* targOut) {

@@ -161,11 +161,14 @@ struct akua_solver {
 
     // learned warm start
     int mode = 0;                  // 0: plain solver, 1: learned warm start with fallback
+    int gateStat = 0;              // 0: max over particles, 1: mean
+    int learnedIters = 1;          // Jacobi iterations after the warm start
     float gateRatio = 1.0f, gateTol = 0.01f;
     bool haveNet = false;
     DevBuf<float> net, feat, targ, lamHat;
     DevBuf<float4> XsP, XsW;
     DevBuf<unsigned int> resid;
+    DevBuf<float2> residPart;
     DevBuf<uint32_t> gate, fallbacks;
     int64_t learnedSteps = 0;
     // training
@@ -211,6 +214,7 @@ struct akua_solver {
         d.nbr = nbr.p; d.cnt = cnt.p; d.nbrSum = nbrSum.p;
         d.feat = feat.p; d.targ = targ.p; d.XsP = XsP.p; d.XsW = XsW.p; d.lamHat = lamHat.p;
         d.resid = resid.p; d.gate = gate.p; d.fallbacks = fallbacks.p;
+        d.residPart = residPart.p; d.nWaves = (nslots + 63) / 64;
         return d;
     }
 
@@ -473,9 +477,15 @@ void enqueue_step(akua_solver* s, hipStream_t st, const StepParams& S, bool time
     L.after("neighbours");
     const int I = s->cfg.solverIterations;
     const bool learned = s->mode == 1 && !record;
+    const int J = learned ? std::min(s->learnedIters, I) : 0;
+    // record: targets are x* before the last Jt iterations and the lambda computed there
+    const int Jt = std::min(std::max(s->learnedIters, 1), I);
     float4* xs[2] = {s->XsA.p, s->XsB.p};
+    // the warm chain: x*_w stays in XsW (for the taps); later iterates alternate between XsL
+    // (free once the neighbour lists are built) and XsB (which the fallback may overwrite only
+    // in scenes whose gate is set, where the warm chain's result is discarded)
+    auto ws = [&](int j) { return j == 0 ? s->XsW.p : (j & 1) ? s->XsL.p : s->XsB.p; };
     if (learned || record) {
-        launch_fill_u32(st, s->resid.p, (uint32_t)(4 * s->scenes.size()), 0u);
         L.before("features");
         launch_features(st, d, P, record, bs);
         L.after("features");
@@ -484,10 +494,19 @@ void enqueue_step(akua_solver* s, hipStream_t st, const StepParams& S, bool time
         L.before("mlp_apply");
         launch_mlp_apply(st, d, s->net.p, P.h, s->XsA.p, s->XsW.p, s->lamHat.p);
         L.after("mlp_apply");
-        L.before("density_lambda_warm");
-        launch_density_lambda(st, d, P, s->XsW.p, bs, 1, false);
-        L.after("density_lambda_warm");
-        launch_gate(st, d, (uint32_t)s->scenes.size(), s->gateRatio, s->gateTol);
+        // J iterations from the warm start; the first density pass is the residual check
+        for (int j = 0; j < J; ++j) {
+            const float4* in = ws(j);
+            L.before("density_lambda_warm");
+            launch_density_lambda(st, d, P, in, bs, j == 0 ? 1 : -1, false);
+            L.after("density_lambda_warm");
+            if (j == 0) launch_gate(st, d, (uint32_t)s->scenes.size(), s->gateStat, s->gateRatio, s->gateTol);
+            if (j + 1 < J) {
+                L.before("delta_correct_warm");
+                launch_delta_correct(st, d, P, S, in, ws(j + 1), false, true, bs);
+                L.after("delta_correct_warm");
+            }
+        }
     }
     if (I == 0) {
         L.before("update_damping");
@@ -499,20 +518,21 @@ void enqueue_step(akua_solver* s, hipStream_t st, const StepParams& S, bool time
         float4* out = xs[(it + 1) & 1];
         const bool last = it + 1 == I;
         L.before(learned ? "density_lambda_fallback" : "density_lambda");
-        launch_density_lambda(st, d, P, in, bs, record && last ? 2 : -1, learned);
+        launch_density_lambda(st, d, P, in, bs, record && it == I - Jt ? 2 : -1, learned);
         L.after(learned ? "density_lambda_fallback" : "density_lambda");
+        if (record && it == I - Jt) {
+            L.before("targets");
+            launch_targets(st, d, P, in);
+            launch_resid_reduce(st, d, (uint32_t)s->scenes.size(), 0x5);
+            L.after("targets");
+        }
         const char* name = last ? "delta_correct_update" : learned ? "delta_correct_fallback" : "delta_correct";
         L.before(name);
         // learned: the fallback's iterations run only where the gate is set; the final
-        // correction reads x*_w where it is clear
+        // correction reads the warm chain's last iterate where it is clear
         launch_delta_correct(st, d, P, S, in, out, last, true, bs, learned ? (last ? 2 : 1) : 0,
-                             s->XsW.p);
+                             learned ? ws(J - 1) : nullptr);
         L.after(name);
-    }
-    if (record) {
-        L.before("targets");
-        launch_targets(st, d, P);
-        L.after("targets");
     }
     L.before("vorticity");
     launch_vorticity(st, d, P, bs);
@@ -830,14 +850,16 @@ void ensure_learned(akua_solver* s) {
     s->feat.alloc(ns * kMlpFeatures);
     s->lamHat.alloc(ns);
     s->XsW.alloc(ns);
-    s->resid.alloc(4 * nsc);
+    s->resid.alloc(kResidStride * nsc);
+    s->residPart.alloc(3 * ((ns + 63) / 64));
+    HIP_CHECK(hipMemset(s->residPart.p, 0, 3 * ((ns + 63) / 64) * sizeof(float2)));
     s->gate.alloc(nsc);
     s->fallbacks.alloc(nsc);
     HIP_CHECK(hipMemset(s->net.p, 0, kMlpParams * sizeof(float)));
     HIP_CHECK(hipMemset(s->feat.p, 0, ns * kMlpFeatures * sizeof(float)));
     HIP_CHECK(hipMemset(s->lamHat.p, 0, ns * sizeof(float)));
     HIP_CHECK(hipMemset(s->XsW.p, 0, ns * sizeof(float4)));
-    HIP_CHECK(hipMemset(s->resid.p, 0, 4 * nsc * sizeof(unsigned int)));
+    HIP_CHECK(hipMemset(s->resid.p, 0, kResidStride * nsc * sizeof(unsigned int)));
     HIP_CHECK(hipMemset(s->gate.p, 0, nsc * sizeof(uint32_t)));
     HIP_CHECK(hipMemset(s->fallbacks.p, 0, nsc * sizeof(uint32_t)));
     s->graphDirty = true;
@@ -909,30 +931,42 @@ struct HostRng {
 };
 
 // He-normal weights, zero biases, the last layer scaled down so that the untrained network's
-// warm start is close to none; normalisation from the first batch of samples.
-std::vector<float> initial_network(uint64_t seed, const std::vector<float>& F,
-                                   const std::vector<float>& T, uint32_t count) {
+// warm start is close to none.  Normalisation: feature means and standard deviations, and the
+// targets' root mean squares, from the given FP64 sums over count samples.
+struct SampleStats {
+    double count = 0.0;
+    double fs[kMlpIn] = {}, fq[kMlpIn] = {}, tq[kMlpOut] = {};
+    void add(const std::vector<float>& F, const std::vector<float>& T, uint32_t n) {
+        for (uint32_t b = 0; b < n; ++b) {
+            for (int f = 0; f < kMlpIn; ++f) {
+                const double x = F[(size_t)b * kMlpIn + f];
+                fs[f] += x;
+                fq[f] += x * x;
+            }
+            for (int o = 0; o < kMlpOut; ++o) {
+                const double t = T[(size_t)b * kMlpOut + o];
+                tq[o] += t * t;
+            }
+        }
+        count += n;
+    }
+};
+
+std::vector<float> initial_network(uint64_t seed, const SampleStats& st) {
     std::vector<float> p(kMlpParams, 0.0f);
     HostRng rng{seed ^ 0x6A09E667F3BCC909ull};
     for (int k = 0; k < kMlpIn * kMlpHid; ++k) p[kOffW1 + k] = (float)(rng.normal() * std::sqrt(2.0 / kMlpIn));
     for (int k = 0; k < kMlpHid * kMlpHid; ++k) p[kOffW2 + k] = (float)(rng.normal() * std::sqrt(2.0 / kMlpHid));
     for (int k = 0; k < kMlpHid * kMlpOut; ++k) p[kOffW3 + k] = (float)(rng.normal() * 0.1 * std::sqrt(1.0 / kMlpHid));
     for (int f = 0; f < kMlpIn; ++f) {
-        double m = 0.0, q = 0.0;
-        for (uint32_t b = 0; b < count; ++b) m += F[(size_t)b * kMlpIn + f];
-        m /= count;
-        for (uint32_t b = 0; b < count; ++b) {
-            const double x = F[(size_t)b * kMlpIn + f] - m;
-            q += x * x;
-        }
-        const double sd = std::sqrt(q / count);
+        const double m = st.fs[f] / st.count;
+        const double var = std::max(0.0, st.fq[f] / st.count - m * m);
+        const double sd = std::sqrt(var);
         p[kOffMean + f] = (float)m;
         p[kOffInv + f] = sd > 1e-30 && std::isfinite(sd) ? (float)(1.0 / sd) : 1.0f;
     }
     for (int o = 0; o < kMlpOut; ++o) {
-        double q = 0.0;
-        for (uint32_t b = 0; b < count; ++b) q += (double)T[(size_t)b * 4 + o] * T[(size_t)b * 4 + o];
-        const double rms = std::sqrt(q / count);
+        const double rms = std::sqrt(st.tq[o] / st.count);
         p[kOffYmul + o] = rms > 1e-30 && std::isfinite(rms) ? (float)rms : 1.0f;
     }
     return p;
@@ -944,6 +978,7 @@ void validate_train_config(const akua_train_config& c) {
     if (c.minibatch < 16 || c.minibatch > c.samplesPerStep)
         fail(AKUA_ERR_INVALID_ARG, "minibatch must be in [16, samplesPerStep]");
     if (c.epochs < 1 || c.epochs > 64) fail(AKUA_ERR_INVALID_ARG, "epochs must be in [1, 64]");
+    if (c.normSteps < 1 || c.normSteps > 100000) fail(AKUA_ERR_INVALID_ARG, "normSteps must be in [1, 10^5]");
     if (!(c.learningRate > 0.0f) || !std::isfinite(c.learningRate))
         fail(AKUA_ERR_INVALID_ARG, "learningRate must be > 0");
     for (float w : c.lossWeights)
@@ -952,16 +987,21 @@ void validate_train_config(const akua_train_config& c) {
 
 // Online training on the solver's own plain trajectory: each step is a plain step that also
 // records features and targets; samplesPerStep random particles of it feed epochs passes of
-// Adam updates over minibatches.  ratio[k] = (max C at the last plain density pass) / (max C at
-// x*_pred) of step k, the plain solver's own residual reduction.
+// Adam updates over minibatches.  A solver without a network first gathers normalisation
+// statistics over normSteps steps (no updates), then starts from He-normal weights.
+// resid[4k..4k+3] of step k: max and mean over particles of max(0, C) at x*_pred, then max and
+// mean at the last plain density pass.
 void train(akua_solver* s, float dt, const float* bmin, const float* bmax, int steps,
-           const akua_train_config& c, float* lossOut, float* ratioOut) {
+           const akua_train_config& c, float* lossOut, float* residOut) {
     if (s->scenes.size() != 1) fail(AKUA_ERR_UNSUPPORTED, "training runs on a single-scene solver");
     if (!s->hasState) fail(AKUA_ERR_STATE, "set the particles before training");
     if (!(dt > 0.0f) || !std::isfinite(dt)) fail(AKUA_ERR_INVALID_ARG, "deltaTime must be > 0");
     if (steps < 1 || steps > 1000000) fail(AKUA_ERR_INVALID_ARG, "steps must be in [1, 10^6]");
     if (s->cfg.solverIterations < 1) fail(AKUA_ERR_UNSUPPORTED, "training needs solverIterations >= 1");
     validate_train_config(c);
+    if (c.targetIterations < 1 || c.targetIterations > s->cfg.solverIterations)
+        fail(AKUA_ERR_INVALID_ARG, "targetIterations must be in [1, solverIterations]");
+    s->learnedIters = c.targetIterations;
     check_box(bmin, bmax);
     set_box(s, 0, bmin, bmax);
     upload_scenes(s);
@@ -969,11 +1009,16 @@ void train(akua_solver* s, float dt, const float* bmin, const float* bmax, int s
     const uint32_t nb = B / M;
     const uint32_t perStep = nb * (uint32_t)c.epochs;
     ensure_train(s, B, perStep * (uint32_t)steps);
+    HIP_CHECK(hipMemset(s->lossBuf.p, 0, (size_t)perStep * steps * sizeof(float)));
     hipStream_t st = s->stream;
     const StepParams S{dt, {s->cfg.gravity[0], s->cfg.gravity[1], s->cfg.gravity[2]}};
     MlpLossWeights lw{{c.lossWeights[0], c.lossWeights[1], c.lossWeights[2], c.lossWeights[3]}};
     const float b1 = 0.9f, b2 = 0.999f, eps = 1e-8f;
-    std::vector<unsigned int> res(4);
+    const double count = (double)s->nparticles;
+    std::vector<unsigned int> res(kResidStride);
+    SampleStats stats;
+    int normLeft = s->haveNet ? 0 : c.normSteps;
+    std::vector<char> trained(steps, 0);
     for (int k = 0; k < steps; ++k) {
         enqueue_step(s, st, S, false, true);
         s->stepsSinceStats++;
@@ -981,33 +1026,42 @@ void train(akua_solver* s, float dt, const float* bmin, const float* bmax, int s
         launch_sample(st, d, c.seed + 0x2545F4914F6CDD1Dull * (uint64_t)(s->adamT + k + 1), B,
                       s->batchF.p, s->batchT.p);
         HIP_CHECK(hipGetLastError());
-        if (!s->haveNet) {
+        if (normLeft > 0) {
             std::vector<float> F((size_t)B * kMlpFeatures), T((size_t)B * 4);
             HIP_CHECK(hipStreamSynchronize(st));
             HIP_CHECK(hipMemcpy(F.data(), s->batchF.p, F.size() * 4, hipMemcpyDeviceToHost));
             HIP_CHECK(hipMemcpy(T.data(), s->batchT.p, T.size() * 4, hipMemcpyDeviceToHost));
-            const std::vector<float> p = initial_network(c.seed, F, T, B);
-            upload_network(s, p.data());
-        }
-        for (int e = 0; e < c.epochs; ++e) {
-            for (uint32_t mb = 0; mb < nb; ++mb) {
-                launch_mlp_grad(st, s->net.p, s->batchF.p + (size_t)mb * M * kMlpFeatures,
-                                s->batchT.p + (size_t)mb * M * 4, M, lw, s->part.p);
-                s->adamT++;
-                const float c1 = (float)(1.0 / (1.0 - std::pow(0.9, (double)s->adamT)));
-                const float c2 = (float)(1.0 / (1.0 - std::pow(0.999, (double)s->adamT)));
-                launch_mlp_adam(st, s->part.p, s->net.p, s->adamM.p, s->adamV.p, c.learningRate,
-                                b1, b2, eps, c1, c2,
-                                s->lossBuf.p + (size_t)k * perStep + (size_t)e * nb + mb);
+            stats.add(F, T, B);
+            if (--normLeft == 0) {
+                const std::vector<float> p = initial_network(c.seed, stats);
+                upload_network(s, p.data());
             }
+        } else {
+            for (int e = 0; e < c.epochs; ++e) {
+                for (uint32_t mb = 0; mb < nb; ++mb) {
+                    launch_mlp_grad(st, s->net.p, s->batchF.p + (size_t)mb * M * kMlpFeatures,
+                                    s->batchT.p + (size_t)mb * M * 4, M, lw, s->part.p);
+                    s->adamT++;
+                    const float c1 = (float)(1.0 / (1.0 - std::pow(0.9, (double)s->adamT)));
+                    const float c2 = (float)(1.0 / (1.0 - std::pow(0.999, (double)s->adamT)));
+                    launch_mlp_adam(st, s->part.p, s->net.p, s->adamM.p, s->adamV.p,
+                                    c.learningRate, b1, b2, eps, c1, c2,
+                                    s->lossBuf.p + (size_t)k * perStep + (size_t)e * nb + mb);
+                }
+            }
+            trained[k] = 1;
         }
         HIP_CHECK(hipGetLastError());
         HIP_CHECK(hipStreamSynchronize(st));
-        HIP_CHECK(hipMemcpy(res.data(), s->resid.p, 4 * sizeof(unsigned int), hipMemcpyDeviceToHost));
-        float rp, r3;
-        std::memcpy(&rp, &res[0], 4);
-        std::memcpy(&r3, &res[2], 4);
-        if (ratioOut) ratioOut[k] = rp > 0.0f ? r3 / rp : 0.0f;
+        HIP_CHECK(hipMemcpy(res.data(), s->resid.p, kResidStride * sizeof(unsigned int), hipMemcpyDeviceToHost));
+        if (residOut) {
+            float v[kResidStride];
+            std::memcpy(v, res.data(), sizeof(v));
+            residOut[4 * k + 0] = v[0];
+            residOut[4 * k + 1] = (float)(v[kResidSum] / count);
+            residOut[4 * k + 2] = v[2];
+            residOut[4 * k + 3] = (float)(v[kResidSum + 2] / count);
+        }
     }
     std::vector<float> L((size_t)perStep * steps);
     HIP_CHECK(hipMemcpy(L.data(), s->lossBuf.p, L.size() * 4, hipMemcpyDeviceToHost));
@@ -1015,7 +1069,7 @@ void train(akua_solver* s, float dt, const float* bmin, const float* bmax, int s
         for (int k = 0; k < steps; ++k) {
             double a = 0.0;
             for (uint32_t u = 0; u < perStep; ++u) a += L[(size_t)k * perStep + u];
-            lossOut[k] = (float)(a / perStep);
+            lossOut[k] = trained[k] ? (float)(a / perStep) : std::nanf("");
         }
 }
 
@@ -1398,7 +1452,8 @@ akua_status akua_solver_get_network(akua_solver* s, float* params, int32_t count
     });
 }
 
-akua_status akua_solver_set_learned(akua_solver* s, int32_t enable, float ratio, float tolerance) {
+akua_status akua_solver_set_learned(akua_solver* s, int32_t enable, int32_t iterations,
+                                    int32_t statistic, float ratio, float tolerance) {
     return guarded([&] {
         if (!s) fail(AKUA_ERR_INVALID_ARG, "NULL solver");
         if (enable) {
@@ -1407,8 +1462,14 @@ akua_status akua_solver_set_learned(akua_solver* s, int32_t enable, float ratio,
                 fail(AKUA_ERR_UNSUPPORTED, "the learned step needs solverIterations >= 1");
             if (!std::isfinite(ratio) || !std::isfinite(tolerance) || ratio < 0.0f)
                 fail(AKUA_ERR_INVALID_ARG, "ratio must be >= 0 and tolerance finite");
+            if (statistic != 0 && statistic != 1)
+                fail(AKUA_ERR_INVALID_ARG, "statistic must be 0 (max) or 1 (mean)");
+            if (iterations < 1 || iterations > s->cfg.solverIterations)
+                fail(AKUA_ERR_INVALID_ARG, "iterations must be in [1, solverIterations]");
+            s->learnedIters = iterations;
         }
         s->mode = enable ? 1 : 0;
+        s->gateStat = statistic;
         s->gateRatio = ratio;
         s->gateTol = tolerance;
         s->graphDirty = true;
@@ -1420,25 +1481,32 @@ akua_status akua_solver_learned_stats(akua_solver* s, int64_t* learnedSteps, int
     return guarded([&] {
         if (!s) fail(AKUA_ERR_INVALID_ARG, "NULL solver");
         int64_t fb = 0;
-        float r[4] = {0.0f, 0.0f, 0.0f, 0.0f};
+        float r[kResidStride] = {};
         if (s->fallbacks.p) {
             auto f = download(s, s->fallbacks.p, s->scenes.size());
             for (auto v : f) fb += v;
-            auto q = download(s, s->resid.p, 4);
+            auto q = download(s, s->resid.p, kResidStride);
             std::memcpy(r, q.data(), sizeof(r));
         }
         if (learnedSteps) *learnedSteps = s->learnedSteps;
         if (fallbacks) *fallbacks = fb;
-        if (residual) std::memcpy(residual, r, 3 * sizeof(float));
+        if (residual) {
+            const SceneDev& sc = s->scenes[0];
+            const float cnt = (float)std::max<uint32_t>(1u, sc.end - sc.begin);
+            for (int k = 0; k < 3; ++k) {
+                residual[k] = r[k];
+                residual[3 + k] = r[kResidSum + k] / cnt;
+            }
+        }
     });
 }
 
 akua_status akua_solver_train(akua_solver* s, float dt, const float boxMin[3], const float boxMax[3],
                               int32_t steps, const akua_train_config* config, float* loss,
-                              float* residualRatio) {
+                              float* residuals) {
     return guarded([&] {
         if (!s || !config || !boxMin || !boxMax) fail(AKUA_ERR_INVALID_ARG, "NULL argument");
-        train(s, dt, boxMin, boxMax, steps, *config, loss, residualRatio);
+        train(s, dt, boxMin, boxMax, steps, *config, loss, residuals);
     });
 }
 

@@ -149,34 +149,40 @@ class PBFSolver:
         _lib.check(self._L.akua_solver_get_network(self._h, _p(p), _lib.MLP_PARAMS))
         return p
 
-    def set_learned(self, enable: bool, ratio: float = 1.0, tolerance: float = 0.01) -> None:
-        """Learned steps: warm start + one iteration, per-scene fallback to the plain solve when
-        max C(x*_w) > ratio * max C(x*_pred) + tolerance."""
-        _lib.check(self._L.akua_solver_set_learned(self._h, 1 if enable else 0, float(ratio),
-                                                   float(tolerance)))
+    def set_learned(self, enable: bool, ratio: float = 1.0, tolerance: float = 0.01,
+                    statistic: str = "max", iterations: int = 1) -> None:
+        """Learned steps: warm start + the last `iterations` Jacobi iterations, per-scene fallback
+        to the plain solve when r(x*_w) > ratio * r(x*_pred) + tolerance, r the max or mean of
+        max(0, C)."""
+        st = {"max": 0, "mean": 1}[statistic]
+        _lib.check(self._L.akua_solver_set_learned(self._h, 1 if enable else 0, int(iterations), st,
+                                                   float(ratio), float(tolerance)))
 
     def learned_stats(self) -> Dict[str, object]:
         steps, fb = ctypes.c_int64(), ctypes.c_int64()
-        r = np.zeros(3, np.float32)
+        r = np.zeros(6, np.float32)
         _lib.check(self._L.akua_solver_learned_stats(self._h, ctypes.byref(steps), ctypes.byref(fb), _p(r)))
         return dict(steps=int(steps.value), fallbacks=int(fb.value),
-                    resid_pred=float(r[0]), resid_warm=float(r[1]), resid_plain=float(r[2]))
+                    max_pred=float(r[0]), max_warm=float(r[1]), max_plain=float(r[2]),
+                    mean_pred=float(r[3]), mean_warm=float(r[4]), mean_plain=float(r[5]))
 
     def train(self, deltaTime: float, boxMin, boxMax, steps: int, samplesPerStep: int = 65536,
               minibatch: int = 8192, epochs: int = 1, learningRate: float = 1e-3, seed: int = 1,
-              lossWeights=(1.0, 1.0, 1.0, 0.25)):
-        """Online training on this scene's plain trajectory; returns (loss per step, residual
-        ratio per step)."""
+              lossWeights=(1.0, 1.0, 1.0, 0.25), normSteps: int = 20, targetIterations: int = 1):
+        """Online training on this scene's plain trajectory.  Returns (loss per step, residuals
+        (steps, 4): max and mean of max(0, C) at x*_pred, then at the last plain density pass)."""
         c = _lib.AkuaTrainConfig()
         c.samplesPerStep, c.minibatch, c.epochs = int(samplesPerStep), int(minibatch), int(epochs)
+        c.normSteps = int(normSteps)
+        c.targetIterations = int(targetIterations)
         c.learningRate, c.seed = float(learningRate), int(seed)
         for k in range(4):
             c.lossWeights[k] = float(lossWeights[k])
         loss = np.zeros(int(steps), np.float32)
-        ratio = np.zeros(int(steps), np.float32)
+        resid = np.zeros((int(steps), 4), np.float32)
         _lib.check(self._L.akua_solver_train(self._h, float(deltaTime), _p(_f3(boxMin)), _p(_f3(boxMax)),
-                                             int(steps), ctypes.byref(c), _p(loss), _p(ratio)))
-        return loss, ratio
+                                             int(steps), ctypes.byref(c), _p(loss), _p(resid)))
+        return loss, resid
 
     def debug_features(self, targets: bool = False):
         F = np.zeros((self.numParticles, 16), np.float32)

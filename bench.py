@@ -14,7 +14,20 @@ kernel (algorithmic bytes per launch / mean launch time from HIP events on the s
 against the 8 TB/s HBM peak), the maximum and mean density-constraint error after the timed
 steps, and the CPU oracle timed on the host's cores on a bounded sample of the same workload.
 
+Also reported, each in its own object of the same line:
+  learned   -- learned warm start vs plain solver at equal error.  The headline parameter set
+               drives the 1M column into a degenerate state (max density error ~88 after 100
+               steps, velocities in the hundreds to thousands), where density error says
+               nothing, so the comparison runs on the same 1M scene with the reference
+               application's own solver parameters (PBFConfig defaults) at a 2 ms substep: plain
+               with 1-4 iterations and learned with J = 1, 2 (networks from
+               akuaengine_amd/networks, trained by akuaengine_amd/train_warm_start.py), 300
+               steps each, density error sampled every 25 steps.
+  dam_break_8m, batched_256 -- throughput of the 8M two-column scene and of 256 independent
+               scenes in one batched solver.
+
 Usage: python bench.py [--steps K] [--warmup W] [--gpus 1] [--cpu-seconds S] [--no-cpu-baseline]
+                       [--no-learned] [--no-extra]
 """
 from __future__ import annotations
 
@@ -84,6 +97,124 @@ def cpu_baseline(scene, cfg, corr, dt, budget_s: float):
                       f"{threads} OpenMP threads, {el:.1f} s"}
 
 
+def time_steps(s, dt, bmin, bmax, steps: int) -> float:
+    """Seconds of `steps` steps, synchronised on both sides."""
+    s.synchronize()
+    t0 = time.perf_counter()
+    s.step_n(dt, bmin, bmax, steps)
+    s.synchronize()
+    return time.perf_counter() - t0
+
+
+def learned_comparison(log_steps: int = 300, sample: int = 25, skip: int = 50):
+    """Plain solver with 1-4 iterations and learned warm start with J = 1, 2 on the 1M scene with
+    the reference application's parameters at a 2 ms substep."""
+    from dataclasses import replace
+    from akuaengine_amd import learned, scenes
+    from akuaengine_amd.config import LambdaCorrParams, PBFConfig
+    from akuaengine_amd.solver import PBFSolver
+    cfg, corr, dt = PBFConfig(), LambdaCorrParams(), 0.002
+    sc = scenes.dam_break_1m()
+    nets = {}
+    for J in (1, 2):
+        path = os.path.join(learned.NETWORK_DIR, f"dam_break_1m_reference_J{J}.npy")
+        if os.path.exists(path):
+            with open(os.path.splitext(path)[0] + ".json") as f:
+                meta = json.load(f)
+            nets[J] = (learned.load_network(path), meta)
+        else:
+            from akuaengine_amd.train_warm_start import train
+            log(f"[bench] training the J={J} network (no file at {path})")
+            nets[J] = train("reference", J, log=log)
+
+    def run(I, J=None):
+        s = PBFSolver(sc.n, replace(cfg, solverIterations=I), corr)
+        s.set_state(sc.pos, sc.vel, sc.mass)
+        if J is not None:
+            net, meta = nets[J]
+            s.set_network(net)
+            s.set_learned(True, meta["gate_ratio_p90"], 0.01, "max", J)
+        errs, t, done = [], 0.0, 0
+        while done < log_steps:
+            el = time_steps(s, dt, sc.box_min, sc.box_max, sample)
+            if done >= skip:
+                t += el
+            done += sample
+            errs.append(s.density_error())
+        ms = t / (log_steps - skip) * 1e3
+        errs = [[float(a), float(b)] for a, b in errs]
+        r = {"iterations": I, "ms_per_step": round(ms, 4), "value": round(sc.n / ms * 1e3, 1),
+             "density_error_max_avg": round(sum(e[0] for e in errs) / len(errs), 5),
+             "density_error_mean_avg": round(sum(e[1] for e in errs) / len(errs), 6),
+             "density_error_final": [round(errs[-1][0], 5), round(errs[-1][1], 6)]}
+        if J is not None:
+            st = s.learned_stats()
+            r.update({"learned_iterations": J, "fallback_steps": st["fallbacks"],
+                      "gate": {"statistic": "max", "ratio": round(nets[J][1]["gate_ratio_p90"], 4),
+                               "tolerance": 0.01}})
+        log(f"[bench] {'learned J=%d' % J if J else 'plain I=%d' % I}: {ms:.3f} ms/step, "
+            f"error max {r['density_error_max_avg']:.4f} mean {r['density_error_mean_avg']:.5f}")
+        return r
+
+    plain = {f"plain_I{I}": run(I) for I in (4, 3, 2, 1)}
+    learned_runs = {f"learned_J{J}": run(4, J) for J in (1, 2)}
+    out = {"workload": "dam_break_1m_reference_substep", "particles": sc.n, "dt": dt,
+           "parameters": "PBFConfig/LambdaCorrParams defaults (PBFConfig.h:10-29)",
+           "steps": log_steps, "timed_steps": log_steps - skip, "error_samples_every": sample,
+           **plain, **learned_runs}
+    # equal error: the cheapest plain solver whose time-averaged errors are both no larger
+    for name, r in learned_runs.items():
+        ok = [p for p in plain.values() if p["density_error_max_avg"] <= r["density_error_max_avg"]
+              and p["density_error_mean_avg"] <= r["density_error_mean_avg"]]
+        best = min(ok, key=lambda p: p["ms_per_step"]) if ok else None
+        r["speedup_vs_plain_I4"] = round(plain["plain_I4"]["ms_per_step"] / r["ms_per_step"], 4)
+        r["speedup_vs_plain_at_equal_error"] = (
+            round(best["ms_per_step"] / r["ms_per_step"], 4) if best else None)
+        r["plain_at_equal_error"] = f"plain_I{best['iterations']}" if best else None
+    out["max_density_error_le_1pct"] = {
+        k: v["density_error_max_avg"] <= 0.01 for k, v in {**plain, **learned_runs}.items()}
+    return out
+
+
+def extra_workloads(steps: int = 20, warmup: int = 3):
+    """Throughput of the 8M two-column scene and of the 256-scene batch (plain, 4 iterations)."""
+    import numpy as np
+    from akuaengine_amd import scenes
+    from akuaengine_amd.config import DAM_BREAK_DT, dam_break_params
+    from akuaengine_amd.solver import BatchSolver, PBFSolver
+    cfg, corr = dam_break_params(4)
+    dt = DAM_BREAK_DT
+    out = {}
+    sc = scenes.dam_break_8m()
+    s = PBFSolver(sc.n, cfg, corr)
+    s.set_state(sc.pos, sc.vel, sc.mass)
+    s.step_n(dt, sc.box_min, sc.box_max, warmup)
+    el = time_steps(s, dt, sc.box_min, sc.box_max, steps)
+    out["dam_break_8m"] = {"particles": sc.n, "steps": steps, "ms_per_step": round(el / steps * 1e3, 4),
+                           "value": round(sc.n * steps / el, 1), "unit": "particle-substeps/s"}
+    log(f"[bench] 8M: {el / steps * 1e3:.3f} ms/step")
+    del s
+    scs = scenes.batched_scenes(256, seed=3)
+    counts = [x.n for x in scs]
+    b = BatchSolver(counts, cfg, corr)
+    b.set_state(np.concatenate([x.pos for x in scs]), np.concatenate([x.vel for x in scs]))
+    for i, x in enumerate(scs):
+        b.set_box(i, x.box_min, x.box_max)
+    b.autotune(dt, trialSteps=3)
+    b.step_batch(dt, warmup)
+    b.synchronize()
+    t0 = time.perf_counter()
+    b.step_batch(dt, steps)
+    b.synchronize()
+    el = time.perf_counter() - t0
+    total = int(sum(counts))
+    out["batched_256"] = {"scenes": 256, "particles": total, "steps": steps,
+                          "ms_per_step": round(el / steps * 1e3, 4), "value": round(total * steps / el, 1),
+                          "unit": "particle-substeps/s", "tuning": b.get_tuning()}
+    log(f"[bench] batched 256 scenes ({total} particles): {el / steps * 1e3:.3f} ms/step")
+    return out
+
+
 def main() -> int:
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
     ap.add_argument("--gpus", type=int, default=1)
@@ -93,6 +224,8 @@ def main() -> int:
                     help="steps with per-kernel HIP events after the timed region")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-learned", action="store_true")
+    ap.add_argument("--no-extra", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU: every box in this setup has one MI355X, so --gpus "
@@ -110,6 +243,8 @@ def main() -> int:
     log(f"[bench] workload dam_break_1m N={sc.n}")
     s = PBFSolver(sc.n, cfg, corr)
     s.set_state(sc.pos, sc.vel, sc.mass)
+    tuning = s.autotune(dt, sc.box_min, sc.box_max, trialSteps=3)
+    log(f"[bench] tuning {tuning}")
 
     s.step_n(dt, sc.box_min, sc.box_max, args.warmup)
     s.synchronize()
@@ -169,6 +304,10 @@ def main() -> int:
     emax, emean = s.density_error()
     log(f"[bench] density error after the timed steps: max {emax:.4f} mean {emean:.5f}")
 
+    del s
+    learned_out = None if args.no_learned else learned_comparison()
+    extra = {} if args.no_extra else extra_workloads()
+
     cpu = None
     if not args.no_cpu_baseline:
         cpu = cpu_baseline(sc, cfg, corr, dt, args.cpu_seconds)
@@ -181,7 +320,7 @@ def main() -> int:
         "config": {"workload": "dam_break_1m_plain", "particles": sc.n, "solver": "plain",
                    "solverIterations": cfg.solverIterations, "dt": dt, "restDensity": cfg.restDensity,
                    "corr_k": corr.k, "corr_n": corr.n, "corr_delta_q": corr.delta_q,
-                   "box_max": list(sc.box_max), "parallelism": "single-gpu"},
+                   "box_max": list(sc.box_max), "parallelism": "single-gpu", "tuning": tuning},
         "roofline": roofline,
         "roofline_density_lambda": roof_dl,
         "roofline_step": roof_step,
@@ -191,6 +330,8 @@ def main() -> int:
                           "definition": "max/mean over particles of max(0, rho/rho0 - 1), exact SPH "
                                         "density at the positions after the timed steps"},
         "kernels": kernels,
+        "learned": learned_out,
+        **extra,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out))

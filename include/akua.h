@@ -164,9 +164,11 @@ akua_status akua_solver_autotune(akua_solver* s, float deltaTime, const float bo
 /* A per-particle network (16 features -> 64 -> 64 -> 4, ReLU, BF16 weights and activations on the
  * matrix cores, FP32 accumulation) predicts the position correction of the plain solve from
  * features at the predicted positions x*_pred.  A learned step moves x* to the prediction x*_w,
- * runs one Jacobi iteration from there, and per scene falls back to the plain solve from x*_pred
- * (all solverIterations iterations, bit-identical to a plain step) when
- *     max_i max(0, rho_i(x*_w) / rho0 - 1)  >  ratio * max_i max(0, rho_i(x*_pred) / rho0 - 1) + tolerance.
+ * runs J Jacobi iterations from there (the last J of the plain solve), and per scene falls back
+ * to the plain solve from x*_pred (all solverIterations iterations, bit-identical to a plain
+ * step) when
+ *     r(x*_w)  >  ratio * r(x*_pred) + tolerance,
+ * r the max (statistic 0) or the mean (statistic 1) over particles of max(0, rho_i / rho0 - 1).
  * No reference counterpart (the reference always runs solverIterations iterations,
  * ConstraintSolverCUDA.cu:173-222).
  *
@@ -177,7 +179,9 @@ akua_status akua_solver_autotune(akua_solver* s, float deltaTime, const float bo
  *   4-6 (x*_pred - x) / h, 7-9 2 lambda (1/rho0) sum_j m_j grad W_ij / h,
  *   10-12 (1/rho0) sum_j m_j s_corr,ij grad W_ij / h,
  *   13-15 per axis clamp((x - lo) / h, 0, 2) - clamp((hi - x) / h, 0, 2).
- * Outputs: (x*_final - x*_pred) / h of the plain solve (3), and its last lambda (1). */
+ * Outputs: (x*_{I-J} - x*_pred) / h of the plain solve with I iterations, x*_{I-J} the positions
+ * its last J iterations start from (3), and the lambda computed there (1).  Normalised targets
+ * are clipped to [-8, 8] in the training loss. */
 #define AKUA_MLP_PARAMS 5544
 #define AKUA_MLP_TRAINABLE 5508
 
@@ -185,6 +189,9 @@ typedef struct akua_train_config {
     int32_t samplesPerStep;   /* random particles recorded per plain step */
     int32_t minibatch;        /* samples per Adam update */
     int32_t epochs;           /* passes over each step's samples */
+    int32_t normSteps;        /* a solver without a network: steps whose samples set the
+                                 normalisation before training starts */
+    int32_t targetIterations; /* J: the network learns x* before the last J iterations */
     float learningRate;       /* Adam, beta1 0.9, beta2 0.999, eps 1e-8 */
     uint64_t seed;            /* sampling and (for a solver without a network) initial weights */
     float lossWeights[4];     /* per output, on outputs normalised by ymul */
@@ -192,22 +199,26 @@ typedef struct akua_train_config {
 
 akua_status akua_solver_set_network(akua_solver* s, const float* params, int32_t count);
 akua_status akua_solver_get_network(akua_solver* s, float* params, int32_t count);
-/* enable 1: learned steps with the fallback rule above; 0: plain steps. */
-akua_status akua_solver_set_learned(akua_solver* s, int32_t enable, float ratio, float tolerance);
-/* Steps run in learned mode, scene-steps that fell back, and the last step's residuals of scene 0
- * (residual[0] at x*_pred, [1] at x*_w, [2] at the last plain density pass of a recorded step). */
+/* enable 1: learned steps with the fallback rule above (iterations = J, 1..solverIterations,
+ * the J the network was trained for); 0: plain steps. */
+akua_status akua_solver_set_learned(akua_solver* s, int32_t enable, int32_t iterations,
+                                    int32_t statistic, float ratio, float tolerance);
+/* Steps run in learned mode, scene-steps that fell back, and the last step's residuals of scene 0:
+ * residual[0..2] the max at x*_pred, at x*_w and at the last plain density pass of a recorded
+ * step; residual[3..5] the means. */
 akua_status akua_solver_learned_stats(akua_solver* s, int64_t* learnedSteps, int64_t* fallbacks,
-                                      float residual[3]);
+                                      float residual[6]);
 /* Trains the solver's network on its own plain trajectory (single-scene solvers): `steps` plain
  * steps (the state advances exactly as akua_solver_step would), each recording features and
  * targets of config->samplesPerStep random particles and running config->epochs passes of Adam
- * over them in minibatches.  A solver without a network starts from He-normal weights and takes
- * the feature and target scales from the first step's samples.  loss[k] (may be NULL): mean
- * minibatch loss of step k; residualRatio[k] (may be NULL): residual at the last plain density
- * pass over residual at x*_pred. */
+ * over them in minibatches.  A solver without a network first takes the feature and target
+ * scales from the samples of config->normSteps steps, then starts from He-normal weights.
+ * loss[k] (may be NULL): mean minibatch loss of step k (NaN for a normalisation step);
+ * residuals[4k..4k+3] (may be NULL): max and mean of max(0, C) at x*_pred, then at x*_{I-J},
+ * of step k. */
 akua_status akua_solver_train(akua_solver* s, float deltaTime, const float boxMin[3],
                               const float boxMax[3], int32_t steps, const akua_train_config* config,
-                              float* loss, float* residualRatio);
+                              float* loss, float* residuals);
 /* Features (16 per particle) and, after a training step, targets (4 per particle) of the last
  * learned or recorded step, by logical index (the order of akua_solver_get_state). */
 akua_status akua_debug_features(akua_solver* s, float* features, float* targets, int32_t n);

@@ -14,8 +14,8 @@ Two modes:
                (and rarely one BF16 ulp of an activation), not bit for bit.
 
 Parameter blob (floats): W1[16][64] b1[64] W2[64][64] b2[64] W3[64][4] b3[4] fmean[16] finv[16]
-ymul[4]; weights are input-major.  Loss: (1/n) sum_rows sum_o w_o (y_o - t_o / ymul_o)^2 with y
-the normalised outputs.
+ymul[4]; weights are input-major.  Loss: (1/n) sum_rows sum_o w_o (y_o - clip(t_o / ymul_o))^2
+with y the normalised outputs and clip to [-8, 8].
 """
 from __future__ import annotations
 
@@ -34,6 +34,7 @@ OFF_INV = OFF_MEAN + NIN
 OFF_YMUL = OFF_INV + NIN
 NPARAMS = OFF_YMUL + NOUT
 assert (TRAINABLE, NPARAMS) == (5508, 5544)
+TARGET_CLIP = 8.0  # normalised targets are clipped to [-8, 8] in the loss
 
 
 def bf16(x) -> np.ndarray:
@@ -98,7 +99,7 @@ def loss_and_grad(params, F, T, weights=(1.0, 1.0, 1.0, 1.0), exact: bool = True
     w = np.asarray(weights, dtype=np.float64)
     _, c = forward(params, F, exact=exact, cache=True)
     q = c["q"]
-    tn = np.asarray(T, dtype=np.float64) / q["ymul"]
+    tn = np.clip(np.asarray(T, dtype=np.float64) / q["ymul"], -TARGET_CLIP, TARGET_CLIP)
     diff = c["y"] - tn
     loss = float((w * diff * diff).sum() / n)
     G3 = 2.0 * w * diff / n
@@ -121,7 +122,7 @@ def loss_only(params, F, T, weights=(1.0, 1.0, 1.0, 1.0)) -> float:
     """FP64 loss (for finite differences)."""
     q = unpack(params)
     y = forward(params, F, exact=True) / q["ymul"]
-    d = y - np.asarray(T, dtype=np.float64) / q["ymul"]
+    d = y - np.clip(np.asarray(T, dtype=np.float64) / q["ymul"], -TARGET_CLIP, TARGET_CLIP)
     return float((np.asarray(weights) * d * d).sum() / len(F))
 
 

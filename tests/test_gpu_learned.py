@@ -59,6 +59,20 @@ def test_gradient_matches_oracle(dev, n):
     assert loss == pytest.approx(lx, rel=2e-2)
 
 
+def test_gradient_with_clipped_targets_matches_oracle(dev):
+    """Outlier targets beyond 8 normalised units enter the loss clipped."""
+    learned, _ = dev
+    rng = np.random.default_rng(12)
+    F, T = _data(rng, 5000)
+    p = M.random_params(rng, F, T).astype(np.float32)
+    T[::20] *= 100.0
+    assert (np.abs(T / p[M.OFF_YMUL:]) > M.TARGET_CLIP).any()
+    loss, g = learned.mlp_gradient(p, F, T)
+    le, ge = M.loss_and_grad(p, F, T, exact=False)
+    assert loss == pytest.approx(le, rel=2e-3)
+    assert np.linalg.norm(g - ge) < 2e-3 * np.linalg.norm(ge)
+
+
 def test_adam_step_matches_oracle(dev):
     learned, _ = dev
     rng = np.random.default_rng(7)
@@ -116,17 +130,18 @@ def _random_net(rng):
     return p.astype(np.float32)
 
 
-def test_features_and_learned_step_match_oracle(dev, oracle_mod, dam_params):
+@pytest.mark.parametrize("J", [1, 2, 4])
+def test_features_and_learned_step_match_oracle(dev, oracle_mod, dam_params, J):
     """After a plain step, one learned step: the features equal the oracle's (bit for bit), the
     warm start equals x*_pred + h * network(features), and the step continued by the oracle from
-    that warm start (one iteration, then the rest of the step) equals the device's."""
+    that warm start (J iterations, then the rest of the step) equals the device's."""
     learned, PBFSolver = dev
     sc, s, o = _scene_pair(PBFSolver, oracle_mod, dam_params)
     s.step(DT, sc.box_min, sc.box_max)
     o.step(DT, sc.box_min, sc.box_max)
     net = _random_net(np.random.default_rng(1))
     s.set_network(net)
-    s.set_learned(True, ratio=1.0, tolerance=1e30)   # never falls back
+    s.set_learned(True, ratio=1.0, tolerance=1e30, iterations=J)   # never falls back
     s.step(DT, sc.box_min, sc.box_max)
     st = s.learned_stats()
     assert st["steps"] == 1 and st["fallbacks"] == 0
@@ -142,8 +157,9 @@ def test_features_and_learned_step_match_oracle(dev, oracle_mod, dam_params):
     np.testing.assert_array_equal(xw, xp + h * y[:, :3])
     np.testing.assert_array_equal(lam_hat, y[:, 3])
     o.set_xs(xw)
-    for name in ("densities", "lambdas", "deltas", "correct", "update", "damping", "vorticity",
-                 "confinement", "xsph"):
+    names = ("densities", "lambdas", "deltas", "correct") * J + (
+        "update", "damping", "vorticity", "confinement", "xsph")
+    for name in names:
         o.stage(name, DT, sc.box_min, sc.box_max)
     got, want = s.get_state(), o.state()
     np.testing.assert_array_equal(got["ids"], want["ids"])
@@ -152,7 +168,8 @@ def test_features_and_learned_step_match_oracle(dev, oracle_mod, dam_params):
     assert np.abs(xw - xp).max() > 0  # the warm start moved the particles
 
 
-def test_fallback_reproduces_the_plain_solver(dev, dam_params):
+@pytest.mark.parametrize("J", [1, 3])
+def test_fallback_reproduces_the_plain_solver(dev, dam_params, J):
     """With a gate that always fails, every learned step falls back: the trajectory is the plain
     solver's, bit for bit."""
     _, PBFSolver = dev
@@ -164,7 +181,7 @@ def test_fallback_reproduces_the_plain_solver(dev, dam_params):
     for s in (a, b):
         s.set_state(sc.pos, sc.vel)
     b.set_network(_random_net(np.random.default_rng(2)))
-    b.set_learned(True, ratio=0.0, tolerance=-1.0)
+    b.set_learned(True, ratio=0.0, tolerance=-1.0, iterations=J)
     a.step_n(DT, sc.box_min, sc.box_max, 10)
     b.step_n(DT, sc.box_min, sc.box_max, 10)
     st = b.learned_stats()

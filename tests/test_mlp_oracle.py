@@ -31,6 +31,22 @@ def test_exact_gradient_matches_finite_differences(seed):
         assert abs(fd - g[k]) <= 1e-5 * (abs(fd) + 1e-3), (k, fd, g[k])
 
 
+def test_clipped_targets_keep_the_gradient_exact():
+    rng = np.random.default_rng(9)
+    F, T = _data(rng)
+    p = M.random_params(rng, F, T)
+    T[::5] *= 50.0
+    assert (np.abs(T / p[M.OFF_YMUL:]) > M.TARGET_CLIP).any()
+    _, g = M.loss_and_grad(p, F, T)
+    for k in rng.choice(M.TRAINABLE, 40, replace=False):
+        e = 1e-6 * max(1.0, abs(p[k]))
+        pp, pm = p.copy(), p.copy()
+        pp[k] += e
+        pm[k] -= e
+        fd = (M.loss_only(pp, F, T) - M.loss_only(pm, F, T)) / (2 * e)
+        assert abs(fd - g[k]) <= 1e-5 * (abs(fd) + 1e-3)
+
+
 def test_a_plant_in_the_backward_pass_is_caught():
     """The finite-difference check fails for a gradient with a dropped ReLU mask."""
     rng = np.random.default_rng(5)
