@@ -299,7 +299,12 @@ __global__ __launch_bounds__(kInferBlock) void k_mlp_apply(DeviceState d, const 
         const uint32_t p = t * 16 + (uint32_t)l;
         if (l < 16 && p < n && p < d.scenes[d.chunkScene[p >> 8]].end) {
             const float4 x = XsPred[p];
-            const float dx = o[0] * W.ymul[0], dy = o[1] * W.ymul[1], dz = o[2] * W.ymul[2];
+            // the network was trained on targets within +-kMlpTargetClip normalised units: the
+            // warm start does not extrapolate beyond them
+            const float c = kMlpTargetClip;
+            const float dx = fminf(fmaxf(o[0], -c), c) * W.ymul[0];
+            const float dy = fminf(fmaxf(o[1], -c), c) * W.ymul[1];
+            const float dz = fminf(fmaxf(o[2], -c), c) * W.ymul[2];
             XsWarm[p] = make_float4(x.x + h * dx, x.y + h * dy, x.z + h * dz, x.w);
             lamHat[p] = o[3] * W.ymul[3];
         }

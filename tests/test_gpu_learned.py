@@ -154,7 +154,9 @@ def test_features_and_learned_step_match_oracle(dev, oracle_mod, dam_params, J):
     np.testing.assert_array_equal(xp, o.stage_arrays()["xs"].astype(np.float32))
     y = learned.mlp_forward(net, F)
     h = np.float32(dam_params[0].smoothRadius)
-    np.testing.assert_array_equal(xw, xp + h * y[:, :3])
+    ymul = net[M.OFF_YMUL:M.OFF_YMUL + 3]
+    yc = np.clip(y[:, :3] / ymul, -M.TARGET_CLIP, M.TARGET_CLIP).astype(np.float32) * ymul
+    np.testing.assert_array_equal(xw, xp + h * yc)
     np.testing.assert_array_equal(lam_hat, y[:, 3])
     o.set_xs(xw)
     names = ("densities", "lambdas", "deltas", "correct") * J + (
@@ -221,3 +223,32 @@ def test_gate_decides_per_scene(dev, dam_params):
         for k in ("pos", "vel"):
             np.testing.assert_array_equal(got[k][sl], want[k], err_msg=f"scene {i} {k}")
         off += s.n
+
+
+def test_trained_network_gains_over_the_step_without_it(dev):
+    """The committed J = 1 network (trained on the GPU by train_warm_start.py) against the same
+    learned step with a zero network (no warm start: one Jacobi iteration), on the 1M evaluation
+    scene with the reference application's parameters at a 2 ms substep, 150 steps, no fallback.
+    The trained warm start must cut the time-averaged mean density error; the max is reported."""
+    learned, PBFSolver = dev
+    from akuaengine_amd import scenes
+    from akuaengine_amd.config import LambdaCorrParams, PBFConfig
+    import os
+    path = os.path.join(learned.NETWORK_DIR, "dam_break_1m_reference_J1.npy")
+    net = learned.load_network(path)
+    zero = net.copy()
+    zero[:M.TRAINABLE] = 0.0
+    sc = scenes.dam_break_1m()
+    res = {}
+    for name, p in (("trained", net), ("zero", zero)):
+        s = PBFSolver(sc.n, PBFConfig(), LambdaCorrParams())
+        s.set_state(sc.pos, sc.vel)
+        s.set_network(p)
+        s.set_learned(True, ratio=1.0, tolerance=1e30, iterations=1)
+        errs = []
+        for _ in range(6):
+            s.step_n(0.002, sc.box_min, sc.box_max, 25)
+            errs.append(s.density_error())
+        res[name] = np.array(errs).mean(0)
+    print("time-averaged density error (max, mean):", res)
+    assert res["trained"][1] < 0.7 * res["zero"][1]
