@@ -67,7 +67,7 @@ struct DeviceState {
     uint32_t tableTotal;
     uint32_t* nbr;
     uint32_t* cnt;
-    unsigned long long* nbrSum;  // running sum of neighbour counts (statistics)
+    unsigned long long* nbrSum;  // per-workgroup partial sums of neighbour counts (statistics)
     // learned warm start (null when unused)
     float* feat;               // 16 features per slot at x*_pred
     float* targ;               // training targets per slot: (x*_final - x*_pred) / h, lambda
@@ -100,6 +100,8 @@ void launch_gather(hipStream_t st, const DeviceState& d, const SphParams& P, con
 void launch_bucket_phys(hipStream_t st, const DeviceState& d);
 // bs: workgroup size of the neighbour kernels (128, 256 or 512; results do not depend on it).
 void launch_neighbours(hipStream_t st, const DeviceState& d, const SphParams& P, int bs);
+// Partial sums of the neighbour counts (blocks partials into part).
+void launch_count_sum(hipStream_t st, const DeviceState& d, unsigned long long* part, uint32_t blocks);
 // residSlot >= 0: also the per-scene max and sum of max(0, C) into check residSlot of d.resid.
 // gated: only scenes whose d.gate is set run (the fallback's iterations).
 void launch_density_lambda(hipStream_t st, const DeviceState& d, const SphParams& P,

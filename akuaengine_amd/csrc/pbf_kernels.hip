@@ -329,10 +329,22 @@ __global__ __launch_bounds__(BS) void k_neighbours(DeviceState d, SphParams P) {
         }
         d.cnt[p] = count;
     }
-    // neighbour statistics: one atomic per wave
-    unsigned long long c = count;
-    for (int off = 32; off > 0; off >>= 1) c += __shfl_down(c, off, 64);
-    if (__lane_id() == 0 && c) atomicAdd(d.nbrSum, c);
+}
+
+// Sum of the neighbour counts of every particle slot (statistics, on demand): one partial per
+// workgroup, summed by the host.
+__global__ __launch_bounds__(256) void k_count_sum(const DeviceState d, unsigned long long* part) {
+    __shared__ unsigned long long sm[256];
+    unsigned long long t = 0;
+    for (uint32_t i = blockIdx.x * 256 + threadIdx.x; i < d.nslots; i += gridDim.x * 256)
+        if (i < scene_of(d, i).end) t += d.cnt[i];
+    sm[threadIdx.x] = t;
+    __syncthreads();
+    for (int off = 128; off > 0; off >>= 1) {
+        if ((int)threadIdx.x < off) sm[threadIdx.x] += sm[threadIdx.x + off];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) part[blockIdx.x] = sm[0];
 }
 
 // kernel_calculate_densities (ConstraintSolverCUDA.cu:16-42) and kernel_calculate_lambdas
@@ -824,6 +836,9 @@ void launch_density_lambda(hipStream_t st, const DeviceState& d, const SphParams
                            const float4* Xs, int bs, int residSlot, bool gated) {
     AKUA_BY_BLOCK(bs, (k_density_lambda<BS><<<grid_xcd(d.nslots, BS), BS, 0, st>>>(
                           d, P, Xs, residSlot, gated ? 1 : 0)));
+}
+void launch_count_sum(hipStream_t st, const DeviceState& d, unsigned long long* part, uint32_t blocks) {
+    k_count_sum<<<blocks, 256, 0, st>>>(d, part);
 }
 void launch_features(hipStream_t st, const DeviceState& d, const SphParams& P, bool record, int bs) {
     AKUA_BY_BLOCK(bs, (k_features<BS><<<grid_xcd(d.nslots, BS), BS, 0, st>>>(d, P, record ? 1 : 0)));

@@ -115,6 +115,8 @@ struct DevBuf {
     ~DevBuf() { release(); }
 };
 
+constexpr uint32_t kCountBlocks = 512;
+
 struct TimedLaunch {
     const char* name;
     hipEvent_t a, b;
@@ -347,7 +349,7 @@ void setup(akua_solver* s, const std::vector<uint32_t>& counts) {
     s->cnt.alloc(ns);
     s->dTiles.alloc(s->tilesHost.size());
     s->dSortScenes.alloc(s->sortScenesHost.size());
-    s->nbrSum.alloc(1);
+    s->nbrSum.alloc(kCountBlocks);
     s->probeRho.alloc(ns);
     s->probeMax.alloc(s->scenes.size());
     s->probeSum.alloc(s->scenes.size());
@@ -356,7 +358,7 @@ void setup(akua_solver* s, const std::vector<uint32_t>& counts) {
                         hipMemcpyHostToDevice));
     HIP_CHECK(hipMemcpy(s->dSortScenes.p, s->sortScenesHost.data(),
                         s->sortScenesHost.size() * sizeof(SortScene), hipMemcpyHostToDevice));
-    HIP_CHECK(hipMemset(s->nbrSum.p, 0, sizeof(unsigned long long)));
+    HIP_CHECK(hipMemset(s->nbrSum.p, 0, kCountBlocks * sizeof(unsigned long long)));
     HIP_CHECK(hipMemset(s->W.p, 0, ns * sizeof(float4)));
     HIP_CHECK(hipMemset(s->s1KA.p, 0, ns * 4));
     HIP_CHECK(hipMemset(s->lidT.p, 0, ns * 4));
@@ -1765,11 +1767,16 @@ akua_status akua_debug_probe_density(akua_solver* s, float* rho, int32_t n) {
 akua_status akua_solver_mean_neighbours(akua_solver* s, double* mean) {
     return guarded([&] {
         if (!s || !mean) fail(AKUA_ERR_INVALID_ARG, "NULL argument");
-        auto v = download(s, s->nbrSum.p, 1);
-        double steps = (double)s->stepsSinceStats;
-        *mean = steps > 0 ? (double)v[0] / (steps * (double)s->nparticles) : 0.0;
-        HIP_CHECK(hipMemset(s->nbrSum.p, 0, sizeof(unsigned long long)));
-        s->stepsSinceStats = 0;
+        if (s->stepsSinceStats == 0) {
+            *mean = 0.0;
+            return;
+        }
+        launch_count_sum(s->stream, s->devState(), s->nbrSum.p, kCountBlocks);
+        HIP_CHECK(hipGetLastError());
+        auto v = download(s, s->nbrSum.p, kCountBlocks);
+        unsigned long long t = 0;
+        for (auto x : v) t += x;
+        *mean = (double)t / (double)s->nparticles;
     });
 }
 

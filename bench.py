@@ -106,9 +106,10 @@ def time_steps(s, dt, bmin, bmax, steps: int) -> float:
     return time.perf_counter() - t0
 
 
-def learned_comparison(log_steps: int = 300, sample: int = 25, skip: int = 50):
+def learned_comparison(log_steps: int = 300, sample: int = 25, skip: int = 50, only: str = None):
     """Plain solver with 1-4 iterations and learned warm start with J = 1, 2 on the 1M scene with
-    the reference application's parameters at a 2 ms substep."""
+    the reference application's parameters at a 2 ms substep.  only: run just one configuration
+    ("plain_I<n>" or "learned_J<n>") and return its record."""
     from dataclasses import replace
     from akuaengine_amd import learned, scenes
     from akuaengine_amd.config import LambdaCorrParams, PBFConfig
@@ -156,6 +157,9 @@ def learned_comparison(log_steps: int = 300, sample: int = 25, skip: int = 50):
             f"error max {r['density_error_max_avg']:.4f} mean {r['density_error_mean_avg']:.5f}")
         return r
 
+    if only:
+        kind, n = only.split("_")
+        return {only: run(int(n[1:])) if kind == "plain" else run(4, int(n[1:]))}
     plain = {f"plain_I{I}": run(I) for I in (4, 3, 2, 1)}
     learned_runs = {f"learned_J{J}": run(4, J) for J in (1, 2)}
     out = {"workload": "dam_break_1m_reference_substep", "particles": sc.n, "dt": dt,
@@ -226,12 +230,19 @@ def main() -> int:
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-learned", action="store_true")
     ap.add_argument("--no-extra", action="store_true")
+    ap.add_argument("--learned-run", default=None,
+                    help="run only this configuration of the learned comparison (plain_I1..4, "
+                         "learned_J1, learned_J2), print its record and exit (for profiling)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU: every box in this setup has one MI355X, so --gpus "
               f"{args.gpus} is not supported (the multi-GPU decomposition is a design note only).",
               file=sys.stderr)
         return 2
+
+    if args.learned_run:
+        print(json.dumps(learned_comparison(only=args.learned_run)))
+        return 0
 
     from akuaengine_amd import _lib, scenes
     from akuaengine_amd.config import DAM_BREAK_DT, dam_break_params
@@ -248,7 +259,6 @@ def main() -> int:
 
     s.step_n(dt, sc.box_min, sc.box_max, args.warmup)
     s.synchronize()
-    s.mean_neighbours()  # resets the neighbour statistics
     log(f"[bench] warm-up {args.warmup} steps done; timing {args.steps} steps")
     t0 = time.perf_counter()
     s.step_n(dt, sc.box_min, sc.box_max, args.steps)

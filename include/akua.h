@@ -298,7 +298,8 @@ akua_status akua_stream_copy_bandwidth(int64_t bytes, int32_t reps, double* gbps
  * [3]: square root over squared lengths in [0, 2^-2).  All four are 0 on a conforming device. */
 akua_status akua_selftest_arith(uint64_t n, uint64_t seed, uint64_t mismatches[4]);
 
-/* Mean neighbour count of the last step (device reduction), for the byte model. */
+/* Mean neighbour count over the particles of the last step (device reduction, on demand), for
+ * the byte model; 0 before the first step. */
 akua_status akua_solver_mean_neighbours(akua_solver* s, double* mean);
 
 #ifdef __cplusplus
