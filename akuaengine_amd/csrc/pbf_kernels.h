@@ -19,9 +19,10 @@
 //   lid  uint32                   logical index                         4 B
 // Per-step scratch: X0 (old positions), XsA/XsB (x* ping-pong), W (vorticity and its length),
 // Vc (velocity after vorticity confinement), two sorts' keys/values, the cell table, and the
-// neighbour list.  The list holds physical indices and is wave-interleaved: entry k of slot p is
-// at ((p / 64) * maxN + k) * 64 + p % 64, so the 64 lanes of a wave read entry k of their 64
-// lists as one 256-byte line.
+// neighbour list.  The list holds physical indices in chunks of four entries, wave-interleaved:
+// chunk c (entries 4c..4c+3) of slot p is the uint4 at ((p / 64) * ceil(maxN / 4) + c) * 64 +
+// p % 64, so the 64 lanes of a wave read chunk c of their 64 lists as one 1 KB block and a lane
+// writes its list in whole 16-byte chunks.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -59,13 +60,12 @@ struct DeviceState {
     const uint32_t* perm1;     // logical index -> previous logical index
     uint32_t* keysM;           // super-cell Morton keys in logical order (second sort's input)
     const uint32_t* perm2;     // physical slot -> logical index
-    uint32_t* tkey;            // cell table keys (all scenes)
-    uint4* tinfo;              // cell table entries: (first logical index, one past the last,
-                               // first physical slot, 1 if the bucket's physical slots are
-                               // consecutive in logical order)
+    uint4* table;              // cell table records (all scenes): (hash key, first logical
+                               // index, one past the last, first physical slot | 2^31 if the
+                               // bucket's physical slots are consecutive in logical order)
     uint32_t* bslot;           // table slot of the bucket a logical index starts (first only)
     uint32_t tableTotal;
-    uint32_t* nbr;
+    uint4* nbr4;               // neighbour lists in four-entry chunks (see list_base)
     uint32_t* cnt;
     unsigned long long* nbrSum;  // per-workgroup partial sums of neighbour counts (statistics)
     // learned warm start (null when unused)
@@ -91,6 +91,8 @@ constexpr int kResidSum = 4;
 constexpr int kBlock = 256;
 
 void launch_fill_u32(hipStream_t st, uint32_t* p, uint32_t n, uint32_t v);
+// Every record of the cell table empty.
+void launch_clear_table(hipStream_t st, uint4* table, uint32_t n);
 void launch_predict_hash(hipStream_t st, const DeviceState& d, const SphParams& P,
                          const StepParams& S);
 void launch_cell_keys(hipStream_t st, const DeviceState& d, const SphParams& P,

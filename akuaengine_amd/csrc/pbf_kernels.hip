@@ -34,27 +34,41 @@ __device__ __forceinline__ const SceneDev& scene_of(const DeviceState& d, uint32
     return d.scenes[d.chunkScene[i >> 8]];
 }
 
+// The neighbour list in chunks of four entries: chunk c (entries 4c..4c+3) of slot i is the
+// uint4 at list_base(i) + c * 64, so the 64 lanes of a wave read chunk c of their lists as one
+// 1 KB block and every lane writes whole 16-byte chunks.
+__device__ __forceinline__ uint32_t list_chunks(int maxN) { return ((uint32_t)maxN + 3u) >> 2; }
 __device__ __forceinline__ size_t list_base(uint32_t i, int maxN) {
-    return (size_t)(i >> 6) * (size_t)maxN * 64u + (i & 63u);
+    return (size_t)(i >> 6) * (size_t)list_chunks(maxN) * 64u + (i & 63u);
 }
 
 // Visits the n entries of a neighbour list in order.  Memory requests are issued kListBatch at a
 // time (list entries, then the neighbours' data through load(j)), then body(value) runs for each
 // entry in list order, so every per-particle sum keeps its reference order.
 constexpr int kListBatch = 8;
+// per-kernel batch: 4 where the 8-entry batch's registers would cost occupancy
+constexpr int kBatchDelta = 4, kBatchVort = 4, kBatchConf = 4, kBatchXsph = 8;
 
-template <class T, class Load, class Body>
-__device__ __forceinline__ void walk_list(const uint32_t* __restrict__ lp, uint32_t n, Load load,
+template <class T, int B = kListBatch, class Load, class Body>
+__device__ __forceinline__ void walk_list(const uint4* __restrict__ lp, uint32_t n, Load load,
                                           Body body) {
-    for (uint32_t k0 = 0; k0 < n; k0 += kListBatch) {
-        uint32_t j[kListBatch];
+    static_assert(B == 4 || B == 8, "one or two chunks per batch");
+    for (uint32_t k0 = 0; k0 < n; k0 += B) {
+        uint32_t j[8];
+        const uint4 a = lp[(size_t)(k0 >> 2) * 64u];
+        j[0] = a.x; j[1] = a.y; j[2] = a.z; j[3] = a.w;
+        if (B == 8) {
+            const uint4 b = k0 + 4u < n ? lp[(size_t)((k0 >> 2) + 1u) * 64u] : a;
+            j[4] = b.x; j[5] = b.y; j[6] = b.z; j[7] = b.w;
+        }
+        // entries past n are not list entries: load a valid one in their place (discarded)
 #pragma unroll
-        for (int u = 0; u < kListBatch; ++u) j[u] = lp[(size_t)min(k0 + u, n - 1u) * 64u];
-        T v[kListBatch];
+        for (int u = 1; u < B; ++u) j[u] = k0 + u < n ? j[u] : j[0];
+        T v[B];
 #pragma unroll
-        for (int u = 0; u < kListBatch; ++u) v[u] = load(j[u]);
+        for (int u = 0; u < B; ++u) v[u] = load(j[u]);
 #pragma unroll
-        for (int u = 0; u < kListBatch; ++u)
+        for (int u = 0; u < B; ++u)
             if (k0 + u < n) body(v[u]);
     }
 }
@@ -88,6 +102,11 @@ __device__ __forceinline__ uint32_t xcd_slot() {
 __global__ void k_fill_u32(uint32_t* __restrict__ p, uint32_t n, uint32_t v) {
     uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) p[i] = v;
+}
+
+__global__ void k_fill_u4(uint4* __restrict__ p, uint32_t n, uint32_t v) {
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = make_uint4(v, v, v, v);
 }
 
 __device__ __forceinline__ uint32_t spread3(uint32_t x) {
@@ -129,13 +148,13 @@ __global__ __launch_bounds__(kBlock) void k_predict_hash(DeviceState d, SphParam
     d.keysLay[k] = layout_key(cx, cy, cz, sc);
 }
 
-__device__ __forceinline__ uint32_t table_insert(uint32_t* tkey, const SceneDev& sc, uint32_t h) {
+__device__ __forceinline__ uint32_t table_insert(uint4* table, const SceneDev& sc, uint32_t h) {
     const uint32_t mask = (1u << sc.tabLog2) - 1u;
     uint32_t s = table_slot(h, sc.tabLog2);
     // The table holds at least twice as many slots as the scene has particles, so an empty
     // slot is always found; the probe count is bounded all the same.
     for (uint32_t probe = 0; probe <= mask; ++probe) {
-        uint32_t prev = atomicCAS(&tkey[sc.tabOffset + s], kEmptyKey, h);
+        uint32_t prev = atomicCAS(&table[sc.tabOffset + s].x, kEmptyKey, h);
         if (prev == kEmptyKey || prev == h) return sc.tabOffset + s;
         s = (s + 1u) & mask;
     }
@@ -158,10 +177,10 @@ __global__ __launch_bounds__(kBlock) void k_cell_keys(DeviceState d, SphParams P
     const bool first = (k == sc.begin) || (d.skeys[k - 1] != h);
     const bool last = (k + 1 == sc.end) || (d.skeys[k + 1] != h);
     if (first || last) {
-        uint32_t slot = table_insert(d.tkey, sc, h);
+        uint32_t slot = table_insert(d.table, sc, h);
         if (slot != kEmptyKey) {
-            if (first) d.tinfo[slot].x = k;
-            if (last) d.tinfo[slot].y = k + 1;
+            if (first) d.table[slot].y = k;
+            if (last) d.table[slot].z = k + 1;
         }
         if (first) d.bslot[k] = slot;
     }
@@ -203,17 +222,18 @@ __global__ __launch_bounds__(kBlock) void k_bucket_phys(DeviceState d) {
     if (!(k == sc.begin || d.skeys[k - 1] != d.skeys[k])) return;
     const uint32_t slot = d.bslot[k];
     if (slot >= d.tableTotal) return;
-    const uint32_t len = d.tinfo[slot].y - k;
+    const uint32_t len = d.table[slot].z - k;
     const uint32_t p0 = d.physT[k];
     bool run = len <= 256u && k + len <= sc.end;
     for (uint32_t t = 1; run && t < len; ++t) run = d.physT[k + t] == p0 + t;
-    d.tinfo[slot].z = p0;
-    d.tinfo[slot].w = run ? 1u : 0u;
+    d.table[slot].w = p0 | (run ? 0x80000000u : 0u);
 }
 
-// A bucket's entry with its ranges clamped to the scene (a consistent table never needs it, but
-// no entry can then index outside the scene).
-__device__ __forceinline__ uint4 clamp_entry(uint4 e, const SceneDev& sc) {
+// A bucket's entry (first logical index, one past the last, first physical slot, contiguity)
+// from its table record, with its ranges clamped to the scene (a consistent table never needs
+// it, but no entry can then index outside the scene).
+__device__ __forceinline__ uint4 clamp_entry(uint4 r, const SceneDev& sc) {
+    uint4 e = make_uint4(r.y, r.z, r.w & 0x7FFFFFFFu, r.w >> 31);
     e.x = max(e.x, sc.begin);
     e.y = min(e.y, sc.end);
     if (e.y < e.x) e.y = e.x;
@@ -221,18 +241,17 @@ __device__ __forceinline__ uint4 clamp_entry(uint4 e, const SceneDev& sc) {
     return e;
 }
 
-__device__ __forceinline__ bool table_find(const uint32_t* __restrict__ tkey,
-                                           const uint4* __restrict__ tinfo, const SceneDev& sc,
+__device__ __forceinline__ bool table_find(const uint4* __restrict__ table, const SceneDev& sc,
                                            uint32_t h, uint4& entry) {
     const uint32_t mask = (1u << sc.tabLog2) - 1u;
     uint32_t s = table_slot(h, sc.tabLog2);
     for (uint32_t probe = 0; probe <= mask; ++probe) {
-        uint32_t k = tkey[sc.tabOffset + s];
-        if (k == h) {
-            entry = clamp_entry(tinfo[sc.tabOffset + s], sc);
+        const uint4 r = table[sc.tabOffset + s];
+        if (r.x == h) {
+            entry = clamp_entry(r, sc);
             return true;
         }
-        if (k == kEmptyKey) return false;
+        if (r.x == kEmptyKey) return false;
         s = (s + 1u) & mask;
     }
     return false;
@@ -247,6 +266,7 @@ __device__ __forceinline__ bool table_find(const uint32_t* __restrict__ tkey,
 // one dx slab together, then each bucket's candidates eight at a time (position and physical
 // slot), after which the candidates are tested and appended strictly in order.
 constexpr int kWalkBatch = 8;
+constexpr int kProbeGroup = 3;
 
 template <int BS>
 __global__ __launch_bounds__(BS) void k_neighbours(DeviceState d, SphParams P) {
@@ -258,31 +278,34 @@ __global__ __launch_bounds__(BS) void k_neighbours(DeviceState d, SphParams P) {
             const f3 xi = make3(d.XsA[p]);
             int cx, cy, cz;
             discretize(xi, P.cellSearch, cx, cy, cz);
-            uint32_t* out = d.nbr + list_base(p, P.maxN);
+            uint4* out = d.nbr4 + list_base(p, P.maxN);
+            uint32_t b0 = 0u, b1 = 0u, b2 = 0u, b3 = 0u, bn = 0u, chunk = 0u;
             const uint32_t maxN = (uint32_t)P.maxN;
-            for (int dx = -1; dx <= 1 && count < maxN; ++dx) {
-                uint32_t hq[9], sl[9], kq[9];
-                uint4 e[9];
+            // stencil cells c = 0..26 in (dx, dy, dz) order, probed kProbeGroup at a time
+            for (int g = 0; g < 27 / kProbeGroup && count < maxN; ++g) {
+                uint32_t hq[kProbeGroup], sl[kProbeGroup];
+                uint4 rq[kProbeGroup], e[kProbeGroup];
 #pragma unroll
-                for (int q = 0; q < 9; ++q) {
-                    hq[q] = cell_hash((int)((uint32_t)cx + (uint32_t)dx),
-                                      (int)((uint32_t)cy + (uint32_t)(q / 3 - 1)),
-                                      (int)((uint32_t)cz + (uint32_t)(q % 3 - 1)), sc);
+                for (int q = 0; q < kProbeGroup; ++q) {
+                    const int c = g * kProbeGroup + q;
+                    hq[q] = cell_hash((int)((uint32_t)cx + (uint32_t)(c / 9 - 1)),
+                                      (int)((uint32_t)cy + (uint32_t)((c / 3) % 3 - 1)),
+                                      (int)((uint32_t)cz + (uint32_t)(c % 3 - 1)), sc);
                     sl[q] = sc.tabOffset + table_slot(hq[q], sc.tabLog2);
                 }
 #pragma unroll
-                for (int q = 0; q < 9; ++q) kq[q] = d.tkey[sl[q]];
+                for (int q = 0; q < kProbeGroup; ++q) rq[q] = d.table[sl[q]];
 #pragma unroll
-                for (int q = 0; q < 9; ++q) {
+                for (int q = 0; q < kProbeGroup; ++q) {
                     e[q] = make_uint4(0u, 0u, 0u, 0u);
-                    if (kq[q] == hq[q]) {
-                        e[q] = clamp_entry(d.tinfo[sl[q]], sc);
-                    } else if (kq[q] != kEmptyKey) {
-                        if (!table_find(d.tkey, d.tinfo, sc, hq[q], e[q])) e[q] = make_uint4(0u, 0u, 0u, 0u);
+                    if (rq[q].x == hq[q]) {
+                        e[q] = clamp_entry(rq[q], sc);
+                    } else if (rq[q].x != kEmptyKey) {
+                        if (!table_find(d.table, sc, hq[q], e[q])) e[q] = make_uint4(0u, 0u, 0u, 0u);
                     }
                 }
 #pragma unroll
-                for (int q = 0; q < 9; ++q) {
+                for (int q = 0; q < kProbeGroup; ++q) {
                     const uint32_t a = e[q].x, b = e[q].y;
                     // bucket of consecutive physical slots: read the cell-ordered positions
                     // directly; otherwise go through the logical-order copy
@@ -313,12 +336,21 @@ __global__ __launch_bounds__(BS) void k_neighbours(DeviceState d, SphParams P) {
                         const uint32_t room = maxN - count;
                         while ((uint32_t)__popc(acc) > room) acc &= ~(1u << (31 - __clz(acc)));
                         if (acc) {
-                            uint32_t* o = out + (size_t)count * 64u;
+                            // append in order through a four-entry register buffer, storing
+                            // whole chunks
 #pragma unroll
                             for (int u = 0; u < kWalkBatch; ++u) {
                                 if (acc & (1u << u)) {
-                                    *o = ph[u];
-                                    o += 64;
+                                    const uint32_t v = ph[u];
+                                    b0 = bn == 0u ? v : b0;
+                                    b1 = bn == 1u ? v : b1;
+                                    b2 = bn == 2u ? v : b2;
+                                    b3 = bn == 3u ? v : b3;
+                                    if (++bn == 4u) {
+                                        out[(size_t)chunk * 64u] = make_uint4(b0, b1, b2, b3);
+                                        ++chunk;
+                                        bn = 0u;
+                                    }
                                 }
                             }
                             count += (uint32_t)__popc(acc);
@@ -326,6 +358,7 @@ __global__ __launch_bounds__(BS) void k_neighbours(DeviceState d, SphParams P) {
                     }
                 }
             }
+            if (bn) out[(size_t)chunk * 64u] = make_uint4(b0, bn > 1u ? b1 : 0u, bn > 2u ? b2 : 0u, 0u);
         }
         d.cnt[p] = count;
     }
@@ -383,7 +416,7 @@ __global__ __launch_bounds__(BS) void k_density_lambda(DeviceState d, SphParams 
         const float4 xi4 = Xs[i];
         const f3 xi = make3(xi4);
         const uint32_t n = d.cnt[i];
-        const uint32_t* lp = d.nbr + list_base(i, P.maxN);
+        const uint4* lp = d.nbr4 + list_base(i, P.maxN);
         float rho = xi4.w * P.w0;
         f3 gi{0.0f, 0.0f, 0.0f};
         float sum = 0.0f;
@@ -428,7 +461,7 @@ __global__ __launch_bounds__(BS) void k_features(DeviceState d, SphParams P, int
         const float4 xi4 = d.XsA[i];
         const f3 xi = make3(xi4);
         const uint32_t n = d.cnt[i];
-        const uint32_t* lp = d.nbr + list_base(i, P.maxN);
+        const uint4* lp = d.nbr4 + list_base(i, P.maxN);
         float rho = xi4.w * P.w0;
         f3 gi{0.0f, 0.0f, 0.0f}, ac{0.0f, 0.0f, 0.0f};
         float sum = 0.0f;
@@ -606,10 +639,10 @@ __global__ __launch_bounds__(BS) void k_delta_correct(DeviceState d, SphParams P
     if (kApplyDelta) {
         const float li = d.lam[i];
         const uint32_t n = d.cnt[i];
-        const uint32_t* lp = d.nbr + list_base(i, P.maxN);
+        const uint4* lp = d.nbr4 + list_base(i, P.maxN);
         f3 dp{0.0f, 0.0f, 0.0f};
         const Recip Rdq = recip(P.wdq);
-        walk_list<XLam>(
+        walk_list<XLam, kBatchDelta>(
             lp, n, [&](uint32_t j) { return XLam{XsIn[j], d.lam[j]}; },
             [&](const XLam& nb) {
                 f3 sep = xi - make3(nb.x);
@@ -658,9 +691,9 @@ __global__ __launch_bounds__(BS) void k_vorticity(DeviceState d, SphParams P) {
     const f3 xi = make3(d.X[i]);
     const f3 vi = make3(d.V[i]);
     const uint32_t n = d.cnt[i];
-    const uint32_t* lp = d.nbr + list_base(i, P.maxN);
+    const uint4* lp = d.nbr4 + list_base(i, P.maxN);
     f3 w{0.0f, 0.0f, 0.0f};
-    walk_list<XV>(
+    walk_list<XV, kBatchVort>(
         lp, n, [&](uint32_t j) { return XV{d.X[j], d.V[j]}; },
         [&](const XV& nb) {
             f3 vij = make3(nb.v) - vi;
@@ -680,9 +713,9 @@ __global__ __launch_bounds__(BS) void k_confinement(DeviceState d, SphParams P, 
     const float4 wi = d.W[i];
     const float invD = 1.0f / d.dens[i];
     const uint32_t n = d.cnt[i];
-    const uint32_t* lp = d.nbr + list_base(i, P.maxN);
+    const uint4* lp = d.nbr4 + list_base(i, P.maxN);
     f3 eta{0.0f, 0.0f, 0.0f};
-    walk_list<XW>(
+    walk_list<XW, kBatchConf>(
         lp, n, [&](uint32_t j) { return XW{d.X[j], d.W[j].w}; },
         [&](const XW& nb) {
             float diff = wi.w - nb.w;
@@ -710,9 +743,9 @@ __global__ __launch_bounds__(BS) void k_xsph(DeviceState d, SphParams P) {
     const f3 xi = make3(d.X[i]);
     const f3 vi = make3(d.Vc[i]);
     const uint32_t n = d.cnt[i];
-    const uint32_t* lp = d.nbr + list_base(i, P.maxN);
+    const uint4* lp = d.nbr4 + list_base(i, P.maxN);
     f3 dv{0.0f, 0.0f, 0.0f};
-    walk_list<XVRho>(
+    walk_list<XVRho, kBatchXsph>(
         lp, n, [&](uint32_t j) { return XVRho{d.X[j], d.Vc[j], d.dens[j]}; },
         [&](const XVRho& nb) {
             f3 vij = make3(nb.v) - vi;
@@ -760,7 +793,7 @@ __global__ __launch_bounds__(BS) void k_density_probe(DeviceState d, SphParams P
 #pragma unroll
                 for (int r = 0; r < q; ++r) dup |= hs[r] == hs[q];
                 uint4 e;
-                if (dup || !table_find(d.tkey, d.tinfo, sc, hs[q], e)) continue;
+                if (dup || !table_find(d.table, sc, hs[q], e)) continue;
                 const int32_t shift = (int32_t)e.z - (int32_t)e.x;
                 for (uint32_t m = e.x; m < e.y; ++m) {
                     if (m == self) continue;
@@ -814,6 +847,9 @@ inline unsigned grid_xcd(uint32_t n, int bs) {
 
 void launch_fill_u32(hipStream_t st, uint32_t* p, uint32_t n, uint32_t v) {
     if (n) k_fill_u32<<<grid(n), kBlock, 0, st>>>(p, n, v);
+}
+void launch_clear_table(hipStream_t st, uint4* table, uint32_t n) {
+    if (n) k_fill_u4<<<grid(n), kBlock, 0, st>>>(table, n, kEmptyKey);
 }
 void launch_predict_hash(hipStream_t st, const DeviceState& d, const SphParams& P,
                          const StepParams& S) {

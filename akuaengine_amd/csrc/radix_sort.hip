@@ -50,56 +50,47 @@ __device__ __forceinline__ uint32_t block_exclusive_scan_256(uint32_t v, uint32_
     return base + inc - v;
 }
 
-__global__ void rs_zero(uint32_t* __restrict__ p, uint32_t n) {
-    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) p[i] = 0u;
-}
-
 template <int D>
 __global__ __launch_bounds__(kSortBlock) void rs_hist(const uint32_t* __restrict__ keys,
                                                       const SortTile* __restrict__ tiles,
-                                                      uint32_t shift, uint32_t* __restrict__ counts,
-                                                      uint32_t* __restrict__ totals) {
+                                                      uint32_t shift, uint32_t* __restrict__ counts) {
     constexpr uint32_t R = 1u << D;
-    __shared__ uint32_t hist[R];
+    constexpr int kWaves = kSortBlock / 64;
+    __shared__ uint32_t hist[kWaves][R];  // one histogram per wave: fewer atomic collisions
     const SortTile t = tiles[blockIdx.x];
-    for (uint32_t d = threadIdx.x; d < R; d += kSortBlock) hist[d] = 0;
+    const uint32_t wave = threadIdx.x >> 6;
+    for (uint32_t d = threadIdx.x; d < R; d += kSortBlock)
+        for (int w = 0; w < kWaves; ++w) hist[w][d] = 0;
     __syncthreads();
-    const uint32_t lane = lane_id();
-#pragma unroll 4
+    uint32_t dg[kSortItems];
+#pragma unroll
     for (int it = 0; it < kSortItems; ++it) {
-        uint32_t idx = t.begin + it * kSortBlock + threadIdx.x;
-        bool valid = idx < t.end;
-        uint32_t d = valid ? (keys[idx] >> shift) & (R - 1) : 0u;
-        uint64_t vmask = __ballot(valid);
-        uint64_t peers = match_digit<D>(d, vmask);
-        // the highest lane of each group adds the group's size
-        if (valid && (peers >> lane) == 1ull) atomicAdd(&hist[d], (uint32_t)__popcll(peers));
+        const uint32_t idx = t.begin + it * kSortBlock + threadIdx.x;
+        dg[it] = idx < t.end ? (keys[idx] >> shift) & (R - 1) : 0xFFFFFFFFu;
     }
+#pragma unroll
+    for (int it = 0; it < kSortItems; ++it)
+        if (dg[it] != 0xFFFFFFFFu) atomicAdd(&hist[wave][dg[it]], 1u);
     __syncthreads();
     for (uint32_t d = threadIdx.x; d < R; d += kSortBlock) {
-        const uint32_t c = hist[d];
+        uint32_t c = 0;
+        for (int w = 0; w < kWaves; ++w) c += hist[w][d];
         counts[t.countBase * R + d * t.sceneTiles + t.localTile] = c;
-        if (c) atomicAdd(&totals[t.scene * R + d], c);
     }
 }
 
-// One wave per (scene, digit): offsets[scene][digit][tile] = first output slot of the tile's keys
-// with that digit (the scene's first slot, plus the scene's keys with a smaller digit, plus the
-// keys of this digit in earlier tiles).
+// One wave per (scene, digit): offsets[scene][digit][tile] = the keys of this digit in earlier
+// tiles of the scene, and totals[scene][digit] = the scene's keys with this digit.  The scatter
+// adds the scene's first slot and the keys of smaller digits (no atomics anywhere).
 template <int D>
 __global__ __launch_bounds__(64) void rs_scan(uint32_t* __restrict__ counts,
-                                              const uint32_t* __restrict__ totals,
+                                              uint32_t* __restrict__ totals,
                                               const SortScene* __restrict__ scenes) {
     constexpr uint32_t R = 1u << D;
     const uint32_t s = blockIdx.x / R, d = blockIdx.x % R;
     const SortScene sc = scenes[s];
     const uint32_t lane = lane_id();
-    uint32_t below = 0;
-    for (uint32_t q = lane; q < d; q += 64) below += totals[s * R + q];
-#pragma unroll
-    for (int off = 32; off > 0; off >>= 1) below += __shfl_xor(below, off, 64);
-    uint32_t carry = sc.begin + below;
+    uint32_t carry = 0;
     uint32_t* c = counts + (size_t)sc.countBase * R + (size_t)d * sc.tiles;
     for (uint32_t base = 0; base < sc.tiles; base += 64) {
         const uint32_t t = base + lane;
@@ -113,6 +104,7 @@ __global__ __launch_bounds__(64) void rs_scan(uint32_t* __restrict__ counts,
         if (t < sc.tiles) c[t] = carry + inc - v;
         carry += __shfl(inc, 63, 64);
     }
+    if (lane == 0) totals[s * R + d] = carry;
 }
 
 template <int D, bool kIdentityVals>
@@ -120,7 +112,7 @@ __global__ __launch_bounds__(kSortBlock) void rs_scatter(
     const uint32_t* __restrict__ keysIn, const uint32_t* __restrict__ valsIn,
     uint32_t* __restrict__ keysOut, uint32_t* __restrict__ valsOut,
     const SortTile* __restrict__ tiles, uint32_t shift, const uint32_t* __restrict__ offsets,
-    uint32_t limit) {
+    const uint32_t* __restrict__ totals, uint32_t limit) {
     constexpr uint32_t R = 1u << D;
     constexpr int kWaves = kSortBlock / 64;
     constexpr int kRounds = kSortTile / kSortBlock;  // rounds of 64 keys per wave
@@ -172,9 +164,12 @@ __global__ __launch_bounds__(kSortBlock) void rs_scatter(
             }
         }
         uint32_t start = block_exclusive_scan_256(d < R ? tot : 0u, scanTmp);
+        // the scene's keys with a smaller digit
+        const uint32_t below = block_exclusive_scan_256(d < R ? totals[t.scene * R + d] : 0u, scanTmp);
         if (d < R) {
             tileStart[d] = start;
-            globalBase[d] = offsets[t.countBase * R + d * t.sceneTiles + t.localTile];
+            const uint32_t sceneBegin = t.begin - t.localTile * (uint32_t)kSortTile;
+            globalBase[d] = sceneBegin + below + offsets[t.countBase * R + d * t.sceneTiles + t.localTile];
         }
     }
     __syncthreads();
@@ -211,16 +206,18 @@ void launch_pass(hipStream_t st, bool first, const uint32_t* kin, const uint32_t
                  const SortScene* scenes, uint32_t nscenes, uint32_t shift, uint32_t* counts,
                  uint32_t* totals, uint32_t limit, LaunchHook hook, void* ctx) {
     if (hook) hook(ctx, "rs_hist", 0);
-    rs_hist<D><<<ntiles, kSortBlock, 0, st>>>(kin, tiles, shift, counts, totals);
+    rs_hist<D><<<ntiles, kSortBlock, 0, st>>>(kin, tiles, shift, counts);
     if (hook) hook(ctx, "rs_hist", 1);
     if (hook) hook(ctx, "rs_scan", 0);
     rs_scan<D><<<nscenes * (1u << D), 64, 0, st>>>(counts, totals, scenes);
     if (hook) hook(ctx, "rs_scan", 1);
     if (hook) hook(ctx, "rs_scatter", 0);
     if (first)
-        rs_scatter<D, true><<<ntiles, kSortBlock, 0, st>>>(kin, vin, kout, vout, tiles, shift, counts, limit);
+        rs_scatter<D, true><<<ntiles, kSortBlock, 0, st>>>(kin, vin, kout, vout, tiles, shift, counts,
+                                                           totals, limit);
     else
-        rs_scatter<D, false><<<ntiles, kSortBlock, 0, st>>>(kin, vin, kout, vout, tiles, shift, counts, limit);
+        rs_scatter<D, false><<<ntiles, kSortBlock, 0, st>>>(kin, vin, kout, vout, tiles, shift, counts,
+                                                            totals, limit);
     if (hook) hook(ctx, "rs_scatter", 1);
 }
 
@@ -264,10 +261,6 @@ int radix_sort_pairs(hipStream_t stream, const SortPlan& plan, const SortBuffers
                      uint32_t numScenes, uint32_t limit, LaunchHook hook, void* hookCtx) {
     const int P = plan.passes();
     const uint32_t R = 1u << plan.digitBits;
-    const uint32_t nTotals = (uint32_t)P * R * numScenes;
-    if (hook) hook(hookCtx, "rs_zero", 0);
-    rs_zero<<<(nTotals + 255) / 256, 256, 0, stream>>>(buf.totals, nTotals);
-    if (hook) hook(hookCtx, "rs_zero", 1);
     uint32_t* kin = buf.keysA;
     uint32_t* vin = buf.valsA;
     uint32_t* kout = buf.keysB;

@@ -117,6 +117,9 @@ struct DevBuf {
 
 constexpr uint32_t kCountBlocks = 512;
 
+// Four-entry chunks per neighbour list (the kernels' list_chunks).
+inline size_t list_chunk_count(int maxN) { return ((size_t)maxN + 3) / 4; }
+
 struct TimedLaunch {
     const char* name;
     hipEvent_t a, b;
@@ -150,9 +153,9 @@ struct akua_solver {
     DevBuf<uint32_t> dChunkScene;
     DevBuf<float4> X, V, X0, XsA, XsB, XsL, W, Vc, color;
     DevBuf<float> dens, lam, Tdens, Tlam, size;
-    DevBuf<uint32_t> id, Tid, lid, phys, lidT, physT, srcL, keysLay, bslot, tkey, nbr, cnt;
+    DevBuf<uint32_t> id, Tid, lid, phys, lidT, physT, srcL, keysLay, bslot, nbr, cnt;
     DevBuf<uint32_t> s1KA, s1VA, s1KB, s1VB, s2KA, s2VA, s2KB, s2VB, counts, totals;
-    DevBuf<uint4> tinfo;
+    DevBuf<uint4> table;
     DevBuf<SortTile> dTiles;
     DevBuf<SortScene> dSortScenes;
     DevBuf<unsigned long long> nbrSum;
@@ -212,8 +215,8 @@ struct akua_solver {
         d.perm1 = r1 ? s1VB.p : s1VA.p;
         d.keysM = s2KA.p;
         d.perm2 = r2 ? s2VB.p : s2VA.p;
-        d.tkey = tkey.p; d.tinfo = tinfo.p; d.bslot = bslot.p; d.tableTotal = tableTotal;
-        d.nbr = nbr.p; d.cnt = cnt.p; d.nbrSum = nbrSum.p;
+        d.table = table.p; d.bslot = bslot.p; d.tableTotal = tableTotal;
+        d.nbr4 = reinterpret_cast<uint4*>(nbr.p); d.cnt = cnt.p; d.nbrSum = nbrSum.p;
         d.feat = feat.p; d.targ = targ.p; d.XsP = XsP.p; d.XsW = XsW.p; d.lamHat = lamHat.p;
         d.resid = resid.p; d.gate = gate.p; d.fallbacks = fallbacks.p;
         d.residPart = residPart.p; d.nWaves = (nslots + 63) / 64;
@@ -342,10 +345,9 @@ void setup(akua_solver* s, const std::vector<uint32_t>& counts) {
     s->s2KA.alloc(ns); s->s2VA.alloc(ns); s->s2KB.alloc(ns); s->s2VB.alloc(ns);
     s->counts.alloc((size_t)s->numTiles * (1u << kMaxDigitBits));
     s->totals.alloc((size_t)kMaxPasses * (1u << kMaxDigitBits) * s->scenes.size());
-    s->tkey.alloc(s->tableTotal);
-    s->tinfo.alloc(s->tableTotal);
+    s->table.alloc(s->tableTotal);
     s->bslot.alloc(ns);
-    s->nbr.alloc(nsList * (size_t)s->cfg.maxNeighbours);
+    s->nbr.alloc(nsList * (size_t)list_chunk_count(s->cfg.maxNeighbours) * 4u);
     s->cnt.alloc(ns);
     s->dTiles.alloc(s->tilesHost.size());
     s->dSortScenes.alloc(s->sortScenesHost.size());
@@ -364,7 +366,7 @@ void setup(akua_solver* s, const std::vector<uint32_t>& counts) {
     HIP_CHECK(hipMemset(s->lidT.p, 0, ns * 4));
     HIP_CHECK(hipMemset(s->srcL.p, 0, ns * 4));
     HIP_CHECK(hipMemset(s->bslot.p, 0xFF, ns * 4));
-    HIP_CHECK(hipMemset(s->tinfo.p, 0, (size_t)s->tableTotal * sizeof(uint4)));
+    HIP_CHECK(hipMemset(s->table.p, 0xFF, (size_t)s->tableTotal * sizeof(uint4)));
     HIP_CHECK(hipMemset(s->keysLay.p, 0, ns * 4));
     HIP_CHECK(hipMemset(s->XsL.p, 0, ns * sizeof(float4)));
     HIP_CHECK(hipMemset(s->physT.p, 0, ns * 4));
@@ -457,7 +459,7 @@ void enqueue_step(akua_solver* s, hipStream_t st, const StepParams& S, bool time
     DeviceState d = s->devState();
     const SphParams& P = s->sph;
     L.before("fill_table");
-    launch_fill_u32(st, s->tkey.p, s->tableTotal, kEmptyKey);
+    launch_clear_table(st, s->table.p, s->tableTotal);
     L.after("fill_table");
     L.before("predict_hash");
     launch_predict_hash(st, d, P, S);
@@ -652,7 +654,7 @@ void run_density_probe(akua_solver* s) {
     DeviceState d = s->devState();
     HIP_CHECK(hipMemsetAsync(s->probeMax.p, 0, s->scenes.size() * sizeof(unsigned int), st));
     HIP_CHECK(hipMemsetAsync(s->probeSum.p, 0, s->scenes.size() * sizeof(double), st));
-    launch_fill_u32(st, s->tkey.p, s->tableTotal, kEmptyKey);
+    launch_clear_table(st, s->table.p, s->tableTotal);
     launch_predict_hash(st, d, s->sph, S0);
     sort_hash(s, st, false);
     launch_cell_keys(st, d, s->sph, S0);
@@ -1646,8 +1648,7 @@ akua_status akua_debug_cell_ranges(akua_solver* s, uint32_t* hash, uint32_t* sta
         if (!s || !ncells) fail(AKUA_ERR_INVALID_ARG, "NULL argument");
         if (s->scenes.size() != 1) fail(AKUA_ERR_UNSUPPORTED, "cell ranges are per single-scene solver");
         auto keys = download(s, s->devState().skeys, s->nslots);
-        auto tk = download(s, s->tkey.p, s->tableTotal);
-        auto tr = download(s, s->tinfo.p, s->tableTotal);
+        auto tr = download(s, s->table.p, s->tableTotal);
         const SceneDev& sc = s->scenes[0];
         std::vector<uint32_t> H, S, E;
         for (uint32_t k = sc.begin; k < sc.end; ++k) {
@@ -1658,8 +1659,8 @@ akua_status akua_debug_cell_ranges(akua_solver* s, uint32_t* hash, uint32_t* sta
             uint32_t slot = (h * 2654435761u) >> (32u - sc.tabLog2);
             uint32_t st = 0xFFFFFFFFu, en = 0xFFFFFFFFu;
             for (uint32_t probe = 0; probe <= mask; ++probe) {
-                uint32_t kk = tk[sc.tabOffset + slot];
-                if (kk == h) { st = tr[sc.tabOffset + slot].x; en = tr[sc.tabOffset + slot].y; break; }
+                uint32_t kk = tr[sc.tabOffset + slot].x;
+                if (kk == h) { st = tr[sc.tabOffset + slot].y; en = tr[sc.tabOffset + slot].z; break; }
                 if (kk == kEmptyKey) break;
                 slot = (slot + 1u) & mask;
             }
@@ -1689,10 +1690,13 @@ akua_status akua_debug_neighbours(akua_solver* s, uint32_t* nbrOut, uint32_t* cn
                 const size_t k = base0 + (LI[i] - sc.begin);  // the row of particle i's logical index
                 if (cntOut) cntOut[k] = C[i];
                 if (!nbrOut) continue;
-                size_t base = (size_t)(i >> 6) * maxN * 64 + (i & 63);
+                // chunk c of slot i: the uint4 at ((i / 64) * chunks + c) * 64 + i % 64
+                const size_t chunks = list_chunk_count(maxN);
+                const size_t base = (size_t)(i >> 6) * chunks * 64 + (i & 63);
                 for (int j = 0; j < maxN; ++j)
                     nbrOut[k * maxN + j] =
-                        (uint32_t)j < C[i] ? LI[L[base + (size_t)j * 64]] - sc.begin : 0u;
+                        (uint32_t)j < C[i] ? LI[L[(base + (size_t)(j >> 2) * 64) * 4 + (j & 3)]] - sc.begin
+                                           : 0u;
             }
             base0 += sc.end - sc.begin;
         }
