@@ -85,11 +85,6 @@ struct XW {
     float4 x;
     float w;
 };
-struct XVRho {
-    float4 x;
-    float4 v;
-    float rho;
-};
 
 // Physical slot of this thread under the XCD-contiguous workgroup order (the grid is a multiple
 // of 8 workgroups of BS threads).
@@ -703,13 +698,17 @@ __global__ __launch_bounds__(BS) void k_vorticity(DeviceState d, SphParams P) {
     d.W[i] = make_float4(w.x, w.y, w.z, length3(w));
 }
 
-// kernel_apply_vorticity_confinement (IntegrationCUDA.cu:130-165), writing Vc.
+// kernel_apply_vorticity_confinement (IntegrationCUDA.cu:130-165), writing Vc: the confined
+// velocity, with the particle's m / rho in .w for the XSPH pass (the quotient the reference
+// forms per neighbour, IntegrationCUDA.cu:184, computed once per particle instead).
 template <int BS>
 __global__ __launch_bounds__(BS) void k_confinement(DeviceState d, SphParams P, StepParams S) {
     const uint32_t i = xcd_slot<BS>();
     if (i >= d.nslots) return;
     if (i >= scene_of(d, i).end) return;
-    const f3 xi = make3(d.X[i]);
+    const float4 xi4 = d.X[i];
+    const f3 xi = make3(xi4);
+    const float xi4w = xi4.w;
     const float4 wi = d.W[i];
     const float invD = 1.0f / d.dens[i];
     const uint32_t n = d.cnt[i];
@@ -731,6 +730,7 @@ __global__ __launch_bounds__(BS) void k_confinement(DeviceState d, SphParams P, 
         f3 vn = make3(v) + S.dt * f;
         v = make_float4(vn.x, vn.y, vn.z, 0.0f);
     }
+    v.w = xi4w / d.dens[i];
     d.Vc[i] = v;
 }
 
@@ -745,13 +745,13 @@ __global__ __launch_bounds__(BS) void k_xsph(DeviceState d, SphParams P) {
     const uint32_t n = d.cnt[i];
     const uint4* lp = d.nbr4 + list_base(i, P.maxN);
     f3 dv{0.0f, 0.0f, 0.0f};
-    walk_list<XVRho, kBatchXsph>(
-        lp, n, [&](uint32_t j) { return XVRho{d.X[j], d.Vc[j], d.dens[j]}; },
-        [&](const XVRho& nb) {
+    walk_list<XV, kBatchXsph>(
+        lp, n, [&](uint32_t j) { return XV{d.X[j], d.Vc[j]}; },
+        [&](const XV& nb) {
             f3 vij = make3(nb.v) - vi;
             f3 sep = xi - make3(nb.x);
             float w = poly6(dot3(sep, sep), P);
-            dv = dv + ((nb.x.w / nb.rho) * vij) * w;
+            dv = dv + (nb.v.w * vij) * w;  // nb.v.w = m_j / rho_j
         });
     f3 vn = vi + P.visc * dv;
     d.V[i] = make_float4(vn.x, vn.y, vn.z, 0.0f);

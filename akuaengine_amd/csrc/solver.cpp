@@ -99,13 +99,18 @@ uint32_t ceil_log2(uint64_t v) {
     return r;
 }
 
+// Owning device allocation (not copyable; alloc replaces any earlier allocation).
 template <class T>
 struct DevBuf {
     T* p = nullptr;
     size_t n = 0;
+    DevBuf() = default;
+    DevBuf(const DevBuf&) = delete;
+    DevBuf& operator=(const DevBuf&) = delete;
     void alloc(size_t count) {
-        n = count;
+        release();
         if (count) HIP_CHECK(hipMalloc(&p, count * sizeof(T)));
+        n = count;
     }
     void release() {
         if (p) (void)hipFree(p);
@@ -1109,11 +1114,31 @@ void autotune(akua_solver* s, float dt, const float* bmin, const float* bmax, in
         }
     }
     const bool timing = s->timing;
-    s->timing = false;
     const bool graph = s->useGraph;
-    s->useGraph = true;
+    const akua_tuning before = s->tuning;
     Snapshot q;
     take_snapshot(s, q);
+    s->timing = false;
+    s->useGraph = true;
+    // on any failure the solver is left as it was found: state, setting and launch mode
+    struct Restore {
+        akua_solver* s;
+        const Snapshot& q;
+        bool timing, graph;
+        akua_tuning tuning;
+        bool armed = true;
+        ~Restore() {
+            if (!armed) return;
+            try {
+                s->timing = timing;
+                s->useGraph = graph;
+                apply_tuning(s, tuning);
+                restore_snapshot(s, q);
+                (void)hipStreamSynchronize(s->stream);
+            } catch (...) {
+            }
+        }
+    } guard{s, q, timing, graph, before};
     akua_tuning best = s->tuning;
     apply_tuning(s, best);
     double bestMs = time_trial(s, q, dt, trialSteps);
@@ -1137,6 +1162,7 @@ void autotune(akua_solver* s, float dt, const float* bmin, const float* bmax, in
     apply_tuning(s, best);
     restore_snapshot(s, q);
     HIP_CHECK(hipStreamSynchronize(s->stream));
+    guard.armed = false;
     s->timing = timing;
     s->useGraph = graph;
     if (cachePath) cache_store(cachePath, key, best);
