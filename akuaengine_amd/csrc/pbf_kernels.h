@@ -21,8 +21,7 @@
 // Vc (velocity after vorticity confinement), two sorts' keys/values, the cell table, and the
 // neighbour list.  The list holds physical indices in chunks of four entries, wave-interleaved:
 // chunk c (entries 4c..4c+3) of slot p is the uint4 at ((p / 64) * ceil(maxN / 4) + c) * 64 +
-// p % 64, so the 64 lanes of a wave read chunk c of their 64 lists as one 1 KB block and a lane
-// writes its list in whole 16-byte chunks.
+// p % 64, so the 64 lanes of a wave read chunk c of their 64 lists as one 1 KB block.
 #pragma once
 
 #include <hip/hip_runtime.h>

@@ -36,7 +36,7 @@ __device__ __forceinline__ const SceneDev& scene_of(const DeviceState& d, uint32
 
 // The neighbour list in chunks of four entries: chunk c (entries 4c..4c+3) of slot i is the
 // uint4 at list_base(i) + c * 64, so the 64 lanes of a wave read chunk c of their lists as one
-// 1 KB block and every lane writes whole 16-byte chunks.
+// 1 KB block (one 16-byte load per lane for four entries).
 __device__ __forceinline__ uint32_t list_chunks(int maxN) { return ((uint32_t)maxN + 3u) >> 2; }
 __device__ __forceinline__ size_t list_base(uint32_t i, int maxN) {
     return (size_t)(i >> 6) * (size_t)list_chunks(maxN) * 64u + (i & 63u);
@@ -273,8 +273,8 @@ __global__ __launch_bounds__(BS) void k_neighbours(DeviceState d, SphParams P) {
             const f3 xi = make3(d.XsA[p]);
             int cx, cy, cz;
             discretize(xi, P.cellSearch, cx, cy, cz);
-            uint4* out = d.nbr4 + list_base(p, P.maxN);
-            uint32_t b0 = 0u, b1 = 0u, b2 = 0u, b3 = 0u, bn = 0u, chunk = 0u;
+            // entry k of this list: element k % 4 of chunk k / 4
+            uint32_t* out = reinterpret_cast<uint32_t*>(d.nbr4 + list_base(p, P.maxN));
             const uint32_t maxN = (uint32_t)P.maxN;
             // stencil cells c = 0..26 in (dx, dy, dz) order, probed kProbeGroup at a time
             for (int g = 0; g < 27 / kProbeGroup && count < maxN; ++g) {
@@ -331,21 +331,13 @@ __global__ __launch_bounds__(BS) void k_neighbours(DeviceState d, SphParams P) {
                         const uint32_t room = maxN - count;
                         while ((uint32_t)__popc(acc) > room) acc &= ~(1u << (31 - __clz(acc)));
                         if (acc) {
-                            // append in order through a four-entry register buffer, storing
-                            // whole chunks
+                            // append in order: the u-th candidate goes to position count + (the
+                            // accepted candidates before it)
 #pragma unroll
                             for (int u = 0; u < kWalkBatch; ++u) {
                                 if (acc & (1u << u)) {
-                                    const uint32_t v = ph[u];
-                                    b0 = bn == 0u ? v : b0;
-                                    b1 = bn == 1u ? v : b1;
-                                    b2 = bn == 2u ? v : b2;
-                                    b3 = bn == 3u ? v : b3;
-                                    if (++bn == 4u) {
-                                        out[(size_t)chunk * 64u] = make_uint4(b0, b1, b2, b3);
-                                        ++chunk;
-                                        bn = 0u;
-                                    }
+                                    const uint32_t k = count + (uint32_t)__popc(acc & ((1u << u) - 1u));
+                                    out[((size_t)(k >> 2) * 64u) * 4u + (k & 3u)] = ph[u];
                                 }
                             }
                             count += (uint32_t)__popc(acc);
@@ -353,7 +345,6 @@ __global__ __launch_bounds__(BS) void k_neighbours(DeviceState d, SphParams P) {
                     }
                 }
             }
-            if (bn) out[(size_t)chunk * 64u] = make_uint4(b0, bn > 1u ? b1 : 0u, bn > 2u ? b2 : 0u, 0u);
         }
         d.cnt[p] = count;
     }
