@@ -96,7 +96,10 @@ void launch_predict_hash(hipStream_t st, const DeviceState& d, const SphParams& 
                          const StepParams& S);
 void launch_cell_keys(hipStream_t st, const DeviceState& d, const SphParams& P,
                       const StepParams& S);
-void launch_gather(hipStream_t st, const DeviceState& d, const SphParams& P, const StepParams& S);
+// carry: also move the previous density and lambda (needed by a step without iterations and by
+// the learned features).
+void launch_gather(hipStream_t st, const DeviceState& d, const SphParams& P, const StepParams& S,
+                   bool carry);
 // Physical start and contiguity of every bucket (after the gather).
 void launch_bucket_phys(hipStream_t st, const DeviceState& d);
 // bs: workgroup size of the neighbour kernels (128, 256 or 512; results do not depend on it).

@@ -22,6 +22,8 @@
 // the hardware places workgroup b on XCD b % 8, so workgroup b takes chunk (b % 8) * (nb / 8) +
 // b / 8 and each XCD's L2 serves one spatially coherent eighth of the particles.
 #include "pbf_kernels.h"
+
+#include <algorithm>
 #include "pbf_math.h"
 
 namespace akua {
@@ -92,6 +94,13 @@ template <int BS>
 __device__ __forceinline__ uint32_t xcd_slot() {
     const uint32_t nb = gridDim.x, b = blockIdx.x;
     return ((b % kXcds) * (nb / kXcds) + b / kXcds) * BS + threadIdx.x;
+}
+
+// The same slot for virtual workgroup vb of nvb (a kernel that loops over virtual workgroups
+// with a smaller grid).
+template <int BS>
+__device__ __forceinline__ uint32_t xcd_slot_of(uint32_t vb, uint32_t nvb) {
+    return ((vb % kXcds) * (nvb / kXcds) + vb / kXcds) * BS + threadIdx.x;
 }
 
 __global__ void k_fill_u32(uint32_t* __restrict__ p, uint32_t n, uint32_t v) {
@@ -184,6 +193,9 @@ __global__ __launch_bounds__(kBlock) void k_cell_keys(DeviceState d, SphParams P
 // Moves each particle to its new physical slot (the record move of the reference's thrust::sort).
 // x* is recomputed from the gathered x and v with the same operations as k_predict_hash, so it is
 // bit-identical to the value that was hashed.
+// kCarry: also the previous density and lambda (read by a step without iterations and by the
+// learned features; a step with iterations rewrites both before anything reads them).
+template <bool kCarry>
 __global__ __launch_bounds__(kBlock) void k_gather(DeviceState d, SphParams P, StepParams S) {
     uint32_t p = blockIdx.x * kBlock + threadIdx.x;
     if (p >= d.nslots) return;
@@ -199,8 +211,10 @@ __global__ __launch_bounds__(kBlock) void k_gather(DeviceState d, SphParams P, S
     d.X0[p] = x;
     d.XsA[p] = xs4;
     d.XsL[k] = xs4;
-    d.Tdens[p] = d.dens[src];
-    d.Tlam[p] = d.lam[src];
+    if (kCarry) {
+        d.Tdens[p] = d.dens[src];
+        d.Tlam[p] = d.lam[src];
+    }
     d.Tid[p] = d.id[src];
     d.lidT[p] = k;
     d.physT[k] = p;
@@ -388,10 +402,9 @@ __device__ __forceinline__ void wave_resid_partial(const DeviceState& d, int slo
 }
 
 template <int BS>
-__global__ __launch_bounds__(BS) void k_density_lambda(DeviceState d, SphParams P,
-                                                           const float4* __restrict__ Xs,
-                                                           int residSlot, int gated) {
-    const uint32_t i = xcd_slot<BS>();
+__device__ __forceinline__ void density_lambda_slot(const DeviceState& d, const SphParams& P,
+                                                    const float4* __restrict__ Xs, int residSlot,
+                                                    int gated, uint32_t i) {
     const uint32_t i0 = i & ~63u;
     if (i0 >= d.nslots) return;  // the whole wave is past the last slot
     const uint32_t sIdx = d.chunkScene[i0 >> 8];  // a wave never spans two scenes
@@ -424,6 +437,16 @@ __global__ __launch_bounds__(BS) void k_density_lambda(DeviceState d, SphParams 
         d.lam[i] = -C / ((sum + dot3(gi, gi)) + P.relax);
     }
     if (residSlot >= 0) wave_resid_partial(d, residSlot, i, valid ? positive_part(C) : 0.0f);
+}
+
+// One workgroup per virtual workgroup (the loop runs once); the loop form lets a launch use any
+// grid size.
+template <int BS>
+__global__ __launch_bounds__(BS) void k_density_lambda(DeviceState d, SphParams P,
+                                                           const float4* __restrict__ Xs,
+                                                           int residSlot, int gated, uint32_t nvb) {
+    for (uint32_t vb = blockIdx.x; vb < nvb; vb += gridDim.x)
+        density_lambda_slot<BS>(d, P, Xs, residSlot, gated, xcd_slot_of<BS>(vb, nvb));
 }
 
 // Features of the learned warm start at x*_pred, one neighbour sweep (list order):
@@ -606,13 +629,12 @@ __device__ __forceinline__ f3 damp_walls(f3 x, f3 v, const SceneDev& sc) {
 // damping, writing the step's positions X, velocities V, ids and the two order maps.
 // gateMode 1: only scenes whose gate is set run; 2: scenes whose gate is clear read XsAlt.
 template <int BS, bool kFinal, bool kApplyDelta>
-__global__ __launch_bounds__(BS) void k_delta_correct(DeviceState d, SphParams P,
-                                                          StepParams S,
-                                                          const float4* __restrict__ XsSet,
-                                                          float4* __restrict__ XsOut,
-                                                          const float4* __restrict__ XsAlt,
-                                                          int gateMode) {
-    const uint32_t i = xcd_slot<BS>();
+__device__ __forceinline__ void delta_correct_slot(const DeviceState& d, const SphParams& P,
+                                                   const StepParams& S,
+                                                   const float4* __restrict__ XsSet,
+                                                   float4* __restrict__ XsOut,
+                                                   const float4* __restrict__ XsAlt, int gateMode,
+                                                   uint32_t i) {
     if (i >= d.nslots) return;
     const uint32_t sIdx = d.chunkScene[i >> 8];
     const SceneDev& sc = d.scenes[sIdx];
@@ -666,6 +688,17 @@ __global__ __launch_bounds__(BS) void k_delta_correct(DeviceState d, SphParams P
         d.dens[i] = d.Tdens[i];
         d.lam[i] = d.Tlam[i];
     }
+}
+
+template <int BS, bool kFinal, bool kApplyDelta>
+__global__ __launch_bounds__(BS) void k_delta_correct(DeviceState d, SphParams P, StepParams S,
+                                                          const float4* __restrict__ XsSet,
+                                                          float4* __restrict__ XsOut,
+                                                          const float4* __restrict__ XsAlt,
+                                                          int gateMode, uint32_t nvb) {
+    for (uint32_t vb = blockIdx.x; vb < nvb; vb += gridDim.x)
+        delta_correct_slot<BS, kFinal, kApplyDelta>(d, P, S, XsSet, XsOut, XsAlt, gateMode,
+                                                    xcd_slot_of<BS>(vb, nvb));
 }
 
 // kernel_compute_vorticities (IntegrationCUDA.cu:104-128).  W.w keeps |omega| for the next pass.
@@ -850,8 +883,12 @@ void launch_cell_keys(hipStream_t st, const DeviceState& d, const SphParams& P,
                       const StepParams& S) {
     k_cell_keys<<<grid(d.nslots), kBlock, 0, st>>>(d, P, S);
 }
-void launch_gather(hipStream_t st, const DeviceState& d, const SphParams& P, const StepParams& S) {
-    k_gather<<<grid(d.nslots), kBlock, 0, st>>>(d, P, S);
+void launch_gather(hipStream_t st, const DeviceState& d, const SphParams& P, const StepParams& S,
+                   bool carry) {
+    if (carry)
+        k_gather<true><<<grid(d.nslots), kBlock, 0, st>>>(d, P, S);
+    else
+        k_gather<false><<<grid(d.nslots), kBlock, 0, st>>>(d, P, S);
 }
 void launch_bucket_phys(hipStream_t st, const DeviceState& d) {
     k_bucket_phys<<<grid(d.nslots), kBlock, 0, st>>>(d);
@@ -859,10 +896,15 @@ void launch_bucket_phys(hipStream_t st, const DeviceState& d) {
 void launch_neighbours(hipStream_t st, const DeviceState& d, const SphParams& P, int bs) {
     AKUA_BY_BLOCK(bs, (k_neighbours<BS><<<grid_xcd(d.nslots, BS), BS, 0, st>>>(d, P)));
 }
+template <int BS>
+void density_lambda_bs(hipStream_t st, const DeviceState& d, const SphParams& P, const float4* Xs,
+                       int residSlot, bool gated) {
+    const unsigned nvb = grid_xcd(d.nslots, BS);
+    k_density_lambda<BS><<<nvb, BS, 0, st>>>(d, P, Xs, residSlot, gated ? 1 : 0, nvb);
+}
 void launch_density_lambda(hipStream_t st, const DeviceState& d, const SphParams& P,
                            const float4* Xs, int bs, int residSlot, bool gated) {
-    AKUA_BY_BLOCK(bs, (k_density_lambda<BS><<<grid_xcd(d.nslots, BS), BS, 0, st>>>(
-                          d, P, Xs, residSlot, gated ? 1 : 0)));
+    AKUA_BY_BLOCK(bs, (density_lambda_bs<BS>(st, d, P, Xs, residSlot, gated)));
 }
 void launch_count_sum(hipStream_t st, const DeviceState& d, unsigned long long* part, uint32_t blocks) {
     k_count_sum<<<blocks, 256, 0, st>>>(d, part);
@@ -888,13 +930,14 @@ template <int BS>
 void delta_correct_bs(hipStream_t st, const DeviceState& d, const SphParams& P,
                       const StepParams& S, const float4* XsIn, float4* XsOut, bool final,
                       bool applyDelta, int gate, const float4* XsAlt) {
-    const unsigned g = grid_xcd(d.nslots, BS);
+    const unsigned nvb = grid_xcd(d.nslots, BS);
+    const unsigned g = nvb;
     if (final && applyDelta)
-        k_delta_correct<BS, true, true><<<g, BS, 0, st>>>(d, P, S, XsIn, XsOut, XsAlt, gate);
+        k_delta_correct<BS, true, true><<<g, BS, 0, st>>>(d, P, S, XsIn, XsOut, XsAlt, gate, nvb);
     else if (final)
-        k_delta_correct<BS, true, false><<<g, BS, 0, st>>>(d, P, S, XsIn, XsOut, XsAlt, gate);
+        k_delta_correct<BS, true, false><<<g, BS, 0, st>>>(d, P, S, XsIn, XsOut, XsAlt, gate, nvb);
     else
-        k_delta_correct<BS, false, true><<<g, BS, 0, st>>>(d, P, S, XsIn, XsOut, XsAlt, gate);
+        k_delta_correct<BS, false, true><<<g, BS, 0, st>>>(d, P, S, XsIn, XsOut, XsAlt, gate, nvb);
 }
 void launch_delta_correct(hipStream_t st, const DeviceState& d, const SphParams& P,
                           const StepParams& S, const float4* XsIn, float4* XsOut, bool final,

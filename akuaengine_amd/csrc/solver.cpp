@@ -475,7 +475,8 @@ void enqueue_step(akua_solver* s, hipStream_t st, const StepParams& S, bool time
     L.after("cell_keys");
     sort_layout(s, st, timed);
     L.before("gather");
-    launch_gather(st, d, P, S);
+    launch_gather(st, d, P, S,
+                  s->cfg.solverIterations == 0 || record || (s->mode == 1 && s->haveNet));
     L.after("gather");
     L.before("bucket_phys");
     launch_bucket_phys(st, d);
@@ -664,7 +665,7 @@ void run_density_probe(akua_solver* s) {
     sort_hash(s, st, false);
     launch_cell_keys(st, d, s->sph, S0);
     sort_layout(s, st, false);
-    launch_gather(st, d, s->sph, S0);
+    launch_gather(st, d, s->sph, S0, false);
     launch_bucket_phys(st, d);
     launch_density_probe(st, d, s->sph, s->probeRho.p, s->probeMax.p, s->probeSum.p);
     HIP_CHECK(hipGetLastError());
