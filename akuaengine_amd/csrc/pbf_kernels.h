@@ -59,6 +59,7 @@ struct DeviceState {
     const uint32_t* perm1;     // logical index -> previous logical index
     uint32_t* keysM;           // super-cell Morton keys in logical order (second sort's input)
     const uint32_t* perm2;     // physical slot -> logical index
+    uint32_t* perm2keep;       // the same map for a step that keeps the physical layout
     uint4* table;              // cell table records (all scenes): (hash key, first logical
                                // index, one past the last, first physical slot | 2^31 if the
                                // bucket's physical slots are consecutive in logical order)
@@ -94,8 +95,10 @@ void launch_fill_u32(hipStream_t st, uint32_t* p, uint32_t n, uint32_t v);
 void launch_clear_table(hipStream_t st, uint4* table, uint32_t n);
 void launch_predict_hash(hipStream_t st, const DeviceState& d, const SphParams& P,
                          const StepParams& S);
+// keepLayout: the step keeps every particle in its physical slot (no layout sort); the map
+// physical slot -> logical index is then written to perm2keep.
 void launch_cell_keys(hipStream_t st, const DeviceState& d, const SphParams& P,
-                      const StepParams& S);
+                      const StepParams& S, bool keepLayout);
 // carry: also move the previous density and lambda (needed by a step without iterations and by
 // the learned features).
 void launch_gather(hipStream_t st, const DeviceState& d, const SphParams& P, const StepParams& S,

@@ -168,6 +168,9 @@ __device__ __forceinline__ uint32_t table_insert(uint4* table, const SceneDev& s
 // Per logical index k: the layout key (second sort's input), the previous physical slot of the
 // particle, and the cell-table entry of k's bucket (kernel_build_hash_table,
 // NeighbourSearchCUDA.cu:52-65, plus the bucket end the reference finds by scanning).
+// kKeep: the physical layout is kept this step (every particle stays in its slot), so instead of
+// the layout key, the inverse map physical slot -> new logical index is written directly.
+template <bool kKeep>
 __global__ __launch_bounds__(kBlock) void k_cell_keys(DeviceState d, SphParams P, StepParams S) {
     uint32_t k = blockIdx.x * kBlock + threadIdx.x;
     if (k >= d.nslots) return;
@@ -175,8 +178,13 @@ __global__ __launch_bounds__(kBlock) void k_cell_keys(DeviceState d, SphParams P
     if (k >= sc.end) return;
     const uint32_t prevLogical = d.perm1[k];
     if (prevLogical >= d.nslots) return;  // a consistent sort never produces this
-    d.keysM[k] = d.keysLay[prevLogical];
-    d.srcL[k] = d.phys[prevLogical];
+    const uint32_t src = d.phys[prevLogical];
+    if (kKeep) {
+        if (src < d.nslots) d.perm2keep[src] = k;
+    } else {
+        d.keysM[k] = d.keysLay[prevLogical];
+    }
+    d.srcL[k] = src;
     const uint32_t h = d.skeys[k];
     const bool first = (k == sc.begin) || (d.skeys[k - 1] != h);
     const bool last = (k + 1 == sc.end) || (d.skeys[k + 1] != h);
@@ -880,8 +888,11 @@ void launch_predict_hash(hipStream_t st, const DeviceState& d, const SphParams& 
     k_predict_hash<<<grid(d.nslots), kBlock, 0, st>>>(d, P, S);
 }
 void launch_cell_keys(hipStream_t st, const DeviceState& d, const SphParams& P,
-                      const StepParams& S) {
-    k_cell_keys<<<grid(d.nslots), kBlock, 0, st>>>(d, P, S);
+                      const StepParams& S, bool keepLayout) {
+    if (keepLayout)
+        k_cell_keys<true><<<grid(d.nslots), kBlock, 0, st>>>(d, P, S);
+    else
+        k_cell_keys<false><<<grid(d.nslots), kBlock, 0, st>>>(d, P, S);
 }
 void launch_gather(hipStream_t st, const DeviceState& d, const SphParams& P, const StepParams& S,
                    bool carry) {

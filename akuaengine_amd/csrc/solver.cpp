@@ -147,7 +147,8 @@ struct akua_solver {
     uint32_t tableTotal = 0;
     SortPlan plan1;          // hash sort (logical order)
     SortPlan plan2;          // super-cell sort (physical layout)
-    akua_tuning tuning{256, 1, 0, 0};  // block, layout level, digit bits (0: balanced)
+    akua_tuning tuning{256, 1, 0, 0, 1};  // block, layout level, digit bits (0: balanced), interval
+    int layoutPhase = 0;           // steps since the physical layout was last re-sorted (0: now)
     std::vector<SortTile> tilesHost;
     std::vector<SortScene> sortScenesHost;
     uint32_t numTiles = 0;
@@ -158,7 +159,7 @@ struct akua_solver {
     DevBuf<uint32_t> dChunkScene;
     DevBuf<float4> X, V, X0, XsA, XsB, XsL, W, Vc, color;
     DevBuf<float> dens, lam, Tdens, Tlam, size;
-    DevBuf<uint32_t> id, Tid, lid, phys, lidT, physT, srcL, keysLay, bslot, nbr, cnt;
+    DevBuf<uint32_t> id, Tid, lid, phys, lidT, physT, srcL, keysLay, bslot, nbr, cnt, perm2keep;
     DevBuf<uint32_t> s1KA, s1VA, s1KB, s1VB, s2KA, s2VA, s2KB, s2VB, counts, totals;
     DevBuf<uint4> table;
     DevBuf<SortTile> dTiles;
@@ -186,12 +187,16 @@ struct akua_solver {
     uint32_t batchCap = 0, lossCap = 0;
     int64_t adamT = 0;
 
-    // step graph
+    // step graphs: [0] re-sorts the physical layout, [1] keeps it
+    struct StepGraph {
+        hipGraphExec_t exec = nullptr;
+        StepParams params{};
+        int plan2 = -1;
+        akua_tuning tuning{};
+        bool dirty = true;
+    };
     bool useGraph = true;
-    hipGraphExec_t graphExec = nullptr;
-    StepParams graphParams{};
-    int graphPlan2 = -1;
-    akua_tuning graphTuning{0, 0, 0, 0};
+    StepGraph graphs[2];
     bool graphDirty = false;
     uint64_t stepsSinceStats = 0;
 
@@ -203,7 +208,7 @@ struct akua_solver {
     std::map<std::string, std::pair<double, int64_t>> timeAcc;
     std::vector<std::string> timeOrder;
 
-    DeviceState devState() {
+    DeviceState devState(bool keepLayout = false) {
         DeviceState d{};
         d.nslots = nslots;
         d.scenes = dScenes.p;
@@ -219,7 +224,8 @@ struct akua_solver {
         d.skeys = r1 ? s1KB.p : s1KA.p;
         d.perm1 = r1 ? s1VB.p : s1VA.p;
         d.keysM = s2KA.p;
-        d.perm2 = r2 ? s2VB.p : s2VA.p;
+        d.perm2 = keepLayout ? perm2keep.p : (r2 ? s2VB.p : s2VA.p);
+        d.perm2keep = perm2keep.p;
         d.table = table.p; d.bslot = bslot.p; d.tableTotal = tableTotal;
         d.nbr4 = reinterpret_cast<uint4*>(nbr.p); d.cnt = cnt.p; d.nbrSum = nbrSum.p;
         d.feat = feat.p; d.targ = targ.p; d.XsP = XsP.p; d.XsW = XsW.p; d.lamHat = lamHat.p;
@@ -229,7 +235,8 @@ struct akua_solver {
     }
 
     ~akua_solver() {
-        if (graphExec) (void)hipGraphExecDestroy(graphExec);
+        for (auto& g : graphs)
+            if (g.exec) (void)hipGraphExecDestroy(g.exec);
         for (auto e : eventPool) (void)hipEventDestroy(e);
         if (stream) (void)hipStreamDestroy(stream);
     }
@@ -345,7 +352,7 @@ void setup(akua_solver* s, const std::vector<uint32_t>& counts) {
     s->dens.alloc(ns); s->lam.alloc(ns); s->Tdens.alloc(ns); s->Tlam.alloc(ns); s->size.alloc(ns);
     s->id.alloc(ns); s->Tid.alloc(ns);
     s->lid.alloc(ns); s->phys.alloc(ns); s->lidT.alloc(ns); s->physT.alloc(ns);
-    s->XsL.alloc(ns); s->srcL.alloc(ns); s->keysLay.alloc(ns);
+    s->XsL.alloc(ns); s->srcL.alloc(ns); s->keysLay.alloc(ns); s->perm2keep.alloc(ns);
     s->s1KA.alloc(ns); s->s1VA.alloc(ns); s->s1KB.alloc(ns); s->s1VB.alloc(ns);
     s->s2KA.alloc(ns); s->s2VA.alloc(ns); s->s2KB.alloc(ns); s->s2VB.alloc(ns);
     s->counts.alloc((size_t)s->numTiles * (1u << kMaxDigitBits));
@@ -370,6 +377,7 @@ void setup(akua_solver* s, const std::vector<uint32_t>& counts) {
     HIP_CHECK(hipMemset(s->s1KA.p, 0, ns * 4));
     HIP_CHECK(hipMemset(s->lidT.p, 0, ns * 4));
     HIP_CHECK(hipMemset(s->srcL.p, 0, ns * 4));
+    HIP_CHECK(hipMemset(s->perm2keep.p, 0, ns * 4));
     HIP_CHECK(hipMemset(s->bslot.p, 0xFF, ns * 4));
     HIP_CHECK(hipMemset(s->table.p, 0xFF, (size_t)s->tableTotal * sizeof(uint4)));
     HIP_CHECK(hipMemset(s->keysLay.p, 0, ns * 4));
@@ -459,9 +467,9 @@ void sort_layout(akua_solver* s, hipStream_t st, bool timed) {
 // the step is the warm start, one Jacobi iteration and, per scene, the plain iterations again
 // from x*_pred when the residual check fails.
 void enqueue_step(akua_solver* s, hipStream_t st, const StepParams& S, bool timed,
-                  bool record = false) {
+                  bool record = false, bool keepLayout = false) {
     Launcher L{s, st, timed};
-    DeviceState d = s->devState();
+    DeviceState d = s->devState(keepLayout);
     const SphParams& P = s->sph;
     L.before("fill_table");
     launch_clear_table(st, s->table.p, s->tableTotal);
@@ -471,9 +479,9 @@ void enqueue_step(akua_solver* s, hipStream_t st, const StepParams& S, bool time
     L.after("predict_hash");
     sort_hash(s, st, timed);
     L.before("cell_keys");
-    launch_cell_keys(st, d, P, S);
+    launch_cell_keys(st, d, P, S, keepLayout);
     L.after("cell_keys");
-    sort_layout(s, st, timed);
+    if (!keepLayout) sort_layout(s, st, timed);
     L.before("gather");
     launch_gather(st, d, P, S,
                   s->cfg.solverIterations == 0 || record || (s->mode == 1 && s->haveNet));
@@ -613,37 +621,44 @@ void run_steps(akua_solver* s, float dt, int nsteps) {
     if (!(dt > 0.0f) || !std::isfinite(dt)) fail(AKUA_ERR_INVALID_ARG, "deltaTime must be > 0");
     upload_scenes(s);
     StepParams S{dt, {s->cfg.gravity[0], s->cfg.gravity[1], s->cfg.gravity[2]}};
+    if (s->graphDirty) {
+        for (auto& g : s->graphs) g.dirty = true;
+        s->graphDirty = false;
+    }
     for (int k = 0; k < nsteps; ++k) {
+        // the physical layout is re-sorted every layoutInterval steps; it never changes a result
+        const bool keep = s->layoutPhase != 0;
         if (s->timing) {
-            enqueue_step(s, s->stream, S, true);
+            enqueue_step(s, s->stream, S, true, false, keep);
             collect_timing(s);
         } else if (s->useGraph) {
-            bool same = s->graphExec && !s->graphDirty &&
-                        std::memcmp(&S, &s->graphParams, sizeof(S)) == 0 &&
-                        s->graphPlan2 == s->plan2.bits &&
-                        std::memcmp(&s->graphTuning, &s->tuning, sizeof(akua_tuning)) == 0;
+            auto& G = s->graphs[keep ? 1 : 0];
+            const bool same = G.exec && !G.dirty && std::memcmp(&S, &G.params, sizeof(S)) == 0 &&
+                              G.plan2 == s->plan2.bits &&
+                              std::memcmp(&G.tuning, &s->tuning, sizeof(akua_tuning)) == 0;
             if (!same) {
-                if (s->graphExec) {
-                    HIP_CHECK(hipGraphExecDestroy(s->graphExec));
-                    s->graphExec = nullptr;
+                if (G.exec) {
+                    HIP_CHECK(hipGraphExecDestroy(G.exec));
+                    G.exec = nullptr;
                 }
                 hipGraph_t g;
                 HIP_CHECK(hipStreamBeginCapture(s->stream, hipStreamCaptureModeThreadLocal));
-                enqueue_step(s, s->stream, S, false);
+                enqueue_step(s, s->stream, S, false, false, keep);
                 HIP_CHECK(hipStreamEndCapture(s->stream, &g));
-                hipError_t e = hipGraphInstantiate(&s->graphExec, g, nullptr, nullptr, 0);
+                hipError_t e = hipGraphInstantiate(&G.exec, g, nullptr, nullptr, 0);
                 (void)hipGraphDestroy(g);
                 HIP_CHECK(e);
-                s->graphParams = S;
-                s->graphPlan2 = s->plan2.bits;
-                s->graphTuning = s->tuning;
-                s->graphDirty = false;
+                G.params = S;
+                G.plan2 = s->plan2.bits;
+                G.tuning = s->tuning;
+                G.dirty = false;
             }
-            HIP_CHECK(hipGraphLaunch(s->graphExec, s->stream));
+            HIP_CHECK(hipGraphLaunch(G.exec, s->stream));
         } else {
-            enqueue_step(s, s->stream, S, false);
+            enqueue_step(s, s->stream, S, false, false, keep);
         }
         HIP_CHECK(hipGetLastError());
+        s->layoutPhase = (s->layoutPhase + 1) % std::max(1, s->tuning.layoutInterval);
         s->stepsSinceStats++;
         if (s->mode == 1) s->learnedSteps++;
     }
@@ -663,7 +678,7 @@ void run_density_probe(akua_solver* s) {
     launch_clear_table(st, s->table.p, s->tableTotal);
     launch_predict_hash(st, d, s->sph, S0);
     sort_hash(s, st, false);
-    launch_cell_keys(st, d, s->sph, S0);
+    launch_cell_keys(st, d, s->sph, S0, false);
     sort_layout(s, st, false);
     launch_gather(st, d, s->sph, S0, false);
     launch_bucket_phys(st, d);
@@ -728,6 +743,7 @@ void upload_soa(akua_solver* s, const float* pos, const float* vel, const float*
     HIP_CHECK(hipMemcpy(s->color.p, C.data(), C.size() * sizeof(float4), hipMemcpyHostToDevice));
     HIP_CHECK(hipMemcpy(s->size.p, Z.data(), Z.size() * 4, hipMemcpyHostToDevice));
     s->hasState = true;
+    s->layoutPhase = 0;  // the uploaded order is not cell ordered: re-sort on the next step
 }
 
 template <class T>
@@ -748,10 +764,13 @@ void validate_tuning(const akua_tuning& t) {
     for (int b : {t.hashDigitBits, t.layoutDigitBits})
         if (b != 0 && (b < 5 || b > kMaxDigitBits))
             fail(AKUA_ERR_INVALID_ARG, "tuning digit widths must be 0 or 5..8");
+    if (t.layoutInterval < 1 || t.layoutInterval > 8)
+        fail(AKUA_ERR_INVALID_ARG, "tuning.layoutInterval must be in [1, 8]");
 }
 
 void apply_tuning(akua_solver* s, const akua_tuning& t) {
     validate_tuning(t);
+    if (t.layoutInterval != s->tuning.layoutInterval) s->layoutPhase = 0;
     s->tuning = t;
     update_layout(s);
 }
@@ -781,6 +800,7 @@ void take_snapshot(akua_solver* s, Snapshot& q) {
 
 void restore_snapshot(akua_solver* s, const Snapshot& q) {
     const size_t n = s->nslots;
+    s->layoutPhase = 0;
     hipStream_t st = s->stream;
     copy_dev(s->X.p, q.X.p, n, st); copy_dev(s->V.p, q.V.p, n, st); copy_dev(s->W.p, q.W.p, n, st);
     copy_dev(s->dens.p, q.dens.p, n, st); copy_dev(s->lam.p, q.lam.p, n, st);
@@ -832,7 +852,8 @@ bool cache_lookup(const char* path, const std::string& key, akua_tuning& t) {
         if (bar == std::string::npos || line.compare(0, bar, key) != 0 || bar != key.size()) continue;
         std::istringstream v(line.substr(bar + 3));
         akua_tuning c{};
-        if (!(v >> c.block >> c.layoutLevel >> c.hashDigitBits >> c.layoutDigitBits)) continue;
+        if (!(v >> c.block >> c.layoutLevel >> c.hashDigitBits >> c.layoutDigitBits >> c.layoutInterval))
+            continue;
         try {
             validate_tuning(c);
         } catch (const AkuaError&) {
@@ -848,7 +869,7 @@ void cache_store(const char* path, const std::string& key, const akua_tuning& t)
     std::ofstream f(path, std::ios::app);
     if (!f) fail(AKUA_ERR_INVALID_ARG, std::string("cannot append to the tuning cache ") + path);
     f << key << " | " << t.block << ' ' << t.layoutLevel << ' ' << t.hashDigitBits << ' '
-      << t.layoutDigitBits << '\n';
+      << t.layoutDigitBits << ' ' << t.layoutInterval << '\n';
 }
 
 // ---- learned warm start ----------------------------------------------------------------------
@@ -1160,6 +1181,7 @@ void autotune(akua_solver* s, float dt, const float* bmin, const float* bmax, in
     sweep(&akua_tuning::layoutLevel, {0, 1, 2});
     sweep(&akua_tuning::hashDigitBits, {0, 5, 6, 8});
     sweep(&akua_tuning::layoutDigitBits, {0, 5, 8});
+    sweep(&akua_tuning::layoutInterval, {1, 2, 4});
     apply_tuning(s, best);
     restore_snapshot(s, q);
     HIP_CHECK(hipStreamSynchronize(s->stream));
@@ -1398,6 +1420,7 @@ akua_status akua_solver_step_device(akua_solver* s, void* dParticles, int32_t n,
         launch_import_aos(s->stream, buf, (uint32_t)n, s->X.p, s->V.p, s->dens.p, s->lam.p, s->id.p,
                           s->lid.p, s->phys.p, s->color.p, s->size.p);
         s->hasState = true;
+        s->layoutPhase = 0;
         run_steps(s, dt, 1);
         DeviceState ds = s->devState();
         launch_export_aos(s->stream, buf, (uint32_t)n, s->X.p, s->V.p, s->dens.p, s->lam.p, s->id.p,

@@ -119,8 +119,8 @@ class PBFSolver:
 
     # -- tuning -------------------------------------------------------------------------------
     def set_tuning(self, block: int = 256, layoutLevel: int = 1, hashDigitBits: int = 0,
-                   layoutDigitBits: int = 0) -> None:
-        t = _lib.AkuaTuning(block, layoutLevel, hashDigitBits, layoutDigitBits)
+                   layoutDigitBits: int = 0, layoutInterval: int = 1) -> None:
+        t = _lib.AkuaTuning(block, layoutLevel, hashDigitBits, layoutDigitBits, layoutInterval)
         _lib.check(self._L.akua_solver_set_tuning(self._h, ctypes.byref(t)))
 
     def get_tuning(self) -> Dict[str, int]:

@@ -144,12 +144,14 @@ typedef struct akua_tuning {
     int32_t layoutLevel;      /* physical layout: super-cells of 2^level hash cells per axis, 0..3 */
     int32_t hashDigitBits;    /* radix digit width of the hash sort: 0 (balanced) or 5..8 */
     int32_t layoutDigitBits;  /* radix digit width of the layout sort: 0 (balanced) or 5..8 */
+    int32_t layoutInterval;   /* the physical layout is re-sorted every layoutInterval steps (1..8);
+                                 in between, particles keep their memory slots */
 } akua_tuning;
 
 akua_status akua_solver_set_tuning(akua_solver* s, const akua_tuning* t);
 akua_status akua_solver_get_tuning(const akua_solver* s, akua_tuning* t);
 /* Times short trials of candidate settings by coordinate descent (block, then layout level, then
- * the two digit widths) and keeps the fastest.  Each trial restarts from a copy of the current
+ * the two digit widths, then the layout interval) and keeps the fastest.  Each trial restarts from a copy of the current
  * particle state, which is restored afterwards, so the call leaves the state as it found it.
  * boxMin/boxMax: the box the trials step in (NULL: each scene's current box).  trialSteps: timed
  * steps per candidate (1..1000).  cachePath (NULL: no cache): a text file of earlier results keyed

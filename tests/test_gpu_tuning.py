@@ -7,10 +7,10 @@ import pytest
 pytestmark = pytest.mark.gpu
 DT = 0.0083
 SETTINGS = [
-    dict(block=256, layoutLevel=1, hashDigitBits=0, layoutDigitBits=0),
-    dict(block=128, layoutLevel=0, hashDigitBits=5, layoutDigitBits=5),
-    dict(block=512, layoutLevel=2, hashDigitBits=8, layoutDigitBits=8),
-    dict(block=256, layoutLevel=3, hashDigitBits=6, layoutDigitBits=0),
+    dict(block=256, layoutLevel=1, hashDigitBits=0, layoutDigitBits=0, layoutInterval=1),
+    dict(block=128, layoutLevel=0, hashDigitBits=5, layoutDigitBits=5, layoutInterval=3),
+    dict(block=512, layoutLevel=2, hashDigitBits=8, layoutDigitBits=8, layoutInterval=2),
+    dict(block=256, layoutLevel=3, hashDigitBits=6, layoutDigitBits=0, layoutInterval=8),
 ]
 
 
@@ -32,7 +32,7 @@ def _run(PBFSolver, sc, cfg, corr, steps, tuning=None):
     return s
 
 
-@pytest.mark.parametrize("tuning", SETTINGS[1:], ids=lambda t: "b{block}-L{layoutLevel}-h{hashDigitBits}-l{layoutDigitBits}".format(**t))
+@pytest.mark.parametrize("tuning", SETTINGS[1:], ids=lambda t: "b{block}-L{layoutLevel}-h{hashDigitBits}-l{layoutDigitBits}-i{layoutInterval}".format(**t))
 def test_every_setting_gives_identical_trajectories(classes, dam_params, tuning):
     _, PBFSolver = classes
     from akuaengine_amd import scenes
@@ -67,7 +67,8 @@ def test_invalid_settings_are_rejected(classes, dam_params):
     from akuaengine_amd._lib import AkuaError
     cfg, corr = dam_params
     s = PBFSolver(1000, cfg, corr)
-    for bad in (dict(block=64), dict(layoutLevel=4), dict(hashDigitBits=4), dict(layoutDigitBits=9)):
+    for bad in (dict(block=64), dict(layoutLevel=4), dict(hashDigitBits=4), dict(layoutDigitBits=9),
+                dict(layoutInterval=0), dict(layoutInterval=9)):
         with pytest.raises(AkuaError):
             s.set_tuning(**bad)
     assert s.get_tuning() == SETTINGS[0]
