@@ -75,14 +75,27 @@ __device__ __forceinline__ float div_rn_unchecked(float a, const Recip& R) {
     return a == 0.0f ? a : q;
 }
 
+// Range tests on the bit patterns: |x| in [2^-40, 2^40] (0x2B800000 .. 0x53800000) or x == 0.
+// For several numerators at once, (|bits| - 1) maps zero above every finite pattern, so one
+// unsigned min and one max test them all.
+constexpr uint32_t kDivLo = 0x2B800000u, kDivHi = 0x53800000u;
+
+__device__ __forceinline__ uint32_t mag_bits(float x) { return __float_as_uint(x) & 0x7FFFFFFFu; }
+
 __device__ __forceinline__ bool in_div_range(float x) {
-    const float m = fabsf(x);
-    return (m >= 0x1p-40f && m <= 0x1p40f) || x == 0.0f;
+    const uint32_t m = mag_bits(x);
+    return m - 1u >= kDivLo - 1u && m <= kDivHi;
+}
+
+// b > 0 and |b| in [2^-40, 2^40]
+__device__ __forceinline__ bool divisor_ok(float b) {
+    const uint32_t u = __float_as_uint(b);
+    return u >= kDivLo && u <= kDivHi;
 }
 
 __device__ __forceinline__ float div_rn(float a, const Recip& R) {
     float q = div_rn_unchecked(a, R);
-    if (!(in_div_range(a) && in_div_range(R.b) && R.b > 0.0f)) q = a / R.b;
+    if (!(in_div_range(a) && divisor_ok(R.b))) q = a / R.b;
     return q;
 }
 
@@ -90,8 +103,10 @@ __device__ __forceinline__ float div_rn(float a, const Recip& R) {
 __device__ __forceinline__ f3 div3_rn(f3 v, float b) {
     const Recip R = recip(b);
     f3 q{div_rn_unchecked(v.x, R), div_rn_unchecked(v.y, R), div_rn_unchecked(v.z, R)};
-    const bool ok = b > 0.0f && in_div_range(b) && in_div_range(v.x) && in_div_range(v.y) &&
-                    in_div_range(v.z);
+    const uint32_t mx = mag_bits(v.x), my = mag_bits(v.y), mz = mag_bits(v.z);
+    const uint32_t lo = min(min(mx - 1u, my - 1u), mz - 1u);
+    const uint32_t hi = max(max(mx, my), mz);
+    const bool ok = divisor_ok(b) && lo >= kDivLo - 1u && hi <= kDivHi;
     if (!ok) q = {v.x / b, v.y / b, v.z / b};
     return q;
 }
