@@ -328,50 +328,42 @@ __global__ __launch_bounds__(BS) void k_neighbours(DeviceState d, SphParams P) {
                     // directly; otherwise go through the logical-order copy
                     const bool run = e[q].w != 0u;
                     const int32_t shift = (int32_t)e[q].z - (int32_t)a;  // physical - logical
-                    // one bucket's candidates, kWalkBatch at a time: load(m, x, slot) fetches
-                    // the candidate at logical index m
-                    auto walk = [&](auto load) {
-                        for (uint32_t m0 = a; m0 < b && count < maxN; m0 += kWalkBatch) {
-                            float4 c[kWalkBatch];
-                            uint32_t ph[kWalkBatch];
+                    for (uint32_t m0 = a; m0 < b && count < maxN; m0 += kWalkBatch) {
+                        float4 c[kWalkBatch];
+                        uint32_t ph[kWalkBatch];
 #pragma unroll
-                            for (int u = 0; u < kWalkBatch; ++u) load(min(m0 + u, b - 1u), c[u], ph[u]);
-                            // test the whole batch without branches, then append in order
-                            uint32_t acc = 0u;
-#pragma unroll
-                            for (int u = 0; u < kWalkBatch; ++u) {
-                                const f3 sep = xi - make3(c[u]);
-                                const bool ok = (dot3(sep, sep) < P.h2) & (ph[u] != p) & (m0 + u < b);
-                                acc |= (uint32_t)ok << u;
-                            }
-                            const uint32_t room = maxN - count;
-                            while ((uint32_t)__popc(acc) > room) acc &= ~(1u << (31 - __clz(acc)));
-                            if (acc) {
-                                // append in order: the u-th candidate goes to position count +
-                                // (the accepted candidates before it)
-#pragma unroll
-                                for (int u = 0; u < kWalkBatch; ++u) {
-                                    if (acc & (1u << u)) {
-                                        const uint32_t k = count + (uint32_t)__popc(acc & ((1u << u) - 1u));
-                                        out[((size_t)(k >> 2) * 64u) * 4u + (k & 3u)] = ph[u];
-                                    }
-                                }
-                                count += (uint32_t)__popc(acc);
+                        for (int u = 0; u < kWalkBatch; ++u) {
+                            const uint32_t m = min(m0 + u, b - 1u);
+                            if (run) {
+                                ph[u] = (uint32_t)((int32_t)m + shift);
+                                c[u] = d.XsA[ph[u]];
+                            } else {
+                                c[u] = d.XsL[m];
+                                ph[u] = d.physT[m];
                             }
                         }
-                    };
-                    if (run) {
-                        // bucket of consecutive physical slots: the cell-ordered positions
-                        walk([&](uint32_t m, float4& x, uint32_t& slot) {
-                            slot = (uint32_t)((int32_t)m + shift);
-                            x = d.XsA[slot];
-                        });
-                    } else {
-                        // otherwise through the logical-order copy
-                        walk([&](uint32_t m, float4& x, uint32_t& slot) {
-                            x = d.XsL[m];
-                            slot = d.physT[m];
-                        });
+                        // test the whole batch without branches, then append in order
+                        uint32_t acc = 0u;
+#pragma unroll
+                        for (int u = 0; u < kWalkBatch; ++u) {
+                            const f3 sep = xi - make3(c[u]);
+                            const bool ok = (dot3(sep, sep) < P.h2) & (ph[u] != p) & (m0 + u < b);
+                            acc |= (uint32_t)ok << u;
+                        }
+                        const uint32_t room = maxN - count;
+                        while ((uint32_t)__popc(acc) > room) acc &= ~(1u << (31 - __clz(acc)));
+                        if (acc) {
+                            // append in order: the u-th candidate goes to position count + (the
+                            // accepted candidates before it)
+#pragma unroll
+                            for (int u = 0; u < kWalkBatch; ++u) {
+                                if (acc & (1u << u)) {
+                                    const uint32_t k = count + (uint32_t)__popc(acc & ((1u << u) - 1u));
+                                    out[((size_t)(k >> 2) * 64u) * 4u + (k & 3u)] = ph[u];
+                                }
+                            }
+                            count += (uint32_t)__popc(acc);
+                        }
                     }
                 }
             }
