@@ -80,6 +80,8 @@ struct DeviceState {
                                // the last plain density pass
     float2* residPart;         // per check (3) and wave: (max, sum) of max(0, C) over its lanes
     uint32_t nWaves;           // waves of 64 slots: ceil(nslots / 64)
+    float2* residBlk;          // per scene, block and check: the first reduction stage
+    uint32_t residBlocks;      // blocks per scene of the first stage
     uint32_t* gate;            // per scene: 1 when this step falls back to the plain solve
     uint32_t* fallbacks;       // per scene: running count of fallback steps
 };
@@ -130,6 +132,8 @@ void launch_gate(hipStream_t st, const DeviceState& d, uint32_t nscenes, int sta
                  float ratio, float tolerance);
 // Per-scene max and sum of the residual checks in slotMask (bit k: check k) into d.resid.
 void launch_resid_reduce(hipStream_t st, const DeviceState& d, uint32_t nscenes, int slotMask);
+// First-stage blocks per scene for scenes of at most maxSceneSlots slots.
+uint32_t resid_blocks_for(uint32_t maxSceneSlots);
 // Training targets of every slot after a recorded plain step; XsLast holds x* before the last
 // iteration.
 void launch_targets(hipStream_t st, const DeviceState& d, const SphParams& P, const float4* XsLast);

@@ -523,43 +523,87 @@ __global__ __launch_bounds__(BS) void k_features(DeviceState d, SphParams P, int
     wave_resid_partial(d, 0, i, valid ? positive_part(C) : 0.0f);
 }
 
-// One workgroup per scene: max and sum over the scene's wave partials of each check in
-// slotMask (fixed reduction order), into the scene's record; with gate, also the fallback
-// decision from checks 0 (x*_pred) and 1 (x*_w).
+// Residual statistics per scene in two fixed-order stages (deterministic sums, no atomics).
+// Stage 1: workgroup (b, s) reduces up to kResidWavesPerBlock wave partials of scene s for each
+// check in slotMask into one block partial.
 constexpr int kReduceBlock = 256;
-__global__ __launch_bounds__(kReduceBlock) void k_resid_reduce(DeviceState d, int slotMask,
-                                                               int gate, int statistic, float ratio,
-                                                               float tolerance) {
-    __shared__ float smax[kReduceBlock], ssum[kReduceBlock];
+constexpr uint32_t kResidWavesPerBlock = 8u * kReduceBlock;
+
+__device__ __forceinline__ void block_max_sum(float& m, float& t, float* sm, float* st) {
+    for (int off = 32; off > 0; off >>= 1) {
+        m = fmaxf(m, __shfl_xor(m, off, 64));
+        t += __shfl_xor(t, off, 64);
+    }
+    const int w = threadIdx.x >> 6;
+    if (__lane_id() == 0) {
+        sm[w] = m;
+        st[w] = t;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int k = 1; k < kReduceBlock / 64; ++k) {
+            sm[0] = fmaxf(sm[0], sm[k]);
+            st[0] += st[k];
+        }
+    }
+    __syncthreads();
+    m = sm[0];
+    t = st[0];
+    __syncthreads();
+}
+
+__global__ __launch_bounds__(kReduceBlock) void k_resid_blocks(DeviceState d, int slotMask) {
+    __shared__ float sm[kReduceBlock / 64], st[kReduceBlock / 64];
+    const uint32_t b = blockIdx.x, s = blockIdx.y;
+    const SceneDev& sc = d.scenes[s];
+    const uint32_t w0 = (sc.begin >> 6) + b * kResidWavesPerBlock;
+    const uint32_t w1 = min((sc.end + 63) >> 6, w0 + kResidWavesPerBlock);
+    for (int k = 0; k < 3; ++k) {
+        if (!((slotMask >> k) & 1)) continue;
+        float m = 0.0f, t = 0.0f;
+        const float2* part = d.residPart + (size_t)k * d.nWaves;
+#pragma unroll
+        for (uint32_t r = 0; r < kResidWavesPerBlock / kReduceBlock; ++r) {
+            const uint32_t w = w0 + r * kReduceBlock + threadIdx.x;
+            if (w < w1) {
+                const float2 p = part[w];
+                m = fmaxf(m, p.x);
+                t += p.y;
+            }
+        }
+        block_max_sum(m, t, sm, st);
+        if (threadIdx.x == 0)
+            d.residBlk[((size_t)s * d.residBlocks + b) * 3 + k] = make_float2(m, t);
+    }
+}
+
+// Stage 2, one wave per scene: the scene's block partials into its record; with gate, also the
+// fallback decision from checks 0 (x*_pred) and 1 (x*_w).
+__global__ __launch_bounds__(64) void k_resid_reduce(DeviceState d, int slotMask, int gate,
+                                                     int statistic, float ratio, float tolerance) {
     const uint32_t s = blockIdx.x;
     const SceneDev& sc = d.scenes[s];
-    const uint32_t w0 = sc.begin >> 6, w1 = (sc.end + 63) >> 6;
+    const uint32_t nb = ((((sc.end + 63) >> 6) - (sc.begin >> 6)) + kResidWavesPerBlock - 1) /
+                        kResidWavesPerBlock;
     unsigned int* rec = d.resid + s * kResidStride;
     for (int k = 0; k < 3; ++k) {
         if (!((slotMask >> k) & 1)) continue;
         float m = 0.0f, t = 0.0f;
-        for (uint32_t w = w0 + threadIdx.x; w < w1; w += kReduceBlock) {
-            const float2 p = d.residPart[(size_t)k * d.nWaves + w];
+        for (uint32_t b = __lane_id(); b < nb; b += 64) {
+            const float2 p = d.residBlk[((size_t)s * d.residBlocks + b) * 3 + k];
             m = fmaxf(m, p.x);
             t += p.y;
         }
-        smax[threadIdx.x] = m;
-        ssum[threadIdx.x] = t;
-        __syncthreads();
-        for (int off = kReduceBlock / 2; off > 0; off >>= 1) {
-            if ((int)threadIdx.x < off) {
-                smax[threadIdx.x] = fmaxf(smax[threadIdx.x], smax[threadIdx.x + off]);
-                ssum[threadIdx.x] += ssum[threadIdx.x + off];
-            }
-            __syncthreads();
+        for (int off = 32; off > 0; off >>= 1) {
+            m = fmaxf(m, __shfl_xor(m, off, 64));
+            t += __shfl_xor(t, off, 64);
         }
-        if (threadIdx.x == 0) {
-            rec[k] = __float_as_uint(smax[0]);
-            rec[kResidSum + k] = __float_as_uint(ssum[0]);
+        if (__lane_id() == 0) {
+            rec[k] = __float_as_uint(m);
+            rec[kResidSum + k] = __float_as_uint(t);
         }
-        __syncthreads();
     }
-    if (gate && threadIdx.x == 0) {
+    if (gate && __lane_id() == 0) {
         float rp, rw;
         if (statistic == 0) {
             rp = __uint_as_float(rec[0]);
@@ -925,10 +969,16 @@ void launch_features(hipStream_t st, const DeviceState& d, const SphParams& P, b
 }
 void launch_gate(hipStream_t st, const DeviceState& d, uint32_t nscenes, int statistic,
                  float ratio, float tolerance) {
-    k_resid_reduce<<<nscenes, kReduceBlock, 0, st>>>(d, 3, 1, statistic, ratio, tolerance);
+    k_resid_blocks<<<dim3(d.residBlocks, nscenes), kReduceBlock, 0, st>>>(d, 3);
+    k_resid_reduce<<<nscenes, 64, 0, st>>>(d, 3, 1, statistic, ratio, tolerance);
 }
 void launch_resid_reduce(hipStream_t st, const DeviceState& d, uint32_t nscenes, int slotMask) {
-    k_resid_reduce<<<nscenes, kReduceBlock, 0, st>>>(d, slotMask, 0, 0, 0.0f, 0.0f);
+    k_resid_blocks<<<dim3(d.residBlocks, nscenes), kReduceBlock, 0, st>>>(d, slotMask);
+    k_resid_reduce<<<nscenes, 64, 0, st>>>(d, slotMask, 0, 0, 0.0f, 0.0f);
+}
+uint32_t resid_blocks_for(uint32_t maxSceneSlots) {
+    const uint32_t waves = (maxSceneSlots + 63) / 64 + 1;  // +1: a scene may start mid-wave-row
+    return std::max(1u, (waves + kResidWavesPerBlock - 1) / kResidWavesPerBlock);
 }
 void launch_targets(hipStream_t st, const DeviceState& d, const SphParams& P, const float4* XsLast) {
     k_targets<<<grid(d.nslots), kBlock, 0, st>>>(d, P, XsLast);

@@ -179,7 +179,8 @@ struct akua_solver {
     DevBuf<float> net, feat, targ, lamHat;
     DevBuf<float4> XsP, XsW;
     DevBuf<unsigned int> resid;
-    DevBuf<float2> residPart;
+    DevBuf<float2> residPart, residBlk;
+    uint32_t residBlocks = 1;
     DevBuf<uint32_t> gate, fallbacks;
     int64_t learnedSteps = 0;
     // training
@@ -231,6 +232,7 @@ struct akua_solver {
         d.feat = feat.p; d.targ = targ.p; d.XsP = XsP.p; d.XsW = XsW.p; d.lamHat = lamHat.p;
         d.resid = resid.p; d.gate = gate.p; d.fallbacks = fallbacks.p;
         d.residPart = residPart.p; d.nWaves = (nslots + 63) / 64;
+        d.residBlk = residBlk.p; d.residBlocks = residBlocks;
         return d;
     }
 
@@ -884,6 +886,11 @@ void ensure_learned(akua_solver* s) {
     s->resid.alloc(kResidStride * nsc);
     s->residPart.alloc(3 * ((ns + 63) / 64));
     HIP_CHECK(hipMemset(s->residPart.p, 0, 3 * ((ns + 63) / 64) * sizeof(float2)));
+    uint32_t maxScene = 1;
+    for (const SceneDev& sc : s->scenes) maxScene = std::max(maxScene, sc.end - sc.begin);
+    s->residBlocks = resid_blocks_for(maxScene);
+    s->residBlk.alloc((size_t)nsc * s->residBlocks * 3);
+    HIP_CHECK(hipMemset(s->residBlk.p, 0, (size_t)nsc * s->residBlocks * 3 * sizeof(float2)));
     s->gate.alloc(nsc);
     s->fallbacks.alloc(nsc);
     HIP_CHECK(hipMemset(s->net.p, 0, kMlpParams * sizeof(float)));
