@@ -186,6 +186,37 @@ def test_everything_in_one_cell(gpu, oracle_mod, dam_params):
     _assert_state_equal(s, o)
 
 
+def _occupancy_cases():
+    rng = np.random.default_rng(5)
+    # 4000 particles in one cell: one bucket far longer than the neighbour cap
+    dense = (np.float32(1.01) + rng.uniform(0, 0.08, size=(4000, 3))).astype(np.float32)
+    # one particle per cell: every workgroup's particles in distinct cells
+    g = np.arange(12, dtype=np.float32) * np.float32(0.1) + np.float32(0.05)
+    single = np.stack(np.meshgrid(g, g, g, indexing="ij"), -1).reshape(-1, 3)
+    # pairs three cells apart: no stencil bucket shared between pairs
+    g3 = np.arange(10, dtype=np.float32) * np.float32(0.3) + np.float32(0.05)
+    c = np.stack(np.meshgrid(g3, g3, g3, indexing="ij"), -1).reshape(-1, 3)
+    pairs = np.concatenate([c, c + np.float32(0.03)]).astype(np.float32)
+    return [("dense_cell", dense), ("one_per_cell", single), ("sparse_pairs", pairs)]
+
+
+@pytest.mark.parametrize("block", [128, 256, 512])
+@pytest.mark.parametrize("idx", range(3), ids=[n for n, _ in _occupancy_cases()])
+def test_extreme_cell_occupancies(gpu, oracle_mod, dam_params, idx, block):
+    """Lists, states and taps equal the oracle's for a 4000-particle cell, one particle per cell
+    and isolated pairs, under every workgroup size of the neighbour kernels."""
+    name, pos = _occupancy_cases()[idx]
+    cfg, corr = dam_params
+    vel = np.zeros_like(pos)
+    _say(f"[parity] occupancy {name} N={len(pos)} block={block}")
+    s, o = _pair(gpu, oracle_mod, pos, vel, cfg, corr)
+    s.set_tuning(block=block, layoutLevel=0)
+    s.step(DT, (0, 0, 0), (3.2, 3.2, 3.2))
+    o.step(DT, (0, 0, 0), (3.2, 3.2, 3.2))
+    _assert_taps_equal(s, o)
+    _assert_state_equal(s, o)
+
+
 def test_non_integer_corr_exponent_within_tolerance(gpu, oracle_mod, dam_params):
     """n = 3.5 takes powf on both sides: device and host powf may differ in the last bit.  Run at
     the reference's default k = 1e-4: with k = 0.1 the first step is violent enough to amplify a
