@@ -409,7 +409,8 @@ __device__ __forceinline__ void wave_resid_partial(const DeviceState& d, int slo
     if (__lane_id() == 0) d.residPart[(size_t)slot * d.nWaves + (i >> 6)] = make_float2(m, t);
 }
 
-template <int BS>
+// kFast (SphParams::fast): unit masses, range-tested quotients bounded in advance.
+template <int BS, bool kFast>
 __device__ __forceinline__ void density_lambda_slot(const DeviceState& d, const SphParams& P,
                                                     const float4* __restrict__ Xs, int residSlot,
                                                     int gated, uint32_t i) {
@@ -433,11 +434,18 @@ __device__ __forceinline__ void density_lambda_slot(const DeviceState& d, const 
             [&](const float4& xj4) {
                 f3 sep = xi - make3(xj4);
                 float d2 = dot3(sep, sep);
-                rho = rho + xj4.w * poly6(d2, P);
-                f3 g = grad_spiky(sep, P);
-                gi = gi + xj4.w * g;
-                f3 gj = (negInv * xj4.w) * g;
-                sum = sum + dot3(gj, gj);
+                f3 g = grad_spiky<kFast>(sep, P);
+                if (kFast) {  // m_j == 1: every product by m_j is the other factor itself
+                    rho = rho + poly6(d2, P);
+                    gi = gi + g;
+                    f3 gj = negInv * g;
+                    sum = sum + dot3(gj, gj);
+                } else {
+                    rho = rho + xj4.w * poly6(d2, P);
+                    gi = gi + xj4.w * g;
+                    f3 gj = (negInv * xj4.w) * g;
+                    sum = sum + dot3(gj, gj);
+                }
             });
         gi = gi * P.invRest;
         C = rho * P.invRest - 1.0f;
@@ -449,12 +457,12 @@ __device__ __forceinline__ void density_lambda_slot(const DeviceState& d, const 
 
 // One workgroup per virtual workgroup (the loop runs once); the loop form lets a launch use any
 // grid size.
-template <int BS>
+template <int BS, bool kFast>
 __global__ __launch_bounds__(BS) void k_density_lambda(DeviceState d, SphParams P,
                                                            const float4* __restrict__ Xs,
                                                            int residSlot, int gated, uint32_t nvb) {
     for (uint32_t vb = blockIdx.x; vb < nvb; vb += gridDim.x)
-        density_lambda_slot<BS>(d, P, Xs, residSlot, gated, xcd_slot_of<BS>(vb, nvb));
+        density_lambda_slot<BS, kFast>(d, P, Xs, residSlot, gated, xcd_slot_of<BS>(vb, nvb));
 }
 
 // Features of the learned warm start at x*_pred, one neighbour sweep (list order):
@@ -680,7 +688,7 @@ __device__ __forceinline__ f3 damp_walls(f3 x, f3 v, const SceneDev& sc) {
 // kFinal: also kernel_update_position_and_velocity (IntegrationCUDA.cu:38-49) and the wall
 // damping, writing the step's positions X, velocities V, ids and the two order maps.
 // gateMode 1: only scenes whose gate is set run; 2: scenes whose gate is clear read XsAlt.
-template <int BS, bool kFinal, bool kApplyDelta>
+template <int BS, bool kFinal, bool kApplyDelta, bool kFast>
 __device__ __forceinline__ void delta_correct_slot(const DeviceState& d, const SphParams& P,
                                                    const StepParams& S,
                                                    const float4* __restrict__ XsSet,
@@ -707,9 +715,13 @@ __device__ __forceinline__ void delta_correct_slot(const DeviceState& d, const S
             [&](const XLam& nb) {
                 f3 sep = xi - make3(nb.x);
                 float d2 = dot3(sep, sep);
-                float lc = P.negCorrK * corr_pow(div_rn(poly6(d2, P), Rdq), P);
-                f3 g = grad_spiky(sep, P);
-                dp = dp + (((li + nb.lam) + lc) * nb.x.w) * g;
+                const float w = poly6(d2, P);
+                float lc = P.negCorrK * corr_pow(kFast ? div_rn_small(w, Rdq) : div_rn(w, Rdq), P);
+                f3 g = grad_spiky<kFast>(sep, P);
+                if (kFast)  // m_j == 1
+                    dp = dp + ((li + nb.lam) + lc) * g;
+                else
+                    dp = dp + (((li + nb.lam) + lc) * nb.x.w) * g;
             });
         dp = dp * P.invRest;
         q = xi + dp;
@@ -742,19 +754,19 @@ __device__ __forceinline__ void delta_correct_slot(const DeviceState& d, const S
     }
 }
 
-template <int BS, bool kFinal, bool kApplyDelta>
+template <int BS, bool kFinal, bool kApplyDelta, bool kFast>
 __global__ __launch_bounds__(BS) void k_delta_correct(DeviceState d, SphParams P, StepParams S,
                                                           const float4* __restrict__ XsSet,
                                                           float4* __restrict__ XsOut,
                                                           const float4* __restrict__ XsAlt,
                                                           int gateMode, uint32_t nvb) {
     for (uint32_t vb = blockIdx.x; vb < nvb; vb += gridDim.x)
-        delta_correct_slot<BS, kFinal, kApplyDelta>(d, P, S, XsSet, XsOut, XsAlt, gateMode,
-                                                    xcd_slot_of<BS>(vb, nvb));
+        delta_correct_slot<BS, kFinal, kApplyDelta, kFast>(d, P, S, XsSet, XsOut, XsAlt, gateMode,
+                                                           xcd_slot_of<BS>(vb, nvb));
 }
 
 // kernel_compute_vorticities (IntegrationCUDA.cu:104-128).  W.w keeps |omega| for the next pass.
-template <int BS>
+template <int BS, bool kFast>
 __global__ __launch_bounds__(BS) void k_vorticity(DeviceState d, SphParams P) {
     const uint32_t i = xcd_slot<BS>();
     if (i >= d.nslots) return;
@@ -769,7 +781,8 @@ __global__ __launch_bounds__(BS) void k_vorticity(DeviceState d, SphParams P) {
         [&](const XV& nb) {
             f3 vij = make3(nb.v) - vi;
             f3 sep = xi - make3(nb.x);
-            w = w + (-nb.x.w) * cross3(vij, grad_spiky(sep, P));
+            const f3 c = cross3(vij, grad_spiky<kFast>(sep, P));
+            w = w + (kFast ? -1.0f : -nb.x.w) * c;  // m_j == 1 under kFast
         });
     d.W[i] = make_float4(w.x, w.y, w.z, length3(w));
 }
@@ -777,7 +790,7 @@ __global__ __launch_bounds__(BS) void k_vorticity(DeviceState d, SphParams P) {
 // kernel_apply_vorticity_confinement (IntegrationCUDA.cu:130-165), writing Vc: the confined
 // velocity, with the particle's m / rho in .w for the XSPH pass (the quotient the reference
 // forms per neighbour, IntegrationCUDA.cu:184, computed once per particle instead).
-template <int BS>
+template <int BS, bool kFast>
 __global__ __launch_bounds__(BS) void k_confinement(DeviceState d, SphParams P, StepParams S) {
     const uint32_t i = xcd_slot<BS>();
     if (i >= d.nslots) return;
@@ -795,7 +808,8 @@ __global__ __launch_bounds__(BS) void k_confinement(DeviceState d, SphParams P, 
         [&](const XW& nb) {
             float diff = wi.w - nb.w;
             f3 sep = xi - make3(nb.x);
-            eta = eta + (nb.x.w * diff) * grad_spiky(sep, P);
+            const f3 g = grad_spiky<kFast>(sep, P);
+            eta = eta + (kFast ? diff : nb.x.w * diff) * g;  // m_j == 1 under kFast
         });
     eta = eta * invD;
     float el = length3(eta);
@@ -955,7 +969,10 @@ template <int BS>
 void density_lambda_bs(hipStream_t st, const DeviceState& d, const SphParams& P, const float4* Xs,
                        int residSlot, bool gated) {
     const unsigned nvb = grid_xcd(d.nslots, BS);
-    k_density_lambda<BS><<<nvb, BS, 0, st>>>(d, P, Xs, residSlot, gated ? 1 : 0, nvb);
+    if (P.fast)
+        k_density_lambda<BS, true><<<nvb, BS, 0, st>>>(d, P, Xs, residSlot, gated ? 1 : 0, nvb);
+    else
+        k_density_lambda<BS, false><<<nvb, BS, 0, st>>>(d, P, Xs, residSlot, gated ? 1 : 0, nvb);
 }
 void launch_density_lambda(hipStream_t st, const DeviceState& d, const SphParams& P,
                            const float4* Xs, int bs, int residSlot, bool gated) {
@@ -993,12 +1010,18 @@ void delta_correct_bs(hipStream_t st, const DeviceState& d, const SphParams& P,
                       bool applyDelta, int gate, const float4* XsAlt) {
     const unsigned nvb = grid_xcd(d.nslots, BS);
     const unsigned g = nvb;
-    if (final && applyDelta)
-        k_delta_correct<BS, true, true><<<g, BS, 0, st>>>(d, P, S, XsIn, XsOut, XsAlt, gate, nvb);
-    else if (final)
-        k_delta_correct<BS, true, false><<<g, BS, 0, st>>>(d, P, S, XsIn, XsOut, XsAlt, gate, nvb);
-    else
-        k_delta_correct<BS, false, true><<<g, BS, 0, st>>>(d, P, S, XsIn, XsOut, XsAlt, gate, nvb);
+    if (final && applyDelta) {
+        if (P.fast)
+            k_delta_correct<BS, true, true, true><<<g, BS, 0, st>>>(d, P, S, XsIn, XsOut, XsAlt, gate, nvb);
+        else
+            k_delta_correct<BS, true, true, false><<<g, BS, 0, st>>>(d, P, S, XsIn, XsOut, XsAlt, gate, nvb);
+    } else if (final) {
+        k_delta_correct<BS, true, false, false><<<g, BS, 0, st>>>(d, P, S, XsIn, XsOut, XsAlt, gate, nvb);
+    } else if (P.fast) {
+        k_delta_correct<BS, false, true, true><<<g, BS, 0, st>>>(d, P, S, XsIn, XsOut, XsAlt, gate, nvb);
+    } else {
+        k_delta_correct<BS, false, true, false><<<g, BS, 0, st>>>(d, P, S, XsIn, XsOut, XsAlt, gate, nvb);
+    }
 }
 void launch_delta_correct(hipStream_t st, const DeviceState& d, const SphParams& P,
                           const StepParams& S, const float4* XsIn, float4* XsOut, bool final,
@@ -1007,11 +1030,17 @@ void launch_delta_correct(hipStream_t st, const DeviceState& d, const SphParams&
                                             XsAlt)));
 }
 void launch_vorticity(hipStream_t st, const DeviceState& d, const SphParams& P, int bs) {
-    AKUA_BY_BLOCK(bs, (k_vorticity<BS><<<grid_xcd(d.nslots, BS), BS, 0, st>>>(d, P)));
+    if (P.fast)
+        AKUA_BY_BLOCK(bs, (k_vorticity<BS, true><<<grid_xcd(d.nslots, BS), BS, 0, st>>>(d, P)));
+    else
+        AKUA_BY_BLOCK(bs, (k_vorticity<BS, false><<<grid_xcd(d.nslots, BS), BS, 0, st>>>(d, P)));
 }
 void launch_confinement(hipStream_t st, const DeviceState& d, const SphParams& P,
                         const StepParams& S, int bs) {
-    AKUA_BY_BLOCK(bs, (k_confinement<BS><<<grid_xcd(d.nslots, BS), BS, 0, st>>>(d, P, S)));
+    if (P.fast)
+        AKUA_BY_BLOCK(bs, (k_confinement<BS, true><<<grid_xcd(d.nslots, BS), BS, 0, st>>>(d, P, S)));
+    else
+        AKUA_BY_BLOCK(bs, (k_confinement<BS, false><<<grid_xcd(d.nslots, BS), BS, 0, st>>>(d, P, S)));
 }
 void launch_xsph(hipStream_t st, const DeviceState& d, const SphParams& P, int bs) {
     AKUA_BY_BLOCK(bs, (k_xsph<BS><<<grid_xcd(d.nslots, BS), BS, 0, st>>>(d, P)));

@@ -65,14 +65,16 @@ __device__ __forceinline__ Recip recip(float b) {
 }
 
 // a / b by the shared reciprocal, valid (equal to '/') when in_div_range(a) and
-// in_div_range(b); zero numerators keep their sign.
+// in_div_range(b).  The quotient takes a's sign bit: for b > 0 a nonzero quotient has it
+// already, and a zero numerator gives a quotient of magnitude zero whose sign the fmas may
+// lose (one bit insert instead of a compare and a select).
 __device__ __forceinline__ float div_rn_unchecked(float a, const Recip& R) {
     float q = a * R.r;
     float t = fmaf(-R.b, q, a);
     q = fmaf(t, R.r, q);
     t = fmaf(-R.b, q, a);
     q = fmaf(t, R.r, q);
-    return a == 0.0f ? a : q;
+    return __uint_as_float((__float_as_uint(q) & 0x7FFFFFFFu) | (__float_as_uint(a) & 0x80000000u));
 }
 
 // Range tests on the bit patterns: |x| in [2^-40, 2^40] (0x2B800000 .. 0x53800000) or x == 0.
@@ -111,6 +113,23 @@ __device__ __forceinline__ f3 div3_rn(f3 v, float b) {
     return q;
 }
 
+// a / b where b and an upper bound of |a| are known to lie in the exact-division range: only
+// the lower bound of |a| (zero or >= 2^-40) is tested.
+__device__ __forceinline__ float div_rn_small(float a, const Recip& R) {
+    float q = div_rn_unchecked(a, R);
+    if (mag_bits(a) - 1u < kDivLo - 1u) q = a / R.b;
+    return q;
+}
+
+// (x, y, z) / b on the same terms as div_rn_small.
+__device__ __forceinline__ f3 div3_rn_small(f3 v, float b) {
+    const Recip R = recip(b);
+    f3 q{div_rn_unchecked(v.x, R), div_rn_unchecked(v.y, R), div_rn_unchecked(v.z, R)};
+    const uint32_t lo = min(min(mag_bits(v.x) - 1u, mag_bits(v.y) - 1u), mag_bits(v.z) - 1u);
+    if (lo < kDivLo - 1u) q = {v.x / b, v.y / b, v.z / b};
+    return q;
+}
+
 // Correctly rounded sqrt for x >= 2^-96 (no scaling needed there): the 1-ulp hardware root,
 // corrected by the residuals of its two neighbours, as the compiler's own lowering does.  Below
 // 2^-96 it returns a value below 2^-47 (not necessarily correctly rounded): callers use it only
@@ -128,14 +147,17 @@ __device__ __forceinline__ float sqrt_rn_unscaled(float x) {
 
 // device_gradient_spiky (SmoothingKernelsCUDA.h:23-28): cs * (h - |r|)^2 * r / |r|.  |r| and
 // the three quotients are correctly rounded (the reference's length and '/'): a squared length
-// below 2^-96 gives |r| < 1e-5, the zero branch, whatever its last bits.
+// below 2^-96 gives |r| < 1e-5, the zero branch, whatever its last bits.  kFast: |r| lies in
+// [1e-5, h] and |cs| h^3 bounds the numerators, both inside the exact-division range
+// (SphParams::fast), so only the numerators' lower bound is tested.
+template <bool kFast = false>
 __device__ __forceinline__ f3 grad_spiky(f3 r, const SphParams& P) {
     const float d2 = dot3(r, r);
     const float rm = sqrt_rn_unscaled(d2);
     if (rm > P.h || rm < 1e-5f) return {0.0f, 0.0f, 0.0f};
     float t = P.h - rm;
     float s = P.cs * (t * t);
-    return div3_rn(s * r, rm);
+    return kFast ? div3_rn_small(s * r, rm) : div3_rn(s * r, rm);
 }
 
 // powf(x, n) of the artificial pressure (ConstraintSolverCUDA.cu:124): an integer n is the n-fold

@@ -44,6 +44,11 @@ struct SphParams {
     float vortEps;
     float visc;
     int32_t maxN;
+    // 1 when every particle's mass is exactly 1 and the quotients of the spiky gradient and of
+    // the artificial pressure need no upper-bound or divisor range test (their bounds, from h,
+    // the kernel constants and delta_q, lie inside the exact-division range): the neighbour
+    // sweeps then skip the multiplications by a neighbour's mass (identities) and those tests.
+    int32_t fast;
 };
 
 // Per-step scalars passed by value to the kernels.

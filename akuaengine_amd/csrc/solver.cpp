@@ -93,6 +93,21 @@ SphParams make_sph(const akua_pbf_config& c, const akua_lambda_corr& corr) {
     return P;
 }
 
+// Whether the neighbour sweeps' quotients need no upper-bound or divisor range test: the spiky
+// gradient divides numerators of magnitude at most |cs| h^3 by |r| in [1e-5, h]; the artificial
+// pressure divides poly6 values of at most cp h^6 by poly6(delta_q^2).  Both must lie inside
+// [2^-40, 2^40] (the exact-division range), with a margin for rounding.
+bool quotients_bounded(const SphParams& P) {
+    const double lo = std::ldexp(1.0, -40), hi = std::ldexp(1.0, 40), margin = 1.01;
+    const double h = P.h, cs = std::fabs((double)P.cs), cp = std::fabs((double)P.cp);
+    if (!(h > 0.0 && h <= hi && 1e-5 >= lo)) return false;
+    if (!(cs * h * h * h * margin <= hi)) return false;
+    const double h6 = h * h * h * h * h * h;
+    if (!(cp * h6 * margin <= hi)) return false;
+    const double wdq = P.wdq;
+    return wdq >= lo && wdq <= hi;
+}
+
 uint32_t ceil_log2(uint64_t v) {
     uint32_t r = 0;
     while ((1ull << r) < v) ++r;
@@ -169,6 +184,8 @@ struct akua_solver {
     DevBuf<unsigned int> probeMax;
     DevBuf<double> probeSum;
     bool hasState = false;
+    bool unitMass = false;         // every particle's mass is exactly 1 (known from the upload)
+    bool boundedQuotients = false; // quotients_bounded(sph)
 
     // learned warm start
     int mode = 0;                  // 0: plain solver, 1: learned warm start with fallback
@@ -392,7 +409,18 @@ void setup(akua_solver* s, const std::vector<uint32_t>& counts) {
     HIP_CHECK(hipMemset(s->V.p, 0, ns * sizeof(float4)));
     HIP_CHECK(hipMemset(s->cnt.p, 0, ns * 4));
     s->sph = make_sph(s->cfg, s->corr);
+    s->boundedQuotients = quotients_bounded(s->sph);
     update_layout(s);
+}
+
+// SphParams::fast from the masses and the parameters; the step graphs capture the kernels it
+// selects, so a change re-captures them.
+void update_fast(akua_solver* s) {
+    const int f = (s->unitMass && s->boundedQuotients) ? 1 : 0;
+    if (s->sph.fast != f) {
+        s->sph.fast = f;
+        s->graphDirty = true;
+    }
 }
 
 void timing_hook(void* ctx, const char* name, int phase);
@@ -703,6 +731,7 @@ void upload_soa(akua_solver* s, const float* pos, const float* vel, const float*
     std::vector<uint32_t> I(s->nslots, 0);
     // particles fill the scenes' slot ranges in scene order
     size_t k = 0;
+    bool unit = true;
     for (const SceneDev& sc : s->scenes) {
         for (uint32_t slot = sc.begin; slot < sc.end; ++slot, ++k) {
             float p[3], v[3], m, dd, ll;
@@ -724,6 +753,7 @@ void upload_soa(akua_solver* s, const float* pos, const float* vel, const float*
                 if (!std::isfinite(p[c]) || !std::isfinite(v[c]))
                     fail(AKUA_ERR_NON_FINITE, "particle " + std::to_string(k) + " has a non-finite position or velocity");
             if (!std::isfinite(m)) fail(AKUA_ERR_NON_FINITE, "non-finite mass");
+            unit = unit && m == 1.0f;
             X[slot] = make_float4(p[0], p[1], p[2], m);
             V[slot] = make_float4(v[0], v[1], v[2], 0.0f);
             D[slot] = dd;
@@ -746,6 +776,8 @@ void upload_soa(akua_solver* s, const float* pos, const float* vel, const float*
     HIP_CHECK(hipMemcpy(s->size.p, Z.data(), Z.size() * 4, hipMemcpyHostToDevice));
     s->hasState = true;
     s->layoutPhase = 0;  // the uploaded order is not cell ordered: re-sort on the next step
+    s->unitMass = unit;
+    update_fast(s);
 }
 
 template <class T>
@@ -1428,6 +1460,8 @@ akua_status akua_solver_step_device(akua_solver* s, void* dParticles, int32_t n,
                           s->lid.p, s->phys.p, s->color.p, s->size.p);
         s->hasState = true;
         s->layoutPhase = 0;
+        s->unitMass = false;  // the masses came from device memory: not known here
+        update_fast(s);
         run_steps(s, dt, 1);
         DeviceState ds = s->devState();
         launch_export_aos(s->stream, buf, (uint32_t)n, s->X.p, s->V.p, s->dens.p, s->lam.p, s->id.p,
