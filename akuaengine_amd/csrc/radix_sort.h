@@ -18,7 +18,7 @@
 namespace akua {
 
 constexpr int kSortBlock = 256;   // threads per tile (4 wave64)
-constexpr int kSortItems = 16;    // keys per thread
+constexpr int kSortItems = 8;     // keys per thread
 constexpr int kSortTile = kSortBlock * kSortItems;
 constexpr int kMaxDigitBits = 8;
 constexpr int kMaxPasses = 8;

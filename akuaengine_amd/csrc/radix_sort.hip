@@ -7,7 +7,7 @@
 //   rs_scan    : one wave per (scene, digit): the digit's base slot (scene begin + the totals of
 //                the smaller digits) plus the exclusive scan of its tile counts.
 //   rs_scatter : per tile, stable ranks by wave64 ballots (wave w owns a contiguous quarter of
-//                the tile, in 16 rounds of 64 consecutive keys), a tile-local counting sort in
+//                the tile, in kSortItems rounds of 64 consecutive keys), a tile-local counting sort in
 //                LDS, then coalesced runs written to the global offsets.
 #include "radix_sort.h"
 
