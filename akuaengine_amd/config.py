@@ -20,8 +20,9 @@ from typing import Tuple
 class LambdaCorrParams:
     """Artificial-pressure (s_corr) parameters; PBFConfig.h:10-15.
 
-    As in the reference, ``enabled`` is carried but does not switch the term off
-    (ConstraintSolverCUDA.cu:99-130 never reads it); set ``k = 0`` to disable it.
+    ``enabled = False`` drops the term.  The reference never reads the flag
+    (ConstraintSolverCUDA.cu:99-130 always applies it), so results are the reference's whenever
+    it is set.
     """
 
     enabled: bool = True

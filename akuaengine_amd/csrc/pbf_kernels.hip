@@ -503,8 +503,10 @@ __global__ __launch_bounds__(BS) void k_features(DeviceState d, SphParams P, int
                 gi = gi + xj4.w * g;
                 f3 gj = (negInv * xj4.w) * g;
                 sum = sum + dot3(gj, gj);
-                const float lc = P.negCorrK * corr_pow(div_rn(w, Rdq), P);
-                ac = ac + (lc * xj4.w) * g;
+                if (P.corrOn) {
+                    const float lc = P.negCorrK * corr_pow(div_rn(w, Rdq), P);
+                    ac = ac + (lc * xj4.w) * g;
+                }
             });
         gi = gi * P.invRest;
         C = rho * P.invRest - 1.0f;
@@ -715,13 +717,16 @@ __device__ __forceinline__ void delta_correct_slot(const DeviceState& d, const S
             [&](const XLam& nb) {
                 f3 sep = xi - make3(nb.x);
                 float d2 = dot3(sep, sep);
-                const float w = poly6(d2, P);
-                float lc = P.negCorrK * corr_pow(kFast ? div_rn_small(w, Rdq) : div_rn(w, Rdq), P);
+                float coef = li + nb.lam;
+                if (P.corrOn) {  // uniform: the artificial pressure s_corr
+                    const float w = poly6(d2, P);
+                    coef = coef + P.negCorrK * corr_pow(kFast ? div_rn_small(w, Rdq) : div_rn(w, Rdq), P);
+                }
                 f3 g = grad_spiky<kFast>(sep, P);
                 if (kFast)  // m_j == 1
-                    dp = dp + ((li + nb.lam) + lc) * g;
+                    dp = dp + coef * g;
                 else
-                    dp = dp + (((li + nb.lam) + lc) * nb.x.w) * g;
+                    dp = dp + (coef * nb.x.w) * g;
             });
         dp = dp * P.invRest;
         q = xi + dp;

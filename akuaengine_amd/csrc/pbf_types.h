@@ -37,6 +37,7 @@ struct SphParams {
     float invRest;     // 1 / restDensity
     float relax;       // relaxation (epsilon)
     float negCorrK;    // -k of the artificial pressure
+    int32_t corrOn;    // LambdaCorrParams::enabled: 0 drops the artificial pressure
     float corrN;       // n of the artificial pressure
     int32_t corrNInt;  // n as an integer in [0, 16], or -1 when n is not such an integer
     float cellHash;    // cell size used to hash (smoothRadius, NeighbourSearchCUDA.cu:163)

@@ -83,6 +83,7 @@ SphParams make_sph(const akua_pbf_config& c, const akua_lambda_corr& corr) {
     P.invRest = 1.0f / c.restDensity;
     P.relax = c.relaxation;
     P.negCorrK = -corr.k;
+    P.corrOn = corr.enabled != 0 ? 1 : 0;
     P.corrN = corr.n;
     P.corrNInt = (corr.n >= 0.0f && corr.n <= 16.0f && corr.n == floorf(corr.n)) ? (int)corr.n : -1;
     P.cellHash = c.smoothRadius;
@@ -104,6 +105,7 @@ bool quotients_bounded(const SphParams& P) {
     if (!(cs * h * h * h * margin <= hi)) return false;
     const double h6 = h * h * h * h * h * h;
     if (!(cp * h6 * margin <= hi)) return false;
+    if (!P.corrOn) return true;  // no artificial-pressure quotient
     const double wdq = P.wdq;
     return wdq >= lo && wdq <= hi;
 }

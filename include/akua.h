@@ -52,9 +52,10 @@ typedef struct akua_pbf_config {
     float gravity[3];
 } akua_pbf_config;
 
-/* Mirrors AkuaEngine::LambdaCorrParams (PBFConfig.h:10-15).  As in the reference, `enabled` does
- * not switch the artificial-pressure term off (ConstraintSolverCUDA.cu:99-130 never reads it);
- * set k = 0 for no correction.  Defaults: enabled 1, k 1e-4, n 4, delta_q 0.03. */
+/* Mirrors AkuaEngine::LambdaCorrParams (PBFConfig.h:10-15).  enabled == 0 drops the
+ * artificial-pressure term s_corr from the position delta.  The reference never reads the flag
+ * (ConstraintSolverCUDA.cu:99-130 always applies the term): results are the reference's whenever
+ * it is set.  Defaults: enabled 1, k 1e-4, n 4, delta_q 0.03. */
 typedef struct akua_lambda_corr {
     int32_t enabled;
     float k;

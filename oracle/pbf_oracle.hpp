@@ -49,8 +49,9 @@ struct Config {
     float gravity[3];
 };
 
-// Mirrors AkuaEngine::LambdaCorrParams (PBFConfig.h:10-15).  `enabled` is carried but, as in the
-// reference (ConstraintSolverCUDA.cu:99-130 never reads it), the correction is always applied.
+// Mirrors AkuaEngine::LambdaCorrParams (PBFConfig.h:10-15).  `enabled` == 0 drops the s_corr term
+// from the position delta (and from the learned features).  The reference never reads the flag
+// (ConstraintSolverCUDA.cu:99-130 always applies the term), so the two agree whenever it is set.
 struct LambdaCorr {
     int32_t enabled;
     float k;
@@ -196,6 +197,7 @@ struct Params {
     T h, h2, cellHash, cellSearch, invRest, relax, vortEps, visc, dt;
     T g[3], bmin[3], bmax[3];
     T corrK, corrN, corrDq;
+    bool corrOn;
     uint32_t tableSize;
     int32_t maxN, iters;
 };
@@ -226,6 +228,7 @@ Params<T> make_params(const Config& c, const LambdaCorr& lc, float dt, const flo
     p.corrK = T(lc.k);
     p.corrN = T(lc.n);
     p.corrDq = T(lc.delta_q);
+    p.corrOn = lc.enabled != 0;
     p.tableSize = tableSize;
     p.maxN = c.maxNeighbours;
     p.iters = c.solverIterations;
@@ -408,8 +411,9 @@ void position_deltas(State<T>& s, const Params<T>& p, const std::vector<uint32_t
             uint32_t j = nbr[st + o];
             V3<T> sep = s.xs[i] - s.xs[j];
             T d2 = dot(sep, sep);
-            T lc = -p.corrK * ipow<T>(poly6<T>(d2, p.K) / wdq, p.corrN);
-            d = d + ((s.lambda[i] + s.lambda[j] + lc) * s.mass[j]) * grad_spiky<T>(sep, p.K);
+            T coef = s.lambda[i] + s.lambda[j];
+            if (p.corrOn) coef = coef + -p.corrK * ipow<T>(poly6<T>(d2, p.K) / wdq, p.corrN);
+            d = d + (coef * s.mass[j]) * grad_spiky<T>(sep, p.K);
         }
         s.dp[i] = d * p.invRest;
     }
@@ -578,8 +582,10 @@ void features(const State<T>& s, const Params<T>& p, const std::vector<uint32_t>
             gi = gi + s.mass[j] * g;
             const V3<T> gj = (-p.invRest * s.mass[j]) * g;
             sum = sum + dot(gj, gj);
-            const T lc = -p.corrK * ipow<T>(w / wdq, p.corrN);
-            ac = ac + (lc * s.mass[j]) * g;
+            if (p.corrOn) {
+                const T lc = -p.corrK * ipow<T>(w / wdq, p.corrN);
+                ac = ac + (lc * s.mass[j]) * g;
+            }
         }
         gi = gi * p.invRest;
         const T C = rho * p.invRest - T(1);

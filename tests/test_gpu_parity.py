@@ -213,6 +213,21 @@ def test_non_unit_masses_and_switching(gpu, oracle_mod, dam_params):
         _assert_state_equal(s, o)
 
 
+def test_disabled_artificial_pressure(gpu, oracle_mod, dam_params):
+    """LambdaCorrParams.enabled = False (the term dropped) equals the oracle bit for bit."""
+    from akuaengine_amd import scenes
+    cfg, corr = dam_params
+    corr = dataclasses.replace(corr, enabled=False)
+    sc = scenes.dam_break_scaled(12, seed=5)
+    _say(f"[parity] s_corr disabled N={sc.n}")
+    s, o = _pair(gpu, oracle_mod, sc.pos, sc.vel, cfg, corr)
+    for _ in range(3):
+        s.step(DT, sc.box_min, sc.box_max)
+        o.step(DT, sc.box_min, sc.box_max)
+    _assert_taps_equal(s, o)
+    _assert_state_equal(s, o)
+
+
 def _occupancy_cases():
     rng = np.random.default_rng(5)
     # 4000 particles in one cell: one bucket far longer than the neighbour cap

@@ -276,6 +276,37 @@ def test_two_particle_closed_form(oracle_mod, dam_params, prec, rtol):
     np.testing.assert_allclose(sa["dp"][1 - x0], [-dp, 0.0, 0.0], rtol=rtol, atol=1e-12)
 
 
+def test_disabled_artificial_pressure(oracle_mod, dam_params):
+    """LambdaCorrParams.enabled = False drops s_corr: the two-particle delta is inv * 2 lam * |grad W|
+    (closed form), and a random cluster steps exactly as with k = 0 (where s_corr is -0)."""
+    import dataclasses
+    O = oracle_mod
+    cfg, corr = dam_params
+    off = dataclasses.replace(corr, enabled=False)
+    r = 0.06
+    pos = np.array([[1.0, 1.0, 1.0], [1.0 + r, 1.0, 1.0]], np.float32)
+    o = _setup(O, pos, np.zeros_like(pos), cfg, off, prec=64)
+    o.set_gravity((0.0, 0.0, 0.0))
+    _run_stages(o, ["predict", "hashes", "sort", "neighbours", "densities", "lambdas", "deltas"])
+    st, sa = o.state(), o.stage_arrays()
+    rr = float(np.float32(1.0 + r) - np.float32(1.0))
+    h = float(np.float32(H))
+    g = 45.0 / (PI32 * h ** 6) * (h - rr) ** 2
+    inv = 1.0 / float(np.float32(cfg.restDensity))
+    dp = inv * 2.0 * float(st["lam"][0]) * g
+    x0 = int(np.argmin(st["pos"][:, 0]))
+    np.testing.assert_allclose(sa["dp"][x0], [dp, 0.0, 0.0], rtol=1e-12, atol=1e-15)
+    rng = np.random.default_rng(4)
+    cl = (np.float32(2.0) + rng.uniform(0, 0.3, size=(300, 3))).astype(np.float32)
+    runs = []
+    for c in (off, dataclasses.replace(corr, k=0.0), corr):
+        oc = _setup(O, cl, np.zeros_like(cl), cfg, c)
+        oc.step(0.0083, (0, 0, 0), (5, 5, 5), 3)
+        runs.append(oc.state()["pos"])
+    np.testing.assert_array_equal(runs[0], runs[1])
+    assert np.abs(runs[0] - runs[2]).max() > 1e-6  # s_corr does act when enabled
+
+
 def test_position_deltas_sum_to_zero(oracle_mod, dam_params):
     """For symmetric neighbour lists and equal masses, sum_i dp_i = 0 (pairwise antisymmetry)."""
     O = oracle_mod
