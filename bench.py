@@ -23,6 +23,8 @@ Also reported, each in its own object of the same line:
                with 1-4 iterations and learned with J = 1, 2 (networks from
                akuaengine_amd/networks, trained by akuaengine_amd/train_warm_start.py), 300
                steps each, density error sampled every 25 steps.
+  learned_8m -- the same comparison on the 8M two-column scene (plain with 3 and 4 iterations,
+               learned with J = 1 and the 1M-trained network), 150 steps each.
   dam_break_8m, batched_256 -- throughput of the 8M two-column scene and of 256 independent
                scenes in one batched solver.
 
@@ -106,18 +108,20 @@ def time_steps(s, dt, bmin, bmax, steps: int) -> float:
     return time.perf_counter() - t0
 
 
-def learned_comparison(log_steps: int = 300, sample: int = 25, skip: int = 50, only: str = None):
-    """Plain solver with 1-4 iterations and learned warm start with J = 1, 2 on the 1M scene with
-    the reference application's parameters at a 2 ms substep.  only: run just one configuration
+def learned_comparison(log_steps: int = 300, sample: int = 25, skip: int = 50, only: str = None,
+                       scene: str = "1m", plain_iters=(4, 3, 2, 1), learned_js=(1, 2)):
+    """Plain solver with 1-4 iterations and learned warm start with J = 1, 2 on the 1M scene (or
+    the 8M two-column scene, scene="8m") with the reference application's parameters at a 2 ms
+    substep; the networks are the ones trained on 1M scenes.  only: run just one configuration
     ("plain_I<n>" or "learned_J<n>") and return its record."""
     from dataclasses import replace
     from akuaengine_amd import learned, scenes
     from akuaengine_amd.config import LambdaCorrParams, PBFConfig
     from akuaengine_amd.solver import PBFSolver
     cfg, corr, dt = PBFConfig(), LambdaCorrParams(), 0.002
-    sc = scenes.dam_break_1m()
+    sc = scenes.dam_break_8m() if scene == "8m" else scenes.dam_break_1m()
     nets = {}
-    for J in (1, 2):
+    for J in learned_js:
         path = os.path.join(learned.NETWORK_DIR, f"dam_break_1m_reference_J{J}.npy")
         if os.path.exists(path):
             with open(os.path.splitext(path)[0] + ".json") as f:
@@ -153,16 +157,16 @@ def learned_comparison(log_steps: int = 300, sample: int = 25, skip: int = 50, o
             r.update({"learned_iterations": J, "fallback_steps": st["fallbacks"],
                       "gate": {"statistic": "max", "ratio": round(nets[J][1]["gate_ratio_p90"], 4),
                                "tolerance": 0.01}})
-        log(f"[bench] {'learned J=%d' % J if J else 'plain I=%d' % I}: {ms:.3f} ms/step, "
+        log(f"[bench] {scene} {'learned J=%d' % J if J else 'plain I=%d' % I}: {ms:.3f} ms/step, "
             f"error max {r['density_error_max_avg']:.4f} mean {r['density_error_mean_avg']:.5f}")
         return r
 
     if only:
         kind, n = only.split("_")
         return {only: run(int(n[1:])) if kind == "plain" else run(4, int(n[1:]))}
-    plain = {f"plain_I{I}": run(I) for I in (4, 3, 2, 1)}
-    learned_runs = {f"learned_J{J}": run(4, J) for J in (1, 2)}
-    out = {"workload": "dam_break_1m_reference_substep", "particles": sc.n, "dt": dt,
+    plain = {f"plain_I{I}": run(I) for I in plain_iters}
+    learned_runs = {f"learned_J{J}": run(4, J) for J in learned_js}
+    out = {"workload": f"dam_break_{scene}_reference_substep", "particles": sc.n, "dt": dt,
            "parameters": "PBFConfig/LambdaCorrParams defaults (PBFConfig.h:10-29)",
            "steps": log_steps, "timed_steps": log_steps - skip, "error_samples_every": sample,
            **plain, **learned_runs}
@@ -318,6 +322,9 @@ def main() -> int:
 
     del s
     learned_out = None if args.no_learned else learned_comparison()
+    # the 8M scene of the learned workload: plain 4 and 3 iterations against the J = 1 network
+    learned_8m = None if args.no_learned or args.no_extra else learned_comparison(
+        log_steps=150, sample=25, skip=25, scene="8m", plain_iters=(4, 3), learned_js=(1,))
     extra = {} if args.no_extra else extra_workloads()
 
     cpu = None
@@ -343,6 +350,7 @@ def main() -> int:
                                         "density at the positions after the timed steps"},
         "kernels": kernels,
         "learned": learned_out,
+        "learned_8m": learned_8m,
         **extra,
         "cpu_baseline": cpu,
     }
