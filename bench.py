@@ -239,9 +239,8 @@ def main() -> int:
                          "learned_J1, learned_J2), print its record and exit (for profiling)")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(f"bench.py measures one GPU: every box in this setup has one MI355X, so --gpus "
-              f"{args.gpus} is not supported (the multi-GPU decomposition is a design note only).",
-              file=sys.stderr)
+        print(f"bench.py measures one GPU: --gpus {args.gpus} is not supported (the solver runs "
+              f"on one device; it has no multi-GPU decomposition).", file=sys.stderr)
         return 2
 
     if args.learned_run:
