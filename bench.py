@@ -29,7 +29,7 @@ Also reported, each in its own object of the same line:
                scenes in one batched solver.
 
 Usage: python bench.py [--steps K] [--warmup W] [--gpus 1] [--cpu-seconds S] [--no-cpu-baseline]
-                       [--no-learned] [--no-extra]
+                       [--no-learned] [--no-extra] [--tuning block,level,hashBits,layoutBits,interval]
 """
 from __future__ import annotations
 
@@ -234,6 +234,10 @@ def main() -> int:
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-learned", action="store_true")
     ap.add_argument("--no-extra", action="store_true")
+    ap.add_argument("--tuning", default=None,
+                    help="block,level,hashBits,layoutBits,interval: apply this setting instead of "
+                         "autotuning (every setting gives identical results; for profiling runs, "
+                         "whose own timings would steer the tuner)")
     ap.add_argument("--learned-run", default=None,
                     help="run only this configuration of the learned comparison (plain_I1..4, "
                          "learned_J1, learned_J2), print its record and exit (for profiling)")
@@ -259,7 +263,13 @@ def main() -> int:
     s.set_state(sc.pos, sc.vel, sc.mass)
     # tune on a developed state (the initial lattice is unrepresentatively regular), then warm up
     s.step_n(dt, sc.box_min, sc.box_max, args.warmup)
-    tuning = s.autotune(dt, sc.box_min, sc.box_max, trialSteps=5)
+    if args.tuning:
+        b_, l_, h_, d_, i_ = (int(v) for v in args.tuning.split(","))
+        s.set_tuning(block=b_, layoutLevel=l_, hashDigitBits=h_, layoutDigitBits=d_,
+                     layoutInterval=i_)
+        tuning = s.get_tuning()
+    else:
+        tuning = s.autotune(dt, sc.box_min, sc.box_max, trialSteps=5)
     log(f"[bench] tuning {tuning}")
 
     s.step_n(dt, sc.box_min, sc.box_max, args.warmup)
