@@ -87,21 +87,32 @@ __global__ __launch_bounds__(64) void rs_scan(uint32_t* __restrict__ counts,
                                               uint32_t* __restrict__ totals,
                                               const SortScene* __restrict__ scenes) {
     constexpr uint32_t R = 1u << D;
+    constexpr uint32_t kPer = 8;  // consecutive tiles per lane: one memory round trip per 512
     const uint32_t s = blockIdx.x / R, d = blockIdx.x % R;
     const SortScene sc = scenes[s];
     const uint32_t lane = lane_id();
     uint32_t carry = 0;
     uint32_t* c = counts + (size_t)sc.countBase * R + (size_t)d * sc.tiles;
-    for (uint32_t base = 0; base < sc.tiles; base += 64) {
-        const uint32_t t = base + lane;
-        const uint32_t v = t < sc.tiles ? c[t] : 0u;
-        uint32_t inc = v;
+    for (uint32_t base = 0; base < sc.tiles; base += 64u * kPer) {
+        const uint32_t t0 = base + lane * kPer;
+        uint32_t v[kPer];
+#pragma unroll
+        for (uint32_t i = 0; i < kPer; ++i) v[i] = t0 + i < sc.tiles ? c[t0 + i] : 0u;
+        uint32_t sum = 0;
+#pragma unroll
+        for (uint32_t i = 0; i < kPer; ++i) sum += v[i];
+        uint32_t inc = sum;
 #pragma unroll
         for (int off = 1; off < 64; off <<= 1) {
             uint32_t y = __shfl_up(inc, off, 64);
             if (lane >= (uint32_t)off) inc += y;
         }
-        if (t < sc.tiles) c[t] = carry + inc - v;
+        uint32_t run = carry + inc - sum;
+#pragma unroll
+        for (uint32_t i = 0; i < kPer; ++i) {
+            if (t0 + i < sc.tiles) c[t0 + i] = run;
+            run += v[i];
+        }
         carry += __shfl(inc, 63, 64);
     }
     if (lane == 0) totals[s * R + d] = carry;
