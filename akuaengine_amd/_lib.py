@@ -38,6 +38,13 @@ EXPORTED = [
     "akua_mlp_gradient", "akua_mlp_adam_step",
 ]
 
+# Every symbol include/akua_io.h declares (scenes and trajectory files; no GPU needed except by
+# akua_traj_append_solver).
+EXPORTED_IO = [
+    "akua_scene_describe", "akua_scene_generate", "akua_traj_create", "akua_traj_append",
+    "akua_traj_append_solver", "akua_traj_open", "akua_traj_read", "akua_traj_close",
+]
+
 MLP_PARAMS = 5544
 MLP_TRAINABLE = 5508
 
@@ -82,6 +89,24 @@ class AkuaTrainConfig(ctypes.Structure):
                 ("targetIterations", ctypes.c_int32),
                 ("learningRate", ctypes.c_float),
                 ("seed", ctypes.c_uint64), ("lossWeights", ctypes.c_float * 4)]
+
+class AkuaSceneInfo(ctypes.Structure):
+    """``akua_scene_info`` (include/akua_io.h)."""
+
+    _fields_ = [("numParticles", ctypes.c_int32), ("boxMin", ctypes.c_float * 3),
+                ("boxMax", ctypes.c_float * 3), ("deltaTime", ctypes.c_float),
+                ("referenceParams", ctypes.c_int32)]
+
+
+class AkuaTrajInfo(ctypes.Structure):
+    """``akua_traj_info`` (include/akua_io.h)."""
+
+    _fields_ = [("version", ctypes.c_uint32), ("numParticles", ctypes.c_int32),
+                ("fields", ctypes.c_uint32), ("deltaTime", ctypes.c_float),
+                ("boxMin", ctypes.c_float * 3), ("boxMax", ctypes.c_float * 3),
+                ("gravity", ctypes.c_float * 3), ("frameBytes", ctypes.c_uint64),
+                ("frames", ctypes.c_int64), ("partialFrame", ctypes.c_int32)]
+
 
 _lib = None
 
@@ -143,6 +168,15 @@ def load() -> ctypes.CDLL:
         "akua_mlp_forward": [P, P, i32, P],
         "akua_mlp_gradient": [P, P, P, i32, P, P, P],
         "akua_mlp_adam_step": [P, P, P, ctypes.c_int64, f32, P, P, i32, P, P],
+        "akua_scene_describe": [ctypes.c_char_p, f32, ctypes.c_int64, ctypes.POINTER(AkuaSceneInfo)],
+        "akua_scene_generate": [ctypes.c_char_p, f32, ctypes.c_int64, P, i32],
+        "akua_traj_create": [ctypes.c_char_p, i32, ctypes.c_uint32, f32, P, P, P, ctypes.POINTER(P)],
+        "akua_traj_append": [P, ctypes.c_int64, ctypes.c_double, P, P, P, P, P, P],
+        "akua_traj_append_solver": [P, P, ctypes.c_int64, ctypes.c_double, P],
+        "akua_traj_open": [ctypes.c_char_p, ctypes.POINTER(AkuaTrajInfo), ctypes.POINTER(P)],
+        "akua_traj_read": [P, ctypes.c_int64, ctypes.POINTER(ctypes.c_int64), ctypes.POINTER(ctypes.c_double),
+                           P, P, P, P, P, P],
+        "akua_traj_close": [P],
     }
     for name, args in sigs.items():
         fn = getattr(L, name)

@@ -22,7 +22,8 @@ OBJ = os.path.join(HERE, "_build")
 
 ARCH = os.environ.get("AKUA_OFFLOAD_ARCH", "gfx950")
 SOURCES = ["radix_sort.hip", "pbf_kernels.hip", "mlp.hip", "aos.hip", "stream_copy.hip", "selftest.hip",
-           "solver.cpp"]
+           "solver.cpp", "scene_io.cpp"]
+RUNNER = os.path.join(HERE, "akua_run")
 
 HIPFLAGS = [
     f"--offload-arch={ARCH}", "-O3", "-std=c++17", "-fPIC",
@@ -60,9 +61,25 @@ def build(jobs: int = 4, verbose: bool = False) -> str:
     if r.returncode != 0:
         raise RuntimeError(f"link failed:\n{r.stdout}\n{r.stderr}")
     os.replace(tmp, LIB)
+    build_runner()
     if verbose:
-        print(f"built {LIB}")
+        print(f"built {LIB} and {RUNNER}")
     return LIB
+
+
+def build_runner() -> str:
+    """The headless driver akua_run: host C++ over the C ABI (include/akua.h, akua_io.h)."""
+    cxx = os.environ.get("CXX") or shutil.which("g++") or shutil.which("c++")
+    if not cxx:
+        raise RuntimeError("no host C++ compiler (g++) found")
+    tmp = RUNNER + ".tmp"
+    cmd = [cxx, "-O2", "-std=c++17", "-Wall", "-Wextra", f"-I{os.path.join(ROOT, 'include')}",
+           os.path.join(CSRC, "akua_run.cpp"), "-o", tmp, f"-L{HERE}", "-lakua", "-Wl,-rpath,$ORIGIN"]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"akua_run build failed:\n{r.stdout}\n{r.stderr}")
+    os.replace(tmp, RUNNER)
+    return RUNNER
 
 
 if __name__ == "__main__":

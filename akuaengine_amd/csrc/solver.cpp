@@ -19,6 +19,7 @@
 
 #include "../../include/akua.h"
 #include "aos.h"
+#include "last_error.h"
 #include "mlp.h"
 #include "pbf_kernels.h"
 #include "pbf_types.h"
@@ -1233,6 +1234,8 @@ void autotune(akua_solver* s, float dt, const float* bmin, const float* bmax, in
 }
 
 }  // namespace
+
+void akua::set_last_error(const std::string& message) { g_lastError = message; }
 
 extern "C" {
 

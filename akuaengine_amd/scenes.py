@@ -7,6 +7,10 @@ optional per-axis uniform jitter in [-j, +j] drawn from SplitMix64 (seeded, so e
 reproducible bit-for-bit).  Velocities start at zero and every mass is 1 (Application.cpp:184-185).
 
 Workloads:
+  * ``app_dam_break``   -- the reference application's own scene (Application.cpp:12-16, 159-190):
+                           a lattice from (2, 1, 2) to (3.5, 2.5, 3.5) at the particle spacing
+                           (30^3 at 0.05), box [1.5,4.5]x[0,4]x[1.5,4.5], time step 0.016, the
+                           reference's default parameters.
   * ``dam_break_27k``   -- 30^3 lattice, spacing 0.05, origin 0.025, box [0,2]x[0,1.5]x[0,1]
                            (the lattice reaches z = 1.475, so the top 10 z-layers start outside
                            the box and are pushed in by the wall clamp).
@@ -81,6 +85,22 @@ def _scene(name, pos, box_min, box_max) -> Scene:
     return Scene(name, np.ascontiguousarray(pos, dtype=np.float32),
                  np.zeros((n, 3), np.float32), np.ones(n, np.float32),
                  tuple(float(b) for b in box_min), tuple(float(b) for b in box_max))
+
+
+APP_DT = 0.016  # Application::_deltaTime (Application.h:32)
+
+
+def app_dam_break(spacing: float = 0.05) -> Scene:
+    """Application::prepareDamBreak (Application.cpp:159-190): numX = int((maxPos - minPos).x /
+    particle_spacing) in FP32, position = minPos + index * spacing, box BOX_MIN..BOX_MAX
+    (Application.cpp:14-15)."""
+    s = np.float32(spacing)
+    lo = np.array([2.0, 1.0, 2.0], np.float32)
+    hi = np.array([3.5, 2.5, 3.5], np.float32)
+    dims = hi - lo
+    nx, ny, nz = (int(np.float32(dims[a] / s)) for a in range(3))
+    pos = lattice(nx, ny, nz, float(s), lo)
+    return _scene("app_dam_break", pos, (1.5, 0.0, 1.5), (4.5, 4.0, 4.5))
 
 
 def dam_break_27k() -> Scene:

@@ -1,0 +1,139 @@
+/* akua_io.h -- headless driver support of libakua.so: deterministic scenes and the trajectory
+ * file format.
+ *
+ * The reference drives its solver from a windowed application: Application::prepareDamBreak
+ * (Application.cpp:126-204) fills the particle VBO with a lattice, and Application::run
+ * (Application.cpp:37-70) advances a fixed-step accumulator, at most MAX_STEPS_PER_FRAME = 3
+ * steps per rendered frame, each step reading and writing the GL-owned particle buffer.  The
+ * headless counterparts here are
+ *   - akua_scene_*: the same lattice scenes, generated on the host bit for bit as the Python
+ *     generators (akuaengine_amd/scenes.py) and the reference's loop order produce them;
+ *   - akua_traj_*: a versioned binary file of particle frames, the output a renderer would
+ *     consume (Renderer::uploadParticles, Renderer.cpp:185-219) and the input of training and
+ *     fixtures;
+ *   - the runner akuaengine_amd/akua_run, which combines them with the solver.
+ * None of these functions touches a GPU, except akua_traj_append_solver (it reads the solver's
+ * state).
+ *
+ * Trajectory file, version 1 (all little-endian):
+ *   header, 128 bytes:
+ *     0   char[8]  magic "AKUATRJ\0"
+ *     8   uint32   version (1)
+ *     12  uint32   header bytes (128)
+ *     16  int32    particles n
+ *     20  uint32   fields (AKUA_FIELD_* bits)
+ *     24  float    time step
+ *     28  float[3] box min
+ *     40  float[3] box max
+ *     52  float[3] gravity
+ *     64  uint64   bytes per frame
+ *     72  ...      zero
+ *   then frames of (bytes per frame) each:
+ *     int64 step, double time, then one array per field present, in bit order:
+ *     position float[n][3], velocity float[n][3], mass float[n], density float[n],
+ *     lambda float[n], id uint32[n].
+ * Arrays are in the solver's logical order (the reference's sorted order); id[k] is the original
+ * index of the particle in slot k.  A file whose length is not a whole number of frames has a
+ * partial last frame (a writer that stopped mid-frame): readers report the whole frames only.
+ */
+#ifndef AKUA_IO_H
+#define AKUA_IO_H
+
+#include "akua.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define AKUA_TRAJ_VERSION 1
+#define AKUA_TRAJ_HEADER_BYTES 128
+
+enum {
+    AKUA_FIELD_POSITION = 1,
+    AKUA_FIELD_VELOCITY = 2,
+    AKUA_FIELD_MASS = 4,
+    AKUA_FIELD_DENSITY = 8,
+    AKUA_FIELD_LAMBDA = 16,
+    AKUA_FIELD_ID = 32,
+    AKUA_FIELD_ALL = 63
+};
+
+/* ---- scenes ------------------------------------------------------------------------------ */
+
+/* Scene names:
+ *   "app_dam_break"  the reference application's own scene (Application.cpp:12-16, 159-190):
+ *                    a lattice from (2, 1, 2) to (3.5, 2.5, 3.5) at the configuration's
+ *                    particle spacing (30^3 = 27,000 particles at 0.05), box
+ *                    [1.5, 4.5] x [0, 4] x [1.5, 4.5], time step 0.016 (Application.h:32);
+ *   "dam_break_27k"  30^3 lattice, spacing 0.05, origin 0.025, box [0,2] x [0,1.5] x [0,1];
+ *   "dam_break_1m"   100^3 lattice, jitter +-0.005 (seed 1), box [0,10] x [0,6] x [0,5];
+ *   "dam_break_8m"   two 200 x 200 x 100 columns at x = 0.025 and x = 20.025, jitter +-0.005
+ *                    (seeds s and s + 1000, s = 2), box [0,30] x [0,12] x [0,5];
+ *   "cube:<k>"       a k^3 lattice, jitter +-0.005 (seed 1), box [0,2e] x [0,1.2e] x [0,e] with
+ *                    e = 0.05 k.
+ * Lattices run x-major, then y, then z (Application.cpp:170-176); a position is
+ * origin + index * spacing in FP32; jitter adds (2u - 1) * j per coordinate, u the top 24 bits
+ * of consecutive SplitMix64 outputs times 2^-24.  Velocities start at 0 and masses at 1. */
+typedef struct akua_scene_info {
+    int32_t numParticles;
+    float boxMin[3];
+    float boxMax[3];
+    float deltaTime;        /* the scene's time step: 0.016 for app_dam_break, else 0.0083 */
+    int32_t referenceParams;/* 1: the reference's default parameters (akua_default_config);
+                               0: the dam-break parameter set (rest density 6378, s_corr k 0.1,
+                               delta_q 0.02) */
+} akua_scene_info;
+
+/* spacing: the lattice spacing of app_dam_break (the configuration's particle_spacing; <= 0
+ * means 0.05); the other scenes ignore it.  seed < 0: the scene's own seed. */
+akua_status akua_scene_describe(const char* name, float spacing, int64_t seed,
+                                akua_scene_info* info);
+/* Positions (3 n floats, xyz per particle) of the scene; n must equal numParticles. */
+akua_status akua_scene_generate(const char* name, float spacing, int64_t seed, float* pos,
+                                int32_t n);
+
+/* ---- trajectory files -------------------------------------------------------------------- */
+
+typedef struct akua_traj akua_traj;
+
+typedef struct akua_traj_info {
+    uint32_t version;
+    int32_t numParticles;
+    uint32_t fields;
+    float deltaTime;
+    float boxMin[3];
+    float boxMax[3];
+    float gravity[3];
+    uint64_t frameBytes;
+    int64_t frames;         /* whole frames in the file */
+    int32_t partialFrame;   /* 1 if the file ends in a partial frame */
+} akua_traj_info;
+
+/* Creates (truncates) a file and writes its header.  fields: AKUA_FIELD_* bits, at least one. */
+akua_status akua_traj_create(const char* path, int32_t n, uint32_t fields, float deltaTime,
+                             const float boxMin[3], const float boxMax[3],
+                             const float gravity[3], akua_traj** out);
+/* Appends one frame.  Every array of a field the file holds must be non-NULL; the others are
+ * ignored. */
+akua_status akua_traj_append(akua_traj* t, int64_t step, double time, const float* pos,
+                             const float* vel, const float* mass, const float* density,
+                             const float* lambda, const uint32_t* ids);
+/* Appends the solver's current state (akua_solver_get_state: synchronises the solver).
+ * idMap (optional, n entries): written ids are idMap[id] instead of id, for a run that
+ * continues a trajectory whose particles were renumbered on loading. */
+akua_status akua_traj_append_solver(akua_traj* t, akua_solver* s, int64_t step, double time,
+                                    const uint32_t* idMap);
+/* Opens a file for reading: checks the magic, the version and that the header is consistent. */
+akua_status akua_traj_open(const char* path, akua_traj_info* info, akua_traj** out);
+/* Reads whole frame `frame` (0-based; negative counts from the end).  Any output may be NULL;
+ * requesting a field the file lacks is an error. */
+akua_status akua_traj_read(akua_traj* t, int64_t frame, int64_t* step, double* time, float* pos,
+                           float* vel, float* mass, float* density, float* lambda, uint32_t* ids);
+/* Closes a file opened by create or open (flushing a written one). */
+akua_status akua_traj_close(akua_traj* t);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* AKUA_IO_H */

@@ -10,13 +10,18 @@ import pytest
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 HEADER = os.path.join(ROOT, "include", "akua.h")
+HEADERS = sorted(os.path.join(ROOT, "include", f) for f in os.listdir(os.path.join(ROOT, "include"))
+                 if f.endswith(".h"))
 
 
-def _declared_functions():
-    text = open(HEADER).read()
-    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
-    names = re.findall(r"^\s*(?:const\s+)?[a-z_0-9]+\s*\*?\s*(akua_[a-z_0-9]+)\s*\(", text, flags=re.M)
-    return sorted(set(names))
+def _declared_functions(headers=(HEADER,)):
+    names = set()
+    for h in headers:
+        text = open(h).read()
+        text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+        names.update(re.findall(r"^\s*(?:const\s+)?[a-z_0-9]+\s*\*?\s*(akua_[a-z_0-9]+)\s*\(", text,
+                                flags=re.M))
+    return sorted(names)
 
 
 @pytest.fixture(scope="module")
@@ -30,13 +35,16 @@ def test_header_declares_the_abi():
     assert "akua_solver_create" in names and "akua_solver_step" in names
     from akuaengine_amd import _lib
     assert sorted(_lib.EXPORTED) == names
+    io = os.path.join(ROOT, "include", "akua_io.h")
+    assert sorted(_lib.EXPORTED_IO) == _declared_functions((io,))
+    assert sorted(_lib.EXPORTED + _lib.EXPORTED_IO) == _declared_functions(HEADERS)
 
 
 def test_every_declared_symbol_is_exported(lib):
     out = subprocess.run(["nm", "-D", "--defined-only", os.path.join(ROOT, "akuaengine_amd", "libakua.so")],
                          capture_output=True, text=True, check=True).stdout
     exported = {line.split()[-1] for line in out.splitlines() if line.strip()}
-    for name in _declared_functions():
+    for name in _declared_functions(HEADERS):
         assert name in exported, name
         assert getattr(lib, name) is not None
 
