@@ -67,6 +67,10 @@ def lib():
         L.orc_grad_spiky.argtypes = [ctypes.c_int, P, d, P]
         L.orc_cell_hash.restype = u32
         L.orc_cell_hash.argtypes = [i32, i32, i32, u32]
+        L.orc_set_boundary.argtypes = [vp, ctypes.c_int, f]
+        L.orc_boundary_samples.restype = lng
+        L.orc_boundary_samples.argtypes = [vp, P, P, P, P, lng]
+        L.orc_boundary_sums.argtypes = [vp, P, P, P, P]
         L.orc_set_threads.argtypes = [ctypes.c_int]
         L.orc_max_threads.restype = ctypes.c_int
         _lib = L
@@ -190,6 +194,26 @@ class Oracle:
         """Replaces x* (sorted order): continues a step from a warm start."""
         a = np.ascontiguousarray(xs, np.float32).reshape(self.n, 3)
         lib().orc_set_xs(self._h, _p(a))
+
+    def set_boundary(self, layers: int, spacing: float = 0.0) -> None:
+        """Boundary samples on the box walls (layers 0: off; spacing <= 0: particle_spacing)."""
+        lib().orc_set_boundary(self._h, int(layers), float(spacing))
+
+    def boundary_samples(self, box_min, box_max):
+        """(positions (m, 3), psi (m,)) of the samples for this box, in index order."""
+        lo, hi = _f3(box_min), _f3(box_max)
+        m = lib().orc_boundary_samples(self._h, _p(lo), _p(hi), None, None, 0)
+        if m < 0:
+            raise ValueError("no boundary for this box and spacing")
+        pos, psi = np.zeros((m, 3)), np.zeros(m)
+        lib().orc_boundary_samples(self._h, _p(lo), _p(hi), _p(pos), _p(psi), m)
+        return pos, psi
+
+    def boundary_sums(self, box_min, box_max):
+        """(rhoB, gB) at x* of the current state, sorted order."""
+        rb, gb = np.zeros(self.n), np.zeros((self.n, 3))
+        lib().orc_boundary_sums(self._h, _p(_f3(box_min)), _p(_f3(box_max)), _p(rb), _p(gb))
+        return rb, gb
 
     def true_density(self):
         """Exact SPH density at the current positions: (by sorted slot, by particle id)."""

@@ -22,7 +22,39 @@ struct Handle {
     std::vector<uint32_t> preSortHash;
     std::vector<float> featF;
     std::vector<double> featD;
+    // boundary samples (opt-in): settings, and the samples built for the last box seen
+    int bndLayers = 0;          // 0: off
+    float bndSpacing = 0.0f;
+    bool bndBuilt = false;
+    float bndBox[6] = {0, 0, 0, 0, 0, 0};
+    Boundary<float> bf;
+    Boundary<double> bd;
 };
+
+template <class T>
+Boundary<T>& bnd(Handle* h);
+template <>
+Boundary<float>& bnd<float>(Handle* h) { return h->bf; }
+template <>
+Boundary<double>& bnd<double>(Handle* h) { return h->bd; }
+
+// The boundary for this box (rebuilt when the box changes); null when off or not admissible.
+template <class T>
+const Boundary<T>* boundary_for(Handle* h, const float* bmin, const float* bmax) {
+    if (h->bndLayers <= 0) return nullptr;
+    const float box[6] = {bmin[0], bmin[1], bmin[2], bmax[0], bmax[1], bmax[2]};
+    Boundary<T>& B = bnd<T>(h);
+    if (!h->bndBuilt || std::memcmp(box, h->bndBox, sizeof(box)) != 0) {
+        const T lo[3] = {T(bmin[0]), T(bmin[1]), T(bmin[2])};
+        const T hi[3] = {T(bmax[0]), T(bmax[1]), T(bmax[2])};
+        const T sp = T(h->bndSpacing > 0.0f ? h->bndSpacing : h->cfg.particle_spacing);
+        if (!make_boundary<T>(B, lo, hi, sp, h->bndLayers)) return nullptr;
+        boundary_psi<T>(B, Kern<T>(T(h->cfg.smoothRadius)), T(h->cfg.restDensity));
+        std::memcpy(h->bndBox, box, sizeof(box));
+        h->bndBuilt = true;
+    }
+    return &B;
+}
 
 template <class T>
 State<T>& st(Handle* h);
@@ -87,6 +119,7 @@ void get_stage(Handle* h, double* xs, double* vs, double* dp, double* w) {
 template <class T>
 void run_steps(Handle* h, float dt, const float* bmin, const float* bmax, int nsteps) {
     Params<T> p = make_params<T>(h->cfg, h->corr, dt, bmin, bmax, h->tableSize);
+    p.bnd = boundary_for<T>(h, bmin, bmax);
     for (int k = 0; k < nsteps; ++k) step(st<T>(h), p, &h->taps);
 }
 
@@ -94,8 +127,10 @@ void run_steps(Handle* h, float dt, const float* bmin, const float* bmax, int ns
 template <class T>
 void run_stage(Handle* h, int stage, float dt, const float* bmin, const float* bmax) {
     Params<T> p = make_params<T>(h->cfg, h->corr, dt, bmin, bmax, h->tableSize);
+    p.bnd = boundary_for<T>(h, bmin, bmax);
     State<T>& s = st<T>(h);
     Taps& t = h->taps;
+    if (p.bnd && stage == 4) boundary_sums(s, p);
     switch (stage) {
         case 0: predict(s, p); break;
         case 1: compute_hashes(s, p); h->preSortHash = s.hash; break;
@@ -205,6 +240,51 @@ void orc_set_xs(void* hp, const float* xs) {
         if (h->prec == 64) h->sd.xs[i] = {xs[3 * i], xs[3 * i + 1], xs[3 * i + 2]};
         else h->sf.xs[i] = {xs[3 * i], xs[3 * i + 1], xs[3 * i + 2]};
     }
+}
+
+// Boundary samples (layers 0: off; spacing <= 0: the configuration's particle spacing).
+void orc_set_boundary(void* hp, int layers, float spacing) {
+    auto* h = static_cast<Handle*>(hp);
+    h->bndLayers = layers;
+    h->bndSpacing = spacing;
+    h->bndBuilt = false;
+}
+
+// The samples for this box: returns their count (-1: not admissible); pos (3 per sample) and psi
+// receive up to `cap` samples in index order.  Either may be NULL.
+long orc_boundary_samples(void* hp, const float bmin[3], const float bmax[3], double* pos, double* psi,
+                          long cap) {
+    auto* h = static_cast<Handle*>(hp);
+    auto out = [&](auto* B) -> long {
+        if (!B) return -1;
+        const long m = (long)B->size();
+        for (long k = 0; k < std::min(m, cap); ++k) {
+            const auto x = boundary_sample(*B, (size_t)k);
+            if (pos) { pos[3 * k] = x.x; pos[3 * k + 1] = x.y; pos[3 * k + 2] = x.z; }
+            if (psi) psi[k] = B->psi[(size_t)k];
+        }
+        return m;
+    };
+    if (h->prec == 64) return out(boundary_for<double>(h, bmin, bmax));
+    return out(boundary_for<float>(h, bmin, bmax));
+}
+
+// Boundary sums at x* of the current state (rhoB per particle, gB 3 per particle, sorted order).
+void orc_boundary_sums(void* hp, const float bmin[3], const float bmax[3], double* rhoB, double* gB) {
+    auto* h = static_cast<Handle*>(hp);
+    auto run = [&](auto& s, const auto* B, auto tag) {
+        using T = decltype(tag);
+        if (!B) return;
+        Params<T> p = make_params<T>(h->cfg, h->corr, 0.0f, bmin, bmax, h->tableSize);
+        p.bnd = B;
+        boundary_sums(s, p);
+        for (size_t i = 0; i < s.size(); ++i) {
+            if (rhoB) rhoB[i] = s.rhoB[i];
+            if (gB) { gB[3 * i] = s.gB[i].x; gB[3 * i + 1] = s.gB[i].y; gB[3 * i + 2] = s.gB[i].z; }
+        }
+    };
+    if (h->prec == 64) run(h->sd, boundary_for<double>(h, bmin, bmax), 0.0);
+    else run(h->sf, boundary_for<float>(h, bmin, bmax), 0.0f);
 }
 
 long orc_ncells(void* hp) { return (long)static_cast<Handle*>(hp)->taps.cell_hash.size(); }

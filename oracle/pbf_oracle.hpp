@@ -19,6 +19,8 @@
 //     squares of the SPH kernels, and s_corr's exponent n when it is an integer) is the k-fold
 //     product x*x*...*x evaluated left to right; a non-integer n calls the libm powf.  The
 //     per-call constants powf(h, 9) and powf(h, 6) call the libm powf as written.
+//   * boundary samples (opt-in, no reference counterpart; the reference plans them as "virtual
+//     particles", ConstraintSolverCUDA.cu:132-135): the definition above make_boundary().
 // Parity status of each function is listed in the comment above it ("pinned by ...").
 #pragma once
 
@@ -174,6 +176,8 @@ struct State {
     std::vector<V3<T>> w;     // vorticity
     std::vector<T> mass, density, lambda;
     std::vector<uint32_t> hash, id;
+    std::vector<T> rhoB;      // boundary samples' density contribution at x* (boundary on)
+    std::vector<V3<T>> gB;    // sum_b psi_b grad W(x*_i - x_b) (boundary on)
     size_t size() const { return x.size(); }
     void resize(size_t n) {
         x.resize(n); v.resize(n); xs.resize(n); vs.resize(n); dp.resize(n); w.resize(n);
@@ -191,8 +195,174 @@ struct Taps {
     std::vector<uint32_t> cell_end;      // one past the last sorted index
 };
 
+// ---- boundary samples (opt-in; no reference counterpart) --------------------------------------
+// The reference keeps particles in the box with a soft clamp and plans "virtual particles" in its
+// place (ConstraintSolverCUDA.cu:132-135).  This is that extension, after Akinci et al. 2012
+// ("Versatile rigid-fluid coupling for incompressible SPH") in the PBF form of Bender's
+// PositionBasedDynamics library: static samples b on the six walls, each with the volume weight
+// psi_b = rho0 / sum_k W(x_b - x_k) over the samples k, add
+//   psi_b W(x*_i - x_b) to rho_i,  psi_b grad W(x*_i - x_b) to the sum behind grad_i C_i,
+//   and lambda_i psi_b grad W(x*_i - x_b) to the sum behind dp_i (both sums times 1/rho0).
+// The samples do not move and have no lambda of their own; the clamp and damping stay.
+// Definition (the library implements the same):
+//   per axis a: n_a = max(2, int(ceil((hi_a - lo_a) / s)) + 1), d_a = (hi_a - lo_a) / (n_a - 1);
+//   wall w = 0..5: normal axis a = w / 2, side w % 2 (lo, hi), tangential axes b < c;
+//   layer l = 0..L-1 at lo_a - l s (side 0) or hi_a + l s (side 1);
+//   tangential index t along axis e: t in [-(L-1), n_e - 1 + (L-1)] when e > a (the wall owns
+//   that edge), t in [1, n_e - 2] when e < a; coordinate lo_e + t d_e;
+//   sample index base_w + (l N_b + (t_b - t0_b)) N_c + (t_c - t0_c), base_w the samples of the
+//   walls before w.
+//   For a point x, the samples that count are those of the index window below with
+//   |x - x_b|^2 < h^2 (dot in the usual order); sums run in ascending sample index.  The window
+//   per wall and layer: layers with (x_a - plane)^2 < h^2, then t_e from
+//   floor((x_e - lo_e - h) / d_e) - 1 to floor((x_e - lo_e + h) / d_e) + 1 (clamped to the wall),
+//   rows and columns with (x_e - coord)^2 >= h^2 skipped.  Pinned by brute force over all
+//   samples, by psi's invariant sum_k psi_k W_bk = rho0 on a flat wall, and by closed forms.
+template <class T>
+struct Boundary {
+    int L = 0;
+    T s{}, lo[3]{}, hi[3]{}, d[3]{};
+    int n[3]{};
+    int t0[6][2]{}, tn[6][2]{};
+    size_t base[7]{};
+    std::vector<T> psi;
+    size_t size() const { return base[6]; }
+};
+
+inline void wall_axes(int w, int& a, int& b, int& c) {
+    a = w / 2;
+    b = a == 0 ? 1 : 0;
+    c = a == 2 ? 1 : 2;
+}
+
+template <class T>
+inline long long floor_ll(T v) {
+    if (!(v == v)) return 0;
+    if (v >= T(1e15)) return (long long)1e15;
+    if (v <= T(-1e15)) return -(long long)1e15;
+    return (long long)std::floor(v);
+}
+
+// Returns false when the box or the spacing admits no boundary (a non-positive extent, or more
+// than 2^26 samples).
+template <class T>
+bool make_boundary(Boundary<T>& B, const T lo[3], const T hi[3], T s, int L) {
+    B = Boundary<T>{};
+    if (!(s > T(0)) || L < 1 || L > 4) return false;
+    B.L = L;
+    B.s = s;
+    for (int a = 0; a < 3; ++a) {
+        B.lo[a] = lo[a];
+        B.hi[a] = hi[a];
+        const T E = hi[a] - lo[a];
+        if (!(E > T(0))) return false;
+        const T q = std::ceil(E / s);
+        if (!(q < T(1 << 20))) return false;
+        B.n[a] = std::max(2, (int)q + 1);
+        B.d[a] = E / T(B.n[a] - 1);
+    }
+    B.base[0] = 0;
+    for (int w = 0; w < 6; ++w) {
+        int a, b, c;
+        wall_axes(w, a, b, c);
+        const int e2[2] = {b, c};
+        for (int k = 0; k < 2; ++k) {
+            const int e = e2[k];
+            if (e > a) {
+                B.t0[w][k] = -(L - 1);
+                B.tn[w][k] = B.n[e] + 2 * (L - 1);
+            } else {
+                B.t0[w][k] = 1;
+                B.tn[w][k] = std::max(0, B.n[e] - 2);
+            }
+        }
+        B.base[w + 1] = B.base[w] + (size_t)L * (size_t)B.tn[w][0] * (size_t)B.tn[w][1];
+    }
+    return B.size() <= (size_t(1) << 26);
+}
+
+template <class T>
+V3<T> boundary_sample(const Boundary<T>& B, size_t idx) {
+    int w = 0;
+    while (w < 5 && idx >= B.base[w + 1]) ++w;
+    int a, b, c;
+    wall_axes(w, a, b, c);
+    size_t r = idx - B.base[w];
+    const size_t nc = (size_t)B.tn[w][1], nb = (size_t)B.tn[w][0];
+    const int tc = (int)(r % nc) + B.t0[w][1];
+    r /= nc;
+    const int tb = (int)(r % nb) + B.t0[w][0];
+    const int l = (int)(r / nb);
+    T sp[3];
+    sp[a] = (w % 2) ? B.hi[a] + T(l) * B.s : B.lo[a] - T(l) * B.s;
+    sp[b] = B.lo[b] + T(tb) * B.d[b];
+    sp[c] = B.lo[c] + T(tc) * B.d[c];
+    return {sp[0], sp[1], sp[2]};
+}
+
+// Calls f(index, x - x_b, |x - x_b|^2) for every sample that counts for x, in ascending index.
+template <class T, class F>
+void for_boundary_near(const Boundary<T>& B, V3<T> x, const Kern<T>& K, F&& f) {
+    const T xa[3] = {x.x, x.y, x.z};
+    for (int w = 0; w < 6; ++w) {
+        int a, b, c;
+        wall_axes(w, a, b, c);
+        if (B.tn[w][0] == 0 || B.tn[w][1] == 0) continue;
+        auto window = [&](int e, int k, long long& lo, long long& hi) {
+            lo = std::max<long long>(B.t0[w][k], floor_ll<T>((xa[e] - B.lo[e] - K.h) / B.d[e]) - 1);
+            hi = std::min<long long>(B.t0[w][k] + B.tn[w][k] - 1,
+                                     floor_ll<T>((xa[e] - B.lo[e] + K.h) / B.d[e]) + 1);
+        };
+        long long b0, b1, c0, c1;
+        window(b, 0, b0, b1);
+        window(c, 1, c0, c1);
+        for (int l = 0; l < B.L; ++l) {
+            const T na = (w % 2) ? B.hi[a] + T(l) * B.s : B.lo[a] - T(l) * B.s;
+            const T ra = xa[a] - na;
+            if (ra * ra >= K.h2) continue;
+            for (long long tb = b0; tb <= b1; ++tb) {
+                const T cb = B.lo[b] + T(tb) * B.d[b];
+                const T rb = xa[b] - cb;
+                if (rb * rb >= K.h2) continue;
+                for (long long tc = c0; tc <= c1; ++tc) {
+                    const T cc = B.lo[c] + T(tc) * B.d[c];
+                    const T rc = xa[c] - cc;
+                    if (rc * rc >= K.h2) continue;
+                    T sp[3];
+                    sp[a] = na;
+                    sp[b] = cb;
+                    sp[c] = cc;
+                    const V3<T> sep = x - V3<T>{sp[0], sp[1], sp[2]};
+                    const T d2 = dot(sep, sep);
+                    if (!(d2 < K.h2)) continue;
+                    const size_t idx = B.base[w] +
+                                       ((size_t)l * (size_t)B.tn[w][0] + (size_t)(tb - B.t0[w][0])) *
+                                           (size_t)B.tn[w][1] +
+                                       (size_t)(tc - B.t0[w][1]);
+                    f(idx, sep, d2);
+                }
+            }
+        }
+    }
+}
+
+// psi_b = rho0 / sum_k W(x_b - x_k), the sum over the samples that count for x_b (b included).
+template <class T>
+void boundary_psi(Boundary<T>& B, const Kern<T>& K, T rest) {
+    const long m = (long)B.size();
+    B.psi.assign((size_t)m, T(0));
+#pragma omp parallel for schedule(static)
+    for (long k = 0; k < m; ++k) {
+        T delta = T(0);
+        for_boundary_near(B, boundary_sample(B, (size_t)k), K,
+                          [&](size_t, V3<T>, T d2) { delta = delta + poly6<T>(d2, K); });
+        B.psi[(size_t)k] = rest / delta;
+    }
+}
+
 template <class T>
 struct Params {
+    const Boundary<T>* bnd = nullptr;  // boundary samples (null: the reference's walls only)
     Kern<T> K{T(0.1f)};
     T h, h2, cellHash, cellSearch, invRest, relax, vortEps, visc, dt;
     T g[3], bmin[3], bmax[3];
@@ -363,6 +533,7 @@ void densities(State<T>& s, const Params<T>& p, const std::vector<uint32_t>& nbr
             T d2 = dot(sep, sep);
             rho = rho + s.mass[j] * poly6<T>(d2, p.K);
         }
+        if (p.bnd) rho = rho + s.rhoB[i];
         s.density[i] = rho;
     }
 }
@@ -382,6 +553,7 @@ void lambdas(State<T>& s, const Params<T>& p, const std::vector<uint32_t>& nbr,
             V3<T> sep = s.xs[i] - s.xs[j];
             gi = gi + s.mass[j] * grad_spiky<T>(sep, p.K);
         }
+        if (p.bnd) gi = gi + s.gB[i];
         gi = gi * p.invRest;  // grad_pi_Ci *= inverseRestDensity
         T sum = T(0);
         for (uint32_t o = 0; o < cnt[i]; ++o) {
@@ -415,6 +587,7 @@ void position_deltas(State<T>& s, const Params<T>& p, const std::vector<uint32_t
             if (p.corrOn) coef = coef + -p.corrK * ipow<T>(poly6<T>(d2, p.K) / wdq, p.corrN);
             d = d + (coef * s.mass[j]) * grad_spiky<T>(sep, p.K);
         }
+        if (p.bnd) d = d + s.lambda[i] * s.gB[i];
         s.dp[i] = d * p.invRest;
     }
 }
@@ -611,10 +784,33 @@ void features(const State<T>& s, const Params<T>& p, const std::vector<uint32_t>
     }
 }
 
+// Boundary sums at x* (boundary on): rhoB_i = sum_b psi_b W(x*_i - x_b) and
+// gB_i = sum_b psi_b grad W(x*_i - x_b), in ascending sample index.
+template <class T>
+void boundary_sums(State<T>& s, const Params<T>& p) {
+    const long n = (long)s.size();
+    s.rhoB.assign((size_t)n, T(0));
+    s.gB.assign((size_t)n, V3<T>{T(0), T(0), T(0)});
+    const Boundary<T>& B = *p.bnd;
+#pragma omp parallel for schedule(dynamic, 256)
+    for (long i = 0; i < n; ++i) {
+        T rb = T(0);
+        V3<T> gb{T(0), T(0), T(0)};
+        for_boundary_near(B, s.xs[i], p.K, [&](size_t idx, V3<T> sep, T d2) {
+            const T psi = B.psi[idx];
+            rb = rb + psi * poly6<T>(d2, p.K);
+            gb = gb + psi * grad_spiky<T>(sep, p.K);
+        });
+        s.rhoB[i] = rb;
+        s.gB[i] = gb;
+    }
+}
+
 // One iteration of runConstraintSolverCUDA's loop (ConstraintSolverCUDA.cu:206-218).
 template <class T>
 void constraint_iteration(State<T>& s, const Params<T>& p, const std::vector<uint32_t>& nbr,
                           const std::vector<uint32_t>& cnt) {
+    if (p.bnd) boundary_sums(s, p);
     densities(s, p, nbr, cnt);
     lambdas(s, p, nbr, cnt);
     position_deltas(s, p, nbr, cnt);
