@@ -46,6 +46,8 @@ const char* kUsage =
     "  --out FILE              write a trajectory: the initial state, then every --every steps\n"
     "  --every K               (default 1)\n"
     "  --fields LIST           pos,vel,mass,density,lambda,id or all (default pos,vel,density,id)\n"
+    "  --boundary L            L layers of wall samples (0 = off, the reference's walls; 1..4)\n"
+    "  --boundary-spacing S    their spacing (default: particle_spacing)\n"
     "  --autotune              pick launch and layout settings before the run (results unchanged)\n"
     "  --network FILE.npy      learned warm start with this parameter blob (float32 .npy)\n"
     "  --learned-iterations J  (default 1)\n"
@@ -71,6 +73,8 @@ struct Options {
     int64_t every = 1;
     uint32_t fields = AKUA_FIELD_POSITION | AKUA_FIELD_VELOCITY | AKUA_FIELD_DENSITY | AKUA_FIELD_ID;
     bool autotune = false;
+    int boundary = 0;
+    float boundarySpacing = 0.0f;
     std::string network;
     int learnedIters = 1;
     float gateRatio = 1.0f;
@@ -154,6 +158,8 @@ Options parse(int argc, char** argv) {
         else if (a == "--every") o.every = parse_int(a, val(), 1, INT32_MAX);
         else if (a == "--fields") o.fields = parse_fields(val());
         else if (a == "--autotune") o.autotune = true;
+        else if (a == "--boundary") o.boundary = (int)parse_int(a, val(), 0, 4);
+        else if (a == "--boundary-spacing") o.boundarySpacing = (float)parse_num(a, val());
         else if (a == "--network") o.network = val();
         else if (a == "--learned-iterations") o.learnedIters = (int)parse_int(a, val(), 1, 64);
         else if (a == "--gate-ratio") o.gateRatio = (float)parse_num(a, val());
@@ -359,6 +365,7 @@ int main(int argc, char** argv) {
     akua_solver* s = nullptr;
     check(akua_solver_create(n, &cfg, &corr, &s), "creating the solver");
     check(akua_solver_set_graph(s, o.graph ? 1 : 0), "set_graph");
+    if (o.boundary > 0) check(akua_solver_set_boundary(s, o.boundary, o.boundarySpacing), "set_boundary");
     check(akua_solver_set_state(s, pos.data(), vel.data(), mass.data(), dens.data(), lam.data(), n),
           "uploading the particles");
     if (!o.network.empty()) {
@@ -407,12 +414,12 @@ int main(int argc, char** argv) {
     std::printf("{\"source\": %s, \"particles\": %d, \"params\": \"%s\", \"iterations\": %d, \"dt\": %.9g, "
                 "\"frames\": %lld, \"steps\": %lld, \"seconds\": %.6f, \"io_seconds\": %.6f, "
                 "\"particle_substeps_per_s\": %.6g, \"max_density_error\": %.6g, \"mean_density_error\": %.6g, "
-                "\"tuning\": [%d, %d, %d, %d, %d], \"graph\": %s",
+                "\"tuning\": [%d, %d, %d, %d, %d], \"graph\": %s, \"boundary_layers\": %d",
                 json_str(source).c_str(), n, referenceParams ? "reference" : "dam_break", o.iterations, dt,
                 (long long)frames, (long long)done, total, ioSeconds,
                 stepSeconds > 0.0 ? (double)n * (double)done / stepSeconds : 0.0, maxErr, meanErr, tuning.block,
                 tuning.layoutLevel, tuning.hashDigitBits, tuning.layoutDigitBits, tuning.layoutInterval,
-                o.graph ? "true" : "false");
+                o.graph ? "true" : "false", o.boundary);
     if (!o.network.empty())
         std::printf(", \"learned\": {\"iterations\": %d, \"learned_steps\": %lld, \"fallbacks\": %lld}", o.learnedIters,
                     (long long)learnedSteps, (long long)fallbacks);

@@ -46,6 +46,7 @@ struct DeviceState {
     float4* XsB;
     float4* W;
     float4* Vc;
+    float4* Bg;                // boundary gradient sum of each slot's last density pass (boundary on)
     float* Tdens;
     float* Tlam;
     uint32_t* Tid;

@@ -24,6 +24,7 @@
 #include "pbf_kernels.h"
 
 #include <algorithm>
+#include "boundary.h"
 #include "pbf_math.h"
 
 namespace akua {
@@ -410,7 +411,10 @@ __device__ __forceinline__ void wave_resid_partial(const DeviceState& d, int slo
 }
 
 // kFast (SphParams::fast): unit masses, range-tested quotients bounded in advance.
-template <int BS, bool kFast>
+// kBnd: boundary samples (SphParams::bd): their density and gradient sums join rho and grad_i C
+// after the neighbour sweep; the gradient sum is kept in Bg for the position-delta pass, which
+// runs at the same x*.
+template <int BS, bool kFast, bool kBnd>
 __device__ __forceinline__ void density_lambda_slot(const DeviceState& d, const SphParams& P,
                                                     const float4* __restrict__ Xs, int residSlot,
                                                     int gated, uint32_t i) {
@@ -447,6 +451,21 @@ __device__ __forceinline__ void density_lambda_slot(const DeviceState& d, const 
                     sum = sum + dot3(gj, gj);
                 }
             });
+        if (kBnd) {
+            float rb = 0.0f;
+            f3 gb{0.0f, 0.0f, 0.0f};
+            if (boundary_near(P.bd, xi.x, xi.y, xi.z, P.h2)) {
+                boundary_visit(P.bd, xi.x, xi.y, xi.z, P.h, P.h2,
+                               [&](uint32_t idx, float s0, float s1, float s2, float d2) {
+                                   const float psi = P.bd.psi[idx];
+                                   rb = rb + psi * poly6(d2, P);
+                                   gb = gb + psi * grad_spiky<kFast>(f3{s0, s1, s2}, P);
+                               });
+                d.Bg[i] = make_float4(gb.x, gb.y, gb.z, 0.0f);
+            }
+            rho = rho + rb;
+            gi = gi + gb;
+        }
         gi = gi * P.invRest;
         C = rho * P.invRest - 1.0f;
         d.dens[i] = rho;
@@ -457,12 +476,12 @@ __device__ __forceinline__ void density_lambda_slot(const DeviceState& d, const 
 
 // One workgroup per virtual workgroup (the loop runs once); the loop form lets a launch use any
 // grid size.
-template <int BS, bool kFast>
+template <int BS, bool kFast, bool kBnd>
 __global__ __launch_bounds__(BS) void k_density_lambda(DeviceState d, SphParams P,
                                                            const float4* __restrict__ Xs,
                                                            int residSlot, int gated, uint32_t nvb) {
     for (uint32_t vb = blockIdx.x; vb < nvb; vb += gridDim.x)
-        density_lambda_slot<BS, kFast>(d, P, Xs, residSlot, gated, xcd_slot_of<BS>(vb, nvb));
+        density_lambda_slot<BS, kFast, kBnd>(d, P, Xs, residSlot, gated, xcd_slot_of<BS>(vb, nvb));
 }
 
 // Features of the learned warm start at x*_pred, one neighbour sweep (list order):
@@ -690,7 +709,7 @@ __device__ __forceinline__ f3 damp_walls(f3 x, f3 v, const SceneDev& sc) {
 // kFinal: also kernel_update_position_and_velocity (IntegrationCUDA.cu:38-49) and the wall
 // damping, writing the step's positions X, velocities V, ids and the two order maps.
 // gateMode 1: only scenes whose gate is set run; 2: scenes whose gate is clear read XsAlt.
-template <int BS, bool kFinal, bool kApplyDelta, bool kFast>
+template <int BS, bool kFinal, bool kApplyDelta, bool kFast, bool kBnd>
 __device__ __forceinline__ void delta_correct_slot(const DeviceState& d, const SphParams& P,
                                                    const StepParams& S,
                                                    const float4* __restrict__ XsSet,
@@ -728,6 +747,11 @@ __device__ __forceinline__ void delta_correct_slot(const DeviceState& d, const S
                 else
                     dp = dp + (coef * nb.x.w) * g;
             });
+        if (kBnd) {  // lambda_i times the boundary gradient sum of the density pass (same x*)
+            f3 gb{0.0f, 0.0f, 0.0f};
+            if (boundary_near(P.bd, xi.x, xi.y, xi.z, P.h2)) gb = make3(d.Bg[i]);
+            dp = dp + li * gb;
+        }
         dp = dp * P.invRest;
         q = xi + dp;
         const float minDist = 0.025f, stiff = 0.5f;
@@ -759,15 +783,15 @@ __device__ __forceinline__ void delta_correct_slot(const DeviceState& d, const S
     }
 }
 
-template <int BS, bool kFinal, bool kApplyDelta, bool kFast>
+template <int BS, bool kFinal, bool kApplyDelta, bool kFast, bool kBnd>
 __global__ __launch_bounds__(BS) void k_delta_correct(DeviceState d, SphParams P, StepParams S,
                                                           const float4* __restrict__ XsSet,
                                                           float4* __restrict__ XsOut,
                                                           const float4* __restrict__ XsAlt,
                                                           int gateMode, uint32_t nvb) {
     for (uint32_t vb = blockIdx.x; vb < nvb; vb += gridDim.x)
-        delta_correct_slot<BS, kFinal, kApplyDelta, kFast>(d, P, S, XsSet, XsOut, XsAlt, gateMode,
-                                                           xcd_slot_of<BS>(vb, nvb));
+        delta_correct_slot<BS, kFinal, kApplyDelta, kFast, kBnd>(d, P, S, XsSet, XsOut, XsAlt, gateMode,
+                                                                 xcd_slot_of<BS>(vb, nvb));
 }
 
 // kernel_compute_vorticities (IntegrationCUDA.cu:104-128).  W.w keeps |omega| for the next pass.
@@ -974,10 +998,17 @@ template <int BS>
 void density_lambda_bs(hipStream_t st, const DeviceState& d, const SphParams& P, const float4* Xs,
                        int residSlot, bool gated) {
     const unsigned nvb = grid_xcd(d.nslots, BS);
-    if (P.fast)
-        k_density_lambda<BS, true><<<nvb, BS, 0, st>>>(d, P, Xs, residSlot, gated ? 1 : 0, nvb);
-    else
-        k_density_lambda<BS, false><<<nvb, BS, 0, st>>>(d, P, Xs, residSlot, gated ? 1 : 0, nvb);
+    const int g = gated ? 1 : 0;
+    if (P.bd.on) {
+        if (P.fast)
+            k_density_lambda<BS, true, true><<<nvb, BS, 0, st>>>(d, P, Xs, residSlot, g, nvb);
+        else
+            k_density_lambda<BS, false, true><<<nvb, BS, 0, st>>>(d, P, Xs, residSlot, g, nvb);
+    } else if (P.fast) {
+        k_density_lambda<BS, true, false><<<nvb, BS, 0, st>>>(d, P, Xs, residSlot, g, nvb);
+    } else {
+        k_density_lambda<BS, false, false><<<nvb, BS, 0, st>>>(d, P, Xs, residSlot, g, nvb);
+    }
 }
 void launch_density_lambda(hipStream_t st, const DeviceState& d, const SphParams& P,
                            const float4* Xs, int bs, int residSlot, bool gated) {
@@ -1015,18 +1046,22 @@ void delta_correct_bs(hipStream_t st, const DeviceState& d, const SphParams& P,
                       bool applyDelta, int gate, const float4* XsAlt) {
     const unsigned nvb = grid_xcd(d.nslots, BS);
     const unsigned g = nvb;
-    if (final && applyDelta) {
-        if (P.fast)
-            k_delta_correct<BS, true, true, true><<<g, BS, 0, st>>>(d, P, S, XsIn, XsOut, XsAlt, gate, nvb);
-        else
-            k_delta_correct<BS, true, true, false><<<g, BS, 0, st>>>(d, P, S, XsIn, XsOut, XsAlt, gate, nvb);
+#define AKUA_DC(F, A, K, B) \
+    k_delta_correct<BS, F, A, K, B><<<g, BS, 0, st>>>(d, P, S, XsIn, XsOut, XsAlt, gate, nvb)
+    if (final && !applyDelta) {
+        AKUA_DC(true, false, false, false);  // a step without iterations
+    } else if (P.bd.on) {
+        if (final) {
+            if (P.fast) AKUA_DC(true, true, true, true); else AKUA_DC(true, true, false, true);
+        } else {
+            if (P.fast) AKUA_DC(false, true, true, true); else AKUA_DC(false, true, false, true);
+        }
     } else if (final) {
-        k_delta_correct<BS, true, false, false><<<g, BS, 0, st>>>(d, P, S, XsIn, XsOut, XsAlt, gate, nvb);
-    } else if (P.fast) {
-        k_delta_correct<BS, false, true, true><<<g, BS, 0, st>>>(d, P, S, XsIn, XsOut, XsAlt, gate, nvb);
+        if (P.fast) AKUA_DC(true, true, true, false); else AKUA_DC(true, true, false, false);
     } else {
-        k_delta_correct<BS, false, true, false><<<g, BS, 0, st>>>(d, P, S, XsIn, XsOut, XsAlt, gate, nvb);
+        if (P.fast) AKUA_DC(false, true, true, false); else AKUA_DC(false, true, false, false);
     }
+#undef AKUA_DC
 }
 void launch_delta_correct(hipStream_t st, const DeviceState& d, const SphParams& P,
                           const StepParams& S, const float4* XsIn, float4* XsOut, bool final,

@@ -24,6 +24,20 @@ struct SceneDev {
     float bmax[4];
 };
 
+// Boundary samples on the six box walls (opt-in; no reference counterpart).  Their layout,
+// volume weights and the order in which a particle visits them are defined in boundary.h.
+struct BoundaryDev {
+    int32_t on;          // 0: no samples (the reference's clamp and damping only)
+    int32_t layers;      // L, 1..4
+    float s;             // layer spacing
+    float lo[3], hi[3];  // the box the samples were built for
+    float d[3];          // tangential spacing per axis
+    int32_t t0[6][2];    // first tangential index per wall and tangential axis (b < c)
+    int32_t tn[6][2];    // tangential index counts
+    uint32_t base[7];    // first sample index of each wall; base[6] = number of samples
+    const float* psi;    // volume weight of each sample (device memory)
+};
+
 // SPH and solver constants.  Every value is computed on the host with the same float expressions
 // the reference evaluates inline (SmoothingKernelsCUDA.h:16-28, ConstraintSolverCUDA.cu:201,
 // :123-124), so device code never re-derives a constant.
@@ -50,6 +64,7 @@ struct SphParams {
     // the kernel constants and delta_q, lie inside the exact-division range): the neighbour
     // sweeps then skip the multiplications by a neighbour's mass (identities) and those tests.
     int32_t fast;
+    BoundaryDev bd;    // boundary samples (bd.on == 0: none)
 };
 
 // Per-step scalars passed by value to the kernels.

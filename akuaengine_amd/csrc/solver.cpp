@@ -19,6 +19,7 @@
 
 #include "../../include/akua.h"
 #include "aos.h"
+#include "boundary.h"
 #include "last_error.h"
 #include "mlp.h"
 #include "pbf_kernels.h"
@@ -196,6 +197,13 @@ struct akua_solver {
     int learnedIters = 1;          // Jacobi iterations after the warm start
     float gateRatio = 1.0f, gateTol = 0.01f;
     bool haveNet = false;
+    // boundary samples on the box walls (akua_solver_set_boundary)
+    int bndLayers = 0;             // 0: off
+    float bndSpacing = 0.0f;       // <= 0: the configuration's particle spacing
+    bool bndDirty = true;          // rebuild before the next step (settings or box changed)
+    bool boxSeen = false;          // a step has passed a box
+    DevBuf<float> bndPsi;
+    DevBuf<float4> Bg;
     DevBuf<float> net, feat, targ, lamHat;
     DevBuf<float4> XsP, XsW;
     DevBuf<unsigned int> resid;
@@ -236,7 +244,7 @@ struct akua_solver {
         d.chunkScene = dChunkScene.p;
         d.X = X.p; d.V = V.p; d.dens = dens.p; d.lam = lam.p; d.id = id.p;
         d.lid = lid.p; d.phys = phys.p;
-        d.X0 = X0.p; d.XsA = XsA.p; d.XsB = XsB.p; d.W = W.p; d.Vc = Vc.p;
+        d.X0 = X0.p; d.XsA = XsA.p; d.XsB = XsB.p; d.W = W.p; d.Vc = Vc.p; d.Bg = Bg.p;
         d.Tdens = Tdens.p; d.Tlam = Tlam.p; d.Tid = Tid.p;
         d.lidT = lidT.p; d.physT = physT.p;
         d.XsL = XsL.p; d.srcL = srcL.p; d.keysLay = keysLay.p;
@@ -637,6 +645,7 @@ void update_layout(akua_solver* s) {
 
 void set_box(akua_solver* s, size_t scene, const float* bmin, const float* bmax) {
     SceneDev& sc = s->scenes[scene];
+    s->boxSeen = true;
     bool changed = false;
     for (int k = 0; k < 3; ++k) {
         if (sc.bmin[k] != bmin[k] || sc.bmax[k] != bmax[k]) changed = true;
@@ -645,14 +654,44 @@ void set_box(akua_solver* s, size_t scene, const float* bmin, const float* bmax)
     }
     if (changed) {
         s->scenesDirty = true;
+        s->bndDirty = true;
         update_layout(s);
     }
+}
+
+// Boundary samples for the current box (single-scene solvers): rebuilt when the settings or the
+// box changed; the step graphs capture SphParams::bd, so a rebuild re-captures them.
+void ensure_boundary(akua_solver* s) {
+    if (s->bndLayers <= 0) {
+        if (s->sph.bd.on) {
+            s->sph.bd = BoundaryDev{};
+            s->graphDirty = true;
+        }
+        return;
+    }
+    if (!s->bndDirty && s->sph.bd.on) return;
+    const SceneDev& sc = s->scenes[0];
+    BoundaryDev B{};
+    std::vector<float> psi;
+    std::string err;
+    const float sp = s->bndSpacing > 0.0f ? s->bndSpacing : s->cfg.particle_spacing;
+    if (!build_boundary(sc.bmin, sc.bmax, sp, s->bndLayers, s->sph, s->cfg.restDensity, B, psi, err))
+        fail(AKUA_ERR_INVALID_ARG, err);
+    HIP_CHECK(hipStreamSynchronize(s->stream));  // earlier steps may still read the old weights
+    if (s->bndPsi.n < psi.size()) s->bndPsi.alloc(psi.size());
+    HIP_CHECK(hipMemcpy(s->bndPsi.p, psi.data(), psi.size() * sizeof(float), hipMemcpyHostToDevice));
+    if (s->Bg.n < s->nslots) s->Bg.alloc(s->nslots);
+    B.psi = s->bndPsi.p;
+    s->sph.bd = B;
+    s->bndDirty = false;
+    s->graphDirty = true;
 }
 
 void run_steps(akua_solver* s, float dt, int nsteps) {
     if (!s->hasState) fail(AKUA_ERR_STATE, "set the particles before stepping");
     if (!(dt > 0.0f) || !std::isfinite(dt)) fail(AKUA_ERR_INVALID_ARG, "deltaTime must be > 0");
     upload_scenes(s);
+    ensure_boundary(s);
     StepParams S{dt, {s->cfg.gravity[0], s->cfg.gravity[1], s->cfg.gravity[2]}};
     if (s->graphDirty) {
         for (auto& g : s->graphs) g.dirty = true;
@@ -1558,6 +1597,8 @@ akua_status akua_solver_set_learned(akua_solver* s, int32_t enable, int32_t iter
         if (!s) fail(AKUA_ERR_INVALID_ARG, "NULL solver");
         if (enable) {
             if (!s->haveNet) fail(AKUA_ERR_STATE, "set or train a network first");
+            if (s->bndLayers > 0)
+                fail(AKUA_ERR_UNSUPPORTED, "the learned step does not support boundary samples");
             if (s->cfg.solverIterations < 1)
                 fail(AKUA_ERR_UNSUPPORTED, "the learned step needs solverIterations >= 1");
             if (!std::isfinite(ratio) || !std::isfinite(tolerance) || ratio < 0.0f)
@@ -1573,6 +1614,43 @@ akua_status akua_solver_set_learned(akua_solver* s, int32_t enable, int32_t iter
         s->gateRatio = ratio;
         s->gateTol = tolerance;
         s->graphDirty = true;
+    });
+}
+
+akua_status akua_solver_set_boundary(akua_solver* s, int32_t layers, float spacing) {
+    return guarded([&] {
+        if (!s) fail(AKUA_ERR_INVALID_ARG, "NULL solver");
+        if (layers < 0 || layers > kMaxBoundaryLayers) fail(AKUA_ERR_INVALID_ARG, "layers must be 0..4");
+        if (!std::isfinite(spacing) || spacing < 0.0f) fail(AKUA_ERR_INVALID_ARG, "spacing must be >= 0");
+        if (layers > 0 && s->scenes.size() != 1)
+            fail(AKUA_ERR_UNSUPPORTED, "boundary samples are per single-scene solver");
+        if (layers > 0 && s->mode == 1)
+            fail(AKUA_ERR_UNSUPPORTED, "the learned step does not support boundary samples");
+        s->bndLayers = layers;
+        s->bndSpacing = spacing;
+        s->bndDirty = true;
+        if (layers == 0) s->sph.bd = BoundaryDev{};
+        s->graphDirty = true;
+    });
+}
+
+akua_status akua_debug_boundary(akua_solver* s, float* pos, float* psi, int32_t capacity, int32_t* count) {
+    return guarded([&] {
+        if (!s) fail(AKUA_ERR_INVALID_ARG, "NULL solver");
+        if (capacity < 0) fail(AKUA_ERR_INVALID_ARG, "capacity must be >= 0");
+        if (s->bndLayers <= 0) fail(AKUA_ERR_STATE, "boundary samples are off");
+        if (!s->boxSeen) fail(AKUA_ERR_STATE, "no box yet: the samples follow the box of a step");
+        ensure_boundary(s);
+        const BoundaryDev& B = s->sph.bd;
+        const uint32_t m = B.base[6];
+        if (count) *count = (int32_t)m;
+        const uint32_t k = std::min<uint32_t>(m, (uint32_t)capacity);
+        if (pos)
+            for (uint32_t i = 0; i < k; ++i) boundary_position(B, i, pos + 3 * (size_t)i);
+        if (psi && k) {
+            HIP_CHECK(hipStreamSynchronize(s->stream));
+            HIP_CHECK(hipMemcpy(psi, s->bndPsi.p, (size_t)k * sizeof(float), hipMemcpyDeviceToHost));
+        }
     });
 }
 
@@ -1606,6 +1684,7 @@ akua_status akua_solver_train(akua_solver* s, float dt, const float boxMin[3], c
                               float* residuals) {
     return guarded([&] {
         if (!s || !config || !boxMin || !boxMax) fail(AKUA_ERR_INVALID_ARG, "NULL argument");
+        if (s->bndLayers > 0) fail(AKUA_ERR_UNSUPPORTED, "training does not support boundary samples");
         train(s, dt, boxMin, boxMax, steps, *config, loss, residuals);
     });
 }

@@ -139,6 +139,21 @@ class PBFSolver:
                                                 int(trialSteps), path, ctypes.byref(t)))
         return t.as_dict()
 
+    # -- boundary samples on the walls --------------------------------------------------------
+    def set_boundary(self, layers: int, spacing: float = 0.0) -> None:
+        """Wall samples with Akinci volume weights (layers 0: off; spacing 0: particle_spacing).
+        No reference counterpart: the reference plans them (ConstraintSolverCUDA.cu:132-135)."""
+        _lib.check(self._L.akua_solver_set_boundary(self._h, int(layers), float(spacing)))
+
+    def debug_boundary(self):
+        """(positions (m, 3), psi (m,)) of the samples for the last box, in index order."""
+        m = ctypes.c_int32()
+        _lib.check(self._L.akua_debug_boundary(self._h, None, None, 0, ctypes.byref(m)))
+        pos = np.zeros((m.value, 3), np.float32)
+        psi = np.zeros(m.value, np.float32)
+        _lib.check(self._L.akua_debug_boundary(self._h, _p(pos), _p(psi), m.value, ctypes.byref(m)))
+        return pos, psi
+
     # -- learned warm start -------------------------------------------------------------------
     def set_network(self, params: np.ndarray) -> None:
         p = np.ascontiguousarray(params, np.float32).reshape(_lib.MLP_PARAMS)

@@ -226,7 +226,36 @@ def extra_workloads(steps: int = 20, warmup: int = 3):
                           "ms_per_step": round(el / steps * 1e3, 4), "value": round(total * steps / el, 1),
                           "unit": "particle-substeps/s", "tuning": b.get_tuning()}
     log(f"[bench] batched 256 scenes ({total} particles): {el / steps * 1e3:.3f} ms/step")
+    del b
+    out["boundary_1m"] = boundary_cost(steps, warmup)
     return out
+
+
+def boundary_cost(steps: int, warmup: int):
+    """The 1M dam break with and without two layers of wall samples (same tuning): the cost of
+    the boundary terms in the density/lambda and position-delta passes."""
+    from akuaengine_amd import scenes
+    from akuaengine_amd.config import DAM_BREAK_DT, dam_break_params
+    from akuaengine_amd.solver import PBFSolver
+    cfg, corr = dam_break_params(4)
+    dt = DAM_BREAK_DT
+    sc = scenes.dam_break_1m()
+    res = {"particles": sc.n, "steps": steps, "layers": 2}
+    for layers in (0, 2):
+        s = PBFSolver(sc.n, cfg, corr)
+        s.set_state(sc.pos, sc.vel, sc.mass)
+        s.set_boundary(layers)
+        s.step_n(dt, sc.box_min, sc.box_max, warmup)
+        el = time_steps(s, dt, sc.box_min, sc.box_max, steps)
+        key = "boundary" if layers else "plain"
+        res[f"ms_per_step_{key}"] = round(el / steps * 1e3, 4)
+        if layers:
+            res["samples"] = int(len(s.debug_boundary()[1]))
+        del s
+    res["overhead"] = round(res["ms_per_step_boundary"] / res["ms_per_step_plain"] - 1.0, 4)
+    log(f"[bench] 1M with wall samples: {res['ms_per_step_boundary']:.3f} ms/step "
+        f"(plain {res['ms_per_step_plain']:.3f})")
+    return res
 
 
 def main() -> int:

@@ -163,6 +163,23 @@ akua_status akua_solver_autotune(akua_solver* s, float deltaTime, const float bo
                                  const float boxMax[3], int32_t trialSteps, const char* cachePath,
                                  akua_tuning* best);
 
+/* ---- boundary samples on the walls ------------------------------------------------------------- */
+/* Static samples on the six walls of the box passed to each step (Akinci et al. 2012, in the PBF
+ * form of Bender's PositionBasedDynamics): each sample b carries psi_b = rho0 / sum_k W(x_b - x_k)
+ * over the samples k; it adds psi_b W to a particle's density, psi_b grad W / rho0 to grad_i C_i,
+ * and lambda_i psi_b grad W / rho0 to the position delta.  The soft clamp and the velocity damping
+ * stay.  layers: 0 (off, the default and the reference's behaviour) or 1..4 planes per wall, spaced
+ * `spacing` apart outward (0: particle_spacing).  No reference counterpart: the reference plans
+ * "virtual particles" for this (ConstraintSolverCUDA.cu:132-135).  The samples are rebuilt when the
+ * box changes (a host computation and a graph re-capture).  Single-scene solvers; not with the
+ * learned step or training.  akua_solver_density_error counts fluid particles only. */
+akua_status akua_solver_set_boundary(akua_solver* s, int32_t layers, float spacing);
+/* The samples for the current box (valid once a step has passed a box): positions (3 floats per
+ * sample) and psi, up to `capacity` samples in index order; *count receives the total.  Any
+ * pointer may be NULL. */
+akua_status akua_debug_boundary(akua_solver* s, float* pos, float* psi, int32_t capacity,
+                                int32_t* count);
+
 /* ---- learned warm start ------------------------------------------------------------------------ */
 /* A per-particle network (16 features -> 64 -> 64 -> 4, ReLU, BF16 weights and activations on the
  * matrix cores, FP32 accumulation) predicts the position correction of the plain solve from

@@ -98,3 +98,18 @@ def test_runner_resume_continues_bit_for_bit(tmp_path):
     for k in ("pos", "vel", "mass", "density", "lam", "ids"):
         np.testing.assert_array_equal(b[1][k], a[2][k], err_msg=k)
     np.testing.assert_array_equal(b.by_id(1, "pos"), a.by_id(2, "pos"))
+
+
+def test_runner_boundary_matches_oracle(oracle_mod, tmp_path):
+    path = str(tmp_path / "bnd.akt")
+    out = _run("--scene", "app_dam_break", "--steps", "8", "--every", "8", "--boundary", "2",
+               "--out", path, "--fields", "all")
+    assert out["boundary_layers"] == 2 and out["steps"] == 8
+    sc = scenes.app_dam_break()
+    o = oracle_mod.Oracle(sc.n, PBFConfig(), LambdaCorrParams(), 32)
+    o.set_state(sc.pos, sc.vel, sc.mass)
+    o.set_boundary(2)
+    o.step(0.016, sc.box_min, sc.box_max, 8)
+    f = T.read_trajectory(path)
+    assert len(f) == 2 and f[1]["step"] == 8
+    _assert_frame_equals_oracle(f[1], o)
