@@ -474,10 +474,23 @@ __global__ void k_mlp_adam(const float* __restrict__ part, float* __restrict__ P
 
 }  // namespace
 
+// Workgroups of an inference launch: the waves loop over the 16-sample tiles, and each workgroup
+// first stages the weights in LDS, so the grid is one resident set and no more: the kernels'
+// 128 VGPRs leave room for 4 waves per SIMD, i.e. 4 workgroups of 4 waves per CU (1,024 on 256
+// CUs measured 59 us per 1M-sample launch against 69 us with 2,048).
+static uint32_t infer_blocks(uint32_t tiles) {
+    int dev = 0, cus = 256;
+    if (hipGetDevice(&dev) != hipSuccess ||
+        hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus < 1)
+        cus = 256;
+    const uint32_t resident = 4u * (uint32_t)cus;
+    return std::max<uint32_t>(1u, std::min<uint32_t>((tiles + kInferWaves - 1) / kInferWaves, resident));
+}
+
 void launch_mlp_eval(hipStream_t st, const float* params, const float* feat, uint32_t n, float* y) {
     if (!n) return;
     const uint32_t tiles = (n + 15) / 16;
-    const uint32_t blocks = std::min<uint32_t>((tiles + kInferWaves - 1) / kInferWaves, 2048u);
+    const uint32_t blocks = infer_blocks(tiles);
     k_mlp_eval<<<blocks, kInferBlock, 0, st>>>(params, feat, n, y);
 }
 
@@ -485,7 +498,7 @@ void launch_mlp_apply(hipStream_t st, const DeviceState& d, const float* params,
                       const float4* XsPred, float4* XsWarm, float* lamHat) {
     if (!d.nslots) return;
     const uint32_t tiles = (d.nslots + 15) / 16;
-    const uint32_t blocks = std::min<uint32_t>((tiles + kInferWaves - 1) / kInferWaves, 2048u);
+    const uint32_t blocks = infer_blocks(tiles);
     k_mlp_apply<<<blocks, kInferBlock, 0, st>>>(d, params, h, XsPred, XsWarm, lamHat);
 }
 
