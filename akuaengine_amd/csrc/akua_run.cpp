@@ -249,6 +249,7 @@ int main(int argc, char** argv) {
         akua_traj_info info;
         check(akua_traj_open(path.c_str(), &info, &t), "opening the --init trajectory");
         if (!(info.fields & AKUA_FIELD_POSITION)) usage_error(path + " holds no positions");
+        if (info.frames < 1) usage_error(path + " holds no whole frame");
         n = info.numParticles;
         pos.resize(3 * (size_t)n);
         vel.assign(3 * (size_t)n, 0.0f);

@@ -306,6 +306,11 @@ def test_runner_dry_run_resume_keeps_ids(tmp_path):
     for name in ("pos", "vel", "mass", "density", "lam", "ids"):
         np.testing.assert_array_equal(fr[name], frames[1][name])
     assert fr["step"] == frames[1]["step"] and fr["time"] == frames[1]["time"]
+    # a file without a whole frame is refused before anything is allocated
+    empty = str(tmp_path / "empty.akt")
+    open(empty, "wb").write(open(src, "rb").read()[:128])
+    r = _run("--init", empty, "--steps", "1", "--dry-run", check=False)
+    assert r.returncode == 2 and "no whole frame" in r.stderr
     # ids that are not a permutation are refused
     bad = [dict(fr, ids=np.zeros(64, np.uint32)) for fr in frames[:1]]
     _write(src, bad)
