@@ -36,6 +36,7 @@ EXPORTED = [
     "akua_solver_get_network", "akua_solver_set_learned", "akua_solver_learned_stats",
     "akua_solver_train", "akua_debug_features", "akua_debug_warm_start", "akua_mlp_forward",
     "akua_mlp_gradient", "akua_mlp_adam_step", "akua_solver_set_boundary", "akua_debug_boundary",
+    "akua_boundary_samples",
 ]
 
 # Every symbol include/akua_io.h declares (scenes and trajectory files; no GPU needed except by
@@ -170,6 +171,7 @@ def load() -> ctypes.CDLL:
         "akua_mlp_adam_step": [P, P, P, ctypes.c_int64, f32, P, P, i32, P, P],
         "akua_solver_set_boundary": [P, i32, f32],
         "akua_debug_boundary": [P, P, P, i32, ctypes.POINTER(i32)],
+        "akua_boundary_samples": [ctypes.POINTER(CPBFConfig), P, P, i32, f32, P, P, i32, ctypes.POINTER(i32)],
         "akua_scene_describe": [ctypes.c_char_p, f32, ctypes.c_int64, ctypes.POINTER(AkuaSceneInfo)],
         "akua_scene_generate": [ctypes.c_char_p, f32, ctypes.c_int64, P, i32],
         "akua_traj_create": [ctypes.c_char_p, i32, ctypes.c_uint32, f32, P, P, P, ctypes.POINTER(P)],

@@ -1654,6 +1654,32 @@ akua_status akua_debug_boundary(akua_solver* s, float* pos, float* psi, int32_t 
     });
 }
 
+akua_status akua_boundary_samples(const akua_pbf_config* cfg, const float boxMin[3],
+                                  const float boxMax[3], int32_t layers, float spacing, float* pos,
+                                  float* psi, int32_t capacity, int32_t* count) {
+    return guarded([&] {
+        if (!cfg || !boxMin || !boxMax) fail(AKUA_ERR_INVALID_ARG, "NULL argument");
+        if (layers < 1 || layers > kMaxBoundaryLayers) fail(AKUA_ERR_INVALID_ARG, "layers must be 1..4");
+        if (capacity < 0 || !std::isfinite(spacing) || spacing < 0.0f)
+            fail(AKUA_ERR_INVALID_ARG, "capacity and spacing must be >= 0");
+        akua_lambda_corr corr{1, 0.0001f, 4.0f, 0.03f};
+        const SphParams P = make_sph(*cfg, corr);
+        BoundaryDev B{};
+        std::vector<float> w;
+        std::string err;
+        const float sp = spacing > 0.0f ? spacing : cfg->particle_spacing;
+        if (!build_boundary(boxMin, boxMax, sp, layers, P, cfg->restDensity, B, w, err))
+            fail(AKUA_ERR_INVALID_ARG, err);
+        const uint32_t m = B.base[6];
+        if (count) *count = (int32_t)m;
+        const uint32_t k = std::min<uint32_t>(m, (uint32_t)capacity);
+        for (uint32_t i = 0; i < k; ++i) {
+            if (pos) boundary_position(B, i, pos + 3 * (size_t)i);
+            if (psi) psi[i] = w[i];
+        }
+    });
+}
+
 akua_status akua_solver_learned_stats(akua_solver* s, int64_t* learnedSteps, int64_t* fallbacks,
                                       float* residual) {
     return guarded([&] {

@@ -280,6 +280,22 @@ class PBFSolver:
         return float(v.value)
 
 
+def boundary_samples(config: PBFConfig, box_min, box_max, layers: int, spacing: float = 0.0):
+    """(positions (m, 3), psi (m,)) of the wall samples a solver with this configuration builds for
+    the box: host only, no GPU (akua_boundary_samples)."""
+    L = _lib.load()
+    c = to_c_config(config)
+    lo, hi = _f3(box_min), _f3(box_max)
+    m = ctypes.c_int32()
+    _lib.check(L.akua_boundary_samples(ctypes.byref(c), _p(lo), _p(hi), int(layers), float(spacing),
+                                       None, None, 0, ctypes.byref(m)))
+    pos = np.zeros((m.value, 3), np.float32)
+    psi = np.zeros(m.value, np.float32)
+    _lib.check(L.akua_boundary_samples(ctypes.byref(c), _p(lo), _p(hi), int(layers), float(spacing),
+                                       _p(pos), _p(psi), m.value, ctypes.byref(m)))
+    return pos, psi
+
+
 class BatchSolver(PBFSolver):
     """Many independent scenes stepped together: one kernel launch per stage covers every scene
     and each step is one hipGraph.  Particles are passed scene after scene (concatenated)."""

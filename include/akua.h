@@ -179,6 +179,11 @@ akua_status akua_solver_set_boundary(akua_solver* s, int32_t layers, float spaci
  * pointer may be NULL. */
 akua_status akua_debug_boundary(akua_solver* s, float* pos, float* psi, int32_t capacity,
                                 int32_t* count);
+/* The same samples without a solver (host only, no GPU): those a solver with this configuration
+ * would build for this box.  spacing 0: cfg->particle_spacing. */
+akua_status akua_boundary_samples(const akua_pbf_config* cfg, const float boxMin[3],
+                                  const float boxMax[3], int32_t layers, float spacing, float* pos,
+                                  float* psi, int32_t capacity, int32_t* count);
 
 /* ---- learned warm start ------------------------------------------------------------------------ */
 /* A per-particle network (16 features -> 64 -> 64 -> 4, ReLU, BF16 weights and activations on the

@@ -202,3 +202,41 @@ def test_resting_block_bottom_density(O):
     for layers in (1, 2, 3):
         assert (rho[layers][bottom] > rho[0][bottom]).all()
     assert rho[1][bottom].mean() > rho[2][bottom].mean() >= rho[3][bottom].mean()
+
+
+def test_thin_box_walls_without_interior_rows(O):
+    # an axis shorter than the spacing has n = 2 points, so walls whose tangential range on that
+    # axis is interior-only hold no samples; sums still equal brute force
+    box = ((0.0, 0.0, 0.0), (0.6, 0.02, 0.3))
+    rng = np.random.default_rng(9)
+    pts = np.stack([rng.random(30) * 0.6, rng.random(30) * 0.02, rng.random(30) * 0.3], axis=1)
+    pts = np.concatenate([pts, [[0.01, 0.01, 0.01], [0.59, 0.019, 0.29]]]).astype(np.float32)
+    o = _oracle(O, pts, 2)
+    pos, psi = o.boundary_samples(*box)
+    # z-walls own no edge with y (interior-only on y, n_y = 2): they have no samples at all
+    y = pos[:, 1].astype(np.float32)
+    assert not np.any((np.abs(pos[:, 2]) < 1e-7) & (y > np.float32(0.0)) & (y < np.float32(0.02)))
+    rb, gb = o.boundary_sums(*box)
+    st = o.state()
+    for i in range(len(pts)):
+        want_r, want_g = _brute_sums(O, st["pos"][i], pos, psi)
+        assert np.float32(rb[i]) == want_r, i
+        np.testing.assert_array_equal(gb[i].astype(np.float32), want_g)
+
+
+@pytest.mark.parametrize("box,layers,spacing", [
+    (((0.0, 0.0, 0.0), (0.6, 0.45, 0.3)), 2, 0.0),
+    (((0.0, 0.0, 0.0), (2.0, 1.5, 1.0)), 1, 0.0),
+    (((-0.3, -0.3, -0.3), (1.0, 0.6, 0.2)), 3, 0.0),
+    (((0.0, 0.0, 0.0), (0.6, 0.02, 0.3)), 2, 0.0),
+    (((1.5, 0.0, 1.5), (4.5, 4.0, 4.5)), 2, 0.07),
+])
+def test_library_samples_equal_the_oracle(O, box, layers, spacing):
+    # libakua.so's host construction (the one the solver uploads) against the oracle's, bit for bit
+    from akuaengine_amd.solver import boundary_samples
+    cfg = PBFConfig()
+    pos, psi = boundary_samples(cfg, *box, layers, spacing)
+    o = _oracle(O, [[0.0, 0.0, 0.0]], layers, spacing, cfg=cfg)
+    opos, opsi = o.boundary_samples(*box)
+    np.testing.assert_array_equal(pos, opos.astype(np.float32))
+    np.testing.assert_array_equal(psi, opsi.astype(np.float32))
