@@ -44,6 +44,7 @@ EXPORTED = [
 EXPORTED_IO = [
     "akua_scene_describe", "akua_scene_generate", "akua_traj_create", "akua_traj_append",
     "akua_traj_append_solver", "akua_traj_open", "akua_traj_read", "akua_traj_close",
+    "akua_default_camera", "akua_camera_matrix", "akua_render",
 ]
 
 MLP_PARAMS = 5544
@@ -97,6 +98,13 @@ class AkuaSceneInfo(ctypes.Structure):
     _fields_ = [("numParticles", ctypes.c_int32), ("boxMin", ctypes.c_float * 3),
                 ("boxMax", ctypes.c_float * 3), ("deltaTime", ctypes.c_float),
                 ("referenceParams", ctypes.c_int32)]
+
+
+class AkuaCamera(ctypes.Structure):
+    """``akua_camera`` (include/akua_io.h): the reference's CameraConfig and clip planes."""
+
+    _fields_ = [("position", ctypes.c_float * 3), ("yaw", ctypes.c_float), ("pitch", ctypes.c_float),
+                ("fovY", ctypes.c_float), ("nearZ", ctypes.c_float), ("farZ", ctypes.c_float)]
 
 
 class AkuaTrajInfo(ctypes.Structure):
@@ -181,7 +189,11 @@ def load() -> ctypes.CDLL:
         "akua_traj_read": [P, ctypes.c_int64, ctypes.POINTER(ctypes.c_int64), ctypes.POINTER(ctypes.c_double),
                            P, P, P, P, P, P],
         "akua_traj_close": [P],
+        "akua_camera_matrix": [ctypes.POINTER(AkuaCamera), i32, i32, P],
+        "akua_render": [P, ctypes.POINTER(AkuaCamera), i32, i32, f32, i32, P],
     }
+    L.akua_default_camera.argtypes = [ctypes.POINTER(AkuaCamera)]
+    L.akua_default_camera.restype = None
     for name, args in sigs.items():
         fn = getattr(L, name)
         fn.argtypes = args

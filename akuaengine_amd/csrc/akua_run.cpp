@@ -46,6 +46,11 @@ const char* kUsage =
     "  --out FILE              write a trajectory: the initial state, then every --every steps\n"
     "  --every K               (default 1)\n"
     "  --fields LIST           pos,vel,mass,density,lambda,id or all (default pos,vel,density,id)\n"
+    "  --render-every K        draw the particles every K steps (and the initial state) as the\n"
+    "                          reference's window would show them, as PPM images ...\n"
+    "  --render-dir DIR        ... named DIR/frame_<step>.ppm (default: .)\n"
+    "  --width W --height H    image size (default 1280 x 720, the reference's window)\n"
+    "  --point-size S          sprite size (default 50, the application's particles)\n"
     "  --boundary L            L layers of wall samples (0 = off, the reference's walls; 1..4)\n"
     "  --boundary-spacing S    their spacing (default: particle_spacing)\n"
     "  --autotune              pick launch and layout settings before the run (results unchanged)\n"
@@ -73,6 +78,10 @@ struct Options {
     int64_t every = 1;
     uint32_t fields = AKUA_FIELD_POSITION | AKUA_FIELD_VELOCITY | AKUA_FIELD_DENSITY | AKUA_FIELD_ID;
     bool autotune = false;
+    int64_t renderEvery = 0;
+    std::string renderDir = ".";
+    int width = 1280, height = 720;
+    float pointSize = 50.0f;
     int boundary = 0;
     float boundarySpacing = 0.0f;
     std::string network;
@@ -158,6 +167,11 @@ Options parse(int argc, char** argv) {
         else if (a == "--every") o.every = parse_int(a, val(), 1, INT32_MAX);
         else if (a == "--fields") o.fields = parse_fields(val());
         else if (a == "--autotune") o.autotune = true;
+        else if (a == "--render-every") o.renderEvery = parse_int(a, val(), 1, INT32_MAX);
+        else if (a == "--render-dir") o.renderDir = val();
+        else if (a == "--width") o.width = (int)parse_int(a, val(), 1, 16384);
+        else if (a == "--height") o.height = (int)parse_int(a, val(), 1, 16384);
+        else if (a == "--point-size") o.pointSize = (float)parse_num(a, val());
         else if (a == "--boundary") o.boundary = (int)parse_int(a, val(), 0, 4);
         else if (a == "--boundary-spacing") o.boundarySpacing = (float)parse_num(a, val());
         else if (a == "--network") o.network = val();
@@ -208,6 +222,31 @@ std::vector<float> load_npy_f32(const std::string& path) {
     f.read(reinterpret_cast<char*>(v.data()), (std::streamsize)(count * 4));
     if (!f) usage_error(path + ": truncated data");
     return v;
+}
+
+// Renders the solver's particles with the reference's camera and writes a binary PPM.
+double render_frame(akua_solver* s, const Options& o, int64_t step, std::vector<uint8_t>& img) {
+    const auto t0 = std::chrono::steady_clock::now();
+    akua_camera cam;
+    akua_default_camera(&cam);
+    img.resize((size_t)o.width * (size_t)o.height * 3);
+    const akua_status st = akua_render(s, &cam, o.width, o.height, o.pointSize, 1, img.data());
+    if (st != AKUA_OK) fatal("rendering", st);
+    char name[64];
+    std::snprintf(name, sizeof(name), "/frame_%08lld.ppm", (long long)step);
+    const std::string path = o.renderDir + name;
+    FILE* f = std::fopen(path.c_str(), "wb");
+    if (!f) {
+        std::fprintf(stderr, "akua_run: cannot write %s\n", path.c_str());
+        std::exit(1);
+    }
+    std::fprintf(f, "P6\n%d %d\n255\n", o.width, o.height);
+    const bool ok = std::fwrite(img.data(), 1, img.size(), f) == img.size();
+    if (std::fclose(f) != 0 || !ok) {
+        std::fprintf(stderr, "akua_run: writing %s failed\n", path.c_str());
+        std::exit(1);
+    }
+    return std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
 }
 
 std::string json_str(const std::string& s) {
@@ -379,7 +418,13 @@ int main(int argc, char** argv) {
     check(akua_solver_get_tuning(s, &tuning), "get_tuning");
 
     using clock = std::chrono::steady_clock;
-    double ioSeconds = 0.0;
+    double ioSeconds = 0.0, renderSeconds = 0.0;
+    int64_t images = 0;
+    std::vector<uint8_t> img;
+    if (o.renderEvery > 0) {
+        renderSeconds += render_frame(s, o, step0, img);
+        ++images;
+    }
     const auto t0 = clock::now();
     int64_t step = step0;
     double time = time0;
@@ -397,6 +442,13 @@ int main(int argc, char** argv) {
                 check(akua_traj_append_solver(out, s, step, time, idMap), "writing a frame");
                 ioSeconds += std::chrono::duration<double>(clock::now() - a).count();
                 ++written;
+            }
+            if (o.renderEvery > 0 && done % o.renderEvery == 0) {
+                check(akua_solver_synchronize(s), "synchronize");
+                const double r = render_frame(s, o, step, img);
+                ioSeconds += r;
+                renderSeconds += r;
+                ++images;
             }
         }
     }
@@ -425,6 +477,8 @@ int main(int argc, char** argv) {
         std::printf(", \"learned\": {\"iterations\": %d, \"learned_steps\": %lld, \"fallbacks\": %lld}", o.learnedIters,
                     (long long)learnedSteps, (long long)fallbacks);
     if (out) std::printf(", \"out\": %s, \"frames_written\": %lld", json_str(o.out).c_str(), (long long)written);
+    if (o.renderEvery > 0)
+        std::printf(", \"images\": %lld, \"render_seconds\": %.6f", (long long)images, renderSeconds);
     std::printf("}\n");
     return 0;
 }

@@ -19,8 +19,10 @@
 
 #include "../../include/akua.h"
 #include "aos.h"
+#include "../../include/akua_io.h"
 #include "boundary.h"
 #include "last_error.h"
+#include "render.h"
 #include "mlp.h"
 #include "pbf_kernels.h"
 #include "pbf_types.h"
@@ -204,6 +206,9 @@ struct akua_solver {
     bool boxSeen = false;          // a step has passed a box
     DevBuf<float> bndPsi;
     DevBuf<float4> Bg;
+    // headless rendering (akua_render): per-pixel keys and the RGB8 image
+    DevBuf<unsigned long long> renderKeys;
+    DevBuf<unsigned char> renderRgb;
     DevBuf<float> net, feat, targ, lamHat;
     DevBuf<float4> XsP, XsW;
     DevBuf<unsigned int> resid;
@@ -1677,6 +1682,67 @@ akua_status akua_boundary_samples(const akua_pbf_config* cfg, const float boxMin
             if (pos) boundary_position(B, i, pos + 3 * (size_t)i);
             if (psi) psi[i] = w[i];
         }
+    });
+}
+
+void akua_default_camera(akua_camera* c) {
+    if (!c) return;
+    c->position[0] = 3.0f;
+    c->position[1] = 2.0f;
+    c->position[2] = 6.0f;
+    c->yaw = -90.0f;
+    c->pitch = 0.0f;
+    c->fovY = 45.0f;
+    c->nearZ = 0.1f;
+    c->farZ = 160.0f;
+}
+
+namespace {
+void check_camera(const akua_camera* cam, int32_t width, int32_t height) {
+    if (!cam) fail(AKUA_ERR_INVALID_ARG, "NULL camera");
+    if (width < 1 || height < 1 || (int64_t)width * height > (1 << 26))
+        fail(AKUA_ERR_INVALID_ARG, "width and height must be positive, at most 2^26 pixels");
+    const float v[8] = {cam->position[0], cam->position[1], cam->position[2], cam->yaw, cam->pitch,
+                        cam->fovY, cam->nearZ, cam->farZ};
+    for (float x : v)
+        if (!std::isfinite(x)) fail(AKUA_ERR_INVALID_ARG, "non-finite camera parameter");
+    if (!(cam->fovY > 0.0f && cam->fovY < 180.0f) || !(cam->nearZ > 0.0f) || !(cam->farZ > cam->nearZ) ||
+        !(std::fabs(cam->pitch) < 90.0f))
+        fail(AKUA_ERR_INVALID_ARG, "camera needs 0 < fovY < 180, 0 < nearZ < farZ and |pitch| < 90");
+}
+}  // namespace
+
+akua_status akua_camera_matrix(const akua_camera* cam, int32_t width, int32_t height, float vp[16]) {
+    return guarded([&] {
+        check_camera(cam, width, height);
+        if (!vp) fail(AKUA_ERR_INVALID_ARG, "NULL matrix");
+        RenderParams R{};
+        camera_setup(cam->position, cam->yaw, cam->pitch, cam->fovY, cam->nearZ, cam->farZ, width, height, R);
+        for (int k = 0; k < 16; ++k) vp[k] = R.vp[k];
+    });
+}
+
+akua_status akua_render(akua_solver* s, const akua_camera* cam, int32_t width, int32_t height,
+                        float pointSize, int32_t drawFloor, uint8_t* rgb) {
+    return guarded([&] {
+        if (!s || !rgb) fail(AKUA_ERR_INVALID_ARG, "NULL argument");
+        check_camera(cam, width, height);
+        if (!std::isfinite(pointSize)) fail(AKUA_ERR_INVALID_ARG, "non-finite point size");
+        if (s->scenes.size() != 1) fail(AKUA_ERR_UNSUPPORTED, "rendering is per single-scene solver");
+        if (!s->hasState) fail(AKUA_ERR_STATE, "set the particles first");
+        RenderParams R{};
+        camera_setup(cam->position, cam->yaw, cam->pitch, cam->fovY, cam->nearZ, cam->farZ, width, height, R);
+        R.pointSize = pointSize;
+        R.drawFloor = drawFloor ? 1 : 0;
+        const size_t pixels = (size_t)width * (size_t)height;
+        HIP_CHECK(hipStreamSynchronize(s->stream));
+        if (s->renderKeys.n < pixels) s->renderKeys.alloc(pixels);
+        if (s->renderRgb.n < 3 * pixels) s->renderRgb.alloc(3 * pixels);
+        launch_render(s->stream, s->X.p, s->color.p, s->size.p, s->id.p, s->lid.p, s->phys.p,
+                      s->scenes[0].end, R, s->renderKeys.p, s->renderRgb.p);
+        HIP_CHECK(hipGetLastError());
+        HIP_CHECK(hipStreamSynchronize(s->stream));
+        HIP_CHECK(hipMemcpy(rgb, s->renderRgb.p, 3 * pixels, hipMemcpyDeviceToHost));
     });
 }
 

@@ -139,6 +139,17 @@ class PBFSolver:
                                                 int(trialSteps), path, ctypes.byref(t)))
         return t.as_dict()
 
+    # -- headless rendering -------------------------------------------------------------------
+    def render(self, camera=None, width: int = 1280, height: int = 720, point_size: float = 50.0,
+               floor: bool = True) -> np.ndarray:
+        """(height, width, 3) uint8 image of the particles as the reference draws them (point
+        sprites over the checkered floor); camera: an akua_camera (default: the reference's)."""
+        cam = camera if camera is not None else default_camera()
+        img = np.zeros((int(height), int(width), 3), np.uint8)
+        _lib.check(self._L.akua_render(self._h, ctypes.byref(cam), int(width), int(height),
+                                       float(point_size), 1 if floor else 0, _p(img)))
+        return img
+
     # -- boundary samples on the walls --------------------------------------------------------
     def set_boundary(self, layers: int, spacing: float = 0.0) -> None:
         """Wall samples with Akinci volume weights (layers 0: off; spacing 0: particle_spacing).
@@ -278,6 +289,20 @@ class PBFSolver:
         v = ctypes.c_double(0.0)
         _lib.check(self._L.akua_solver_mean_neighbours(self._h, ctypes.byref(v)))
         return float(v.value)
+
+
+def default_camera():
+    """The reference's camera (CameraConfig.h:8-16) and clip planes (Application.cpp:17-18)."""
+    c = _lib.AkuaCamera()
+    _lib.load().akua_default_camera(ctypes.byref(c))
+    return c
+
+
+def camera_matrix(camera, width: int, height: int) -> np.ndarray:
+    """The renderer's row-major view-projection matrix (4, 4) for this image size."""
+    m = np.zeros(16, np.float32)
+    _lib.check(_lib.load().akua_camera_matrix(ctypes.byref(camera), int(width), int(height), _p(m)))
+    return m.reshape(4, 4)
 
 
 def boundary_samples(config: PBFConfig, box_min, box_max, layers: int, spacing: float = 0.0):

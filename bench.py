@@ -188,7 +188,8 @@ def learned_comparison(log_steps: int = 300, sample: int = 25, skip: int = 50, o
 
 
 def extra_workloads(steps: int = 20, warmup: int = 3):
-    """Throughput of the 8M two-column scene and of the 256-scene batch (plain, 4 iterations)."""
+    """Throughput of the 8M two-column scene and of the 256-scene batch (plain, 4 iterations), the
+    cost of the wall samples and of headless rendering."""
     import numpy as np
     from akuaengine_amd import scenes
     from akuaengine_amd.config import DAM_BREAK_DT, dam_break_params
@@ -228,7 +229,36 @@ def extra_workloads(steps: int = 20, warmup: int = 3):
     log(f"[bench] batched 256 scenes ({total} particles): {el / steps * 1e3:.3f} ms/step")
     del b
     out["boundary_1m"] = boundary_cost(steps, warmup)
+    out["render_1280x720"] = render_cost()
     return out
+
+
+def render_cost(images: int = 10):
+    """Wall time of akua_render per 1280x720 image, the copy of the 2.8 MB image to the host
+    included: the application scene from the default camera (27k particles, 50-pixel sprites), and
+    the 1M dam break from (5, 4, 16), where its whole 10 x 6 x 5 box is in view."""
+    from akuaengine_amd import scenes
+    from akuaengine_amd.config import dam_break_params
+    from akuaengine_amd.solver import PBFSolver, default_camera
+    cfg, corr = dam_break_params(4)
+    res = {"width": 1280, "height": 720, "point_size": 50.0, "images": images}
+    for key, sc, eye in (("app_dam_break", scenes.app_dam_break(), None),
+                         ("dam_break_1m", scenes.dam_break_1m(), (5.0, 4.0, 16.0))):
+        s = PBFSolver(sc.n, cfg, corr)
+        s.set_state(sc.pos, sc.vel, sc.mass)
+        cam = default_camera()
+        if eye is not None:
+            cam.position[:] = list(eye)
+        s.render(cam, 1280, 720)
+        t0 = time.perf_counter()
+        for _ in range(images):
+            s.render(cam, 1280, 720)
+        el = time.perf_counter() - t0
+        res[f"ms_per_image_{key}"] = round(el / images * 1e3, 3)
+        del s
+    log(f"[bench] render 1280x720: {res['ms_per_image_app_dam_break']:.2f} ms (27k), "
+        f"{res['ms_per_image_dam_break_1m']:.2f} ms (1M)")
+    return res
 
 
 def boundary_cost(steps: int, warmup: int):

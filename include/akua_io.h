@@ -1,5 +1,5 @@
-/* akua_io.h -- headless driver support of libakua.so: deterministic scenes and the trajectory
- * file format.
+/* akua_io.h -- headless driver support of libakua.so: deterministic scenes, the trajectory
+ * file format and headless rendering.
  *
  * The reference drives its solver from a windowed application: Application::prepareDamBreak
  * (Application.cpp:126-204) fills the particle VBO with a lattice, and Application::run
@@ -11,9 +11,11 @@
  *   - akua_traj_*: a versioned binary file of particle frames, the output a renderer would
  *     consume (Renderer::uploadParticles, Renderer.cpp:185-219) and the input of training and
  *     fixtures;
+ *   - akua_render: the reference's point-sprite drawing (Renderer::render) done on the GPU
+ *     into an RGB image, without a window;
  *   - the runner akuaengine_amd/akua_run, which combines them with the solver.
  * None of these functions touches a GPU, except akua_traj_append_solver (it reads the solver's
- * state).
+ * state) and akua_render.
  *
  * Trajectory file, version 1 (all little-endian):
  *   header, 128 bytes:
@@ -131,6 +133,32 @@ akua_status akua_traj_read(akua_traj* t, int64_t frame, int64_t* step, double* t
                            float* vel, float* mass, float* density, float* lambda, uint32_t* ids);
 /* Closes a file opened by create or open (flushing a written one). */
 akua_status akua_traj_close(akua_traj* t);
+
+/* ---- headless rendering ------------------------------------------------------------------ */
+/* The reference draws the particle buffer as point sprites (Renderer.cpp:28-56,
+ * fluidParticle.vert/.frag): a sprite of size / distance pixels around each particle, coloured
+ * colour * exp(-3 d^2) inside the disc d <= 0.5 of sprite coordinates, depth-tested (GL_LESS, in
+ * draw order) over a checkered floor (floor.frag, Application.cpp:128-157: y = -0.02, x and z in
+ * [0, 100], unit tiles of 0.8 and 0.5 grey), cleared to (0.15, 0.15, 0.18).  akua_render does the
+ * same on the GPU, headless, from the solver's own buffers (no copy of the particles): one
+ * 64-bit atomic minimum of (window depth, particle slot) per covered pixel keeps the sprite GL's
+ * depth test would keep, then one pass per pixel shades it or the floor. */
+typedef struct akua_camera {
+    float position[3];   /* CameraConfig (CameraConfig.h:8-16): (3, 2, 6) */
+    float yaw, pitch;    /* degrees: -90, 0; gaze (cos yaw cos pitch, sin pitch, sin yaw cos pitch) */
+    float fovY;          /* degrees: 45 */
+    float nearZ, farZ;   /* Application.cpp:17-18: 0.1, 160 */
+} akua_camera;
+
+void akua_default_camera(akua_camera* cam);
+/* The row-major view-projection matrix the renderer uses (glm::lookAt and glm::perspective:
+ * right-handed, depth in [-1, 1]) for an image of width x height. */
+akua_status akua_camera_matrix(const akua_camera* cam, int32_t width, int32_t height, float vp[16]);
+/* Renders the solver's particles (single-scene solvers).  pointSize > 0: every sprite has this size
+ * (the application's particles have 50); otherwise each particle's own size (records).  drawFloor:
+ * 1 draws the floor.  rgb: width * height * 3 bytes, rows from the top.  Synchronises. */
+akua_status akua_render(akua_solver* s, const akua_camera* cam, int32_t width, int32_t height,
+                        float pointSize, int32_t drawFloor, uint8_t* rgb);
 
 #ifdef __cplusplus
 }

@@ -113,3 +113,29 @@ def test_runner_boundary_matches_oracle(oracle_mod, tmp_path):
     f = T.read_trajectory(path)
     assert len(f) == 2 and f[1]["step"] == 8
     _assert_frame_equals_oracle(f[1], o)
+
+
+def _read_ppm(path):
+    data = open(path, "rb").read()
+    parts = data.split(b"\n", 3)
+    assert parts[0] == b"P6" and parts[2] == b"255"
+    w, h = (int(v) for v in parts[1].split())
+    return np.frombuffer(parts[3], np.uint8).reshape(h, w, 3)
+
+
+def test_runner_images_equal_the_library_render(tmp_path):
+    # frame 0 of the runner is the library's render of the scene as uploaded, with the
+    # reference's camera and the application's sprite size
+    from akuaengine_amd.solver import PBFSolver, default_camera
+    out = _run("--scene", "app_dam_break", "--steps", "4", "--render-every", "2", "--render-dir",
+               str(tmp_path), "--width", "320", "--height", "180")
+    assert out["images"] == 3
+    frames = sorted(p.name for p in tmp_path.iterdir())
+    assert frames == ["frame_00000000.ppm", "frame_00000002.ppm", "frame_00000004.ppm"]
+    sc = scenes.app_dam_break()
+    s = PBFSolver(sc.n, PBFConfig(), LambdaCorrParams())
+    s.set_state(sc.pos, sc.vel, sc.mass)
+    img = s.render(default_camera(), 320, 180, point_size=50.0)
+    np.testing.assert_array_equal(_read_ppm(str(tmp_path / frames[0])), img)
+    s.step_n(0.016, sc.box_min, sc.box_max, 4)
+    np.testing.assert_array_equal(_read_ppm(str(tmp_path / frames[2])), s.render(default_camera(), 320, 180))

@@ -316,3 +316,35 @@ def test_runner_dry_run_resume_keeps_ids(tmp_path):
     _write(src, bad)
     r = _run("--init", src, "--steps", "1", "--dry-run", check=False)
     assert r.returncode == 2 and "permutation" in r.stderr
+
+
+# ---- the renderer's camera (host) -----------------------------------------------------------------
+
+def test_camera_matrix_is_glm_lookat_times_perspective(lib):
+    # glm::lookAt (right-handed) and glm::perspective (depth in [-1, 1]) written out in float64,
+    # for the reference's camera (CameraConfig.h:8-16) and two others
+    from akuaengine_amd import solver as S
+    for pos, yaw, pitch, fov, W, H in [((3, 2, 6), -90.0, 0.0, 45.0, 1280, 720),
+                                       ((5, 4, 9), -110.0, -25.0, 60.0, 320, 240),
+                                       ((-1, 0.5, 2), 30.0, 10.0, 35.0, 100, 300)]:
+        c = S.default_camera()
+        c.position[:] = pos
+        c.yaw, c.pitch, c.fovY = yaw, pitch, fov
+        m = S.camera_matrix(c, W, H).astype(np.float64)
+        y, p = np.radians(yaw), np.radians(pitch)
+        f = np.array([np.cos(y) * np.cos(p), np.sin(p), np.sin(y) * np.cos(p)])
+        f /= np.linalg.norm(f)
+        r = np.cross(f, [0, 1, 0]); r /= np.linalg.norm(r)
+        u = np.cross(r, f); u /= np.linalg.norm(u)
+        s = np.cross(f, u); s /= np.linalg.norm(s)
+        uu = np.cross(s, f)
+        e = np.array(pos, float)
+        V = np.array([[*s, -s @ e], [*uu, -uu @ e], [*(-f), f @ e], [0, 0, 0, 1]])
+        t = np.tan(np.radians(fov) / 2)
+        n_, f_ = 0.1, 160.0
+        P = np.array([[1 / (W / H * t), 0, 0, 0], [0, 1 / t, 0, 0],
+                      [0, 0, -(f_ + n_) / (f_ - n_), -2 * f_ * n_ / (f_ - n_)], [0, 0, -1, 0]])
+        np.testing.assert_allclose(m, P @ V, rtol=1e-6, atol=1e-6)
+        # the gaze point lands in the middle of the image
+        g = m @ np.array([*(e + 2 * f), 1.0])
+        assert abs(g[0] / g[3]) < 1e-6 and abs(g[1] / g[3]) < 1e-6
