@@ -158,6 +158,33 @@ def test_neighbour_cap_and_small_table(gpu, oracle_mod, dam_params, cap):
     assert (s.debug_neighbours()[1] == cap).any()
 
 
+@pytest.mark.parametrize("case", ["cell_size_0.12", "cell_size_0.08", "far_from_origin"])
+def test_search_cell_size_and_far_coordinates(gpu, oracle_mod, dam_params, case):
+    """A search cell size other than h (the stencil then covers more or less than h; the
+    reference hashes with h and searches with spatialHashCellSize), and a lattice far from the
+    origin in x and y (cells near -512 and 512, x / h rounded at larger magnitudes) with negative
+    z cells.  Lists and states stay equal to the oracle's."""
+    from akuaengine_amd import scenes
+    cfg, corr = dam_params
+    sc = scenes.dam_break_scaled(12, seed=5)
+    box_min, box_max = sc.box_min, sc.box_max
+    pos = sc.pos
+    if case.startswith("cell_size_"):
+        cfg = dataclasses.replace(cfg, spatialHashCellSize=float(case.split("_")[-1]))
+    else:
+        shift = np.array([-51.5, 51.0, -0.35], np.float32)
+        pos = (sc.pos + shift).astype(np.float32)
+        box_min = tuple(float(a + b) for a, b in zip(sc.box_min, shift))
+        box_max = tuple(float(a + b) for a, b in zip(sc.box_max, shift))
+    _say(f"[parity] {case} N={len(pos)}")
+    s, o = _pair(gpu, oracle_mod, pos, sc.vel, cfg, corr)
+    for _ in range(2):
+        s.step(DT, box_min, box_max)
+        o.step(DT, box_min, box_max)
+    _assert_taps_equal(s, o)
+    _assert_state_equal(s, o)
+
+
 @pytest.mark.parametrize("n", [1, 2, 63, 65, 4095, 4097, 12289])
 def test_particle_counts_across_tile_edges(gpu, oracle_mod, dam_params, n):
     from akuaengine_amd import scenes
