@@ -78,11 +78,20 @@ def log(msg: str) -> None:
     print(msg, file=sys.stderr, flush=True)
 
 
-def cpu_baseline(scene, cfg, corr, dt, budget_s: float):
-    """The FP32 CPU oracle (OpenMP over particles) on the same 1M workload from its initial
-    state: steps until the time budget is spent (at least one)."""
-    from oracle import oracle as O
-    threads = O.max_threads()
+def _cpu_model() -> str:
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def _oracle_rate(O, scene, cfg, corr, dt, budget_s: float, max_steps: int, label: str):
+    """Steps of the FP32 oracle from the scene's initial state until the budget is spent (at
+    least one step, at most max_steps): (particle-substeps/s, steps, seconds)."""
     o = O.Oracle(scene.n, cfg, corr, 32)
     o.set_state(scene.pos, scene.vel, scene.mass)
     steps, t0 = 0, time.perf_counter()
@@ -90,13 +99,38 @@ def cpu_baseline(scene, cfg, corr, dt, budget_s: float):
         o.step(dt, scene.box_min, scene.box_max, 1)
         steps += 1
         el = time.perf_counter() - t0
-        log(f"[bench] cpu oracle step {steps}: {el:.1f} s")
-        if el >= budget_s or steps >= 50:
+        if steps == 1 or steps % 50 == 0:
+            log(f"[bench] cpu oracle {label} step {steps}: {el:.1f} s")
+        if el >= budget_s or steps >= max_steps:
             break
-    return {"value": scene.n * steps / el, "unit": "particle-substeps/s", "cores": threads,
-            "kind": "port",
-            "sample": f"{steps} steps of the 1M dam break from its initial state, FP32 oracle, "
-                      f"{threads} OpenMP threads, {el:.1f} s"}
+    return scene.n * steps / el, steps, el
+
+
+def cpu_baseline(scene, cfg, corr, dt, budget_s: float):
+    """The FP32 CPU oracle (OpenMP over particles) on the same 1M workload from its initial
+    state: steps until the time budget is spent (at least one).  Beside it: one step of the 1M
+    scene on one thread, and the 27k dam break (all 200 steps on every thread, a bounded sample
+    on one thread), with the host's CPU model and core count."""
+    from akuaengine_amd import scenes
+    from oracle import oracle as O
+    threads = O.max_threads()
+    rate, steps, el = _oracle_rate(O, scene, cfg, corr, dt, budget_s, 50, "1M")
+    out = {"value": rate, "unit": "particle-substeps/s", "cores": threads, "kind": "port",
+           "sample": f"{steps} steps of the 1M dam break from its initial state, FP32 oracle, "
+                     f"{threads} OpenMP threads, {el:.1f} s",
+           "host": {"cpu_model": _cpu_model(), "nproc": os.cpu_count(), "threads": threads}}
+    small = scenes.dam_break_27k()
+    r27, s27, e27 = _oracle_rate(O, small, cfg, corr, dt, 60.0, 200, "27k")
+    out["dam_break_27k"] = {"value": r27, "steps": s27, "seconds": round(e27, 2), "threads": threads}
+    O.set_threads(1)
+    try:
+        r1, s1, e1 = _oracle_rate(O, scene, cfg, corr, dt, 0.0, 1, "1M one thread")
+        out["one_thread_1m"] = {"value": r1, "steps": s1, "seconds": round(e1, 2)}
+        r1s, s1s, e1s = _oracle_rate(O, small, cfg, corr, dt, 8.0, 200, "27k one thread")
+        out["one_thread_27k"] = {"value": r1s, "steps": s1s, "seconds": round(e1s, 2)}
+    finally:
+        O.set_threads(threads)
+    return out
 
 
 def time_steps(s, dt, bmin, bmax, steps: int) -> float:
