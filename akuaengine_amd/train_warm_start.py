@@ -3,7 +3,7 @@
 Recipe (the learned-workload definition): dam-break scenes shaped like the 1M workload (100^3
 lattice, spacing 0.05, jitter +-0.005, box [0,10]x[0,6]x[0,5]) with seeds other than the
 evaluation seed 1, stepped by the plain 4-iteration solver while the network trains online on
-their features and targets: 4 scenes x 500 steps = 2,000 solver steps, 65,536 sampled particles
+their features and targets: 8 scenes x 500 steps = 4,000 solver steps, 65,536 sampled particles
 per step, minibatches of 8,192, 2 epochs per step, Adam with the learning rate lowered per scene.
 The plain solver's own residual reduction over the same steps calibrates the gate ratio (its 90th
 percentile).
@@ -31,8 +31,9 @@ PHYSICS = {
 }
 
 
-def train(physics: str = "reference", iterations: int = 1, seeds=(11, 12, 13, 14),
-          steps: int = 500, lrs=(1e-3, 5e-4, 2.5e-4, 1.25e-4), n_side: int = 100, log=print):
+def train(physics: str = "reference", iterations: int = 1, seeds=tuple(range(11, 19)),
+          steps: int = 500, lrs=(1e-3, 5e-4, 2.5e-4, 1.25e-4, 1e-4, 8e-5, 6e-5, 5e-5),
+          n_side: int = 100, log=print):
     cfg, corr, dt = PHYSICS[physics]()
     solver = None
     ratios, losses = [], []
@@ -67,7 +68,7 @@ def main():
     ap.add_argument("--physics", default="reference", choices=sorted(PHYSICS))
     ap.add_argument("--iterations", type=int, default=1)
     ap.add_argument("--steps", type=int, default=500)
-    ap.add_argument("--seeds", default="11,12,13,14")
+    ap.add_argument("--seeds", default="11,12,13,14,15,16,17,18")
     ap.add_argument("--out", required=True)
     a = ap.parse_args()
     net, meta = train(a.physics, a.iterations, tuple(int(s) for s in a.seeds.split(",")), a.steps)
