@@ -242,6 +242,22 @@ void orc_set_xs(void* hp, const float* xs) {
     }
 }
 
+// Replaces the vorticity (sorted order, 3 floats per particle): a confinement pass from a given
+// vorticity field.
+void orc_set_vort(void* hp, const float* w) {
+    auto* h = static_cast<Handle*>(hp);
+    const long n = (long)(h->prec == 64 ? h->sd.size() : h->sf.size());
+    for (long i = 0; i < n; ++i) {
+        if (h->prec == 64) h->sd.w[i] = {w[3 * i], w[3 * i + 1], w[3 * i + 2]};
+        else h->sf.w[i] = {w[3 * i], w[3 * i + 1], w[3 * i + 2]};
+    }
+}
+
+#ifdef ORACLE_PLANTS
+// Turns planted mistake k on (0: none); see the list in pbf_oracle.hpp.
+void orc_set_plant(int k) { planted_mistake() = k; }
+#endif
+
 // Boundary samples (layers 0: off; spacing <= 0: the configuration's particle spacing).
 void orc_set_boundary(void* hp, int layers, float spacing) {
     auto* h = static_cast<Handle*>(hp);

@@ -29,6 +29,7 @@
 #include <cstdint>
 #include <cstring>
 #include <numeric>
+#include <utility>
 #include <vector>
 
 #ifdef _OPENMP
@@ -36,6 +37,33 @@
 #endif
 
 namespace pbf_oracle {
+
+// Planted mistakes, for the tests that show each pin fails when the function it pins is wrong.
+// Only the build with ORACLE_PLANTS (oracle/liboracle_plant.so) carries the run-time switch; the
+// regular build compiles every PLANT(k) to false.  Mistake k, where it is planted:
+//   1 vorticities: v_i - v_j in place of v_j - v_i
+//   2 vorticities: cross(grad W, v_ij) in place of cross(v_ij, grad W)
+//   3 vorticities: m_i in place of m_j
+//   4 vorticity_confinement: |w_j| - |w_i| in place of |w_i| - |w_j|
+//   5 vorticity_confinement: cross(w_i, N) in place of cross(N, w_i)
+//   6 vorticity_confinement: the factor 1 / rho_i dropped
+//   7 vorticity_confinement: m_i in place of m_j
+//   8 xsph: m_i / rho_i in place of m_j / rho_j
+//   9 xsph: v_i - v_j in place of v_j - v_i
+//  10 xsph: velocities updated in place (Gauss-Seidel) instead of Jacobi
+//  11 boundary_damping: the normal of the y-max wall is +y
+//  12 boundary_damping: every wall acts on the velocity from before the damping
+//  13 boundary_damping: the x-max wall placed at boxMin.x
+//  14 boundary_damping: the friction factor applied to the normal part, not the tangential one
+#ifdef ORACLE_PLANTS
+inline int& planted_mistake() {
+    static int k = 0;
+    return k;
+}
+#define PLANT(k) (::pbf_oracle::planted_mistake() == (k))
+#else
+#define PLANT(k) false
+#endif
 
 // Mirrors AkuaEngine::PBFConfig (PBFConfig.h:18-29).  Plain C layout, no glm.
 struct Config {
@@ -626,23 +654,30 @@ void update_position_velocity(State<T>& s, const Params<T>& p) {
 
 // resolve_collision + kernel_apply_boundary_velocity_damping (IntegrationCUDA.cu:51-102), called
 // with restitution 0 and friction 0.95 (PBFSolver.cpp:64).  Planes are applied in order, each
-// seeing the velocity the previous one left.  Pinned by per-plane special cases.
+// seeing the velocity the previous one left.  Pinned by known answers on each of the six walls
+// of an asymmetric box (approaching, resting, leaving), an interior particle left untouched and a
+// corner that needs the sequential order (tests/test_oracle_postpass.py).
 template <class T>
 void boundary_damping(State<T>& s, const Params<T>& p, T restitution, T friction) {
     const long n = (long)s.size();
     const T minDist = T(0.025f);
-    const V3<T> pt[6] = {{p.bmin[0], 0, 0}, {p.bmax[0], 0, 0}, {0, p.bmin[1], 0},
+    const T xMax = PLANT(13) ? p.bmin[0] : p.bmax[0];
+    const T nyMax = PLANT(11) ? T(1) : T(-1);
+    const V3<T> pt[6] = {{p.bmin[0], 0, 0}, {xMax, 0, 0},        {0, p.bmin[1], 0},
                          {0, p.bmax[1], 0}, {0, 0, p.bmin[2]}, {0, 0, p.bmax[2]}};
-    const V3<T> nm[6] = {{1, 0, 0}, {-1, 0, 0}, {0, 1, 0}, {0, -1, 0}, {0, 0, 1}, {0, 0, -1}};
+    const V3<T> nm[6] = {{1, 0, 0}, {-1, 0, 0}, {0, 1, 0}, {0, nyMax, 0}, {0, 0, 1}, {0, 0, -1}};
 #pragma omp parallel for schedule(static)
     for (long i = 0; i < n; ++i) {
-        V3<T> x = s.x[i], v = s.v[i];
+        const V3<T> x = s.x[i], v0 = s.v[i];
+        V3<T> v = v0;
         for (int k = 0; k < 6; ++k) {
+            const V3<T> vin = PLANT(12) ? v0 : v;
             T distance = dot(x - pt[k], nm[k]);
-            T approaching = dot(v, nm[k]);
+            T approaching = dot(vin, nm[k]);
             if (distance < minDist) {
                 V3<T> vn = approaching * nm[k];
-                V3<T> vt = v - vn;
+                V3<T> vt = vin - vn;
+                if (PLANT(14)) std::swap(vn, vt);
                 if (approaching < T(0)) {
                     v = (-restitution) * vn + (T(1) - friction) * vt;
                 } else if (std::fabs(approaching) < T(1e-5f)) {
@@ -655,7 +690,9 @@ void boundary_damping(State<T>& s, const Params<T>& p, T restitution, T friction
 }
 
 // kernel_compute_vorticities (IntegrationCUDA.cu:104-128).  Pinned by zero vorticity under a
-// uniform velocity field and by rigid rotation giving a vorticity parallel to the axis.
+// uniform velocity field, and by rigid rotation v = Omega x r on an interior lattice with
+// checkerboard masses: w_i = (2/3) Omega sum_j m_j |grad W(r_j)| |r_j|, summed over the shells
+// of the 26 neighbours (tests/test_oracle_postpass.py).
 template <class T>
 void vorticities(State<T>& s, const Params<T>& p, const std::vector<uint32_t>& nbr,
                  const std::vector<uint32_t>& cnt) {
@@ -666,16 +703,21 @@ void vorticities(State<T>& s, const Params<T>& p, const std::vector<uint32_t>& n
         V3<T> w{T(0), T(0), T(0)};
         for (uint32_t o = 0; o < cnt[i]; ++o) {
             uint32_t j = nbr[st + o];
-            V3<T> vij = s.v[j] - s.v[i];
+            V3<T> vij = PLANT(1) ? s.v[i] - s.v[j] : s.v[j] - s.v[i];
             V3<T> sep = s.xs[i] - s.xs[j];
-            w = w + (-s.mass[j]) * cross(vij, grad_spiky<T>(sep, p.K));
+            const V3<T> g = grad_spiky<T>(sep, p.K);
+            const T mj = PLANT(3) ? s.mass[i] : s.mass[j];
+            w = w + (-mj) * (PLANT(2) ? cross(g, vij) : cross(vij, g));
         }
         s.w[i] = w;
     }
 }
 
-// kernel_apply_vorticity_confinement (IntegrationCUDA.cu:130-165).  Pinned by "no change when the
-// vorticity field is uniform" (eta = 0).
+// kernel_apply_vorticity_confinement (IntegrationCUDA.cu:130-165).  Pinned by closed forms
+// (tests/test_oracle_postpass.py): on an interior lattice whose |w| grows along x with w along z,
+// eta points along -x and the velocity gains dt * eps * |w_i| along +y; the 1e-5 threshold on
+// |eta| decides at the value the 1 / rho_i factor gives; three particles with unequal masses fix
+// the direction of eta.
 template <class T>
 void vorticity_confinement(State<T>& s, const Params<T>& p, const std::vector<uint32_t>& nbr,
                            const std::vector<uint32_t>& cnt) {
@@ -683,44 +725,54 @@ void vorticity_confinement(State<T>& s, const Params<T>& p, const std::vector<ui
 #pragma omp parallel for schedule(static)
     for (long i = 0; i < n; ++i) {
         const size_t st = (size_t)i * p.maxN;
-        T invD = T(1) / s.density[i];
+        T invD = PLANT(6) ? T(1) : T(1) / s.density[i];
         V3<T> eta{T(0), T(0), T(0)};
         const T li = length(s.w[i]);
         for (uint32_t o = 0; o < cnt[i]; ++o) {
             uint32_t j = nbr[st + o];
-            T diff = li - length(s.w[j]);
+            T diff = PLANT(4) ? length(s.w[j]) - li : li - length(s.w[j]);
             V3<T> sep = s.xs[i] - s.xs[j];
-            eta = eta + (s.mass[j] * diff) * grad_spiky<T>(sep, p.K);
+            const T mj = PLANT(7) ? s.mass[i] : s.mass[j];
+            eta = eta + (mj * diff) * grad_spiky<T>(sep, p.K);
         }
         eta = eta * invD;
         T el = length(eta);
         if (el < T(1e-5f)) continue;
         V3<T> N = eta / el;
-        V3<T> f = p.vortEps * cross(N, s.w[i]);
+        V3<T> f = p.vortEps * (PLANT(5) ? cross(s.w[i], N) : cross(N, s.w[i]));
         s.v[i] = s.v[i] + p.dt * f;
     }
 }
 
 // kernel_apply_xsph_viscosity (IntegrationCUDA.cu:167-195), Jacobi: every particle reads the
-// velocities from before the pass.  Pinned by "no change under a uniform velocity field".
+// velocities from before the pass.  Pinned by "no change under a uniform velocity field", by the
+// two-particle closed form with unequal masses and densities, and by a three-particle chain whose
+// middle particle must see its neighbours' velocities from before the pass
+// (tests/test_oracle_postpass.py).
 template <class T>
 void xsph(State<T>& s, const Params<T>& p, const std::vector<uint32_t>& nbr,
           const std::vector<uint32_t>& cnt, std::vector<V3<T>>& vnew) {
     const long n = (long)s.size();
     vnew.resize(n);
-#pragma omp parallel for schedule(static)
-    for (long i = 0; i < n; ++i) {
+    auto dv_of = [&](long i, const std::vector<V3<T>>& v) {
         const size_t st = (size_t)i * p.maxN;
         V3<T> dv{T(0), T(0), T(0)};
         for (uint32_t o = 0; o < cnt[i]; ++o) {
             uint32_t j = nbr[st + o];
-            V3<T> vij = s.v[j] - s.v[i];
+            V3<T> vij = PLANT(9) ? v[i] - v[j] : v[j] - v[i];
             V3<T> sep = s.xs[i] - s.xs[j];
             T w = poly6<T>(dot(sep, sep), p.K);
-            dv = dv + ((s.mass[j] / s.density[j]) * vij) * w;
+            const T mr = PLANT(8) ? s.mass[i] / s.density[i] : s.mass[j] / s.density[j];
+            dv = dv + (mr * vij) * w;
         }
-        vnew[i] = s.v[i] + p.visc * dv;
+        return dv;
+    };
+    if (PLANT(10)) {  // in place, in slot order
+        for (long i = 0; i < n; ++i) s.v[i] = s.v[i] + p.visc * dv_of(i, s.v);
+        return;
     }
+#pragma omp parallel for schedule(static)
+    for (long i = 0; i < n; ++i) vnew[i] = s.v[i] + p.visc * dv_of(i, s.v);
     s.v.swap(vnew);
 }
 
