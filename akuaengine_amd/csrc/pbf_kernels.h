@@ -17,6 +17,10 @@
 //   dens, lam  float              density and lambda of the last step   4 B each
 //   id   uint32                   original particle index               4 B
 //   lid  uint32                   logical index                         4 B
+//   VS   float4 (xyz, 0)          the step's velocity before the wall damping, vorticity and
+//                                 XSPH: the record's new_velocity       16 B
+//   DP   float4 (xyz, 0)          the last iteration's position delta: the record's
+//                                 position_delta                        16 B
 // Per-step scratch: X0 (old positions), XsA/XsB (x* ping-pong), W (vorticity and its length),
 // Vc (velocity after vorticity confinement), two sorts' keys/values, the cell table, and the
 // neighbour list.  The list holds physical indices in chunks of four entries, wave-interleaved:
@@ -41,6 +45,9 @@ struct DeviceState {
     uint32_t* id;
     uint32_t* lid;             // logical index of each physical slot (persistent)
     uint32_t* phys;            // physical slot of each logical index (persistent)
+    float4* VS;                // new_velocity of the last step (persistent)
+    float4* DP;                // position_delta of the last iteration (persistent)
+    float4* TDP;               // DP moved to the new physical slots (a step without iterations)
     float4* X0;
     float4* XsA;
     float4* XsB;
@@ -103,9 +110,10 @@ void launch_predict_hash(hipStream_t st, const DeviceState& d, const SphParams& 
 void launch_cell_keys(hipStream_t st, const DeviceState& d, const SphParams& P,
                       const StepParams& S, bool keepLayout);
 // carry: also move the previous density and lambda (needed by a step without iterations and by
-// the learned features).
+// the learned features); carryDelta: also the position delta (a step without iterations keeps
+// it, as the reference's record does).
 void launch_gather(hipStream_t st, const DeviceState& d, const SphParams& P, const StepParams& S,
-                   bool carry);
+                   bool carry, bool carryDelta);
 // Physical start and contiguity of every bucket (after the gather).
 void launch_bucket_phys(hipStream_t st, const DeviceState& d);
 // bs: workgroup size of the neighbour kernels (128, 256 or 512; results do not depend on it).

@@ -204,7 +204,8 @@ __global__ __launch_bounds__(kBlock) void k_cell_keys(DeviceState d, SphParams P
 // bit-identical to the value that was hashed.
 // kCarry: also the previous density and lambda (read by a step without iterations and by the
 // learned features; a step with iterations rewrites both before anything reads them).
-template <bool kCarry>
+// kCarryDelta: also the position delta (a step without iterations keeps the previous one).
+template <bool kCarry, bool kCarryDelta>
 __global__ __launch_bounds__(kBlock) void k_gather(DeviceState d, SphParams P, StepParams S) {
     uint32_t p = blockIdx.x * kBlock + threadIdx.x;
     if (p >= d.nslots) return;
@@ -224,6 +225,7 @@ __global__ __launch_bounds__(kBlock) void k_gather(DeviceState d, SphParams P, S
         d.Tdens[p] = d.dens[src];
         d.Tlam[p] = d.lam[src];
     }
+    if (kCarryDelta) d.TDP[p] = d.DP[src];
     d.Tid[p] = d.id[src];
     d.lidT[p] = k;
     d.physT[k] = p;
@@ -725,11 +727,11 @@ __device__ __forceinline__ void delta_correct_slot(const DeviceState& d, const S
     const float4 xi4 = XsIn[i];
     const f3 xi = make3(xi4);
     f3 q = xi;
+    f3 dp{0.0f, 0.0f, 0.0f};
     if (kApplyDelta) {
         const float li = d.lam[i];
         const uint32_t n = d.cnt[i];
         const uint4* lp = d.nbr4 + list_base(i, P.maxN);
-        f3 dp{0.0f, 0.0f, 0.0f};
         const Recip Rdq = recip(P.wdq);
         walk_list<XLam, kBatchDelta>(
             lp, n, [&](uint32_t j) { return XLam{XsIn[j], d.lam[j]}; },
@@ -770,6 +772,7 @@ __device__ __forceinline__ void delta_correct_slot(const DeviceState& d, const S
     }
     const float4 x0 = d.X0[i];
     f3 v = div3_rn(q - make3(x0), S.dt);
+    d.VS[i] = make_float4(v.x, v.y, v.z, 0.0f);  // the record's new_velocity (IntegrationCUDA.cu:45)
     v = damp_walls(q, v, sc);
     d.X[i] = make_float4(q.x, q.y, q.z, xi4.w);
     d.V[i] = make_float4(v.x, v.y, v.z, 0.0f);
@@ -777,9 +780,12 @@ __device__ __forceinline__ void delta_correct_slot(const DeviceState& d, const S
     const uint32_t k = d.lidT[i];
     d.lid[i] = k;
     d.phys[k] = i;
-    if (!kApplyDelta) {  // no iteration ran: density and lambda carry over
+    if (kApplyDelta) {  // the record's position_delta (ConstraintSolverCUDA.cu:129)
+        d.DP[i] = make_float4(dp.x, dp.y, dp.z, 0.0f);
+    } else {  // no iteration ran: density, lambda and the position delta carry over
         d.dens[i] = d.Tdens[i];
         d.lam[i] = d.Tlam[i];
+        d.DP[i] = d.TDP[i];
     }
 }
 
@@ -982,11 +988,13 @@ void launch_cell_keys(hipStream_t st, const DeviceState& d, const SphParams& P,
         k_cell_keys<false><<<grid(d.nslots), kBlock, 0, st>>>(d, P, S);
 }
 void launch_gather(hipStream_t st, const DeviceState& d, const SphParams& P, const StepParams& S,
-                   bool carry) {
-    if (carry)
-        k_gather<true><<<grid(d.nslots), kBlock, 0, st>>>(d, P, S);
+                   bool carry, bool carryDelta) {
+    if (carryDelta)
+        k_gather<true, true><<<grid(d.nslots), kBlock, 0, st>>>(d, P, S);
+    else if (carry)
+        k_gather<true, false><<<grid(d.nslots), kBlock, 0, st>>>(d, P, S);
     else
-        k_gather<false><<<grid(d.nslots), kBlock, 0, st>>>(d, P, S);
+        k_gather<false, false><<<grid(d.nslots), kBlock, 0, st>>>(d, P, S);
 }
 void launch_bucket_phys(hipStream_t st, const DeviceState& d) {
     k_bucket_phys<<<grid(d.nslots), kBlock, 0, st>>>(d);

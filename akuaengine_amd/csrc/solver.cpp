@@ -178,7 +178,7 @@ struct akua_solver {
     // device buffers
     DevBuf<SceneDev> dScenes;
     DevBuf<uint32_t> dChunkScene;
-    DevBuf<float4> X, V, X0, XsA, XsB, XsL, W, Vc, color;
+    DevBuf<float4> X, V, VS, DP, TDP, X0, XsA, XsB, XsL, W, Vc, color;
     DevBuf<float> dens, lam, Tdens, Tlam, size;
     DevBuf<uint32_t> id, Tid, lid, phys, lidT, physT, srcL, keysLay, bslot, nbr, cnt, perm2keep;
     DevBuf<uint32_t> s1KA, s1VA, s1KB, s1VB, s2KA, s2VA, s2KB, s2VB, counts, totals;
@@ -197,6 +197,8 @@ struct akua_solver {
     int mode = 0;                  // 0: plain solver, 1: learned warm start with fallback
     int gateStat = 0;              // 0: max over particles, 1: mean
     int learnedIters = 1;          // Jacobi iterations after the warm start
+    int trainIters = 1;            // recorded steps: targets are x* before the last trainIters
+                                   // iterations (akua_solver_train's targetIterations)
     float gateRatio = 1.0f, gateTol = 0.01f;
     bool haveNet = false;
     // boundary samples on the box walls (akua_solver_set_boundary)
@@ -234,6 +236,11 @@ struct akua_solver {
     bool graphDirty = false;
     uint64_t stepsSinceStats = 0;
 
+    // step_device: orders the solver's stream with the caller's stream (created on first use)
+    hipEvent_t orderEvent = nullptr;
+    // record export staging (get_particles), allocated on first use
+    DevBuf<AosParticle> aosStage;
+
     // timing
     bool timing = false;
     std::vector<TimedLaunch> pending;
@@ -248,6 +255,7 @@ struct akua_solver {
         d.scenes = dScenes.p;
         d.chunkScene = dChunkScene.p;
         d.X = X.p; d.V = V.p; d.dens = dens.p; d.lam = lam.p; d.id = id.p;
+        d.VS = VS.p; d.DP = DP.p; d.TDP = TDP.p;
         d.lid = lid.p; d.phys = phys.p;
         d.X0 = X0.p; d.XsA = XsA.p; d.XsB = XsB.p; d.W = W.p; d.Vc = Vc.p; d.Bg = Bg.p;
         d.Tdens = Tdens.p; d.Tlam = Tlam.p; d.Tid = Tid.p;
@@ -269,11 +277,32 @@ struct akua_solver {
         return d;
     }
 
+    // Teardown: wait for the stream's work, free the device buffers, then destroy the graphs,
+    // the events and last the stream.
     ~akua_solver() {
+        if (stream) (void)hipStreamSynchronize(stream);
+        release_buffers();
         for (auto& g : graphs)
             if (g.exec) (void)hipGraphExecDestroy(g.exec);
         for (auto e : eventPool) (void)hipEventDestroy(e);
+        if (orderEvent) (void)hipEventDestroy(orderEvent);
         if (stream) (void)hipStreamDestroy(stream);
+    }
+
+    void release_buffers() {
+        dScenes.release(); dChunkScene.release();
+        for (auto* b : {&X, &V, &VS, &DP, &TDP, &X0, &XsA, &XsB, &XsL, &W, &Vc, &color, &Bg, &XsP, &XsW})
+            b->release();
+        for (auto* b : {&dens, &lam, &Tdens, &Tlam, &size, &probeRho, &bndPsi, &net, &feat, &targ,
+                        &lamHat, &adamM, &adamV, &part, &batchF, &batchT, &lossBuf})
+            b->release();
+        for (auto* b : {&id, &Tid, &lid, &phys, &lidT, &physT, &srcL, &keysLay, &bslot, &nbr, &cnt,
+                        &perm2keep, &s1KA, &s1VA, &s1KB, &s1VB, &s2KA, &s2VA, &s2KB, &s2VB, &counts,
+                        &totals, &gate, &fallbacks})
+            b->release();
+        table.release(); dTiles.release(); dSortScenes.release(); nbrSum.release();
+        probeMax.release(); probeSum.release(); renderKeys.release(); renderRgb.release();
+        resid.release(); residPart.release(); residBlk.release(); aosStage.release();
     }
 };
 
@@ -383,6 +412,7 @@ void setup(akua_solver* s, const std::vector<uint32_t>& counts) {
     s->dScenes.alloc(s->scenes.size());
     s->dChunkScene.alloc(nChunks);
     s->X.alloc(ns); s->V.alloc(ns); s->X0.alloc(ns); s->XsA.alloc(ns); s->XsB.alloc(ns);
+    s->VS.alloc(ns); s->DP.alloc(ns); s->TDP.alloc(ns);
     s->W.alloc(ns); s->Vc.alloc(ns); s->color.alloc(ns);
     s->dens.alloc(ns); s->lam.alloc(ns); s->Tdens.alloc(ns); s->Tlam.alloc(ns); s->size.alloc(ns);
     s->id.alloc(ns); s->Tid.alloc(ns);
@@ -423,6 +453,9 @@ void setup(akua_solver* s, const std::vector<uint32_t>& counts) {
     HIP_CHECK(hipMemset(s->s2KA.p, 0, ns * 4));
     HIP_CHECK(hipMemset(s->X.p, 0, ns * sizeof(float4)));
     HIP_CHECK(hipMemset(s->V.p, 0, ns * sizeof(float4)));
+    HIP_CHECK(hipMemset(s->VS.p, 0, ns * sizeof(float4)));
+    HIP_CHECK(hipMemset(s->DP.p, 0, ns * sizeof(float4)));
+    HIP_CHECK(hipMemset(s->TDP.p, 0, ns * sizeof(float4)));
     HIP_CHECK(hipMemset(s->cnt.p, 0, ns * 4));
     s->sph = make_sph(s->cfg, s->corr);
     s->boundedQuotients = quotients_bounded(s->sph);
@@ -530,7 +563,8 @@ void enqueue_step(akua_solver* s, hipStream_t st, const StepParams& S, bool time
     if (!keepLayout) sort_layout(s, st, timed);
     L.before("gather");
     launch_gather(st, d, P, S,
-                  s->cfg.solverIterations == 0 || record || (s->mode == 1 && s->haveNet));
+                  s->cfg.solverIterations == 0 || record || (s->mode == 1 && s->haveNet),
+                  s->cfg.solverIterations == 0);
     L.after("gather");
     L.before("bucket_phys");
     launch_bucket_phys(st, d);
@@ -543,7 +577,7 @@ void enqueue_step(akua_solver* s, hipStream_t st, const StepParams& S, bool time
     const bool learned = s->mode == 1 && !record;
     const int J = learned ? std::min(s->learnedIters, I) : 0;
     // record: targets are x* before the last Jt iterations and the lambda computed there
-    const int Jt = std::min(std::max(s->learnedIters, 1), I);
+    const int Jt = std::min(std::max(s->trainIters, 1), I);
     float4* xs[2] = {s->XsA.p, s->XsB.p};
     // the warm chain: x*_w stays in XsW (for the taps); later iterates alternate between XsL
     // (free once the neighbour lists are built) and XsB (which the fallback may overwrite only
@@ -757,7 +791,7 @@ void run_density_probe(akua_solver* s) {
     sort_hash(s, st, false);
     launch_cell_keys(st, d, s->sph, S0, false);
     sort_layout(s, st, false);
-    launch_gather(st, d, s->sph, S0, false);
+    launch_gather(st, d, s->sph, S0, false, false);
     launch_bucket_phys(st, d);
     launch_density_probe(st, d, s->sph, s->probeRho.p, s->probeMax.p, s->probeSum.p);
     HIP_CHECK(hipGetLastError());
@@ -773,6 +807,7 @@ void upload_soa(akua_solver* s, const float* pos, const float* vel, const float*
                 bool fromAos, const akua_particle* aos) {
     const size_t n = s->nparticles;
     std::vector<float4> X(s->nslots, make_float4(0, 0, 0, 0)), V(s->nslots, make_float4(0, 0, 0, 0));
+    std::vector<float4> VS(s->nslots, make_float4(0, 0, 0, 0)), DP(s->nslots, make_float4(0, 0, 0, 0));
     std::vector<float4> C(s->nslots, make_float4(0, 0, 1, 1));
     std::vector<float> D(s->nslots, 0.0f), L(s->nslots, 0.0f), Z(s->nslots, 0.0f);
     std::vector<uint32_t> I(s->nslots, 0);
@@ -786,6 +821,8 @@ void upload_soa(akua_solver* s, const float* pos, const float* vel, const float*
                 const akua_particle& a = aos[k];
                 for (int c = 0; c < 3; ++c) { p[c] = a.position[c]; v[c] = a.velocity[c]; }
                 m = a.mass; dd = a.density; ll = a.lambda;
+                VS[slot] = make_float4(a.new_velocity[0], a.new_velocity[1], a.new_velocity[2], 0.0f);
+                DP[slot] = make_float4(a.position_delta[0], a.position_delta[1], a.position_delta[2], 0.0f);
                 C[k] = make_float4(a.color[0], a.color[1], a.color[2], a.color[3]);
                 Z[k] = a.size;
             } else {
@@ -812,6 +849,8 @@ void upload_soa(akua_solver* s, const float* pos, const float* vel, const float*
     HIP_CHECK(hipStreamSynchronize(s->stream));
     HIP_CHECK(hipMemcpy(s->X.p, X.data(), X.size() * sizeof(float4), hipMemcpyHostToDevice));
     HIP_CHECK(hipMemcpy(s->V.p, V.data(), V.size() * sizeof(float4), hipMemcpyHostToDevice));
+    HIP_CHECK(hipMemcpy(s->VS.p, VS.data(), VS.size() * sizeof(float4), hipMemcpyHostToDevice));
+    HIP_CHECK(hipMemcpy(s->DP.p, DP.data(), DP.size() * sizeof(float4), hipMemcpyHostToDevice));
     HIP_CHECK(hipMemcpy(s->dens.p, D.data(), D.size() * 4, hipMemcpyHostToDevice));
     HIP_CHECK(hipMemcpy(s->lam.p, L.data(), L.size() * 4, hipMemcpyHostToDevice));
     HIP_CHECK(hipMemcpy(s->id.p, I.data(), I.size() * 4, hipMemcpyHostToDevice));
@@ -858,7 +897,7 @@ void apply_tuning(akua_solver* s, const akua_tuning& t) {
 
 // The per-slot state a step reads: a trial restarts from it and the tuner restores it.
 struct Snapshot {
-    DevBuf<float4> X, V, W;
+    DevBuf<float4> X, V, W, VS, DP;
     DevBuf<float> dens, lam;
     DevBuf<uint32_t> id, lid, phys;
 };
@@ -870,10 +909,12 @@ void copy_dev(T* dst, const T* src, size_t n, hipStream_t st) {
 
 void take_snapshot(akua_solver* s, Snapshot& q) {
     const size_t n = s->nslots;
-    q.X.alloc(n); q.V.alloc(n); q.W.alloc(n); q.dens.alloc(n); q.lam.alloc(n);
+    q.X.alloc(n); q.V.alloc(n); q.W.alloc(n); q.VS.alloc(n); q.DP.alloc(n);
+    q.dens.alloc(n); q.lam.alloc(n);
     q.id.alloc(n); q.lid.alloc(n); q.phys.alloc(n);
     hipStream_t st = s->stream;
     copy_dev(q.X.p, s->X.p, n, st); copy_dev(q.V.p, s->V.p, n, st); copy_dev(q.W.p, s->W.p, n, st);
+    copy_dev(q.VS.p, s->VS.p, n, st); copy_dev(q.DP.p, s->DP.p, n, st);
     copy_dev(q.dens.p, s->dens.p, n, st); copy_dev(q.lam.p, s->lam.p, n, st);
     copy_dev(q.id.p, s->id.p, n, st); copy_dev(q.lid.p, s->lid.p, n, st);
     copy_dev(q.phys.p, s->phys.p, n, st);
@@ -884,6 +925,7 @@ void restore_snapshot(akua_solver* s, const Snapshot& q) {
     s->layoutPhase = 0;
     hipStream_t st = s->stream;
     copy_dev(s->X.p, q.X.p, n, st); copy_dev(s->V.p, q.V.p, n, st); copy_dev(s->W.p, q.W.p, n, st);
+    copy_dev(s->VS.p, q.VS.p, n, st); copy_dev(s->DP.p, q.DP.p, n, st);
     copy_dev(s->dens.p, q.dens.p, n, st); copy_dev(s->lam.p, q.lam.p, n, st);
     copy_dev(s->id.p, q.id.p, n, st); copy_dev(s->lid.p, q.lid.p, n, st);
     copy_dev(s->phys.p, q.phys.p, n, st);
@@ -1118,7 +1160,7 @@ void train(akua_solver* s, float dt, const float* bmin, const float* bmax, int s
     validate_train_config(c);
     if (c.targetIterations < 1 || c.targetIterations > s->cfg.solverIterations)
         fail(AKUA_ERR_INVALID_ARG, "targetIterations must be in [1, solverIterations]");
-    s->learnedIters = c.targetIterations;
+    s->trainIters = c.targetIterations;
     check_box(bmin, bmax);
     set_box(s, 0, bmin, bmax);
     upload_scenes(s);
@@ -1456,15 +1498,14 @@ akua_status akua_solver_get_particles(akua_solver* s, akua_particle* out, int32_
         if (!s || !out) fail(AKUA_ERR_INVALID_ARG, "NULL argument");
         check_n(s, n);
         if (s->scenes.size() != 1) fail(AKUA_ERR_UNSUPPORTED, "record export is per single-scene solver");
-        AosParticle* d = nullptr;
-        HIP_CHECK(hipMalloc(&d, (size_t)n * sizeof(AosParticle)));
+        if (s->aosStage.n < (size_t)n) s->aosStage.alloc((size_t)n);
         DeviceState ds = s->devState();
-        launch_export_aos(s->stream, d, (uint32_t)n, s->X.p, s->V.p, s->dens.p, s->lam.p, s->id.p,
-                          s->lid.p, s->W.p, ds.skeys, s->color.p, s->size.p);
-        hipError_t e = hipStreamSynchronize(s->stream);
-        if (e == hipSuccess) e = hipMemcpy(out, d, (size_t)n * sizeof(AosParticle), hipMemcpyDeviceToHost);
-        (void)hipFree(d);
-        HIP_CHECK(e);
+        launch_export_aos(s->stream, s->aosStage.p, (uint32_t)n, s->X.p, s->V.p, s->VS.p, s->DP.p,
+                          s->dens.p, s->lam.p, s->id.p, s->lid.p, s->W.p, ds.skeys, s->color.p,
+                          s->size.p);
+        HIP_CHECK(hipGetLastError());
+        HIP_CHECK(hipStreamSynchronize(s->stream));
+        HIP_CHECK(hipMemcpy(out, s->aosStage.p, (size_t)n * sizeof(AosParticle), hipMemcpyDeviceToHost));
     });
 }
 
@@ -1496,30 +1537,28 @@ akua_status akua_solver_step_device(akua_solver* s, void* dParticles, int32_t n,
         if (s->scenes.size() != 1) fail(AKUA_ERR_UNSUPPORTED, "record stepping is per single-scene solver");
         check_box(boxMin, boxMax);
         set_box(s, 0, boxMin, boxMax);
+        // NULL is the null stream: the solver's own stream does not synchronise with it, so the
+        // order is made explicit for every caller stream
         hipStream_t user = static_cast<hipStream_t>(stream);
-        hipEvent_t ev = nullptr;
-        if (user) {
-            // the solver's stream waits for the caller's work on the buffer
-            HIP_CHECK(hipEventCreateWithFlags(&ev, hipEventDisableTiming));
-            HIP_CHECK(hipEventRecord(ev, user));
-            HIP_CHECK(hipStreamWaitEvent(s->stream, ev, 0));
-        }
+        if (!s->orderEvent) HIP_CHECK(hipEventCreateWithFlags(&s->orderEvent, hipEventDisableTiming));
+        hipEvent_t ev = s->orderEvent;
+        // the solver's stream waits for the caller's earlier work on the buffer
+        HIP_CHECK(hipEventRecord(ev, user));
+        HIP_CHECK(hipStreamWaitEvent(s->stream, ev, 0));
         auto* buf = static_cast<AosParticle*>(dParticles);
-        launch_import_aos(s->stream, buf, (uint32_t)n, s->X.p, s->V.p, s->dens.p, s->lam.p, s->id.p,
-                          s->lid.p, s->phys.p, s->color.p, s->size.p);
+        launch_import_aos(s->stream, buf, (uint32_t)n, s->X.p, s->V.p, s->VS.p, s->DP.p, s->dens.p,
+                          s->lam.p, s->id.p, s->lid.p, s->phys.p, s->color.p, s->size.p);
         s->hasState = true;
         s->layoutPhase = 0;
         s->unitMass = false;  // the masses came from device memory: not known here
         update_fast(s);
         run_steps(s, dt, 1);
         DeviceState ds = s->devState();
-        launch_export_aos(s->stream, buf, (uint32_t)n, s->X.p, s->V.p, s->dens.p, s->lam.p, s->id.p,
-                          s->lid.p, s->W.p, ds.skeys, s->color.p, s->size.p);
-        if (user) {
-            HIP_CHECK(hipEventRecord(ev, s->stream));
-            HIP_CHECK(hipStreamWaitEvent(user, ev, 0));
-            HIP_CHECK(hipEventDestroy(ev));
-        }
+        launch_export_aos(s->stream, buf, (uint32_t)n, s->X.p, s->V.p, s->VS.p, s->DP.p, s->dens.p,
+                          s->lam.p, s->id.p, s->lid.p, s->W.p, ds.skeys, s->color.p, s->size.p);
+        // the caller's later work on its stream waits for the written-back records
+        HIP_CHECK(hipEventRecord(ev, s->stream));
+        HIP_CHECK(hipStreamWaitEvent(user, ev, 0));
         HIP_CHECK(hipGetLastError());
     });
 }

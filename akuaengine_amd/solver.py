@@ -101,7 +101,8 @@ class PBFSolver:
 
     def step_device(self, d_particles: int, deltaTime: float, boxMin, boxMax, stream: int = 0) -> None:
         """Steps a device buffer of numParticles 108-byte records (e.g. a torch uint8 tensor's
-        data_ptr()) on `stream` (0: the solver's own stream)."""
+        data_ptr()), ordered on `stream` (0: the null stream, torch's default stream): the step
+        waits for the work already on that stream, and later work on it waits for the step."""
         _lib.check(self._L.akua_solver_step_device(self._h, ctypes.c_void_p(d_particles),
                                                    self.numParticles, float(deltaTime),
                                                    _p(_f3(boxMin)), _p(_f3(boxMax)),

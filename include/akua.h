@@ -96,12 +96,15 @@ akua_status akua_solver_create(int32_t numParticles, const akua_pbf_config* cfg,
 akua_status akua_solver_destroy(akua_solver* s);
 
 /* Upload particles as the reference's AoS records (Renderer::uploadParticles,
- * Renderer.cpp:185-219).  Reads position, velocity, mass, density, lambda, color and size. */
+ * Renderer.cpp:185-219).  Reads position, velocity, new_velocity, position_delta, mass, density,
+ * lambda, color and size (new_velocity and position_delta are returned unchanged until a step
+ * rewrites them; a step without iterations keeps position_delta, as the reference's does). */
 akua_status akua_solver_set_particles(akua_solver* s, const akua_particle* particles, int32_t n);
 /* Download in the solver's current (sorted) order, as the reference's buffer is physically
  * reordered by each step (NeighbourSearchCUDA.cu:166-170).  Fills every field from the last
- * step: position, velocity, new_position (= position), new_velocity (= the velocity before
- * vorticity and XSPH), position_delta (last iteration), vorticity, mass, density, lambda, hash. */
+ * step: position, velocity, new_position (= position), new_velocity ((x* - x) / dt, the velocity
+ * before the wall damping, vorticity and XSPH, IntegrationCUDA.cu:45), position_delta (the last
+ * iteration's delta, ConstraintSolverCUDA.cu:129), vorticity, mass, density, lambda, hash. */
 akua_status akua_solver_get_particles(akua_solver* s, akua_particle* particles, int32_t n);
 
 /* Structure-of-arrays upload: pos/vel are 3*n floats (xyz interleaved per particle); mass,
@@ -123,9 +126,11 @@ akua_status akua_solver_step(akua_solver* s, float deltaTime, const float boxMin
 akua_status akua_solver_step_n(akua_solver* s, float deltaTime, const float boxMin[3],
                                const float boxMax[3], int32_t nsteps);
 /* The GL-interop replacement: the caller owns a device buffer of n akua_particle records (the
- * role of the mapped VBO, InteropResource.cpp:9-35).  The call imports it, runs one step on
- * `stream` (NULL: the solver's stream) and writes the records back in the new sorted order.
- * The buffer must stay valid until the stream has completed the step. */
+ * role of the mapped VBO, InteropResource.cpp:9-35).  The call imports it, runs one step and
+ * writes the records back in the new sorted order, ordered on the caller's `stream` (NULL: the
+ * null stream): the import waits for the work enqueued on `stream` before the call, and work
+ * enqueued on `stream` after the call waits for the written-back records.  The host does not
+ * wait.  The buffer must stay valid until that work has completed. */
 akua_status akua_solver_step_device(akua_solver* s, void* d_particles, int32_t n, float deltaTime,
                                     const float boxMin[3], const float boxMax[3], void* stream);
 /* PBFSolver::setGravity (PBFSolver.h:18, :29-31). */
@@ -233,7 +238,9 @@ akua_status akua_solver_set_learned(akua_solver* s, int32_t enable, int32_t iter
  * step; residual[3..5] the means. */
 akua_status akua_solver_learned_stats(akua_solver* s, int64_t* learnedSteps, int64_t* fallbacks,
                                       float residual[6]);
-/* Trains the solver's network on its own plain trajectory (single-scene solvers): `steps` plain
+/* Trains the solver's network on its own plain trajectory (single-scene solvers without
+ * boundary samples, AKUA_ERR_UNSUPPORTED otherwise; the learned-step settings of
+ * akua_solver_set_learned are left as they are): `steps` plain
  * steps (the state advances exactly as akua_solver_step would), each recording features and
  * targets of config->samplesPerStep random particles and running config->epochs passes of Adam
  * over them in minibatches.  A solver without a network first takes the feature and target

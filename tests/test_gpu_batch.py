@@ -68,3 +68,59 @@ def test_256_scene_workload_properties(solvers, dam_params):
         ids = st["ids"][off:off + n]
         assert ids.min() == off and ids.max() == off + n - 1 and len(np.unique(ids)) == n
         off += n
+
+
+def _oracle_state(O, s, cfg, corr, steps):
+    o = O.Oracle(s.n, cfg, corr, 32)
+    o.set_state(s.pos, s.vel)
+    o.step(DT, s.box_min, s.box_max, steps)
+    return o.state()
+
+
+def _assert_scene_equals_oracle(got, off, want, i):
+    sl = slice(off, off + len(want["ids"]))
+    for k in ("pos", "vel", "density", "lam"):
+        np.testing.assert_array_equal(got[k][sl], want[k].astype(np.float32), err_msg=f"scene {i} {k}")
+    np.testing.assert_array_equal(got["ids"][sl] - off, want["ids"], err_msg=f"scene {i} ids")
+
+
+def test_batch_scenes_equal_the_oracle(solvers, oracle_mod, dam_params):
+    """Every scene of a 7-scene batch (1 to 8,000 particles, sort-tile edges included) equals the
+    FP32 oracle stepping that scene alone, bit for bit."""
+    BatchSolver, _ = solvers
+    from akuaengine_amd import scenes
+    cfg, corr = dam_params
+    counts = [100, 4097, 2000, 513, 8000, 1, 777]
+    scs = [scenes.batched_scene(n, i, seed=5) for i, n in enumerate(counts)]
+    b = BatchSolver(counts, cfg, corr)
+    b.set_state(np.concatenate([s.pos for s in scs]), np.concatenate([s.vel for s in scs]))
+    for i, s in enumerate(scs):
+        b.set_box(i, s.box_min, s.box_max)
+    b.step_batch(DT, 4)
+    got = b.get_state()
+    off = 0
+    for i, s in enumerate(scs):
+        _assert_scene_equals_oracle(got, off, _oracle_state(oracle_mod, s, cfg, corr, 4), i)
+        off += s.n
+
+
+def test_256_scene_workload_sample_equals_the_oracle(solvers, oracle_mod, dam_params):
+    """The 256-scene workload at full size, two batched steps: the smallest, the largest and six
+    seeded random scenes each equal the FP32 oracle stepping that scene alone, bit for bit."""
+    BatchSolver, _ = solvers
+    from akuaengine_amd import scenes
+    cfg, corr = dam_params
+    scs = scenes.batched_scenes(256, seed=3)
+    counts = [s.n for s in scs]
+    b = BatchSolver(counts, cfg, corr)
+    b.set_state(np.concatenate([s.pos for s in scs]), np.concatenate([s.vel for s in scs]))
+    for i, s in enumerate(scs):
+        b.set_box(i, s.box_min, s.box_max)
+    b.step_batch(DT, 2)
+    got = b.get_state()
+    offs = np.concatenate([[0], np.cumsum(counts)[:-1]])
+    pick = {int(np.argmin(counts)), int(np.argmax(counts))}
+    pick |= set(int(i) for i in np.random.default_rng(17).choice(256, 6, replace=False))
+    for i in sorted(pick):
+        print(f"[batch 256] scene {i}: {counts[i]} particles", flush=True)
+        _assert_scene_equals_oracle(got, int(offs[i]), _oracle_state(oracle_mod, scs[i], cfg, corr, 2), i)

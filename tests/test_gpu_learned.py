@@ -252,3 +252,116 @@ def test_trained_network_gains_over_the_step_without_it(dev):
         res[name] = np.array(errs).mean(0)
     print("time-averaged density error (max, mean):", res)
     assert res["trained"][1] < 0.7 * res["zero"][1]
+
+
+def test_training_records_plain_steps_and_its_targets(dev, oracle_mod, dam_params):
+    """akua_solver_train: the recorded steps advance the state exactly as plain steps do; the
+    features and targets of the last recorded step equal the oracle's staged values (features
+    at x*_pred; targets (x*_{I-J} - x*_pred) / h and the lambda computed at x*_{I-J}); two runs
+    from the same state and seed give the same network, bit for bit; the learned-step iteration
+    count is left alone."""
+    learned, PBFSolver = dev
+    from akuaengine_amd import scenes
+    cfg, corr = dam_params
+    sc = scenes.dam_break_scaled(16, seed=6)
+    net0 = _random_net(np.random.default_rng(4))
+    steps, J = 3, 2
+    nets, states = [], []
+    for run in range(2):
+        s = PBFSolver(sc.n, cfg, corr)
+        s.set_state(sc.pos, sc.vel)
+        s.set_network(net0)
+        loss, resid = s.train(DT, sc.box_min, sc.box_max, steps, samplesPerStep=4096,
+                              minibatch=1024, epochs=2, seed=7, targetIterations=J)
+        assert np.isfinite(loss).all() and np.isfinite(resid).all()
+        nets.append(s.get_network())
+        states.append(s.get_state())
+        F, T = s.debug_features(targets=True)
+    np.testing.assert_array_equal(nets[0], nets[1])
+    assert np.abs(nets[0] - net0).max() > 0            # training moved the weights
+    plain = PBFSolver(sc.n, cfg, corr)
+    plain.set_state(sc.pos, sc.vel)
+    plain.step_n(DT, sc.box_min, sc.box_max, steps)
+    want = plain.get_state()
+    for k in ("pos", "vel", "density", "lam", "ids"):
+        np.testing.assert_array_equal(states[0][k], want[k], err_msg=k)
+    # the oracle: steps - 1 plain steps, then the last step's stages
+    o = oracle_mod.Oracle(sc.n, cfg, corr)
+    o.set_state(sc.pos, sc.vel)
+    o.step(DT, sc.box_min, sc.box_max, steps - 1)
+    for name in ("predict", "hashes", "sort", "neighbours", "features"):
+        o.stage(name, DT, sc.box_min, sc.box_max)
+    np.testing.assert_array_equal(F, o.features().astype(np.float32))
+    xpred = o.stage_arrays()["xs"].astype(np.float32)
+    I = cfg.solverIterations
+    for name in ("densities", "lambdas", "deltas", "correct") * (I - J):
+        o.stage(name, DT, sc.box_min, sc.box_max)
+    o.stage("densities", DT, sc.box_min, sc.box_max)
+    o.stage("lambdas", DT, sc.box_min, sc.box_max)
+    xlast = o.stage_arrays()["xs"].astype(np.float32)
+    inv_h = np.float32(1.0) / np.float32(cfg.smoothRadius)
+    np.testing.assert_array_equal(T[:, :3], (xlast - xpred) * inv_h)
+    np.testing.assert_array_equal(T[:, 3], o.state()["lam"].astype(np.float32))
+    # set_learned's iteration count survives training: after training with J = 2, a learned
+    # step still runs one iteration, as a fresh solver in the same state does
+    s.set_learned(True, ratio=1.0, tolerance=1e30, iterations=1)
+    s.train(DT, sc.box_min, sc.box_max, 1, samplesPerStep=4096, minibatch=1024, seed=8,
+            targetIterations=J)
+    st = s.get_state()
+    ref = PBFSolver(sc.n, cfg, corr)
+    ref.set_state(st["pos"], st["vel"], st["mass"], st["density"], st["lam"])
+    ref.set_network(s.get_network())
+    ref.set_learned(True, ratio=1.0, tolerance=1e30, iterations=1)
+    s.step(DT, sc.box_min, sc.box_max)
+    ref.step(DT, sc.box_min, sc.box_max)
+    a, b = s.get_state(), ref.get_state()
+    np.testing.assert_array_equal(a["pos"], b["pos"])
+    np.testing.assert_array_equal(st["ids"][b["ids"]], a["ids"])
+
+
+def test_learned_step_at_8m_matches_oracle(dev, oracle_mod, dam_params):
+    """The learned step on the 8M two-column scene with the committed J = 1 network: features
+    equal the oracle's bit for bit, x*_w = x*_pred + h * network(features), and the step the
+    oracle continues from x*_w equals the device's."""
+    learned, PBFSolver = dev
+    import os
+    from akuaengine_amd import scenes
+    print("[learned 8m] scene", flush=True)
+    cfg, corr = dam_params
+    sc = scenes.dam_break_8m()
+    net = learned.load_network(os.path.join(learned.NETWORK_DIR, "dam_break_1m_reference_J1.npy"))
+    s = PBFSolver(sc.n, cfg, corr)
+    s.set_state(sc.pos, sc.vel)
+    s.step(DT, sc.box_min, sc.box_max)
+    s.set_network(net)
+    s.set_learned(True, ratio=1.0, tolerance=1e30, iterations=1)   # never falls back
+    s.step(DT, sc.box_min, sc.box_max)
+    assert s.learned_stats()["fallbacks"] == 0
+    F = s.debug_features()
+    xp, xw, lam_hat = s.debug_warm_start()
+    got = s.get_state()
+    del s
+    print("[learned 8m] oracle plain step", flush=True)
+    o = oracle_mod.Oracle(sc.n, cfg, corr)
+    o.set_state(sc.pos, sc.vel)
+    o.step(DT, sc.box_min, sc.box_max)
+    for name in ("predict", "hashes", "sort", "neighbours", "features"):
+        print(f"[learned 8m] oracle {name}", flush=True)
+        o.stage(name, DT, sc.box_min, sc.box_max)
+    np.testing.assert_array_equal(F, o.features().astype(np.float32))
+    np.testing.assert_array_equal(xp, o.stage_arrays()["xs"].astype(np.float32))
+    y = learned.mlp_forward(net, F)
+    h = np.float32(cfg.smoothRadius)
+    ymul = net[M.OFF_YMUL:M.OFF_YMUL + 3]
+    yc = np.clip(y[:, :3] / ymul, -M.TARGET_CLIP, M.TARGET_CLIP).astype(np.float32) * ymul
+    np.testing.assert_array_equal(xw, xp + h * yc)
+    np.testing.assert_array_equal(lam_hat, y[:, 3])
+    o.set_xs(xw)
+    for name in ("densities", "lambdas", "deltas", "correct", "update", "damping", "vorticity",
+                 "confinement", "xsph"):
+        print(f"[learned 8m] oracle {name}", flush=True)
+        o.stage(name, DT, sc.box_min, sc.box_max)
+    want = o.state()
+    np.testing.assert_array_equal(got["ids"], want["ids"])
+    np.testing.assert_array_equal(got["pos"], want["pos"].astype(np.float32))
+    np.testing.assert_array_equal(got["vel"], want["vel"].astype(np.float32))
