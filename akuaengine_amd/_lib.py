@@ -33,6 +33,7 @@ EXPORTED = [
     "akua_selftest_arith", "akua_batch_create", "akua_batch_num_scenes", "akua_batch_set_box",
     "akua_batch_step", "akua_batch_density_error", "akua_solver_set_tuning",
     "akua_solver_get_tuning", "akua_solver_autotune", "akua_solver_set_network",
+    "akua_solver_set_search", "akua_solver_get_search", "akua_solver_set_exact",
     "akua_solver_get_network", "akua_solver_set_learned", "akua_solver_learned_stats",
     "akua_solver_train", "akua_debug_features", "akua_debug_warm_start", "akua_mlp_forward",
     "akua_mlp_gradient", "akua_mlp_adam_step", "akua_solver_set_boundary", "akua_debug_boundary",
@@ -77,7 +78,7 @@ class AkuaTuning(ctypes.Structure):
 
     _fields_ = [("block", ctypes.c_int32), ("layoutLevel", ctypes.c_int32),
                 ("hashDigitBits", ctypes.c_int32), ("layoutDigitBits", ctypes.c_int32),
-                ("layoutInterval", ctypes.c_int32)]
+                ("layoutInterval", ctypes.c_int32), ("balanceTile", ctypes.c_int32)]
 
     def as_dict(self):
         return {k: int(getattr(self, k)) for k, _ in self._fields_}
@@ -166,6 +167,9 @@ def load() -> ctypes.CDLL:
         "akua_batch_density_error": [P, P, P],
         "akua_solver_set_tuning": [P, ctypes.POINTER(AkuaTuning)],
         "akua_solver_get_tuning": [P, ctypes.POINTER(AkuaTuning)],
+        "akua_solver_set_search": [P, i32],
+        "akua_solver_get_search": [P, P],
+        "akua_solver_set_exact": [P, i32],
         "akua_solver_autotune": [P, f32, P, P, i32, ctypes.c_char_p, ctypes.POINTER(AkuaTuning)],
         "akua_solver_set_network": [P, P, i32],
         "akua_solver_get_network": [P, P, i32],

@@ -75,6 +75,12 @@ struct DeviceState {
     uint32_t tableTotal;
     uint4* nbr4;               // neighbour lists in four-entry chunks (see list_base)
     uint32_t* cnt;
+    // The neighbour sweeps' work order.  Work position t (the t-th thread of a sweep) handles
+    // physical slot work[t]; its count is cntW[t] and its list the chunks of position t in nbrW
+    // (same chunk layout as nbr4).  Without balancing, work is null, cntW = cnt, nbrW = nbr4.
+    const uint32_t* work;
+    const uint32_t* cntW;
+    const uint4* nbrW;
     unsigned long long* nbrSum;  // per-workgroup partial sums of neighbour counts (statistics)
     // learned warm start (null when unused)
     float* feat;               // 16 features per slot at x*_pred
@@ -118,6 +124,13 @@ void launch_gather(hipStream_t st, const DeviceState& d, const SphParams& P, con
 void launch_bucket_phys(hipStream_t st, const DeviceState& d);
 // bs: workgroup size of the neighbour kernels (128, 256 or 512; results do not depend on it).
 void launch_neighbours(hipStream_t st, const DeviceState& d, const SphParams& P, int bs);
+// The balanced work order of the neighbour sweeps: within each tile of `tile` physical slots
+// (a multiple of 256, at most kBalanceMaxTile), slots sorted by scene, then by neighbour count
+// (descending), so the 64 lanes of a wave walk lists of similar length; writes work, cntW and
+// the lists in work order (nbrW).  Each wave of work positions stays within one scene.
+constexpr uint32_t kBalanceMaxTile = 2048;
+void launch_balance(hipStream_t st, const DeviceState& d, const SphParams& P, uint32_t tile,
+                    uint32_t* work, uint32_t* cntW, uint4* nbrW);
 // Partial sums of the neighbour counts (blocks partials into part).
 void launch_count_sum(hipStream_t st, const DeviceState& d, unsigned long long* part, uint32_t blocks);
 // residSlot >= 0: also the per-scene max and sum of max(0, C) into check residSlot of d.resid.

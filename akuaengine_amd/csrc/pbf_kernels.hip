@@ -149,7 +149,7 @@ __global__ __launch_bounds__(kBlock) void k_predict_hash(DeviceState d, SphParam
     int cx, cy, cz;
     discretize(xs, P.cellHash, cx, cy, cz);
     const uint32_t k = d.lid[p];
-    d.keysL[k] = cell_hash(cx, cy, cz, sc);
+    d.keysL[k] = P.searchCells == 2 ? cell_key_fine(cx, cy, cz) : cell_hash(cx, cy, cz, sc);
     d.keysLay[k] = layout_key(cx, cy, cz, sc);
 }
 
@@ -375,6 +375,161 @@ __global__ __launch_bounds__(BS) void k_neighbours(DeviceState d, SphParams P) {
     }
 }
 
+// The balanced work order (launch_balance): one workgroup per tile of T physical slots.  The
+// slots are ranked by the key (scene rank within the tile, 254 - min(count, 254)), padding slots
+// past a scene's end last within their scene; a slot's work position is the tile base plus its
+// rank.  A scene owns whole 256-slot chunks, so its slots keep exactly the positions of its own
+// chunks and every wave of work positions belongs to one scene.  Equal keys are ranked by LDS
+// atomics, in no fixed order: the order only decides which lane handles which particle, never a
+// result.  Then each lane copies the list of one work position into nbrW, one 16-byte chunk per
+// four entries, so a wave writes each chunk row as one 1 KB block.
+constexpr uint32_t kBalanceThreads = 256;
+constexpr uint32_t kBalanceBins = (kBalanceMaxTile / 256) * 256;  // scene ranks x count classes
+
+__global__ __launch_bounds__(kBalanceThreads) void k_balance(DeviceState d, SphParams P, uint32_t T,
+                                                             uint32_t* __restrict__ work,
+                                                             uint32_t* __restrict__ cntW,
+                                                             uint4* __restrict__ nbrW) {
+    __shared__ uint32_t bins[kBalanceBins];
+    __shared__ uint32_t scan[kBalanceThreads];
+    __shared__ uint16_t keyOf[kBalanceMaxTile];
+    __shared__ uint16_t rankOf[kBalanceMaxTile];
+    __shared__ uint32_t srcOf[kBalanceMaxTile];
+    const uint32_t base = blockIdx.x * T;
+    if (base >= d.nslots) return;
+    const uint32_t n = min(T, d.nslots - base);
+    const uint32_t tid = threadIdx.x;
+    for (uint32_t b = tid; b < kBalanceBins; b += kBalanceThreads) bins[b] = 0u;
+    __syncthreads();
+    const uint32_t s0 = d.chunkScene[base >> 8];
+    for (uint32_t e = tid; e < n; e += kBalanceThreads) {
+        const uint32_t p = base + e;
+        const uint32_t sc = d.chunkScene[p >> 8];
+        const bool valid = p < d.scenes[sc].end;
+        const uint32_t cls = valid ? 254u - min(d.cnt[p], 254u) : 255u;
+        const uint32_t key = min(sc - s0, kBalanceMaxTile / 256 - 1) * 256u + cls;
+        keyOf[e] = (uint16_t)key;
+        rankOf[e] = (uint16_t)atomicAdd(&bins[key], 1u);
+    }
+    __syncthreads();
+    // exclusive scan of the bins: each thread sums a run of kBalanceBins / kBalanceThreads bins
+    constexpr uint32_t kRun = kBalanceBins / kBalanceThreads;
+    uint32_t run = 0;
+#pragma unroll
+    for (uint32_t k = 0; k < kRun; ++k) run += bins[tid * kRun + k];
+    scan[tid] = run;
+    __syncthreads();
+    for (uint32_t off = 1; off < kBalanceThreads; off <<= 1) {
+        const uint32_t v = tid >= off ? scan[tid - off] : 0u;
+        __syncthreads();
+        scan[tid] += v;
+        __syncthreads();
+    }
+    uint32_t acc = scan[tid] - run;
+#pragma unroll
+    for (uint32_t k = 0; k < kRun; ++k) {
+        const uint32_t c = bins[tid * kRun + k];
+        bins[tid * kRun + k] = acc;
+        acc += c;
+    }
+    __syncthreads();
+    for (uint32_t e = tid; e < n; e += kBalanceThreads) {
+        const uint32_t r = bins[keyOf[e]] + rankOf[e];
+        srcOf[r] = base + e;
+    }
+    __syncthreads();
+    const uint32_t chunks = list_chunks(P.maxN);
+    const uint32_t lane = tid & 63u, wave = tid >> 6;
+    for (uint32_t w = wave; w * 64u < n; w += kBalanceThreads / 64u) {
+        const uint32_t r = w * 64u + lane;
+        if (r >= n) continue;
+        const uint32_t t = base + r, p = srcOf[r];
+        const uint32_t c = d.cnt[p];
+        work[t] = p;
+        cntW[t] = c;
+        const uint4* src = d.nbr4 + list_base(p, P.maxN);
+        uint4* dst = nbrW + list_base(t, P.maxN);
+        const uint32_t nc = min((c + 3u) >> 2, chunks);
+        for (uint32_t k = 0; k < nc; ++k) dst[(size_t)k * 64u] = src[(size_t)k * 64u];
+    }
+}
+
+// Physical slot of work position t of a neighbour sweep (t itself without balancing).
+__device__ __forceinline__ uint32_t work_slot(const DeviceState& d, uint32_t t) {
+    return (d.work != nullptr && t < d.nslots) ? d.work[t] : t;
+}
+
+// The non-parity search (SphParams::searchCells == 2): cells of c = h / 2, keyed by their packed
+// coordinates (cell_key_fine), so every bucket is one cell and none is walked twice.  A cell at
+// offset (dx, dy, dz) from the particle's cell, |d| <= 2, can hold a particle within h only if its
+// box comes within h of x_i: with f the particle's position in its cell (cell units), the box lies
+// d - f away along an axis for d >= 1, f - d - 1 for d <= -1 and 0 for d = 0, and it is walked when
+// the squared distances sum below (h / c)^2 = 4 (with a margin of 0.25% for the rounding of f).
+// Walked cells go in (dx, dy, dz) order, each bucket in logical order, self skipped, r^2 < h^2
+// kept, at most maxNeighbours: the list holds every particle within h (up to the cap).
+template <int BS>
+__global__ __launch_bounds__(BS) void k_neighbours_fine(DeviceState d, SphParams P) {
+    const uint32_t p = xcd_slot<BS>();
+    uint32_t count = 0;
+    if (p < d.nslots) {
+        const SceneDev& sc = scene_of(d, p);
+        if (p < sc.end) {
+            const f3 xi = make3(d.XsA[p]);
+            const float qx = xi.x / P.cellSearch, qy = xi.y / P.cellSearch, qz = xi.z / P.cellSearch;
+            const int cx = to_int_sat(floorf(qx)), cy = to_int_sat(floorf(qy)), cz = to_int_sat(floorf(qz));
+            const float fx = qx - floorf(qx), fy = qy - floorf(qy), fz = qz - floorf(qz);
+            constexpr float kReach2 = 4.01f;
+            uint32_t* out = reinterpret_cast<uint32_t*>(d.nbr4 + list_base(p, P.maxN));
+            const uint32_t maxN = (uint32_t)P.maxN;
+            auto gap = [](int dd, float f) {
+                return dd > 0 ? (float)dd - f : dd < 0 ? f - (float)(dd + 1) : 0.0f;
+            };
+            for (int dx = -2; dx <= 2 && count < maxN; ++dx) {
+                const float gx = gap(dx, fx), gx2 = gx * gx;
+                if (gx2 >= kReach2) continue;
+                for (int dy = -2; dy <= 2 && count < maxN; ++dy) {
+                    const float gy = gap(dy, fy), gxy = gx2 + gy * gy;
+                    if (gxy >= kReach2) continue;
+                    for (int dz = -2; dz <= 2 && count < maxN; ++dz) {
+                        const float gz = gap(dz, fz);
+                        if (gxy + gz * gz >= kReach2) continue;
+                        const uint32_t h = cell_key_fine(cx + dx, cy + dy, cz + dz);
+                        uint4 e;
+                        if (!table_find(d.table, sc, h, e)) continue;
+                        const uint32_t a = e.x, b = e.y;
+                        const bool run = e.w != 0u;
+                        const int32_t shift = (int32_t)e.z - (int32_t)a;
+                        for (uint32_t m0 = a; m0 < b && count < maxN; m0 += 4) {
+                            float4 c[4];
+                            uint32_t ph[4];
+#pragma unroll
+                            for (int u = 0; u < 4; ++u) {
+                                const uint32_t m = min(m0 + u, b - 1u);
+                                if (run) {
+                                    ph[u] = (uint32_t)((int32_t)m + shift);
+                                    c[u] = d.XsA[ph[u]];
+                                } else {
+                                    c[u] = d.XsL[m];
+                                    ph[u] = d.physT[m];
+                                }
+                            }
+#pragma unroll
+                            for (int u = 0; u < 4; ++u) {
+                                const f3 sep = xi - make3(c[u]);
+                                if ((m0 + u < b) & (dot3(sep, sep) < P.h2) & (ph[u] != p) & (count < maxN)) {
+                                    out[((size_t)(count >> 2) * 64u) * 4u + (count & 3u)] = ph[u];
+                                    ++count;
+                                }
+                            }
+                        }
+                    }
+                }
+            }
+        }
+        d.cnt[p] = count;
+    }
+}
+
 // Sum of the neighbour counts of every particle slot (statistics, on demand): one partial per
 // workgroup, summed by the host.
 __global__ __launch_bounds__(256) void k_count_sum(const DeviceState d, unsigned long long* part) {
@@ -419,18 +574,19 @@ __device__ __forceinline__ void wave_resid_partial(const DeviceState& d, int slo
 template <int BS, bool kFast, bool kBnd>
 __device__ __forceinline__ void density_lambda_slot(const DeviceState& d, const SphParams& P,
                                                     const float4* __restrict__ Xs, int residSlot,
-                                                    int gated, uint32_t i) {
-    const uint32_t i0 = i & ~63u;
-    if (i0 >= d.nslots) return;  // the whole wave is past the last slot
-    const uint32_t sIdx = d.chunkScene[i0 >> 8];  // a wave never spans two scenes
+                                                    int gated, uint32_t t) {
+    const uint32_t t0 = t & ~63u;
+    if (t0 >= d.nslots) return;  // the whole wave is past the last slot
+    const uint32_t sIdx = d.chunkScene[t0 >> 8];  // a wave never spans two scenes
     if (gated && !d.gate[sIdx]) return;
-    const bool valid = i < d.scenes[sIdx].end;
+    const uint32_t i = work_slot(d, t);
+    const bool valid = t < d.nslots && i < d.scenes[sIdx].end;
     float C = 0.0f;
     if (valid) {
         const float4 xi4 = Xs[i];
         const f3 xi = make3(xi4);
-        const uint32_t n = d.cnt[i];
-        const uint4* lp = d.nbr4 + list_base(i, P.maxN);
+        const uint32_t n = d.cntW[t];
+        const uint4* lp = d.nbrW + list_base(t, P.maxN);
         float rho = xi4.w * P.w0;
         f3 gi{0.0f, 0.0f, 0.0f};
         float sum = 0.0f;
@@ -473,7 +629,7 @@ __device__ __forceinline__ void density_lambda_slot(const DeviceState& d, const 
         d.dens[i] = rho;
         d.lam[i] = -C / ((sum + dot3(gi, gi)) + P.relax);
     }
-    if (residSlot >= 0) wave_resid_partial(d, residSlot, i, valid ? positive_part(C) : 0.0f);
+    if (residSlot >= 0) wave_resid_partial(d, residSlot, t, valid ? positive_part(C) : 0.0f);
 }
 
 // One workgroup per virtual workgroup (the loop runs once); the loop form lets a launch use any
@@ -496,18 +652,19 @@ __global__ __launch_bounds__(BS) void k_density_lambda(DeviceState d, SphParams 
 //   13-15 wall proximity per axis, clamp((x - lo) / h, 0, 2) - clamp((hi - x) / h, 0, 2)
 template <int BS>
 __global__ __launch_bounds__(BS) void k_features(DeviceState d, SphParams P, int record) {
-    const uint32_t i = xcd_slot<BS>();
-    const uint32_t i0 = i & ~63u;
-    if (i0 >= d.nslots) return;
-    const uint32_t sIdx = d.chunkScene[i0 >> 8];
+    const uint32_t t = xcd_slot<BS>();
+    const uint32_t t0 = t & ~63u;
+    if (t0 >= d.nslots) return;
+    const uint32_t sIdx = d.chunkScene[t0 >> 8];
     const SceneDev& sc = d.scenes[sIdx];
-    const bool valid = i < sc.end;
+    const uint32_t i = work_slot(d, t);
+    const bool valid = t < d.nslots && i < sc.end;
     float C = 0.0f;
     if (valid) {
         const float4 xi4 = d.XsA[i];
         const f3 xi = make3(xi4);
-        const uint32_t n = d.cnt[i];
-        const uint4* lp = d.nbr4 + list_base(i, P.maxN);
+        const uint32_t n = d.cntW[t];
+        const uint4* lp = d.nbrW + list_base(t, P.maxN);
         float rho = xi4.w * P.w0;
         f3 gi{0.0f, 0.0f, 0.0f}, ac{0.0f, 0.0f, 0.0f};
         float sum = 0.0f;
@@ -551,7 +708,7 @@ __global__ __launch_bounds__(BS) void k_features(DeviceState d, SphParams P, int
         f[3] = make_float4(sa.z, wall[0], wall[1], wall[2]);
         if (record) d.XsP[i] = xi4;
     }
-    wave_resid_partial(d, 0, i, valid ? positive_part(C) : 0.0f);
+    wave_resid_partial(d, 0, t, valid ? positive_part(C) : 0.0f);
 }
 
 // Residual statistics per scene in two fixed-order stages (deterministic sums, no atomics).
@@ -717,8 +874,9 @@ __device__ __forceinline__ void delta_correct_slot(const DeviceState& d, const S
                                                    const float4* __restrict__ XsSet,
                                                    float4* __restrict__ XsOut,
                                                    const float4* __restrict__ XsAlt, int gateMode,
-                                                   uint32_t i) {
-    if (i >= d.nslots) return;
+                                                   uint32_t t) {
+    if (t >= d.nslots) return;
+    const uint32_t i = work_slot(d, t);
     const uint32_t sIdx = d.chunkScene[i >> 8];
     const SceneDev& sc = d.scenes[sIdx];
     if (i >= sc.end) return;
@@ -730,8 +888,8 @@ __device__ __forceinline__ void delta_correct_slot(const DeviceState& d, const S
     f3 dp{0.0f, 0.0f, 0.0f};
     if (kApplyDelta) {
         const float li = d.lam[i];
-        const uint32_t n = d.cnt[i];
-        const uint4* lp = d.nbr4 + list_base(i, P.maxN);
+        const uint32_t n = d.cntW[t];
+        const uint4* lp = d.nbrW + list_base(t, P.maxN);
         const Recip Rdq = recip(P.wdq);
         walk_list<XLam, kBatchDelta>(
             lp, n, [&](uint32_t j) { return XLam{XsIn[j], d.lam[j]}; },
@@ -803,13 +961,14 @@ __global__ __launch_bounds__(BS) void k_delta_correct(DeviceState d, SphParams P
 // kernel_compute_vorticities (IntegrationCUDA.cu:104-128).  W.w keeps |omega| for the next pass.
 template <int BS, bool kFast>
 __global__ __launch_bounds__(BS) void k_vorticity(DeviceState d, SphParams P) {
-    const uint32_t i = xcd_slot<BS>();
-    if (i >= d.nslots) return;
+    const uint32_t t = xcd_slot<BS>();
+    if (t >= d.nslots) return;
+    const uint32_t i = work_slot(d, t);
     if (i >= scene_of(d, i).end) return;
     const f3 xi = make3(d.X[i]);
     const f3 vi = make3(d.V[i]);
-    const uint32_t n = d.cnt[i];
-    const uint4* lp = d.nbr4 + list_base(i, P.maxN);
+    const uint32_t n = d.cntW[t];
+    const uint4* lp = d.nbrW + list_base(t, P.maxN);
     f3 w{0.0f, 0.0f, 0.0f};
     walk_list<XV, kBatchVort>(
         lp, n, [&](uint32_t j) { return XV{d.X[j], d.V[j]}; },
@@ -827,16 +986,17 @@ __global__ __launch_bounds__(BS) void k_vorticity(DeviceState d, SphParams P) {
 // forms per neighbour, IntegrationCUDA.cu:184, computed once per particle instead).
 template <int BS, bool kFast>
 __global__ __launch_bounds__(BS) void k_confinement(DeviceState d, SphParams P, StepParams S) {
-    const uint32_t i = xcd_slot<BS>();
-    if (i >= d.nslots) return;
+    const uint32_t t = xcd_slot<BS>();
+    if (t >= d.nslots) return;
+    const uint32_t i = work_slot(d, t);
     if (i >= scene_of(d, i).end) return;
     const float4 xi4 = d.X[i];
     const f3 xi = make3(xi4);
     const float xi4w = xi4.w;
     const float4 wi = d.W[i];
     const float invD = 1.0f / d.dens[i];
-    const uint32_t n = d.cnt[i];
-    const uint4* lp = d.nbr4 + list_base(i, P.maxN);
+    const uint32_t n = d.cntW[t];
+    const uint4* lp = d.nbrW + list_base(t, P.maxN);
     f3 eta{0.0f, 0.0f, 0.0f};
     walk_list<XW, kBatchConf>(
         lp, n, [&](uint32_t j) { return XW{d.X[j], d.W[j].w}; },
@@ -862,13 +1022,14 @@ __global__ __launch_bounds__(BS) void k_confinement(DeviceState d, SphParams P, 
 // kernel_apply_xsph_viscosity (IntegrationCUDA.cu:167-195), Jacobi: reads Vc, writes V.
 template <int BS>
 __global__ __launch_bounds__(BS) void k_xsph(DeviceState d, SphParams P) {
-    const uint32_t i = xcd_slot<BS>();
-    if (i >= d.nslots) return;
+    const uint32_t t = xcd_slot<BS>();
+    if (t >= d.nslots) return;
+    const uint32_t i = work_slot(d, t);
     if (i >= scene_of(d, i).end) return;
     const f3 xi = make3(d.X[i]);
     const f3 vi = make3(d.Vc[i]);
-    const uint32_t n = d.cnt[i];
-    const uint4* lp = d.nbr4 + list_base(i, P.maxN);
+    const uint32_t n = d.cntW[t];
+    const uint4* lp = d.nbrW + list_base(t, P.maxN);
     f3 dv{0.0f, 0.0f, 0.0f};
     walk_list<XV, kBatchXsph>(
         lp, n, [&](uint32_t j) { return XV{d.X[j], d.Vc[j]}; },
@@ -1000,7 +1161,10 @@ void launch_bucket_phys(hipStream_t st, const DeviceState& d) {
     k_bucket_phys<<<grid(d.nslots), kBlock, 0, st>>>(d);
 }
 void launch_neighbours(hipStream_t st, const DeviceState& d, const SphParams& P, int bs) {
-    AKUA_BY_BLOCK(bs, (k_neighbours<BS><<<grid_xcd(d.nslots, BS), BS, 0, st>>>(d, P)));
+    if (P.searchCells == 2)
+        AKUA_BY_BLOCK(bs, (k_neighbours_fine<BS><<<grid_xcd(d.nslots, BS), BS, 0, st>>>(d, P)));
+    else
+        AKUA_BY_BLOCK(bs, (k_neighbours<BS><<<grid_xcd(d.nslots, BS), BS, 0, st>>>(d, P)));
 }
 template <int BS>
 void density_lambda_bs(hipStream_t st, const DeviceState& d, const SphParams& P, const float4* Xs,
@@ -1021,6 +1185,12 @@ void density_lambda_bs(hipStream_t st, const DeviceState& d, const SphParams& P,
 void launch_density_lambda(hipStream_t st, const DeviceState& d, const SphParams& P,
                            const float4* Xs, int bs, int residSlot, bool gated) {
     AKUA_BY_BLOCK(bs, (density_lambda_bs<BS>(st, d, P, Xs, residSlot, gated)));
+}
+void launch_balance(hipStream_t st, const DeviceState& d, const SphParams& P, uint32_t tile,
+                    uint32_t* work, uint32_t* cntW, uint4* nbrW) {
+    if (!d.nslots) return;
+    const uint32_t blocks = (d.nslots + tile - 1) / tile;
+    k_balance<<<blocks, kBalanceThreads, 0, st>>>(d, P, tile, work, cntW, nbrW);
 }
 void launch_count_sum(hipStream_t st, const DeviceState& d, unsigned long long* part, uint32_t blocks) {
     k_count_sum<<<blocks, 256, 0, st>>>(d, part);

@@ -196,6 +196,13 @@ __device__ __forceinline__ uint32_t cell_hash(int cx, int cy, int cz, const Scen
     return fast_mod(a ^ b ^ c, sc.tableSize, sc.modMagic, sc.modShift);
 }
 
+// Key of a cell of the non-parity search over h / 2 cells: the low 10 bits of each coordinate,
+// packed (never kEmptyKey).  Cells less than 1024 apart along every axis get distinct keys, so no
+// two cells of one 5x5x5 stencil share a bucket.
+__device__ __forceinline__ uint32_t cell_key_fine(int cx, int cy, int cz) {
+    return (((uint32_t)cx & 1023u) << 20) | (((uint32_t)cy & 1023u) << 10) | ((uint32_t)cz & 1023u);
+}
+
 // kernel_predict_position (IntegrationCUDA.cu:27-36): v* = v + g dt; x* = x + v* dt.
 __device__ __forceinline__ f3 predict_x(f3 x, f3 v, const StepParams& S) {
     f3 g{S.g[0], S.g[1], S.g[2]};

@@ -56,6 +56,11 @@ struct SphParams {
     int32_t corrNInt;  // n as an integer in [0, 16], or -1 when n is not such an integer
     float cellHash;    // cell size used to hash (smoothRadius, NeighbourSearchCUDA.cu:163)
     float cellSearch;  // cell size used by the stencil search (spatialHashCellSize)
+    // 1: the reference's search (cells as above, the 27-cell stencil).  2: the non-parity search
+    // over cells of h / 2 (cellHash = cellSearch = h / 2) and the 5x5x5 cells that come within h
+    // of the particle; same neighbour set, another list order (results equal the reference's up
+    // to the rounding of the reordered sums).
+    int32_t searchCells;
     float vortEps;
     float visc;
     int32_t maxN;

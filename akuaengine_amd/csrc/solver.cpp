@@ -92,6 +92,7 @@ SphParams make_sph(const akua_pbf_config& c, const akua_lambda_corr& corr) {
     P.corrNInt = (corr.n >= 0.0f && corr.n <= 16.0f && corr.n == floorf(corr.n)) ? (int)corr.n : -1;
     P.cellHash = c.smoothRadius;
     P.cellSearch = c.spatialHashCellSize;
+    P.searchCells = 1;
     P.vortEps = c.vorticityEpsilon;
     P.visc = c.viscosity;
     P.maxN = c.maxNeighbours;
@@ -167,8 +168,10 @@ struct akua_solver {
     uint32_t nparticles = 0;
     uint32_t tableTotal = 0;
     SortPlan plan1;          // hash sort (logical order)
+    int hashBits = 1;        // bits of the reference's hashes (log2 of the largest table size)
     SortPlan plan2;          // super-cell sort (physical layout)
-    akua_tuning tuning{256, 1, 0, 0, 1};  // block, layout level, digit bits (0: balanced), interval
+    // block, layout level, digit bits (0: balanced), layout interval, balance tile
+    akua_tuning tuning{256, 1, 0, 0, 1, 0};
     int layoutPhase = 0;           // steps since the physical layout was last re-sorted (0: now)
     std::vector<SortTile> tilesHost;
     std::vector<SortScene> sortScenesHost;
@@ -183,6 +186,7 @@ struct akua_solver {
     DevBuf<uint32_t> id, Tid, lid, phys, lidT, physT, srcL, keysLay, bslot, nbr, cnt, perm2keep;
     DevBuf<uint32_t> s1KA, s1VA, s1KB, s1VB, s2KA, s2VA, s2KB, s2VB, counts, totals;
     DevBuf<uint4> table;
+    DevBuf<uint32_t> work, cntW, nbrW;  // the sweeps' balanced work order (allocated when used)
     DevBuf<SortTile> dTiles;
     DevBuf<SortScene> dSortScenes;
     DevBuf<unsigned long long> nbrSum;
@@ -190,6 +194,7 @@ struct akua_solver {
     DevBuf<unsigned int> probeMax;
     DevBuf<double> probeSum;
     bool hasState = false;
+    bool exact = true;             // the autotuner keeps the reference's search (akua_solver_set_exact)
     bool unitMass = false;         // every particle's mass is exactly 1 (known from the upload)
     bool boundedQuotients = false; // quotients_bounded(sph)
 
@@ -270,6 +275,11 @@ struct akua_solver {
         d.perm2keep = perm2keep.p;
         d.table = table.p; d.bslot = bslot.p; d.tableTotal = tableTotal;
         d.nbr4 = reinterpret_cast<uint4*>(nbr.p); d.cnt = cnt.p; d.nbrSum = nbrSum.p;
+        if (tuning.balanceTile > 0 && nbrW.p) {
+            d.work = work.p; d.cntW = cntW.p; d.nbrW = reinterpret_cast<const uint4*>(nbrW.p);
+        } else {
+            d.work = nullptr; d.cntW = cnt.p; d.nbrW = d.nbr4;
+        }
         d.feat = feat.p; d.targ = targ.p; d.XsP = XsP.p; d.XsW = XsW.p; d.lamHat = lamHat.p;
         d.resid = resid.p; d.gate = gate.p; d.fallbacks = fallbacks.p;
         d.residPart = residPart.p; d.nWaves = (nslots + 63) / 64;
@@ -298,7 +308,7 @@ struct akua_solver {
             b->release();
         for (auto* b : {&id, &Tid, &lid, &phys, &lidT, &physT, &srcL, &keysLay, &bslot, &nbr, &cnt,
                         &perm2keep, &s1KA, &s1VA, &s1KB, &s1VB, &s2KA, &s2VA, &s2KB, &s2VB, &counts,
-                        &totals, &gate, &fallbacks})
+                        &totals, &gate, &fallbacks, &work, &cntW, &nbrW})
             b->release();
         table.release(); dTiles.release(); dSortScenes.release(); nbrSum.release();
         probeMax.release(); probeSum.release(); renderKeys.release(); renderRgb.release();
@@ -395,7 +405,8 @@ void setup(akua_solver* s, const std::vector<uint32_t>& counts) {
     s->tableTotal = tab;
     s->nparticles = 0;
     for (auto c : counts) s->nparticles += c;
-    s->plan1 = SortPlan::balanced((int)ceil_log2((uint64_t)maxTable));
+    s->hashBits = (int)ceil_log2((uint64_t)maxTable);
+    s->plan1 = SortPlan::balanced(s->hashBits);
     s->plan2 = SortPlan::balanced(3);
     s->numTiles = build_sort_tiles(sb, se, s->tilesHost, s->sortScenesHost);
 
@@ -573,6 +584,12 @@ void enqueue_step(akua_solver* s, hipStream_t st, const StepParams& S, bool time
     L.before("neighbours");
     launch_neighbours(st, d, P, bs);
     L.after("neighbours");
+    if (d.work) {
+        L.before("balance");
+        launch_balance(st, d, P, (uint32_t)s->tuning.balanceTile, s->work.p, s->cntW.p,
+                       reinterpret_cast<uint4*>(s->nbrW.p));
+        L.after("balance");
+    }
     const int I = s->cfg.solverIterations;
     const bool learned = s->mode == 1 && !record;
     const int J = learned ? std::min(s->learnedIters, I) : 0;
@@ -682,6 +699,20 @@ void update_layout(akua_solver* s) {
     s->scenesDirty = true;
 }
 
+// Neighbour search cells per smoothing radius (SphParams::searchCells): 1 the reference's, 2 the
+// non-parity search over h / 2 cells.  The hash cell size follows; the step graphs re-capture.
+void apply_search(akua_solver* s, int cells) {
+    if (s->sph.searchCells == cells) return;
+    const float h = s->cfg.smoothRadius;
+    s->sph.searchCells = cells;
+    s->sph.cellHash = cells == 2 ? h * 0.5f : h;
+    s->sph.cellSearch = cells == 2 ? h * 0.5f : s->cfg.spatialHashCellSize;
+    // the packed cell keys of the h / 2 search take 30 bits
+    s->plan1 = make_plan(cells == 2 ? 30 : s->hashBits, s->tuning.hashDigitBits);
+    s->graphDirty = true;
+    update_layout(s);
+}
+
 void set_box(akua_solver* s, size_t scene, const float* bmin, const float* bmax) {
     SceneDev& sc = s->scenes[scene];
     s->boxSeen = true;
@@ -726,11 +757,23 @@ void ensure_boundary(akua_solver* s) {
     s->graphDirty = true;
 }
 
+// Buffers of the balanced work order, allocated the first time a setting uses them (before any
+// capture: a graph records no allocation).
+void ensure_balance(akua_solver* s) {
+    if (s->tuning.balanceTile <= 0 || s->nbrW.p) return;
+    HIP_CHECK(hipStreamSynchronize(s->stream));
+    s->work.alloc(s->nslots);
+    s->cntW.alloc(s->nslots);
+    s->nbrW.alloc(s->nbr.n);
+    s->graphDirty = true;
+}
+
 void run_steps(akua_solver* s, float dt, int nsteps) {
     if (!s->hasState) fail(AKUA_ERR_STATE, "set the particles before stepping");
     if (!(dt > 0.0f) || !std::isfinite(dt)) fail(AKUA_ERR_INVALID_ARG, "deltaTime must be > 0");
     upload_scenes(s);
     ensure_boundary(s);
+    ensure_balance(s);
     StepParams S{dt, {s->cfg.gravity[0], s->cfg.gravity[1], s->cfg.gravity[2]}};
     if (s->graphDirty) {
         for (auto& g : s->graphs) g.dirty = true;
@@ -784,16 +827,21 @@ void run_density_probe(akua_solver* s) {
     const StepParams S0{0.0f, {0.0f, 0.0f, 0.0f}};
     hipStream_t st = s->stream;
     DeviceState d = s->devState();
+    // the probe's own walk uses cells of h whatever search the steps use
+    SphParams P0 = s->sph;
+    P0.cellHash = s->cfg.smoothRadius;
+    P0.cellSearch = s->cfg.smoothRadius;
+    P0.searchCells = 1;
     HIP_CHECK(hipMemsetAsync(s->probeMax.p, 0, s->scenes.size() * sizeof(unsigned int), st));
     HIP_CHECK(hipMemsetAsync(s->probeSum.p, 0, s->scenes.size() * sizeof(double), st));
     launch_clear_table(st, s->table.p, s->tableTotal);
-    launch_predict_hash(st, d, s->sph, S0);
+    launch_predict_hash(st, d, P0, S0);
     sort_hash(s, st, false);
-    launch_cell_keys(st, d, s->sph, S0, false);
+    launch_cell_keys(st, d, P0, S0, false);
     sort_layout(s, st, false);
-    launch_gather(st, d, s->sph, S0, false, false);
+    launch_gather(st, d, P0, S0, false, false);
     launch_bucket_phys(st, d);
-    launch_density_probe(st, d, s->sph, s->probeRho.p, s->probeMax.p, s->probeSum.p);
+    launch_density_probe(st, d, P0, s->probeRho.p, s->probeMax.p, s->probeSum.p);
     HIP_CHECK(hipGetLastError());
 }
 
@@ -886,6 +934,9 @@ void validate_tuning(const akua_tuning& t) {
             fail(AKUA_ERR_INVALID_ARG, "tuning digit widths must be 0 or 5..8");
     if (t.layoutInterval < 1 || t.layoutInterval > 8)
         fail(AKUA_ERR_INVALID_ARG, "tuning.layoutInterval must be in [1, 8]");
+    if (t.balanceTile != 0 && (t.balanceTile < 256 || t.balanceTile > (int)kBalanceMaxTile ||
+                               t.balanceTile % 256 != 0))
+        fail(AKUA_ERR_INVALID_ARG, "tuning.balanceTile must be 0 or a multiple of 256 in [256, 2048]");
 }
 
 void apply_tuning(akua_solver* s, const akua_tuning& t) {
@@ -960,12 +1011,13 @@ std::string tuning_key(akua_solver* s) {
     std::ostringstream k;
     k << arch << " n2^" << ceil_log2(std::max<uint64_t>(1, s->nparticles)) << " scenes2^"
       << ceil_log2(std::max<uint64_t>(1, s->scenes.size())) << " cap" << s->cfg.maxNeighbours
-      << " it" << s->cfg.solverIterations;
+      << " it" << s->cfg.solverIterations << (s->exact ? "" : " inexact");
     return k.str();
 }
 
-// Cache lines: "<key> | block level hashBits layoutBits".  The last line for a key wins.
-bool cache_lookup(const char* path, const std::string& key, akua_tuning& t) {
+// Cache lines: "<key> | block level hashBits layoutBits interval balanceTile searchCells".  The
+// last line for a key wins.
+bool cache_lookup(const char* path, const std::string& key, akua_tuning& t, int& cells) {
     std::ifstream f(path);
     if (!f) return false;
     std::string line;
@@ -975,24 +1027,28 @@ bool cache_lookup(const char* path, const std::string& key, akua_tuning& t) {
         if (bar == std::string::npos || line.compare(0, bar, key) != 0 || bar != key.size()) continue;
         std::istringstream v(line.substr(bar + 3));
         akua_tuning c{};
-        if (!(v >> c.block >> c.layoutLevel >> c.hashDigitBits >> c.layoutDigitBits >> c.layoutInterval))
+        int sc = 1;
+        if (!(v >> c.block >> c.layoutLevel >> c.hashDigitBits >> c.layoutDigitBits >> c.layoutInterval >>
+              c.balanceTile >> sc))
             continue;
         try {
             validate_tuning(c);
         } catch (const AkuaError&) {
             continue;
         }
+        if (sc != 1 && sc != 2) continue;
         t = c;
+        cells = sc;
         found = true;
     }
     return found;
 }
 
-void cache_store(const char* path, const std::string& key, const akua_tuning& t) {
+void cache_store(const char* path, const std::string& key, const akua_tuning& t, int cells) {
     std::ofstream f(path, std::ios::app);
     if (!f) fail(AKUA_ERR_INVALID_ARG, std::string("cannot append to the tuning cache ") + path);
     f << key << " | " << t.block << ' ' << t.layoutLevel << ' ' << t.hashDigitBits << ' '
-      << t.layoutDigitBits << ' ' << t.layoutInterval << '\n';
+      << t.layoutDigitBits << ' ' << t.layoutInterval << ' ' << t.balanceTile << ' ' << cells << '\n';
 }
 
 // ---- learned warm start ----------------------------------------------------------------------
@@ -1168,6 +1224,7 @@ void train(akua_solver* s, float dt, const float* bmin, const float* bmax, int s
     const uint32_t nb = B / M;
     const uint32_t perStep = nb * (uint32_t)c.epochs;
     ensure_train(s, B, perStep * (uint32_t)steps);
+    ensure_balance(s);
     HIP_CHECK(hipMemset(s->lossBuf.p, 0, (size_t)perStep * steps * sizeof(float)));
     hipStream_t st = s->stream;
     const StepParams S{dt, {s->cfg.gravity[0], s->cfg.gravity[1], s->cfg.gravity[2]}};
@@ -1258,14 +1315,17 @@ void autotune(akua_solver* s, float dt, const float* bmin, const float* bmax, in
     if (cachePath) {
         key = tuning_key(s);
         akua_tuning c{};
-        if (cache_lookup(cachePath, key, c)) {
+        int cells = 1;
+        if (cache_lookup(cachePath, key, c, cells)) {
             apply_tuning(s, c);
+            if (!s->exact) apply_search(s, cells);
             return;
         }
     }
     const bool timing = s->timing;
     const bool graph = s->useGraph;
     const akua_tuning before = s->tuning;
+    const int cellsBefore = s->sph.searchCells;
     Snapshot q;
     take_snapshot(s, q);
     s->timing = false;
@@ -1276,19 +1336,21 @@ void autotune(akua_solver* s, float dt, const float* bmin, const float* bmax, in
         const Snapshot& q;
         bool timing, graph;
         akua_tuning tuning;
+        int cells;
         bool armed = true;
         ~Restore() {
             if (!armed) return;
             try {
                 s->timing = timing;
                 s->useGraph = graph;
+                apply_search(s, cells);
                 apply_tuning(s, tuning);
                 restore_snapshot(s, q);
                 (void)hipStreamSynchronize(s->stream);
             } catch (...) {
             }
         }
-    } guard{s, q, timing, graph, before};
+    } guard{s, q, timing, graph, before, cellsBefore};
     akua_tuning best = s->tuning;
     apply_tuning(s, best);
     double bestMs = time_trial(s, q, dt, trialSteps);
@@ -1310,13 +1372,29 @@ void autotune(akua_solver* s, float dt, const float* bmin, const float* bmax, in
     sweep(&akua_tuning::hashDigitBits, {0, 5, 6, 8});
     sweep(&akua_tuning::layoutDigitBits, {0, 5, 8});
     sweep(&akua_tuning::layoutInterval, {1, 2, 4});
+    sweep(&akua_tuning::balanceTile, {0, 512, 1024, 2048});
+    // the non-parity search over h / 2 cells, only when the caller gave up bit-exactness
+    int bestCells = s->sph.searchCells;
+    if (!s->exact) {
+        apply_tuning(s, best);
+        for (int cells : {1, 2}) {
+            if (cells == bestCells) continue;
+            apply_search(s, cells);
+            const double ms = time_trial(s, q, dt, trialSteps);
+            if (ms < bestMs) {
+                bestMs = ms;
+                bestCells = cells;
+            }
+        }
+        apply_search(s, bestCells);
+    }
     apply_tuning(s, best);
     restore_snapshot(s, q);
     HIP_CHECK(hipStreamSynchronize(s->stream));
     guard.armed = false;
     s->timing = timing;
     s->useGraph = graph;
-    if (cachePath) cache_store(cachePath, key, best);
+    if (cachePath) cache_store(cachePath, key, best, bestCells);
 }
 
 }  // namespace
@@ -1597,6 +1675,28 @@ akua_status akua_solver_set_tuning(akua_solver* s, const akua_tuning* t) {
     return guarded([&] {
         if (!s || !t) fail(AKUA_ERR_INVALID_ARG, "NULL argument");
         apply_tuning(s, *t);
+    });
+}
+
+akua_status akua_solver_set_search(akua_solver* s, int32_t cellsPerH) {
+    return guarded([&] {
+        if (!s) fail(AKUA_ERR_INVALID_ARG, "NULL solver");
+        if (cellsPerH != 1 && cellsPerH != 2) fail(AKUA_ERR_INVALID_ARG, "cellsPerH must be 1 or 2");
+        apply_search(s, cellsPerH);
+    });
+}
+
+akua_status akua_solver_get_search(const akua_solver* s, int32_t* cellsPerH) {
+    return guarded([&] {
+        if (!s || !cellsPerH) fail(AKUA_ERR_INVALID_ARG, "NULL argument");
+        *cellsPerH = s->sph.searchCells;
+    });
+}
+
+akua_status akua_solver_set_exact(akua_solver* s, int32_t exact) {
+    return guarded([&] {
+        if (!s) fail(AKUA_ERR_INVALID_ARG, "NULL solver");
+        s->exact = exact != 0;
     });
 }
 

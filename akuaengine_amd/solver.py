@@ -120,9 +120,24 @@ class PBFSolver:
 
     # -- tuning -------------------------------------------------------------------------------
     def set_tuning(self, block: int = 256, layoutLevel: int = 1, hashDigitBits: int = 0,
-                   layoutDigitBits: int = 0, layoutInterval: int = 1) -> None:
-        t = _lib.AkuaTuning(block, layoutLevel, hashDigitBits, layoutDigitBits, layoutInterval)
+                   layoutDigitBits: int = 0, layoutInterval: int = 1, balanceTile: int = 0) -> None:
+        t = _lib.AkuaTuning(block, layoutLevel, hashDigitBits, layoutDigitBits, layoutInterval,
+                            balanceTile)
         _lib.check(self._L.akua_solver_set_tuning(self._h, ctypes.byref(t)))
+
+    def set_search(self, cellsPerH: int) -> None:
+        """1: the reference's neighbour search (bit-exact); 2: cells of h / 2 (results equal to
+        FP32 rounding, not bit for bit)."""
+        _lib.check(self._L.akua_solver_set_search(self._h, int(cellsPerH)))
+
+    def get_search(self) -> int:
+        c = ctypes.c_int32()
+        _lib.check(self._L.akua_solver_get_search(self._h, ctypes.byref(c)))
+        return int(c.value)
+
+    def set_exact(self, exact: bool) -> None:
+        """False lets autotune() also try the h / 2 search (which is not bit-exact)."""
+        _lib.check(self._L.akua_solver_set_exact(self._h, 1 if exact else 0))
 
     def get_tuning(self) -> Dict[str, int]:
         t = _lib.AkuaTuning()

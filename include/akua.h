@@ -152,12 +152,35 @@ typedef struct akua_tuning {
     int32_t layoutDigitBits;  /* radix digit width of the layout sort: 0 (balanced) or 5..8 */
     int32_t layoutInterval;   /* the physical layout is re-sorted every layoutInterval steps (1..8);
                                  in between, particles keep their memory slots */
+    int32_t balanceTile;      /* 0 (the default): the neighbour sweeps run one thread per physical
+                                 slot; 256..2048 (a multiple of 256): in a per-step work order, the
+                                 slots of each tile of this many sorted by neighbour count, so that
+                                 the lanes of a wave walk lists of similar length (their gathers
+                                 then spread over the tile) */
 } akua_tuning;
 
 akua_status akua_solver_set_tuning(akua_solver* s, const akua_tuning* t);
+/* Neighbour search cells per smoothing radius h.  1 (the default) is the reference's search
+ * (NeighbourSearchCUDA.cu:72-130, :163, :178): hash cells of smoothRadius, the 27-cell stencil of
+ * spatialHashCellSize cells; every result is the reference's, bit for bit.  2 is a search that
+ * does not keep bit-exactness: cells of h / 2 keyed by their packed coordinates (no hash
+ * collisions), walking the cells of the 5x5x5 block around the particle's cell whose boxes come
+ * within h of it (about 15.6 h^3 of candidate volume instead of 27 h^3).  Its lists hold exactly
+ * the particles within h, in another order; the reference's lists also repeat the particles of a
+ * bucket that two stencil cells hash to (NeighbourSearchCUDA.cu:95-127), which this search does
+ * not.  Every neighbour sum is therefore rounded in another order (and a capped list may keep
+ * other neighbours), so results equal the reference's only to rounding where no list of the
+ * reference repeats a neighbour.  The exact density probe (akua_solver_density_error) keeps cells
+ * of h either way.  No reference counterpart. */
+akua_status akua_solver_set_search(akua_solver* s, int32_t cellsPerH);
+akua_status akua_solver_get_search(const akua_solver* s, int32_t* cellsPerH);
+/* exact 1 (the default): akua_solver_autotune keeps the search as it is.  0: the caller gives up
+ * bit-exactness, and the autotuner also times the search over h / 2 cells and keeps the faster
+ * (cache entries of such tunings are keyed apart). */
+akua_status akua_solver_set_exact(akua_solver* s, int32_t exact);
 akua_status akua_solver_get_tuning(const akua_solver* s, akua_tuning* t);
 /* Times short trials of candidate settings by coordinate descent (block, then layout level, then
- * the two digit widths, then the layout interval) and keeps the fastest.  Each trial restarts from a copy of the current
+ * the two digit widths, then the layout interval, then the balance tile) and keeps the fastest.  Each trial restarts from a copy of the current
  * particle state, which is restored afterwards, so the call leaves the state as it found it.
  * boxMin/boxMax: the box the trials step in (NULL: each scene's current box).  trialSteps: timed
  * steps per candidate (1..1000).  cachePath (NULL: no cache): a text file of earlier results keyed

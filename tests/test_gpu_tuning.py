@@ -1,16 +1,23 @@
 """Tuning choices (workgroup size of the neighbour kernels, physical super-cell size, radix digit
-widths) change only speed: every setting yields bit-identical trajectories, and the autotuner
-leaves the particle state as it found it."""
+widths, layout interval, the balanced work order of the neighbour sweeps) change only speed: every
+setting yields bit-identical trajectories, and the autotuner leaves the particle state as it found
+it."""
 import numpy as np
 import pytest
 
 pytestmark = pytest.mark.gpu
 DT = 0.0083
 SETTINGS = [
-    dict(block=256, layoutLevel=1, hashDigitBits=0, layoutDigitBits=0, layoutInterval=1),
-    dict(block=128, layoutLevel=0, hashDigitBits=5, layoutDigitBits=5, layoutInterval=3),
-    dict(block=512, layoutLevel=2, hashDigitBits=8, layoutDigitBits=8, layoutInterval=2),
-    dict(block=256, layoutLevel=3, hashDigitBits=6, layoutDigitBits=0, layoutInterval=8),
+    dict(block=256, layoutLevel=1, hashDigitBits=0, layoutDigitBits=0, layoutInterval=1,
+         balanceTile=0),
+    dict(block=128, layoutLevel=0, hashDigitBits=5, layoutDigitBits=5, layoutInterval=3,
+         balanceTile=1024),
+    dict(block=512, layoutLevel=2, hashDigitBits=8, layoutDigitBits=8, layoutInterval=2,
+         balanceTile=256),
+    dict(block=256, layoutLevel=3, hashDigitBits=6, layoutDigitBits=0, layoutInterval=8,
+         balanceTile=2048),
+    dict(block=128, layoutLevel=1, hashDigitBits=0, layoutDigitBits=0, layoutInterval=1,
+         balanceTile=768),
 ]
 
 
@@ -32,7 +39,7 @@ def _run(PBFSolver, sc, cfg, corr, steps, tuning=None):
     return s
 
 
-@pytest.mark.parametrize("tuning", SETTINGS[1:], ids=lambda t: "b{block}-L{layoutLevel}-h{hashDigitBits}-l{layoutDigitBits}-i{layoutInterval}".format(**t))
+@pytest.mark.parametrize("tuning", SETTINGS[1:], ids=lambda t: "b{block}-L{layoutLevel}-h{hashDigitBits}-l{layoutDigitBits}-i{layoutInterval}-t{balanceTile}".format(**t))
 def test_every_setting_gives_identical_trajectories(classes, dam_params, tuning):
     _, PBFSolver = classes
     from akuaengine_amd import scenes
@@ -51,10 +58,11 @@ def test_setting_changes_between_steps(classes, dam_params):
     from akuaengine_amd import scenes
     cfg, corr = dam_params
     sc = scenes.dam_break_scaled(12)
-    want = _run(PBFSolver, sc, cfg, corr, 6).get_state()
+    seq = SETTINGS + SETTINGS[:2]
+    want = _run(PBFSolver, sc, cfg, corr, len(seq)).get_state()
     s = PBFSolver(sc.n, cfg, corr)
     s.set_state(sc.pos, sc.vel)
-    for t in SETTINGS + SETTINGS[:2]:
+    for t in seq:
         s.set_tuning(**t)
         s.step(DT, sc.box_min, sc.box_max)
     got = s.get_state()
@@ -68,7 +76,8 @@ def test_invalid_settings_are_rejected(classes, dam_params):
     cfg, corr = dam_params
     s = PBFSolver(1000, cfg, corr)
     for bad in (dict(block=64), dict(layoutLevel=4), dict(hashDigitBits=4), dict(layoutDigitBits=9),
-                dict(layoutInterval=0), dict(layoutInterval=9)):
+                dict(layoutInterval=0), dict(layoutInterval=9), dict(balanceTile=100),
+                dict(balanceTile=4096), dict(balanceTile=-256)):
         with pytest.raises(AkuaError):
             s.set_tuning(**bad)
     assert s.get_tuning() == SETTINGS[0]
@@ -106,7 +115,7 @@ def test_batch_settings_are_invariant(classes, dam_params):
     counts = [300, 4097, 2000, 9000]
     scs = [scenes.batched_scene(n, i, seed=9) for i, n in enumerate(counts)]
     out = []
-    for t in (SETTINGS[0], SETTINGS[2], SETTINGS[1]):
+    for t in (SETTINGS[1], SETTINGS[0], SETTINGS[2], SETTINGS[3], SETTINGS[4]):
         b = BatchSolver(counts, cfg, corr)
         b.set_tuning(**t)
         b.set_state(np.concatenate([s.pos for s in scs]), np.concatenate([s.vel for s in scs]))
