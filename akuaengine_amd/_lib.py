@@ -36,7 +36,8 @@ EXPORTED = [
     "akua_solver_set_search", "akua_solver_get_search", "akua_solver_set_exact",
     "akua_solver_get_network", "akua_solver_set_learned", "akua_solver_learned_stats",
     "akua_solver_train", "akua_debug_features", "akua_debug_warm_start", "akua_mlp_forward",
-    "akua_mlp_gradient", "akua_mlp_adam_step", "akua_solver_set_boundary", "akua_debug_boundary",
+    "akua_mlp_gradient", "akua_mlp_adam_step", "akua_mlp_adam_step_device",
+    "akua_solver_set_boundary", "akua_debug_boundary",
     "akua_boundary_samples",
 ]
 
@@ -181,6 +182,7 @@ def load() -> ctypes.CDLL:
         "akua_mlp_forward": [P, P, i32, P],
         "akua_mlp_gradient": [P, P, P, i32, P, P, P],
         "akua_mlp_adam_step": [P, P, P, ctypes.c_int64, f32, P, P, i32, P, P],
+        "akua_mlp_adam_step_device": [P, P, P, ctypes.c_int64, f32, P, P, i32, P, P, P],
         "akua_solver_set_boundary": [P, i32, f32],
         "akua_debug_boundary": [P, P, P, i32, ctypes.POINTER(i32)],
         "akua_boundary_samples": [ctypes.POINTER(CPBFConfig), P, P, i32, f32, P, P, i32, ctypes.POINTER(i32)],

@@ -21,8 +21,9 @@ LIB = os.path.join(HERE, "libakua.so")
 OBJ = os.path.join(HERE, "_build")
 
 ARCH = os.environ.get("AKUA_OFFLOAD_ARCH", "gfx950")
-SOURCES = ["radix_sort.hip", "pbf_kernels.hip", "mlp.hip", "aos.hip", "stream_copy.hip", "selftest.hip",
-           "solver.cpp", "boundary.cpp", "scene_io.cpp", "render.hip"]
+SOURCES = ["radix_sort.hip", "pbf_kernels.hip", "mlp.hip", "mlp_noalloc.hip", "aos.hip",
+           "stream_copy.hip", "selftest.hip", "solver.cpp", "boundary.cpp", "scene_io.cpp",
+           "render.hip"]
 RUNNER = os.path.join(HERE, "akua_run")
 
 HIPFLAGS = [

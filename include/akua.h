@@ -292,6 +292,18 @@ akua_status akua_mlp_gradient(const float* params, const float* features, const 
 akua_status akua_mlp_adam_step(float* params, float* m, float* v, int64_t t, float learningRate,
                                const float* features, const float* targets, int32_t n,
                                const float lossWeights[4], float* loss);
+/* akua_mlp_adam_step on device memory, allocating nothing.  No counterpart in the reference: it is
+ * for callers that cannot allocate device memory (code under HIP graph capture, tight training or
+ * serving loops).  params, m, v, features, targets (n rows) and loss are device pointers, updated
+ * as akua_mlp_adam_step updates them; lossWeights is host memory read during the call.  The work
+ * is enqueued on `stream` (NULL: the null stream) and the call returns without waiting.  The
+ * gradient partials live in one fixed array of the library per device, 256 x 5509 floats
+ * (5.64 MB, held until the process ends), whatever n is, so no n is refused for its size; every
+ * call rewrites all of it, and calls that overlap on one device race on it. */
+akua_status akua_mlp_adam_step_device(float* params, float* m, float* v, int64_t t,
+                                      float learningRate, const float* features,
+                                      const float* targets, int32_t n, const float lossWeights[4],
+                                      float* loss, void* stream);
 
 /* ---- batched scenes ---------------------------------------------------------------------------- */
 /* Many independent scenes in one solver: every kernel of a step covers all scenes in one launch
