@@ -123,7 +123,10 @@ void launch_gather(hipStream_t st, const DeviceState& d, const SphParams& P, con
 // Physical start and contiguity of every bucket (after the gather).
 void launch_bucket_phys(hipStream_t st, const DeviceState& d);
 // bs: workgroup size of the neighbour kernels (128, 256 or 512; results do not depend on it).
-void launch_neighbours(hipStream_t st, const DeviceState& d, const SphParams& P, int bs);
+// slabBatch: 0 walks each stencil bucket in batches of eight candidates; 4 or 8 walks each dx slab
+// of nine buckets as one stream in batches of that many (identical lists).
+void launch_neighbours(hipStream_t st, const DeviceState& d, const SphParams& P, int bs,
+                       int slabBatch);
 // The balanced work order of the neighbour sweeps: within each tile of `tile` physical slots
 // (a multiple of 256, at most kBalanceMaxTile), slots sorted by scene, then by neighbour count
 // (descending), so the 64 lanes of a wave walk lists of similar length; writes work, cntW and

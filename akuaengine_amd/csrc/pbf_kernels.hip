@@ -459,6 +459,123 @@ __device__ __forceinline__ uint32_t work_slot(const DeviceState& d, uint32_t t) 
     return (d.work != nullptr && t < d.nslots) ? d.work[t] : t;
 }
 
+// The list build walking each dx slab of the stencil (its nine cells in (dy, dz) order) as one
+// stream of candidates.  The nine buckets' entries (first slot, length, contiguity) sit in a
+// register queue; a batch of B stream positions may span several buckets, so a lane tests about as
+// many candidates as its slab holds instead of a whole batch per bucket (per-wave candidate slots
+// drop by about half on the 1M dam break).  Each lane first maps its B positions to buckets by
+// shifting the queue past exhausted buckets, then issues the B loads, then tests and appends in
+// stream order, which is the reference's list order.  Lists equal k_neighbours' entry for entry.
+template <int BS, int B>
+__global__ __launch_bounds__(BS) void k_neighbours_slab(DeviceState d, SphParams P) {
+    const uint32_t p = xcd_slot<BS>();
+    uint32_t count = 0;
+    if (p < d.nslots) {
+        const SceneDev& sc = scene_of(d, p);
+        if (p < sc.end) {
+            const f3 xi = make3(d.XsA[p]);
+            int cx, cy, cz;
+            discretize(xi, P.cellSearch, cx, cy, cz);
+            uint32_t* out = reinterpret_cast<uint32_t*>(d.nbr4 + list_base(p, P.maxN));
+            const uint32_t maxN = (uint32_t)P.maxN;
+            constexpr uint32_t kRun = 0x80000000u, kLen = 0x7FFFFFFFu;
+            for (int g = 0; g < 3 && count < maxN; ++g) {
+                // the slab's nine buckets: base (physical start of a contiguous bucket, else the
+                // logical start) and length | contiguity flag
+                uint32_t qb[9], ql[9];
+#pragma unroll
+                for (int r = 0; r < 3; ++r) {
+                    uint32_t hq[3];
+                    uint4 rq[3];
+#pragma unroll
+                    for (int q = 0; q < 3; ++q) {
+                        const int c = g * 9 + r * 3 + q;
+                        hq[q] = cell_hash((int)((uint32_t)cx + (uint32_t)(c / 9 - 1)),
+                                          (int)((uint32_t)cy + (uint32_t)((c / 3) % 3 - 1)),
+                                          (int)((uint32_t)cz + (uint32_t)(c % 3 - 1)), sc);
+                        rq[q] = d.table[sc.tabOffset + table_slot(hq[q], sc.tabLog2)];
+                    }
+#pragma unroll
+                    for (int q = 0; q < 3; ++q) {
+                        uint4 e = make_uint4(0u, 0u, 0u, 0u);
+                        if (rq[q].x == hq[q]) {
+                            e = clamp_entry(rq[q], sc);
+                        } else if (rq[q].x != kEmptyKey) {
+                            if (!table_find(d.table, sc, hq[q], e)) e = make_uint4(0u, 0u, 0u, 0u);
+                        }
+                        qb[r * 3 + q] = e.w ? e.z : e.x;
+                        ql[r * 3 + q] = (e.y - e.x) | (e.w ? kRun : 0u);
+                    }
+                }
+                uint32_t total = 0;
+#pragma unroll
+                for (int q = 0; q < 9; ++q) total += ql[q] & kLen;
+                uint32_t segStart = 0, segEnd = ql[0] & kLen, left = 8;
+                for (uint32_t t0 = 0; t0 < total && count < maxN; t0 += B) {
+                    uint32_t mm[B];
+                    bool rr[B];
+#pragma unroll
+                    for (int u = 0; u < B; ++u) {
+                        const uint32_t t = t0 + u;
+                        // shift the queue past exhausted (and empty) buckets; the lengths sum
+                        // to total, so a position below total always stops in a bucket
+                        while (t < total && t >= segEnd && left > 0) {
+#pragma unroll
+                            for (int k = 0; k < 8; ++k) {
+                                qb[k] = qb[k + 1];
+                                ql[k] = ql[k + 1];
+                            }
+                            ql[8] = 0u;
+                            --left;
+                            segStart = segEnd;
+                            segEnd += ql[0] & kLen;
+                        }
+                        if (t < total) {
+                            mm[u] = qb[0] + (t - segStart);
+                            rr[u] = (ql[0] & kRun) != 0u;
+                        } else {  // past the stream: repeat the previous position (discarded)
+                            mm[u] = u ? mm[u - 1] : qb[0];
+                            rr[u] = u ? rr[u - 1] : true;
+                        }
+                    }
+                    float4 c[B];
+                    uint32_t ph[B];
+#pragma unroll
+                    for (int u = 0; u < B; ++u) {
+                        if (rr[u]) {
+                            ph[u] = mm[u];
+                            c[u] = d.XsA[ph[u]];
+                        } else {
+                            c[u] = d.XsL[mm[u]];
+                            ph[u] = d.physT[mm[u]];
+                        }
+                    }
+                    uint32_t acc = 0u;
+#pragma unroll
+                    for (int u = 0; u < B; ++u) {
+                        const f3 sep = xi - make3(c[u]);
+                        const bool ok = (dot3(sep, sep) < P.h2) & (ph[u] != p) & (t0 + u < total);
+                        acc |= (uint32_t)ok << u;
+                    }
+                    const uint32_t room = maxN - count;
+                    while ((uint32_t)__popc(acc) > room) acc &= ~(1u << (31 - __clz(acc)));
+                    if (acc) {
+#pragma unroll
+                        for (int u = 0; u < B; ++u) {
+                            if (acc & (1u << u)) {
+                                const uint32_t k = count + (uint32_t)__popc(acc & ((1u << u) - 1u));
+                                out[((size_t)(k >> 2) * 64u) * 4u + (k & 3u)] = ph[u];
+                            }
+                        }
+                        count += (uint32_t)__popc(acc);
+                    }
+                }
+            }
+        }
+        d.cnt[p] = count;
+    }
+}
+
 // The non-parity search (SphParams::searchCells == 2): cells of c = h / 2, keyed by their packed
 // coordinates (cell_key_fine), so every bucket is one cell and none is walked twice.  A cell at
 // offset (dx, dy, dz) from the particle's cell, |d| <= 2, can hold a particle within h only if its
@@ -1160,9 +1277,14 @@ void launch_gather(hipStream_t st, const DeviceState& d, const SphParams& P, con
 void launch_bucket_phys(hipStream_t st, const DeviceState& d) {
     k_bucket_phys<<<grid(d.nslots), kBlock, 0, st>>>(d);
 }
-void launch_neighbours(hipStream_t st, const DeviceState& d, const SphParams& P, int bs) {
+void launch_neighbours(hipStream_t st, const DeviceState& d, const SphParams& P, int bs,
+                       int slabBatch) {
     if (P.searchCells == 2)
         AKUA_BY_BLOCK(bs, (k_neighbours_fine<BS><<<grid_xcd(d.nslots, BS), BS, 0, st>>>(d, P)));
+    else if (slabBatch == 4)
+        AKUA_BY_BLOCK(bs, (k_neighbours_slab<BS, 4><<<grid_xcd(d.nslots, BS), BS, 0, st>>>(d, P)));
+    else if (slabBatch == 8)
+        AKUA_BY_BLOCK(bs, (k_neighbours_slab<BS, 8><<<grid_xcd(d.nslots, BS), BS, 0, st>>>(d, P)));
     else
         AKUA_BY_BLOCK(bs, (k_neighbours<BS><<<grid_xcd(d.nslots, BS), BS, 0, st>>>(d, P)));
 }

@@ -171,7 +171,7 @@ struct akua_solver {
     int hashBits = 1;        // bits of the reference's hashes (log2 of the largest table size)
     SortPlan plan2;          // super-cell sort (physical layout)
     // block, layout level, digit bits (0: balanced), layout interval, balance tile
-    akua_tuning tuning{256, 1, 0, 0, 1, 0};
+    akua_tuning tuning{256, 1, 0, 0, 1, 0, 0};
     int layoutPhase = 0;           // steps since the physical layout was last re-sorted (0: now)
     std::vector<SortTile> tilesHost;
     std::vector<SortScene> sortScenesHost;
@@ -582,7 +582,7 @@ void enqueue_step(akua_solver* s, hipStream_t st, const StepParams& S, bool time
     L.after("bucket_phys");
     const int bs = s->tuning.block;
     L.before("neighbours");
-    launch_neighbours(st, d, P, bs);
+    launch_neighbours(st, d, P, bs, s->tuning.slabBatch);
     L.after("neighbours");
     if (d.work) {
         L.before("balance");
@@ -937,6 +937,8 @@ void validate_tuning(const akua_tuning& t) {
     if (t.balanceTile != 0 && (t.balanceTile < 256 || t.balanceTile > (int)kBalanceMaxTile ||
                                t.balanceTile % 256 != 0))
         fail(AKUA_ERR_INVALID_ARG, "tuning.balanceTile must be 0 or a multiple of 256 in [256, 2048]");
+    if (t.slabBatch != 0 && t.slabBatch != 4 && t.slabBatch != 8)
+        fail(AKUA_ERR_INVALID_ARG, "tuning.slabBatch must be 0, 4 or 8");
 }
 
 void apply_tuning(akua_solver* s, const akua_tuning& t) {
@@ -1015,8 +1017,8 @@ std::string tuning_key(akua_solver* s) {
     return k.str();
 }
 
-// Cache lines: "<key> | block level hashBits layoutBits interval balanceTile searchCells".  The
-// last line for a key wins.
+// Cache lines: "<key> | block level hashBits layoutBits interval balanceTile slabBatch
+// searchCells".  The last line for a key wins.
 bool cache_lookup(const char* path, const std::string& key, akua_tuning& t, int& cells) {
     std::ifstream f(path);
     if (!f) return false;
@@ -1029,7 +1031,7 @@ bool cache_lookup(const char* path, const std::string& key, akua_tuning& t, int&
         akua_tuning c{};
         int sc = 1;
         if (!(v >> c.block >> c.layoutLevel >> c.hashDigitBits >> c.layoutDigitBits >> c.layoutInterval >>
-              c.balanceTile >> sc))
+              c.balanceTile >> c.slabBatch >> sc))
             continue;
         try {
             validate_tuning(c);
@@ -1048,7 +1050,8 @@ void cache_store(const char* path, const std::string& key, const akua_tuning& t,
     std::ofstream f(path, std::ios::app);
     if (!f) fail(AKUA_ERR_INVALID_ARG, std::string("cannot append to the tuning cache ") + path);
     f << key << " | " << t.block << ' ' << t.layoutLevel << ' ' << t.hashDigitBits << ' '
-      << t.layoutDigitBits << ' ' << t.layoutInterval << ' ' << t.balanceTile << ' ' << cells << '\n';
+      << t.layoutDigitBits << ' ' << t.layoutInterval << ' ' << t.balanceTile << ' ' << t.slabBatch
+      << ' ' << cells << '\n';
 }
 
 // ---- learned warm start ----------------------------------------------------------------------
@@ -1373,6 +1376,7 @@ void autotune(akua_solver* s, float dt, const float* bmin, const float* bmax, in
     sweep(&akua_tuning::layoutDigitBits, {0, 5, 8});
     sweep(&akua_tuning::layoutInterval, {1, 2, 4});
     sweep(&akua_tuning::balanceTile, {0, 512, 1024, 2048});
+    sweep(&akua_tuning::slabBatch, {0, 4, 8});
     // the non-parity search over h / 2 cells, only when the caller gave up bit-exactness
     int bestCells = s->sph.searchCells;
     if (!s->exact) {

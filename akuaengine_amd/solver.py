@@ -120,9 +120,10 @@ class PBFSolver:
 
     # -- tuning -------------------------------------------------------------------------------
     def set_tuning(self, block: int = 256, layoutLevel: int = 1, hashDigitBits: int = 0,
-                   layoutDigitBits: int = 0, layoutInterval: int = 1, balanceTile: int = 0) -> None:
+                   layoutDigitBits: int = 0, layoutInterval: int = 1, balanceTile: int = 0,
+                   slabBatch: int = 0) -> None:
         t = _lib.AkuaTuning(block, layoutLevel, hashDigitBits, layoutDigitBits, layoutInterval,
-                            balanceTile)
+                            balanceTile, slabBatch)
         _lib.check(self._L.akua_solver_set_tuning(self._h, ctypes.byref(t)))
 
     def set_search(self, cellsPerH: int) -> None:

@@ -157,6 +157,9 @@ typedef struct akua_tuning {
                                  slots of each tile of this many sorted by neighbour count, so that
                                  the lanes of a wave walk lists of similar length (their gathers
                                  then spread over the tile) */
+    int32_t slabBatch;        /* 0: the list build walks each of the 27 stencil buckets in batches
+                                 of 8 candidates; 4 or 8: it walks each dx slab of nine buckets as
+                                 one stream of candidates in batches of this many */
 } akua_tuning;
 
 akua_status akua_solver_set_tuning(akua_solver* s, const akua_tuning* t);
@@ -180,7 +183,8 @@ akua_status akua_solver_get_search(const akua_solver* s, int32_t* cellsPerH);
 akua_status akua_solver_set_exact(akua_solver* s, int32_t exact);
 akua_status akua_solver_get_tuning(const akua_solver* s, akua_tuning* t);
 /* Times short trials of candidate settings by coordinate descent (block, then layout level, then
- * the two digit widths, then the layout interval, then the balance tile) and keeps the fastest.  Each trial restarts from a copy of the current
+ * the two digit widths, then the layout interval, the balance tile and the slab batch) and keeps
+ * the fastest.  Each trial restarts from a copy of the current
  * particle state, which is restored afterwards, so the call leaves the state as it found it.
  * boxMin/boxMax: the box the trials step in (NULL: each scene's current box).  trialSteps: timed
  * steps per candidate (1..1000).  cachePath (NULL: no cache): a text file of earlier results keyed

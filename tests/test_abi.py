@@ -101,8 +101,8 @@ def test_null_pointers_are_rejected(lib):
 
 def test_tuning_struct_and_null_arguments(lib):
     from akuaengine_amd import _lib
-    assert ctypes.sizeof(_lib.AkuaTuning) == 24
-    t = _lib.AkuaTuning(256, 1, 0, 0, 1, 0)
+    assert ctypes.sizeof(_lib.AkuaTuning) == 28
+    t = _lib.AkuaTuning(256, 1, 0, 0, 1, 0, 0)
     assert lib.akua_solver_set_tuning(None, ctypes.byref(t)) == 1
     assert lib.akua_solver_get_tuning(None, ctypes.byref(t)) == 1
     assert lib.akua_solver_autotune(None, ctypes.c_float(0.01), None, None, 5, None, None) == 1

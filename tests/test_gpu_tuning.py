@@ -9,15 +9,15 @@ pytestmark = pytest.mark.gpu
 DT = 0.0083
 SETTINGS = [
     dict(block=256, layoutLevel=1, hashDigitBits=0, layoutDigitBits=0, layoutInterval=1,
-         balanceTile=0),
+         balanceTile=0, slabBatch=0),
     dict(block=128, layoutLevel=0, hashDigitBits=5, layoutDigitBits=5, layoutInterval=3,
-         balanceTile=1024),
+         balanceTile=1024, slabBatch=8),
     dict(block=512, layoutLevel=2, hashDigitBits=8, layoutDigitBits=8, layoutInterval=2,
-         balanceTile=256),
+         balanceTile=256, slabBatch=4),
     dict(block=256, layoutLevel=3, hashDigitBits=6, layoutDigitBits=0, layoutInterval=8,
-         balanceTile=2048),
+         balanceTile=2048, slabBatch=0),
     dict(block=128, layoutLevel=1, hashDigitBits=0, layoutDigitBits=0, layoutInterval=1,
-         balanceTile=768),
+         balanceTile=768, slabBatch=8),
 ]
 
 
@@ -39,7 +39,7 @@ def _run(PBFSolver, sc, cfg, corr, steps, tuning=None):
     return s
 
 
-@pytest.mark.parametrize("tuning", SETTINGS[1:], ids=lambda t: "b{block}-L{layoutLevel}-h{hashDigitBits}-l{layoutDigitBits}-i{layoutInterval}-t{balanceTile}".format(**t))
+@pytest.mark.parametrize("tuning", SETTINGS[1:], ids=lambda t: "b{block}-L{layoutLevel}-h{hashDigitBits}-l{layoutDigitBits}-i{layoutInterval}-t{balanceTile}-s{slabBatch}".format(**t))
 def test_every_setting_gives_identical_trajectories(classes, dam_params, tuning):
     _, PBFSolver = classes
     from akuaengine_amd import scenes
@@ -77,7 +77,8 @@ def test_invalid_settings_are_rejected(classes, dam_params):
     s = PBFSolver(1000, cfg, corr)
     for bad in (dict(block=64), dict(layoutLevel=4), dict(hashDigitBits=4), dict(layoutDigitBits=9),
                 dict(layoutInterval=0), dict(layoutInterval=9), dict(balanceTile=100),
-                dict(balanceTile=4096), dict(balanceTile=-256)):
+                dict(balanceTile=4096), dict(balanceTile=-256), dict(slabBatch=2),
+                dict(slabBatch=16)):
         with pytest.raises(AkuaError):
             s.set_tuning(**bad)
     assert s.get_tuning() == SETTINGS[0]
