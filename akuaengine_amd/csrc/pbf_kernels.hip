@@ -46,9 +46,11 @@ __device__ __forceinline__ size_t list_base(uint32_t i, int maxN) {
 }
 
 // Visits the n entries of a neighbour list in order.  Memory requests are issued kListBatch at a
-// time (list entries, then the neighbours' data through load(j)), then body(value) runs for each
-// entry in list order, so every per-particle sum keeps its reference order.
-constexpr int kListBatch = 8;
+// time (the neighbours' data through load(j)), then body(value) runs for each entry in list order,
+// so every per-particle sum keeps its reference order.  The list chunks of the next batch are
+// loaded while the current batch's data is in flight, so a batch's gathers wait only for
+// themselves, not for a list load before them.
+constexpr int kListBatch = 4;
 // per-kernel batch: 4 where the 8-entry batch's registers would cost occupancy
 constexpr int kBatchDelta = 4, kBatchVort = 4, kBatchConf = 4, kBatchXsph = 8;
 
@@ -56,12 +58,13 @@ template <class T, int B = kListBatch, class Load, class Body>
 __device__ __forceinline__ void walk_list(const uint4* __restrict__ lp, uint32_t n, Load load,
                                           Body body) {
     static_assert(B == 4 || B == 8, "one or two chunks per batch");
+    if (n == 0u) return;
+    uint4 a = lp[0];
+    uint4 b = (B == 8 && n > 4u) ? lp[64] : a;
     for (uint32_t k0 = 0; k0 < n; k0 += B) {
         uint32_t j[8];
-        const uint4 a = lp[(size_t)(k0 >> 2) * 64u];
         j[0] = a.x; j[1] = a.y; j[2] = a.z; j[3] = a.w;
         if (B == 8) {
-            const uint4 b = k0 + 4u < n ? lp[(size_t)((k0 >> 2) + 1u) * 64u] : a;
             j[4] = b.x; j[5] = b.y; j[6] = b.z; j[7] = b.w;
         }
         // entries past n are not list entries: load a valid one in their place (discarded)
@@ -70,6 +73,11 @@ __device__ __forceinline__ void walk_list(const uint4* __restrict__ lp, uint32_t
         T v[B];
 #pragma unroll
         for (int u = 0; u < B; ++u) v[u] = load(j[u]);
+        const uint32_t k1 = k0 + B;
+        if (k1 < n) {
+            a = lp[(size_t)(k1 >> 2) * 64u];
+            if (B == 8) b = k1 + 4u < n ? lp[(size_t)((k1 >> 2) + 1u) * 64u] : a;
+        }
 #pragma unroll
         for (int u = 0; u < B; ++u)
             if (k0 + u < n) body(v[u]);
