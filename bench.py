@@ -157,6 +157,9 @@ def learned_comparison(log_steps: int = 300, sample: int = 25, skip: int = 50, o
     from akuaengine_amd.train_warm_start import PHYSICS
     cfg, corr, dt = PHYSICS[physics]()
     sc = scenes.dam_break_8m() if scene == "8m" else scenes.dam_break_1m()
+    if only:  # load just the network the configuration uses
+        kind, n = only.split("_")
+        learned_js = (int(n[1:]),) if kind == "learned" else ()
     stem = "reference" if physics == "reference" else "benchmark"
     nets = {}
     for J in learned_js:
@@ -372,6 +375,9 @@ def main() -> int:
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-learned", action="store_true")
     ap.add_argument("--no-extra", action="store_true")
+    ap.add_argument("--no-full-run", action="store_true",
+                    help="skip the 500-step run of the headline workload (profiling runs: one solver "
+                         "per process)")
     ap.add_argument("--tuning", default=None,
                     help="block,level,hashBits,layoutBits,interval: apply this setting instead of "
                          "autotuning (every setting gives identical results; for profiling runs, "
@@ -379,6 +385,8 @@ def main() -> int:
     ap.add_argument("--learned-run", default=None,
                     help="run only this configuration of the learned comparison (plain_I1..4, "
                          "learned_J1, learned_J2), print its record and exit (for profiling)")
+    ap.add_argument("--learned-physics", default="reference", choices=("reference", "dam_break"),
+                    help="parameter set of --learned-run")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU: --gpus {args.gpus} is not supported (the solver runs "
@@ -386,7 +394,7 @@ def main() -> int:
         return 2
 
     if args.learned_run:
-        print(json.dumps(learned_comparison(only=args.learned_run)))
+        print(json.dumps(learned_comparison(only=args.learned_run, physics=args.learned_physics)))
         return 0
 
     from akuaengine_amd import _lib, scenes
@@ -470,7 +478,7 @@ def main() -> int:
     log(f"[bench] density error after the timed steps: max {emax:.4f} mean {emean:.5f}")
 
     del s
-    run500 = full_run(tuning)
+    run500 = None if args.no_full_run else full_run(tuning)
     learned_out = None if args.no_learned else learned_comparison()
     # the learned workload at its own stated parameters (the dam-break set, dt 0.0083), where the
     # column degenerates (see learned_config4["note"]): measured and reported as such
