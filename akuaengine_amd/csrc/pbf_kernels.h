@@ -125,8 +125,12 @@ void launch_bucket_phys(hipStream_t st, const DeviceState& d);
 // bs: workgroup size of the neighbour kernels (128, 256 or 512; results do not depend on it).
 // slabBatch: 0 walks each stencil bucket in batches of eight candidates; 4 or 8 walks each dx slab
 // of nine buckets as one stream in batches of that many (identical lists).
+// density: the same launch also runs the first density/lambda sweep at x*_pred (d.XsA), as
+// launch_density_lambda(d.XsA) would; features >= 0: it runs the feature sweep instead, as
+// launch_features(record = features) would.  Either only with the physical work order
+// (d.work == nullptr), slabBatch 0, searchCells 1 and no boundary samples.
 void launch_neighbours(hipStream_t st, const DeviceState& d, const SphParams& P, int bs,
-                       int slabBatch);
+                       int slabBatch, bool density = false, int features = -1);
 // The balanced work order of the neighbour sweeps: within each tile of `tile` physical slots
 // (a multiple of 256, at most kBalanceMaxTile), slots sorted by scene, then by neighbour count
 // (descending), so the 64 lanes of a wave walk lists of similar length; writes work, cntW and

@@ -296,9 +296,8 @@ __device__ __forceinline__ bool table_find(const uint4* __restrict__ table, cons
 constexpr int kWalkBatch = 8;
 constexpr int kProbeGroup = 3;
 
-template <int BS>
-__global__ __launch_bounds__(BS) void k_neighbours(DeviceState d, SphParams P) {
-    const uint32_t p = xcd_slot<BS>();
+__device__ __forceinline__ void neighbours_slot(const DeviceState& d, const SphParams& P,
+                                                uint32_t p) {
     uint32_t count = 0;
     if (p < d.nslots) {
         const SceneDev& sc = scene_of(d, p);
@@ -381,6 +380,11 @@ __global__ __launch_bounds__(BS) void k_neighbours(DeviceState d, SphParams P) {
         }
         d.cnt[p] = count;
     }
+}
+
+template <int BS>
+__global__ __launch_bounds__(BS) void k_neighbours(DeviceState d, SphParams P) {
+    neighbours_slot(d, P, xcd_slot<BS>());
 }
 
 // The balanced work order (launch_balance): one workgroup per tile of T physical slots.  The
@@ -767,6 +771,20 @@ __global__ __launch_bounds__(BS) void k_density_lambda(DeviceState d, SphParams 
         density_lambda_slot<BS, kFast, kBnd>(d, P, Xs, residSlot, gated, xcd_slot_of<BS>(vb, nvb));
 }
 
+// The list build followed, in the same thread, by the first Jacobi iteration's density/lambda
+// sweep at x*_pred over the list just written (each lane reads back only its own list, so no
+// synchronisation is needed; x*_pred is read-only in this kernel).  Results are those of
+// k_neighbours then k_density_lambda; the neighbours' positions are still in the caches.
+// Physical work order only (no balanced order), no boundary samples.
+// Held to the list build's 7 waves per SIMD (at most 72 VGPRs).
+template <int BS, bool kFast>
+__global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(7)))
+void k_neighbours_density(DeviceState d, SphParams P) {
+    const uint32_t p = xcd_slot<BS>();
+    neighbours_slot(d, P, p);
+    density_lambda_slot<BS, kFast, false>(d, P, d.XsA, -1, 0, p);
+}
+
 // Features of the learned warm start at x*_pred, one neighbour sweep (list order):
 //   0 C = rho / rho0 - 1 and 1 lambda, as the first Jacobi iteration computes them
 //   2 neighbour count, 3 lambda of the previous step
@@ -775,9 +793,8 @@ __global__ __launch_bounds__(BS) void k_density_lambda(DeviceState d, SphParams 
 //       particle's lambda
 //   10-12 (1/rho0) sum_j m_j s_corr,ij grad W_ij / h: the artificial-pressure correction
 //   13-15 wall proximity per axis, clamp((x - lo) / h, 0, 2) - clamp((hi - x) / h, 0, 2)
-template <int BS>
-__global__ __launch_bounds__(BS) void k_features(DeviceState d, SphParams P, int record) {
-    const uint32_t t = xcd_slot<BS>();
+__device__ __forceinline__ void features_slot(const DeviceState& d, const SphParams& P, int record,
+                                              uint32_t t) {
     const uint32_t t0 = t & ~63u;
     if (t0 >= d.nslots) return;
     const uint32_t sIdx = d.chunkScene[t0 >> 8];
@@ -834,6 +851,21 @@ __global__ __launch_bounds__(BS) void k_features(DeviceState d, SphParams P, int
         if (record) d.XsP[i] = xi4;
     }
     wave_resid_partial(d, 0, t, valid ? positive_part(C) : 0.0f);
+}
+
+template <int BS>
+__global__ __launch_bounds__(BS) void k_features(DeviceState d, SphParams P, int record) {
+    features_slot(d, P, record, xcd_slot<BS>());
+}
+
+// The list build followed, in the same thread, by the feature sweep over the list just written
+// (as k_neighbours_density does for the density/lambda sweep).
+template <int BS>
+__global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(7)))
+void k_neighbours_features(DeviceState d, SphParams P, int record) {
+    const uint32_t p = xcd_slot<BS>();
+    neighbours_slot(d, P, p);
+    features_slot(d, P, record, p);
 }
 
 // Residual statistics per scene in two fixed-order stages (deterministic sums, no atomics).
@@ -1286,8 +1318,15 @@ void launch_bucket_phys(hipStream_t st, const DeviceState& d) {
     k_bucket_phys<<<grid(d.nslots), kBlock, 0, st>>>(d);
 }
 void launch_neighbours(hipStream_t st, const DeviceState& d, const SphParams& P, int bs,
-                       int slabBatch) {
-    if (P.searchCells == 2)
+                       int slabBatch, bool density, int features) {
+    if (features >= 0) {
+        AKUA_BY_BLOCK(bs, (k_neighbours_features<BS><<<grid_xcd(d.nslots, BS), BS, 0, st>>>(d, P, features)));
+    } else if (density) {
+        if (P.fast)
+            AKUA_BY_BLOCK(bs, (k_neighbours_density<BS, true><<<grid_xcd(d.nslots, BS), BS, 0, st>>>(d, P)));
+        else
+            AKUA_BY_BLOCK(bs, (k_neighbours_density<BS, false><<<grid_xcd(d.nslots, BS), BS, 0, st>>>(d, P)));
+    } else if (P.searchCells == 2)
         AKUA_BY_BLOCK(bs, (k_neighbours_fine<BS><<<grid_xcd(d.nslots, BS), BS, 0, st>>>(d, P)));
     else if (slabBatch == 4)
         AKUA_BY_BLOCK(bs, (k_neighbours_slab<BS, 4><<<grid_xcd(d.nslots, BS), BS, 0, st>>>(d, P)));

@@ -171,7 +171,7 @@ struct akua_solver {
     int hashBits = 1;        // bits of the reference's hashes (log2 of the largest table size)
     SortPlan plan2;          // super-cell sort (physical layout)
     // block, layout level, digit bits (0: balanced), layout interval, balance tile
-    akua_tuning tuning{256, 1, 0, 0, 1, 0, 0};
+    akua_tuning tuning{256, 1, 0, 0, 1, 0, 0, 0};
     int layoutPhase = 0;           // steps since the physical layout was last re-sorted (0: now)
     std::vector<SortTile> tilesHost;
     std::vector<SortScene> sortScenesHost;
@@ -581,17 +581,26 @@ void enqueue_step(akua_solver* s, hipStream_t st, const StepParams& S, bool time
     launch_bucket_phys(st, d);
     L.after("bucket_phys");
     const int bs = s->tuning.block;
-    L.before("neighbours");
-    launch_neighbours(st, d, P, bs, s->tuning.slabBatch);
-    L.after("neighbours");
+    const int I = s->cfg.solverIterations;
+    const bool learned = s->mode == 1 && !record;
+    // the first plain density/lambda sweep, or the learned step's feature sweep, runs inside
+    // the list build's launch
+    const bool fusable = s->tuning.fuseSweep == 1 && !d.work && s->tuning.slabBatch == 0 &&
+                         P.searchCells == 1 && !P.bd.on;
+    const bool fusedDensity = fusable && I > 0 && !learned && !record;
+    const bool fusedFeatures = fusable && (learned || record);
+    const char* nbrName = fusedDensity ? "neighbours_density" : fusedFeatures ? "neighbours_features"
+                                                                              : "neighbours";
+    L.before(nbrName);
+    launch_neighbours(st, d, P, bs, s->tuning.slabBatch, fusedDensity,
+                      fusedFeatures ? (record ? 1 : 0) : -1);
+    L.after(nbrName);
     if (d.work) {
         L.before("balance");
         launch_balance(st, d, P, (uint32_t)s->tuning.balanceTile, s->work.p, s->cntW.p,
                        reinterpret_cast<uint4*>(s->nbrW.p));
         L.after("balance");
     }
-    const int I = s->cfg.solverIterations;
-    const bool learned = s->mode == 1 && !record;
     const int J = learned ? std::min(s->learnedIters, I) : 0;
     // record: targets are x* before the last Jt iterations and the lambda computed there
     const int Jt = std::min(std::max(s->trainIters, 1), I);
@@ -600,7 +609,7 @@ void enqueue_step(akua_solver* s, hipStream_t st, const StepParams& S, bool time
     // (free once the neighbour lists are built) and XsB (which the fallback may overwrite only
     // in scenes whose gate is set, where the warm chain's result is discarded)
     auto ws = [&](int j) { return j == 0 ? s->XsW.p : (j & 1) ? s->XsL.p : s->XsB.p; };
-    if (learned || record) {
+    if ((learned || record) && !fusedFeatures) {
         L.before("features");
         launch_features(st, d, P, record, bs);
         L.after("features");
@@ -632,9 +641,11 @@ void enqueue_step(akua_solver* s, hipStream_t st, const StepParams& S, bool time
         const float4* in = xs[it & 1];
         float4* out = xs[(it + 1) & 1];
         const bool last = it + 1 == I;
-        L.before(learned ? "density_lambda_fallback" : "density_lambda");
-        launch_density_lambda(st, d, P, in, bs, record && it == I - Jt ? 2 : -1, learned);
-        L.after(learned ? "density_lambda_fallback" : "density_lambda");
+        if (!(fusedDensity && it == 0)) {
+            L.before(learned ? "density_lambda_fallback" : "density_lambda");
+            launch_density_lambda(st, d, P, in, bs, record && it == I - Jt ? 2 : -1, learned);
+            L.after(learned ? "density_lambda_fallback" : "density_lambda");
+        }
         if (record && it == I - Jt) {
             L.before("targets");
             launch_targets(st, d, P, in);
@@ -939,6 +950,7 @@ void validate_tuning(const akua_tuning& t) {
         fail(AKUA_ERR_INVALID_ARG, "tuning.balanceTile must be 0 or a multiple of 256 in [256, 2048]");
     if (t.slabBatch != 0 && t.slabBatch != 4 && t.slabBatch != 8)
         fail(AKUA_ERR_INVALID_ARG, "tuning.slabBatch must be 0, 4 or 8");
+    if (t.fuseSweep != 0 && t.fuseSweep != 1) fail(AKUA_ERR_INVALID_ARG, "tuning.fuseSweep must be 0 or 1");
 }
 
 void apply_tuning(akua_solver* s, const akua_tuning& t) {
@@ -1018,7 +1030,8 @@ std::string tuning_key(akua_solver* s) {
 }
 
 // Cache lines: "<key> | block level hashBits layoutBits interval balanceTile slabBatch
-// searchCells".  The last line for a key wins.
+// searchCells fuseSweep".  The last line for a key wins; a line without the last field (written
+// before it existed) reads as fuseSweep 0.
 bool cache_lookup(const char* path, const std::string& key, akua_tuning& t, int& cells) {
     std::ifstream f(path);
     if (!f) return false;
@@ -1033,6 +1046,7 @@ bool cache_lookup(const char* path, const std::string& key, akua_tuning& t, int&
         if (!(v >> c.block >> c.layoutLevel >> c.hashDigitBits >> c.layoutDigitBits >> c.layoutInterval >>
               c.balanceTile >> c.slabBatch >> sc))
             continue;
+        if (!(v >> c.fuseSweep)) c.fuseSweep = 0;
         try {
             validate_tuning(c);
         } catch (const AkuaError&) {
@@ -1051,7 +1065,7 @@ void cache_store(const char* path, const std::string& key, const akua_tuning& t,
     if (!f) fail(AKUA_ERR_INVALID_ARG, std::string("cannot append to the tuning cache ") + path);
     f << key << " | " << t.block << ' ' << t.layoutLevel << ' ' << t.hashDigitBits << ' '
       << t.layoutDigitBits << ' ' << t.layoutInterval << ' ' << t.balanceTile << ' ' << t.slabBatch
-      << ' ' << cells << '\n';
+      << ' ' << cells << ' ' << t.fuseSweep << '\n';
 }
 
 // ---- learned warm start ----------------------------------------------------------------------
@@ -1377,6 +1391,7 @@ void autotune(akua_solver* s, float dt, const float* bmin, const float* bmax, in
     sweep(&akua_tuning::layoutInterval, {1, 2, 4});
     sweep(&akua_tuning::balanceTile, {0, 512, 1024, 2048});
     sweep(&akua_tuning::slabBatch, {0, 4, 8});
+    sweep(&akua_tuning::fuseSweep, {0, 1});
     // the non-parity search over h / 2 cells, only when the caller gave up bit-exactness
     int bestCells = s->sph.searchCells;
     if (!s->exact) {

@@ -30,7 +30,7 @@ Also reported, each in its own object of the same line:
 
 Usage: python bench.py [--steps K] [--warmup W] [--gpus 1] [--cpu-seconds S] [--no-cpu-baseline]
                        [--no-learned] [--no-extra]
-                       [--tuning block,level,hashBits,layoutBits,interval[,balanceTile[,slabBatch]]]
+                       [--tuning block,level,hashBits,layoutBits,interval[,balanceTile[,slabBatch[,fuseSweep]]]]
 """
 from __future__ import annotations
 
@@ -54,6 +54,11 @@ HBM_PEAK_GBPS = 8000.0
 #   delta/correct  : x* 12 + lambda 4 + count 4 + x* out 12 + list 4 nbar
 #   last iteration : the above + update and damping (x 12 in, x 12 + v 12 out, ids 8) 48
 #   neighbours     : x* 12 + count 4 + list 4 nbar
+#   neighbours_density (the list build and the first density/lambda sweep in one launch; the
+#                    sweep reads back the list this thread just wrote, counted once):
+#                    x* 12 + count 4 + list 4 nbar + density 4 + lambda 4
+#   neighbours_features (the list build and the learned step's feature sweep): x* 12 + count 4 +
+#                    list 4 nbar + x 12 + previous lambda 4 + features 64
 #   vorticity      : x 12 + v 12 + count 4 + omega 12 + list 4 nbar
 #   confinement    : x 12 + omega 12 + rho 4 + v 12 + count 4 + v out 12 + list 4 nbar
 #   xsph           : x 12 + v 12 + rho 4 + count 4 + v out 12 + list 4 nbar
@@ -63,6 +68,8 @@ BYTES = {
     "delta_correct": lambda nb: 32 + 4 * nb,
     "delta_correct_update": lambda nb: 80 + 4 * nb,
     "neighbours": lambda nb: 16 + 4 * nb,
+    "neighbours_density": lambda nb: 24 + 4 * nb,
+    "neighbours_features": lambda nb: 96 + 4 * nb,
     "vorticity": lambda nb: 40 + 4 * nb,
     "confinement": lambda nb: 56 + 4 * nb,
     "xsph": lambda nb: 44 + 4 * nb,
@@ -414,7 +421,7 @@ def main() -> int:
         b_, l_, h_, d_, i_ = v[:5]
         s.set_tuning(block=b_, layoutLevel=l_, hashDigitBits=h_, layoutDigitBits=d_,
                      layoutInterval=i_, balanceTile=v[5] if len(v) > 5 else 0,
-                     slabBatch=v[6] if len(v) > 6 else 0)
+                     slabBatch=v[6] if len(v) > 6 else 0, fuseSweep=v[7] if len(v) > 7 else 0)
         tuning = s.get_tuning()
     else:
         tuning = s.autotune(dt, sc.box_min, sc.box_max, trialSteps=5)

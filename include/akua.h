@@ -160,6 +160,11 @@ typedef struct akua_tuning {
     int32_t slabBatch;        /* 0: the list build walks each of the 27 stencil buckets in batches
                                  of 8 candidates; 4 or 8: it walks each dx slab of nine buckets as
                                  one stream of candidates in batches of this many */
+    int32_t fuseSweep;        /* 0 (the default): the list build is a launch of its own; 1: the
+                                 same launch then runs, in each thread over the list it just
+                                 wrote, the first density/lambda sweep (or, in a learned or
+                                 training step, the feature sweep); applies with balanceTile 0,
+                                 slabBatch 0, the reference's search and no boundary samples */
 } akua_tuning;
 
 akua_status akua_solver_set_tuning(akua_solver* s, const akua_tuning* t);

@@ -130,13 +130,16 @@ def _random_net(rng):
     return p.astype(np.float32)
 
 
+@pytest.mark.parametrize("fuse", [0, 1])
 @pytest.mark.parametrize("J", [1, 2, 4])
-def test_features_and_learned_step_match_oracle(dev, oracle_mod, dam_params, J):
+def test_features_and_learned_step_match_oracle(dev, oracle_mod, dam_params, J, fuse):
     """After a plain step, one learned step: the features equal the oracle's (bit for bit), the
     warm start equals x*_pred + h * network(features), and the step continued by the oracle from
-    that warm start (J iterations, then the rest of the step) equals the device's."""
+    that warm start (J iterations, then the rest of the step) equals the device's.  fuse 1: the
+    list build, the first plain sweep and the feature sweep share their launches."""
     learned, PBFSolver = dev
     sc, s, o = _scene_pair(PBFSolver, oracle_mod, dam_params)
+    s.set_tuning(fuseSweep=fuse)
     s.step(DT, sc.box_min, sc.box_max)
     o.step(DT, sc.box_min, sc.box_max)
     net = _random_net(np.random.default_rng(1))
@@ -254,7 +257,8 @@ def test_trained_network_gains_over_the_step_without_it(dev):
     assert res["trained"][1] < 0.7 * res["zero"][1]
 
 
-def test_training_records_plain_steps_and_its_targets(dev, oracle_mod, dam_params):
+@pytest.mark.parametrize("fuse", [0, 1])
+def test_training_records_plain_steps_and_its_targets(dev, oracle_mod, dam_params, fuse):
     """akua_solver_train: the recorded steps advance the state exactly as plain steps do; the
     features and targets of the last recorded step equal the oracle's staged values (features
     at x*_pred; targets (x*_{I-J} - x*_pred) / h and the lambda computed at x*_{I-J}); two runs
@@ -269,6 +273,7 @@ def test_training_records_plain_steps_and_its_targets(dev, oracle_mod, dam_param
     nets, states = [], []
     for run in range(2):
         s = PBFSolver(sc.n, cfg, corr)
+        s.set_tuning(fuseSweep=fuse)
         s.set_state(sc.pos, sc.vel)
         s.set_network(net0)
         loss, resid = s.train(DT, sc.box_min, sc.box_max, steps, samplesPerStep=4096,

@@ -240,6 +240,30 @@ def test_non_unit_masses_and_switching(gpu, oracle_mod, dam_params):
         _assert_state_equal(s, o)
 
 
+@pytest.mark.parametrize("case", ["unit_masses", "random_masses", "one_million"])
+def test_fused_first_sweep_bit_exact(gpu, oracle_mod, dam_params, case):
+    """The list build and the first density/lambda sweep in one launch (tuning fuseSweep 1)
+    equal the oracle bit for bit: unit masses (the sweep without mass products), other masses,
+    and one step of the 1M workload."""
+    from akuaengine_amd import scenes
+    cfg, corr = dam_params
+    if case == "one_million":
+        sc, steps, mass = scenes.dam_break_1m(), 1, None
+    else:
+        sc, steps = scenes.dam_break_scaled(14, seed=6), 3
+        mass = None if case == "unit_masses" else \
+            np.random.default_rng(8).uniform(0.5, 1.5, size=sc.n).astype(np.float32)
+    _say(f"[parity] fused first sweep, {case}, N={sc.n}")
+    s, o = _pair(gpu, oracle_mod, sc.pos, sc.vel, cfg, corr, mass if mass is not None else sc.mass)
+    s.set_tuning(block=128, layoutLevel=0, fuseSweep=1)
+    assert s.get_tuning()["fuseSweep"] == 1
+    for _ in range(steps):
+        s.step(DT, sc.box_min, sc.box_max)
+        o.step(DT, sc.box_min, sc.box_max)
+    _assert_taps_equal(s, o)
+    _assert_state_equal(s, o)
+
+
 def test_disabled_artificial_pressure(gpu, oracle_mod, dam_params):
     """LambdaCorrParams.enabled = False (the term dropped) equals the oracle bit for bit."""
     from akuaengine_amd import scenes
