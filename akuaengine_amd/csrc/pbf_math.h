@@ -84,6 +84,12 @@ constexpr uint32_t kDivLo = 0x2B800000u, kDivHi = 0x53800000u;
 
 __device__ __forceinline__ uint32_t mag_bits(float x) { return __float_as_uint(x) & 0x7FFFFFFFu; }
 
+// 2 (|bits| - 1) mod 2^32, one shift-and-add: the sign bit shifts out, and zero maps to
+// 0xFFFFFFFE, above every finite pattern.  So low2(x) >= kDivLo2 exactly when |x| >= 2^-40 or
+// x == 0, the lower half of in_div_range.
+constexpr uint32_t kDivLo2 = 2u * kDivLo - 2u;
+__device__ __forceinline__ uint32_t low2(float x) { return (__float_as_uint(x) << 1) - 2u; }
+
 __device__ __forceinline__ bool in_div_range(float x) {
     const uint32_t m = mag_bits(x);
     return m - 1u >= kDivLo - 1u && m <= kDivHi;
@@ -117,7 +123,7 @@ __device__ __forceinline__ f3 div3_rn(f3 v, float b) {
 // the lower bound of |a| (zero or >= 2^-40) is tested.
 __device__ __forceinline__ float div_rn_small(float a, const Recip& R) {
     float q = div_rn_unchecked(a, R);
-    if (mag_bits(a) - 1u < kDivLo - 1u) q = a / R.b;
+    if (low2(a) < kDivLo2) q = a / R.b;
     return q;
 }
 
@@ -125,8 +131,8 @@ __device__ __forceinline__ float div_rn_small(float a, const Recip& R) {
 __device__ __forceinline__ f3 div3_rn_small(f3 v, float b) {
     const Recip R = recip(b);
     f3 q{div_rn_unchecked(v.x, R), div_rn_unchecked(v.y, R), div_rn_unchecked(v.z, R)};
-    const uint32_t lo = min(min(mag_bits(v.x) - 1u, mag_bits(v.y) - 1u), mag_bits(v.z) - 1u);
-    if (lo < kDivLo - 1u) q = {v.x / b, v.y / b, v.z / b};
+    const uint32_t lo = min(min(low2(v.x), low2(v.y)), low2(v.z));
+    if (lo < kDivLo2) q = {v.x / b, v.y / b, v.z / b};
     return q;
 }
 
