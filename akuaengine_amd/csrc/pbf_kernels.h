@@ -21,7 +21,8 @@
 //                                 XSPH: the record's new_velocity       16 B
 //   DP   float4 (xyz, 0)          the last iteration's position delta: the record's
 //                                 position_delta                        16 B
-// Per-step scratch: X0 (old positions), XsA/XsB (x* ping-pong), W (vorticity and its length),
+// Per-step scratch: X0 (old positions), XsA/XsB (x* ping-pong; after the last iteration XsB holds
+// (x, |omega|) from the vorticity pass for the confinement pass), W (vorticity and its length),
 // Vc (velocity after vorticity confinement), two sorts' keys/values, the cell table, and the
 // neighbour list.  The list holds physical indices in chunks of four entries, wave-interleaved:
 // chunk c (entries 4c..4c+3) of slot p is the uint4 at ((p / 64) * ceil(maxN / 4) + c) * 64 +

@@ -1115,7 +1115,9 @@ __global__ __launch_bounds__(BS) void k_delta_correct(DeviceState d, SphParams P
                                                                  xcd_slot_of<BS>(vb, nvb));
 }
 
-// kernel_compute_vorticities (IntegrationCUDA.cu:104-128).  W.w keeps |omega| for the next pass.
+// kernel_compute_vorticities (IntegrationCUDA.cu:104-128).  W.w keeps |omega| for the next pass,
+// which also finds it beside the position in XsB (free between the last Jacobi iteration and the
+// next step's): (x, |omega|), one gather per neighbour there.
 template <int BS, bool kFast>
 __global__ __launch_bounds__(BS) void k_vorticity(DeviceState d, SphParams P) {
     const uint32_t t = xcd_slot<BS>();
@@ -1135,7 +1137,9 @@ __global__ __launch_bounds__(BS) void k_vorticity(DeviceState d, SphParams P) {
             const f3 c = cross3(vij, grad_spiky<kFast>(sep, P));
             w = w + (kFast ? -1.0f : -nb.x.w) * c;  // m_j == 1 under kFast
         });
-    d.W[i] = make_float4(w.x, w.y, w.z, length3(w));
+    const float wl = length3(w);
+    d.W[i] = make_float4(w.x, w.y, w.z, wl);
+    d.XsB[i] = make_float4(xi.x, xi.y, xi.z, wl);
 }
 
 // kernel_apply_vorticity_confinement (IntegrationCUDA.cu:130-165), writing Vc: the confined
@@ -1155,13 +1159,14 @@ __global__ __launch_bounds__(BS) void k_confinement(DeviceState d, SphParams P, 
     const uint32_t n = d.cntW[t];
     const uint4* lp = d.nbrW + list_base(t, P.maxN);
     f3 eta{0.0f, 0.0f, 0.0f};
+    // XW.x: (x_j, |omega_j|); XW.w: m_j (read only without unit masses)
     walk_list<XW, kBatchConf>(
-        lp, n, [&](uint32_t j) { return XW{d.X[j], d.W[j].w}; },
+        lp, n, [&](uint32_t j) { return XW{d.XsB[j], kFast ? 1.0f : d.X[j].w}; },
         [&](const XW& nb) {
-            float diff = wi.w - nb.w;
+            float diff = wi.w - nb.x.w;
             f3 sep = xi - make3(nb.x);
             const f3 g = grad_spiky<kFast>(sep, P);
-            eta = eta + (kFast ? diff : nb.x.w * diff) * g;  // m_j == 1 under kFast
+            eta = eta + (kFast ? diff : nb.w * diff) * g;  // m_j == 1 under kFast
         });
     eta = eta * invD;
     float el = length3(eta);
