@@ -756,7 +756,11 @@ __device__ __forceinline__ void density_lambda_slot(const DeviceState& d, const 
         gi = gi * P.invRest;
         C = rho * P.invRest - 1.0f;
         d.dens[i] = rho;
-        d.lam[i] = -C / ((sum + dot3(gi, gi)) + P.relax);
+        const float lam = -C / ((sum + dot3(gi, gi)) + P.relax);
+        d.lam[i] = lam;
+        // unit masses: (x*, lambda) side by side for the position-delta pass, in W (free until
+        // the vorticity pass)
+        if (kFast) d.W[i] = make_float4(xi.x, xi.y, xi.z, lam);
     }
     if (residSlot >= 0) wave_resid_partial(d, residSlot, t, valid ? positive_part(C) : 0.0f);
 }
@@ -1048,8 +1052,16 @@ __device__ __forceinline__ void delta_correct_slot(const DeviceState& d, const S
         const uint32_t n = d.cntW[t];
         const uint4* lp = d.nbrW + list_base(t, P.maxN);
         const Recip Rdq = recip(P.wdq);
+        // unit masses: x*_j and lambda_j from one float4 written by the density pass (W)
         walk_list<XLam, kBatchDelta>(
-            lp, n, [&](uint32_t j) { return XLam{XsIn[j], d.lam[j]}; },
+            lp, n,
+            [&](uint32_t j) {
+                if (kFast) {
+                    const float4 a = d.W[j];
+                    return XLam{a, a.w};
+                }
+                return XLam{XsIn[j], d.lam[j]};
+            },
             [&](const XLam& nb) {
                 f3 sep = xi - make3(nb.x);
                 float d2 = dot3(sep, sep);
