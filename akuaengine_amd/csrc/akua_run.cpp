@@ -467,12 +467,12 @@ int main(int argc, char** argv) {
     std::printf("{\"source\": %s, \"particles\": %d, \"params\": \"%s\", \"iterations\": %d, \"dt\": %.9g, "
                 "\"frames\": %lld, \"steps\": %lld, \"seconds\": %.6f, \"io_seconds\": %.6f, "
                 "\"particle_substeps_per_s\": %.6g, \"max_density_error\": %.6g, \"mean_density_error\": %.6g, "
-                "\"tuning\": [%d, %d, %d, %d, %d, %d, %d, %d], \"graph\": %s, \"boundary_layers\": %d",
+                "\"tuning\": [%d, %d, %d, %d, %d, %d, %d, %d, %d], \"graph\": %s, \"boundary_layers\": %d",
                 json_str(source).c_str(), n, referenceParams ? "reference" : "dam_break", o.iterations, dt,
                 (long long)frames, (long long)done, total, ioSeconds,
                 stepSeconds > 0.0 ? (double)n * (double)done / stepSeconds : 0.0, maxErr, meanErr, tuning.block,
                 tuning.layoutLevel, tuning.hashDigitBits, tuning.layoutDigitBits, tuning.layoutInterval,
-                tuning.balanceTile, tuning.slabBatch, tuning.fuseSweep,
+                tuning.balanceTile, tuning.slabBatch, tuning.fuseSweep, tuning.laneGroup,
                 o.graph ? "true" : "false", o.boundary);
     if (!o.network.empty())
         std::printf(", \"learned\": {\"iterations\": %d, \"learned_steps\": %lld, \"fallbacks\": %lld}", o.learnedIters,

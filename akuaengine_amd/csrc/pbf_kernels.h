@@ -130,8 +130,10 @@ void launch_bucket_phys(hipStream_t st, const DeviceState& d);
 // launch_density_lambda(d.XsA) would; features >= 0: it runs the feature sweep instead, as
 // launch_features(record = features) would.  Either only with the physical work order
 // (d.work == nullptr), slabBatch 0, searchCells 1 and no boundary samples.
+// cells: the same lists with one lane per stencil cell (32 per particle), for small scenes; with
+// searchCells 1, slabBatch 0 and neither sweep.
 void launch_neighbours(hipStream_t st, const DeviceState& d, const SphParams& P, int bs,
-                       int slabBatch, bool density = false, int features = -1);
+                       int slabBatch, bool density = false, int features = -1, bool cells = false);
 // The balanced work order of the neighbour sweeps: within each tile of `tile` physical slots
 // (a multiple of 256, at most kBalanceMaxTile), slots sorted by scene, then by neighbour count
 // (descending), so the 64 lanes of a wave walk lists of similar length; writes work, cntW and
@@ -143,8 +145,12 @@ void launch_balance(hipStream_t st, const DeviceState& d, const SphParams& P, ui
 void launch_count_sum(hipStream_t st, const DeviceState& d, unsigned long long* part, uint32_t blocks);
 // residSlot >= 0: also the per-scene max and sum of max(0, C) into check residSlot of d.resid.
 // gated: only scenes whose d.gate is set run (the fallback's iterations).
+// lanes (4, 8 or 16; 1: one thread per particle): a lane group per particle in the neighbour
+// sweeps, for scenes too small to fill the chip, with results identical to one lane per particle;
+// only for plain steps (residSlot -1, no gate), the physical order and no wall samples.
 void launch_density_lambda(hipStream_t st, const DeviceState& d, const SphParams& P,
-                           const float4* Xs, int bs, int residSlot = -1, bool gated = false);
+                           const float4* Xs, int bs, int residSlot = -1, bool gated = false,
+                           int lanes = 1);
 // Position correction with the wall clamp.  final == true also runs the velocity update and the
 // wall damping and writes X, V, id, lid and phys (the last iteration); applyDelta == false skips
 // the correction (a step with zero iterations).
@@ -152,7 +158,8 @@ void launch_density_lambda(hipStream_t st, const DeviceState& d, const SphParams
 // is clear read XsAlt instead of XsIn (the learned step's final correction).
 void launch_delta_correct(hipStream_t st, const DeviceState& d, const SphParams& P,
                           const StepParams& S, const float4* XsIn, float4* XsOut, bool final,
-                          bool applyDelta, int bs, int gate = 0, const float4* XsAlt = nullptr);
+                          bool applyDelta, int bs, int gate = 0, const float4* XsAlt = nullptr,
+                          int lanes = 1);
 // Features of the learned warm start at x*_pred (XsA) into d.feat, the per-scene residual at
 // x*_pred into resid slot 0, and with record also x*_pred into d.XsP.
 void launch_features(hipStream_t st, const DeviceState& d, const SphParams& P, bool record, int bs);
@@ -170,10 +177,10 @@ void launch_targets(hipStream_t st, const DeviceState& d, const SphParams& P, co
 // Gathers count random samples (slots of scene 0) of features and targets.
 void launch_sample(hipStream_t st, const DeviceState& d, uint64_t seed, uint32_t count,
                    float* featOut, float* targOut);
-void launch_vorticity(hipStream_t st, const DeviceState& d, const SphParams& P, int bs);
+void launch_vorticity(hipStream_t st, const DeviceState& d, const SphParams& P, int bs, int lanes = 1);
 void launch_confinement(hipStream_t st, const DeviceState& d, const SphParams& P,
-                        const StepParams& S, int bs);
-void launch_xsph(hipStream_t st, const DeviceState& d, const SphParams& P, int bs);
+                        const StepParams& S, int bs, int lanes = 1);
+void launch_xsph(hipStream_t st, const DeviceState& d, const SphParams& P, int bs, int lanes = 1);
 // Exact SPH density at the positions held in XsA; rho by logical index, per-scene error max and
 // sum.
 void launch_density_probe(hipStream_t st, const DeviceState& d, const SphParams& P, float* rho,

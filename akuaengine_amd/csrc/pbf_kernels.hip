@@ -387,6 +387,85 @@ __global__ __launch_bounds__(BS) void k_neighbours(DeviceState d, SphParams P) {
     neighbours_slot(d, P, xcd_slot<BS>());
 }
 
+// The same lists with one lane per stencil cell, for scenes too small to fill the chip: 32 lanes
+// per particle (lanes 27-31 idle), lane c on stencil cell c.  Each lane walks its bucket twice:
+// first counting its accepted candidates, then, after an exclusive prefix of the 27 counts in
+// stencil order, storing them at their list positions; positions at or past maxN are dropped, as
+// the one-lane walk stops there.  A bucket reached from two stencil cells is walked by both lanes,
+// as in the reference.
+template <int BS>
+__global__ __launch_bounds__(BS) void k_neighbours_cells(DeviceState d, SphParams P) {
+    const uint32_t T = xcd_slot<BS>();
+    const uint32_t p = T >> 5, c = T & 31u;
+    if (p >= d.nslots) return;  // whole groups
+    const SceneDev& sc = scene_of(d, p);
+    if (p >= sc.end) return;
+    const f3 xi = make3(d.XsA[p]);
+    int cx, cy, cz;
+    discretize(xi, P.cellSearch, cx, cy, cz);
+    uint4 e = make_uint4(0u, 0u, 0u, 0u);
+    if (c < 27u) {
+        const uint32_t h = cell_hash((int)((uint32_t)cx + (c / 9u - 1u)), (int)((uint32_t)cy + ((c / 3u) % 3u - 1u)),
+                                     (int)((uint32_t)cz + (c % 3u - 1u)), sc);
+        if (!table_find(d.table, sc, h, e)) e = make_uint4(0u, 0u, 0u, 0u);
+    }
+    const uint32_t a = e.x, b = e.y;
+    const bool run = e.w != 0u;
+    const int32_t shift = (int32_t)e.z - (int32_t)a;  // physical - logical
+    // the accepted candidates of bucket positions [m0, m0 + kWalkBatch), one bit each
+    const auto batch = [&](uint32_t m0, uint32_t (&ph)[kWalkBatch]) {
+        float4 cand[kWalkBatch];
+#pragma unroll
+        for (int u = 0; u < kWalkBatch; ++u) {
+            const uint32_t m = min(m0 + u, b - 1u);
+            if (run) {
+                ph[u] = (uint32_t)((int32_t)m + shift);
+                cand[u] = d.XsA[ph[u]];
+            } else {
+                cand[u] = d.XsL[m];
+                ph[u] = d.physT[m];
+            }
+        }
+        uint32_t acc = 0u;
+#pragma unroll
+        for (int u = 0; u < kWalkBatch; ++u) {
+            const f3 sep = xi - make3(cand[u]);
+            const bool ok = (dot3(sep, sep) < P.h2) & (ph[u] != p) & (m0 + u < b);
+            acc |= (uint32_t)ok << u;
+        }
+        return acc;
+    };
+    uint32_t mine = 0;
+    for (uint32_t m0 = a; m0 < b; m0 += kWalkBatch) {
+        uint32_t ph[kWalkBatch];
+        mine += (uint32_t)__popc(batch(m0, ph));
+    }
+    uint32_t incl = mine;  // inclusive prefix over the group's 32 lanes (stencil order)
+#pragma unroll
+    for (int off = 1; off < 32; off <<= 1) {
+        const uint32_t y = __shfl_up(incl, off, 32);
+        if ((int)c >= off) incl += y;
+    }
+    const uint32_t total = __shfl(incl, 31, 32);
+    const uint32_t maxN = (uint32_t)P.maxN;
+    uint32_t k = incl - mine;  // this cell's first list position
+    if (mine && k < maxN) {
+        uint32_t* out = reinterpret_cast<uint32_t*>(d.nbr4 + list_base(p, P.maxN));
+        for (uint32_t m0 = a; m0 < b && k < maxN; m0 += kWalkBatch) {
+            uint32_t ph[kWalkBatch];
+            const uint32_t acc = batch(m0, ph);
+#pragma unroll
+            for (int u = 0; u < kWalkBatch; ++u) {
+                if ((acc >> u) & 1u) {
+                    if (k < maxN) out[((size_t)(k >> 2) * 64u) * 4u + (k & 3u)] = ph[u];
+                    ++k;
+                }
+            }
+        }
+    }
+    if (c == 0u) d.cnt[p] = min(total, maxN);
+}
+
 // The balanced work order (launch_balance): one workgroup per tile of T physical slots.  The
 // slots are ranked by the key (scene rank within the tile, 254 - min(count, 254)), padding slots
 // past a scene's end last within their scene; a slot's work position is the tile base plus its
@@ -1023,6 +1102,46 @@ __device__ __forceinline__ f3 damp_walls(f3 x, f3 v, const SceneDev& sc) {
     return v;
 }
 
+// kernel_correct_position with handle_particle_collision (ConstraintSolverCUDA.cu:136-169) after
+// the neighbour sum: dp becomes sum / rho0 and the corrected position is clamped to the walls.
+__device__ __forceinline__ f3 correct_position(f3 xi, f3& dp, const SphParams& P, const SceneDev& sc) {
+    dp = dp * P.invRest;
+    f3 q = xi + dp;
+    const float minDist = 0.025f, stiff = 0.5f;
+    float qa[3] = {q.x, q.y, q.z};
+    float c[3] = {0.0f, 0.0f, 0.0f};
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        if (qa[a] < sc.bmin[a] + minDist) c[a] = c[a] + stiff * (sc.bmin[a] + minDist - qa[a]);
+        if (qa[a] > sc.bmax[a] - minDist) c[a] = c[a] + stiff * (sc.bmax[a] - minDist - qa[a]);
+    }
+    return q + f3{c[0], c[1], c[2]};
+}
+
+// kernel_update_position_and_velocity (IntegrationCUDA.cu:38-49) and the wall damping at the last
+// iteration's corrected position q, and the step's writes of slot i.
+template <bool kApplyDelta>
+__device__ __forceinline__ void finish_step(const DeviceState& d, const StepParams& S,
+                                            const SceneDev& sc, uint32_t i, float4 xi4, f3 q, f3 dp) {
+    const float4 x0 = d.X0[i];
+    f3 v = div3_rn(q - make3(x0), S.dt);
+    d.VS[i] = make_float4(v.x, v.y, v.z, 0.0f);  // the record's new_velocity (IntegrationCUDA.cu:45)
+    v = damp_walls(q, v, sc);
+    d.X[i] = make_float4(q.x, q.y, q.z, xi4.w);
+    d.V[i] = make_float4(v.x, v.y, v.z, 0.0f);
+    d.id[i] = d.Tid[i];
+    const uint32_t k = d.lidT[i];
+    d.lid[i] = k;
+    d.phys[k] = i;
+    if (kApplyDelta) {  // the record's position_delta (ConstraintSolverCUDA.cu:129)
+        d.DP[i] = make_float4(dp.x, dp.y, dp.z, 0.0f);
+    } else {  // no iteration ran: density, lambda and the position delta carry over
+        d.dens[i] = d.Tdens[i];
+        d.lam[i] = d.Tlam[i];
+        d.DP[i] = d.TDP[i];
+    }
+}
+
 // kernel_calculate_position_delta (ConstraintSolverCUDA.cu:99-130), then kernel_correct_position
 // with handle_particle_collision (ConstraintSolverCUDA.cu:136-169).  x* is double-buffered: the
 // kernel reads XsIn and writes XsOut, so every particle sees the positions of the same iteration.
@@ -1081,39 +1200,13 @@ __device__ __forceinline__ void delta_correct_slot(const DeviceState& d, const S
             if (boundary_near(P.bd, xi.x, xi.y, xi.z, P.h2)) gb = make3(d.Bg[i]);
             dp = dp + li * gb;
         }
-        dp = dp * P.invRest;
-        q = xi + dp;
-        const float minDist = 0.025f, stiff = 0.5f;
-        float qa[3] = {q.x, q.y, q.z};
-        float c[3] = {0.0f, 0.0f, 0.0f};
-#pragma unroll
-        for (int a = 0; a < 3; ++a) {
-            if (qa[a] < sc.bmin[a] + minDist) c[a] = c[a] + stiff * (sc.bmin[a] + minDist - qa[a]);
-            if (qa[a] > sc.bmax[a] - minDist) c[a] = c[a] + stiff * (sc.bmax[a] - minDist - qa[a]);
-        }
-        q = q + f3{c[0], c[1], c[2]};
+        q = correct_position(xi, dp, P, sc);
     }
     if (!kFinal) {
         XsOut[i] = make_float4(q.x, q.y, q.z, xi4.w);
         return;
     }
-    const float4 x0 = d.X0[i];
-    f3 v = div3_rn(q - make3(x0), S.dt);
-    d.VS[i] = make_float4(v.x, v.y, v.z, 0.0f);  // the record's new_velocity (IntegrationCUDA.cu:45)
-    v = damp_walls(q, v, sc);
-    d.X[i] = make_float4(q.x, q.y, q.z, xi4.w);
-    d.V[i] = make_float4(v.x, v.y, v.z, 0.0f);
-    d.id[i] = d.Tid[i];
-    const uint32_t k = d.lidT[i];
-    d.lid[i] = k;
-    d.phys[k] = i;
-    if (kApplyDelta) {  // the record's position_delta (ConstraintSolverCUDA.cu:129)
-        d.DP[i] = make_float4(dp.x, dp.y, dp.z, 0.0f);
-    } else {  // no iteration ran: density, lambda and the position delta carry over
-        d.dens[i] = d.Tdens[i];
-        d.lam[i] = d.Tlam[i];
-        d.DP[i] = d.TDP[i];
-    }
+    finish_step<kApplyDelta>(d, S, sc, i, xi4, q, dp);
 }
 
 template <int BS, bool kFinal, bool kApplyDelta, bool kFast, bool kBnd>
@@ -1154,6 +1247,23 @@ __global__ __launch_bounds__(BS) void k_vorticity(DeviceState d, SphParams P) {
     d.XsB[i] = make_float4(xi.x, xi.y, xi.z, wl);
 }
 
+// The confinement force from the summed eta (IntegrationCUDA.cu:146-163) and the write of Vc.
+__device__ __forceinline__ void confine_finish(const DeviceState& d, const SphParams& P,
+                                               const StepParams& S, uint32_t i, float xi4w, float4 wi,
+                                               f3 eta, float invD) {
+    eta = eta * invD;
+    float el = length3(eta);
+    float4 v = d.V[i];
+    if (!(el < 1e-5f)) {
+        f3 N = div3_rn(eta, el);
+        f3 f = P.vortEps * cross3(N, make3(wi));
+        f3 vn = make3(v) + S.dt * f;
+        v = make_float4(vn.x, vn.y, vn.z, 0.0f);
+    }
+    v.w = xi4w / d.dens[i];
+    d.Vc[i] = v;
+}
+
 // kernel_apply_vorticity_confinement (IntegrationCUDA.cu:130-165), writing Vc: the confined
 // velocity, with the particle's m / rho in .w for the XSPH pass (the quotient the reference
 // forms per neighbour, IntegrationCUDA.cu:184, computed once per particle instead).
@@ -1180,17 +1290,7 @@ __global__ __launch_bounds__(BS) void k_confinement(DeviceState d, SphParams P, 
             const f3 g = grad_spiky<kFast>(sep, P);
             eta = eta + (kFast ? diff : nb.w * diff) * g;  // m_j == 1 under kFast
         });
-    eta = eta * invD;
-    float el = length3(eta);
-    float4 v = d.V[i];
-    if (!(el < 1e-5f)) {
-        f3 N = div3_rn(eta, el);
-        f3 f = P.vortEps * cross3(N, make3(wi));
-        f3 vn = make3(v) + S.dt * f;
-        v = make_float4(vn.x, vn.y, vn.z, 0.0f);
-    }
-    v.w = xi4w / d.dens[i];
-    d.Vc[i] = v;
+    confine_finish(d, P, S, i, xi4w, wi, eta, invD);
 }
 
 // kernel_apply_xsph_viscosity (IntegrationCUDA.cu:167-195), Jacobi: reads Vc, writes V.
@@ -1215,6 +1315,217 @@ __global__ __launch_bounds__(BS) void k_xsph(DeviceState d, SphParams P) {
         });
     f3 vn = vi + P.visc * dv;
     d.V[i] = make_float4(vn.x, vn.y, vn.z, 0.0f);
+}
+
+// ---- lane groups: L lanes per particle, for scenes too small to fill the chip ------------------
+// Thread T of a lane-group launch works on particle T / L (physical order).  The group's L lanes
+// take the list entries L at a time, lane r entry k0 + r, kLaneBatch rounds of loads in flight,
+// and compute each entry's terms; then every lane of the group adds the L terms of a round in
+// entry order (lane shuffles), so each sum is the one-lane sweep's sum, term for term and in list
+// order.  Entries past the list's end contribute +0: no sum here starts at or reaches -0 (each
+// starts at +0 or a positive value, and x + (-x) rounds to +0), so adding +0 changes nothing.
+// Only plain steps (no learned gate, no residual check), the physical order and no wall samples.
+constexpr int kLaneBatch = 4;
+
+template <int L>
+struct LaneGroup {
+    uint32_t i;     // particle slot
+    uint32_t r;     // lane within the group
+    uint32_t base;  // first lane of the group within the wave
+};
+
+template <int L>
+__device__ __forceinline__ LaneGroup<L> lane_group(uint32_t T) {
+    static_assert(L >= 2 && L <= 64 && (L & (L - 1)) == 0, "a power of two that divides 64");
+    return {T / (uint32_t)L, T % (uint32_t)L, __lane_id() & ~(uint32_t)(L - 1)};
+}
+
+__device__ __forceinline__ uint32_t list_entry(const uint4* __restrict__ lp, uint32_t k) {
+    return reinterpret_cast<const uint32_t*>(lp + (size_t)(k >> 2) * 64u)[k & 3u];
+}
+
+template <int L, int NA, class T, class Load, class Term>
+__device__ __forceinline__ void walk_split(const uint4* __restrict__ lp, uint32_t n,
+                                           const LaneGroup<L>& g, Load load, Term term,
+                                           float (&acc)[NA]) {
+    for (uint32_t k0 = 0; k0 < n; k0 += (uint32_t)(L * kLaneBatch)) {
+        T v[kLaneBatch];
+        bool ok[kLaneBatch];
+#pragma unroll
+        for (int b = 0; b < kLaneBatch; ++b) {
+            const uint32_t k = k0 + (uint32_t)(b * L) + g.r;
+            ok[b] = k < n;
+            v[b] = load(list_entry(lp, ok[b] ? k : 0u));  // past the end: a valid entry, discarded
+        }
+#pragma unroll
+        for (int b = 0; b < kLaneBatch; ++b) {
+            if (k0 + (uint32_t)(b * L) >= n) break;  // the same for every lane of the group
+            float t[NA];
+            term(v[b], t);
+#pragma unroll
+            for (int c = 0; c < NA; ++c) t[c] = ok[b] ? t[c] : 0.0f;
+#pragma unroll
+            for (int s = 0; s < L; ++s)
+#pragma unroll
+                for (int c = 0; c < NA; ++c) acc[c] = acc[c] + __shfl(t[c], (int)(g.base + s), 64);
+        }
+    }
+}
+
+template <int BS, int L, bool kFast>
+__global__ __launch_bounds__(BS) void k_density_lambda_group(DeviceState d, SphParams P,
+                                                             const float4* __restrict__ Xs) {
+    const LaneGroup<L> g = lane_group<L>(xcd_slot<BS>());
+    if (g.i >= d.nslots || g.i >= scene_of(d, g.i).end) return;  // whole groups
+    const float4 xi4 = Xs[g.i];
+    const f3 xi = make3(xi4);
+    const float negInv = -P.invRest;
+    float acc[5] = {xi4.w * P.w0, 0.0f, 0.0f, 0.0f, 0.0f};  // rho, grad_i C (x, y, z), sum |grad_j C|^2
+    walk_split<L, 5, float4>(
+        d.nbr4 + list_base(g.i, P.maxN), d.cnt[g.i], g, [&](uint32_t j) { return Xs[j]; },
+        [&](const float4& xj4, float (&t)[5]) {
+            f3 sep = xi - make3(xj4);
+            float d2 = dot3(sep, sep);
+            f3 gr = grad_spiky<kFast>(sep, P);
+            if (kFast) {
+                t[0] = poly6(d2, P);
+                t[1] = gr.x; t[2] = gr.y; t[3] = gr.z;
+                f3 gj = negInv * gr;
+                t[4] = dot3(gj, gj);
+            } else {
+                t[0] = xj4.w * poly6(d2, P);
+                f3 mg = xj4.w * gr;
+                t[1] = mg.x; t[2] = mg.y; t[3] = mg.z;
+                f3 gj = (negInv * xj4.w) * gr;
+                t[4] = dot3(gj, gj);
+            }
+        },
+        acc);
+    if (g.r != 0) return;
+    f3 gi{acc[1], acc[2], acc[3]};
+    gi = gi * P.invRest;
+    const float C = acc[0] * P.invRest - 1.0f;
+    d.dens[g.i] = acc[0];
+    const float lam = -C / ((acc[4] + dot3(gi, gi)) + P.relax);
+    d.lam[g.i] = lam;
+    if (kFast) d.W[g.i] = make_float4(xi.x, xi.y, xi.z, lam);
+}
+
+template <int BS, int L, bool kFinal, bool kFast>
+__global__ __launch_bounds__(BS) void k_delta_correct_group(DeviceState d, SphParams P, StepParams S,
+                                                            const float4* __restrict__ XsIn,
+                                                            float4* __restrict__ XsOut) {
+    const LaneGroup<L> g = lane_group<L>(xcd_slot<BS>());
+    if (g.i >= d.nslots) return;
+    const SceneDev& sc = scene_of(d, g.i);
+    if (g.i >= sc.end) return;
+    const float4 xi4 = XsIn[g.i];
+    const f3 xi = make3(xi4);
+    const float li = d.lam[g.i];
+    const Recip Rdq = recip(P.wdq);
+    float acc[3] = {0.0f, 0.0f, 0.0f};
+    walk_split<L, 3, XLam>(
+        d.nbr4 + list_base(g.i, P.maxN), d.cnt[g.i], g,
+        [&](uint32_t j) {
+            if (kFast) {
+                const float4 a = d.W[j];
+                return XLam{a, a.w};
+            }
+            return XLam{XsIn[j], d.lam[j]};
+        },
+        [&](const XLam& nb, float (&t)[3]) {
+            f3 sep = xi - make3(nb.x);
+            float d2 = dot3(sep, sep);
+            float coef = li + nb.lam;
+            if (P.corrOn) {
+                const float w = poly6(d2, P);
+                coef = coef + P.negCorrK * corr_pow(kFast ? div_rn_small(w, Rdq) : div_rn(w, Rdq), P);
+            }
+            f3 gr = grad_spiky<kFast>(sep, P);
+            const f3 u = kFast ? coef * gr : (coef * nb.x.w) * gr;
+            t[0] = u.x; t[1] = u.y; t[2] = u.z;
+        },
+        acc);
+    if (g.r != 0) return;
+    f3 dp{acc[0], acc[1], acc[2]};
+    const f3 q = correct_position(xi, dp, P, sc);
+    if (!kFinal) {
+        XsOut[g.i] = make_float4(q.x, q.y, q.z, xi4.w);
+        return;
+    }
+    finish_step<true>(d, S, sc, g.i, xi4, q, dp);
+}
+
+template <int BS, int L, bool kFast>
+__global__ __launch_bounds__(BS) void k_vorticity_group(DeviceState d, SphParams P) {
+    const LaneGroup<L> g = lane_group<L>(xcd_slot<BS>());
+    if (g.i >= d.nslots || g.i >= scene_of(d, g.i).end) return;
+    const f3 xi = make3(d.X[g.i]);
+    const f3 vi = make3(d.V[g.i]);
+    float acc[3] = {0.0f, 0.0f, 0.0f};
+    walk_split<L, 3, XV>(
+        d.nbr4 + list_base(g.i, P.maxN), d.cnt[g.i], g,
+        [&](uint32_t j) { return XV{d.X[j], d.V[j]}; },
+        [&](const XV& nb, float (&t)[3]) {
+            f3 vij = make3(nb.v) - vi;
+            f3 sep = xi - make3(nb.x);
+            const f3 c = cross3(vij, grad_spiky<kFast>(sep, P));
+            const f3 u = (kFast ? -1.0f : -nb.x.w) * c;
+            t[0] = u.x; t[1] = u.y; t[2] = u.z;
+        },
+        acc);
+    if (g.r != 0) return;
+    const f3 w{acc[0], acc[1], acc[2]};
+    const float wl = length3(w);
+    d.W[g.i] = make_float4(w.x, w.y, w.z, wl);
+    d.XsB[g.i] = make_float4(xi.x, xi.y, xi.z, wl);
+}
+
+template <int BS, int L, bool kFast>
+__global__ __launch_bounds__(BS) void k_confinement_group(DeviceState d, SphParams P, StepParams S) {
+    const LaneGroup<L> g = lane_group<L>(xcd_slot<BS>());
+    if (g.i >= d.nslots || g.i >= scene_of(d, g.i).end) return;
+    const float4 xi4 = d.X[g.i];
+    const f3 xi = make3(xi4);
+    const float4 wi = d.W[g.i];
+    const float invD = 1.0f / d.dens[g.i];
+    float acc[3] = {0.0f, 0.0f, 0.0f};
+    walk_split<L, 3, XW>(
+        d.nbr4 + list_base(g.i, P.maxN), d.cnt[g.i], g,
+        [&](uint32_t j) { return XW{d.XsB[j], kFast ? 1.0f : d.X[j].w}; },
+        [&](const XW& nb, float (&t)[3]) {
+            float diff = wi.w - nb.x.w;
+            f3 sep = xi - make3(nb.x);
+            const f3 gr = grad_spiky<kFast>(sep, P);
+            const f3 u = (kFast ? diff : nb.w * diff) * gr;
+            t[0] = u.x; t[1] = u.y; t[2] = u.z;
+        },
+        acc);
+    if (g.r != 0) return;
+    confine_finish(d, P, S, g.i, xi4.w, wi, f3{acc[0], acc[1], acc[2]}, invD);
+}
+
+template <int BS, int L>
+__global__ __launch_bounds__(BS) void k_xsph_group(DeviceState d, SphParams P) {
+    const LaneGroup<L> g = lane_group<L>(xcd_slot<BS>());
+    if (g.i >= d.nslots || g.i >= scene_of(d, g.i).end) return;
+    const f3 xi = make3(d.X[g.i]);
+    const f3 vi = make3(d.Vc[g.i]);
+    float acc[3] = {0.0f, 0.0f, 0.0f};
+    walk_split<L, 3, XV>(
+        d.nbr4 + list_base(g.i, P.maxN), d.cnt[g.i], g,
+        [&](uint32_t j) { return XV{d.X[j], d.Vc[j]}; },
+        [&](const XV& nb, float (&t)[3]) {
+            f3 vij = make3(nb.v) - vi;
+            f3 sep = xi - make3(nb.x);
+            float w = poly6(dot3(sep, sep), P);
+            const f3 u = (nb.v.w * vij) * w;
+            t[0] = u.x; t[1] = u.y; t[2] = u.z;
+        },
+        acc);
+    if (g.r != 0) return;
+    f3 vn = vi + P.visc * f3{acc[0], acc[1], acc[2]};
+    d.V[g.i] = make_float4(vn.x, vn.y, vn.z, 0.0f);
 }
 
 // Density at the current positions over every particle within h, each counted once: the walk of
@@ -1303,6 +1614,25 @@ inline unsigned grid_xcd(uint32_t n, int bs) {
         }                               \
     } while (0)
 
+// Lane-group launches: L in {4, 8, 16}, nslots * L threads.
+#define AKUA_BY_LANES(lanes, CALL)      \
+    do {                                \
+        if ((lanes) == 4) {             \
+            constexpr int L = 4;        \
+            CALL;                       \
+        } else if ((lanes) == 8) {      \
+            constexpr int L = 8;        \
+            CALL;                       \
+        } else {                        \
+            constexpr int L = 16;       \
+            CALL;                       \
+        }                               \
+    } while (0)
+
+inline unsigned grid_lanes(uint32_t nslots, int lanes, int bs) {
+    return grid_xcd(nslots * (uint32_t)lanes, bs);
+}
+
 }  // namespace
 
 void launch_fill_u32(hipStream_t st, uint32_t* p, uint32_t n, uint32_t v) {
@@ -1335,8 +1665,10 @@ void launch_bucket_phys(hipStream_t st, const DeviceState& d) {
     k_bucket_phys<<<grid(d.nslots), kBlock, 0, st>>>(d);
 }
 void launch_neighbours(hipStream_t st, const DeviceState& d, const SphParams& P, int bs,
-                       int slabBatch, bool density, int features) {
-    if (features >= 0) {
+                       int slabBatch, bool density, int features, bool cells) {
+    if (cells) {
+        AKUA_BY_BLOCK(bs, (k_neighbours_cells<BS><<<grid_lanes(d.nslots, 32, BS), BS, 0, st>>>(d, P)));
+    } else if (features >= 0) {
         AKUA_BY_BLOCK(bs, (k_neighbours_features<BS><<<grid_xcd(d.nslots, BS), BS, 0, st>>>(d, P, features)));
     } else if (density) {
         if (P.fast)
@@ -1369,7 +1701,15 @@ void density_lambda_bs(hipStream_t st, const DeviceState& d, const SphParams& P,
     }
 }
 void launch_density_lambda(hipStream_t st, const DeviceState& d, const SphParams& P,
-                           const float4* Xs, int bs, int residSlot, bool gated) {
+                           const float4* Xs, int bs, int residSlot, bool gated, int lanes) {
+    if (lanes > 1) {
+        const unsigned g = grid_lanes(d.nslots, lanes, bs);
+        if (P.fast)
+            AKUA_BY_BLOCK(bs, AKUA_BY_LANES(lanes, (k_density_lambda_group<BS, L, true><<<g, BS, 0, st>>>(d, P, Xs))));
+        else
+            AKUA_BY_BLOCK(bs, AKUA_BY_LANES(lanes, (k_density_lambda_group<BS, L, false><<<g, BS, 0, st>>>(d, P, Xs))));
+        return;
+    }
     AKUA_BY_BLOCK(bs, (density_lambda_bs<BS>(st, d, P, Xs, residSlot, gated)));
 }
 void launch_balance(hipStream_t st, const DeviceState& d, const SphParams& P, uint32_t tile,
@@ -1429,24 +1769,57 @@ void delta_correct_bs(hipStream_t st, const DeviceState& d, const SphParams& P,
 }
 void launch_delta_correct(hipStream_t st, const DeviceState& d, const SphParams& P,
                           const StepParams& S, const float4* XsIn, float4* XsOut, bool final,
-                          bool applyDelta, int bs, int gate, const float4* XsAlt) {
+                          bool applyDelta, int bs, int gate, const float4* XsAlt, int lanes) {
+    if (lanes > 1 && applyDelta) {
+        const unsigned g = grid_lanes(d.nslots, lanes, bs);
+#define AKUA_DCG(F, K) \
+    AKUA_BY_BLOCK(bs, AKUA_BY_LANES(lanes, (k_delta_correct_group<BS, L, F, K><<<g, BS, 0, st>>>(d, P, S, XsIn, XsOut))))
+        if (final) {
+            if (P.fast) AKUA_DCG(true, true); else AKUA_DCG(true, false);
+        } else {
+            if (P.fast) AKUA_DCG(false, true); else AKUA_DCG(false, false);
+        }
+#undef AKUA_DCG
+        return;
+    }
     AKUA_BY_BLOCK(bs, (delta_correct_bs<BS>(st, d, P, S, XsIn, XsOut, final, applyDelta, gate,
                                             XsAlt)));
 }
-void launch_vorticity(hipStream_t st, const DeviceState& d, const SphParams& P, int bs) {
+void launch_vorticity(hipStream_t st, const DeviceState& d, const SphParams& P, int bs, int lanes) {
+    if (lanes > 1) {
+        const unsigned g = grid_lanes(d.nslots, lanes, bs);
+        if (P.fast)
+            AKUA_BY_BLOCK(bs, AKUA_BY_LANES(lanes, (k_vorticity_group<BS, L, true><<<g, BS, 0, st>>>(d, P))));
+        else
+            AKUA_BY_BLOCK(bs, AKUA_BY_LANES(lanes, (k_vorticity_group<BS, L, false><<<g, BS, 0, st>>>(d, P))));
+        return;
+    }
     if (P.fast)
         AKUA_BY_BLOCK(bs, (k_vorticity<BS, true><<<grid_xcd(d.nslots, BS), BS, 0, st>>>(d, P)));
     else
         AKUA_BY_BLOCK(bs, (k_vorticity<BS, false><<<grid_xcd(d.nslots, BS), BS, 0, st>>>(d, P)));
 }
 void launch_confinement(hipStream_t st, const DeviceState& d, const SphParams& P,
-                        const StepParams& S, int bs) {
+                        const StepParams& S, int bs, int lanes) {
+    if (lanes > 1) {
+        const unsigned g = grid_lanes(d.nslots, lanes, bs);
+        if (P.fast)
+            AKUA_BY_BLOCK(bs, AKUA_BY_LANES(lanes, (k_confinement_group<BS, L, true><<<g, BS, 0, st>>>(d, P, S))));
+        else
+            AKUA_BY_BLOCK(bs, AKUA_BY_LANES(lanes, (k_confinement_group<BS, L, false><<<g, BS, 0, st>>>(d, P, S))));
+        return;
+    }
     if (P.fast)
         AKUA_BY_BLOCK(bs, (k_confinement<BS, true><<<grid_xcd(d.nslots, BS), BS, 0, st>>>(d, P, S)));
     else
         AKUA_BY_BLOCK(bs, (k_confinement<BS, false><<<grid_xcd(d.nslots, BS), BS, 0, st>>>(d, P, S)));
 }
-void launch_xsph(hipStream_t st, const DeviceState& d, const SphParams& P, int bs) {
+void launch_xsph(hipStream_t st, const DeviceState& d, const SphParams& P, int bs, int lanes) {
+    if (lanes > 1) {
+        const unsigned g = grid_lanes(d.nslots, lanes, bs);
+        AKUA_BY_BLOCK(bs, AKUA_BY_LANES(lanes, (k_xsph_group<BS, L><<<g, BS, 0, st>>>(d, P))));
+        return;
+    }
     AKUA_BY_BLOCK(bs, (k_xsph<BS><<<grid_xcd(d.nslots, BS), BS, 0, st>>>(d, P)));
 }
 void launch_density_probe(hipStream_t st, const DeviceState& d, const SphParams& P, float* rho,

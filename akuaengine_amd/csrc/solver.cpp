@@ -171,7 +171,7 @@ struct akua_solver {
     int hashBits = 1;        // bits of the reference's hashes (log2 of the largest table size)
     SortPlan plan2;          // super-cell sort (physical layout)
     // block, layout level, digit bits (0: balanced), layout interval, balance tile
-    akua_tuning tuning{256, 1, 0, 0, 1, 0, 0, 0};
+    akua_tuning tuning{256, 1, 0, 0, 1, 0, 0, 0, 0};
     int layoutPhase = 0;           // steps since the physical layout was last re-sorted (0: now)
     std::vector<SortTile> tilesHost;
     std::vector<SortScene> sortScenesHost;
@@ -583,17 +583,20 @@ void enqueue_step(akua_solver* s, hipStream_t st, const StepParams& S, bool time
     const int bs = s->tuning.block;
     const int I = s->cfg.solverIterations;
     const bool learned = s->mode == 1 && !record;
+    const bool plainOrder = !d.work && s->tuning.slabBatch == 0 && P.searchCells == 1 && !P.bd.on;
+    // small scenes: lane groups in the sweeps of a plain step, one lane per stencil cell in the
+    // list build
+    const int lanes = s->tuning.laneGroup > 1 && plainOrder && !learned && !record ? s->tuning.laneGroup : 1;
     // the first plain density/lambda sweep, or the learned step's feature sweep, runs inside
     // the list build's launch
-    const bool fusable = s->tuning.fuseSweep == 1 && !d.work && s->tuning.slabBatch == 0 &&
-                         P.searchCells == 1 && !P.bd.on;
+    const bool fusable = s->tuning.fuseSweep == 1 && plainOrder && lanes == 1;
     const bool fusedDensity = fusable && I > 0 && !learned && !record;
     const bool fusedFeatures = fusable && (learned || record);
     const char* nbrName = fusedDensity ? "neighbours_density" : fusedFeatures ? "neighbours_features"
                                                                               : "neighbours";
     L.before(nbrName);
     launch_neighbours(st, d, P, bs, s->tuning.slabBatch, fusedDensity,
-                      fusedFeatures ? (record ? 1 : 0) : -1);
+                      fusedFeatures ? (record ? 1 : 0) : -1, lanes > 1);
     L.after(nbrName);
     if (d.work) {
         L.before("balance");
@@ -643,7 +646,7 @@ void enqueue_step(akua_solver* s, hipStream_t st, const StepParams& S, bool time
         const bool last = it + 1 == I;
         if (!(fusedDensity && it == 0)) {
             L.before(learned ? "density_lambda_fallback" : "density_lambda");
-            launch_density_lambda(st, d, P, in, bs, record && it == I - Jt ? 2 : -1, learned);
+            launch_density_lambda(st, d, P, in, bs, record && it == I - Jt ? 2 : -1, learned, lanes);
             L.after(learned ? "density_lambda_fallback" : "density_lambda");
         }
         if (record && it == I - Jt) {
@@ -657,17 +660,17 @@ void enqueue_step(akua_solver* s, hipStream_t st, const StepParams& S, bool time
         // learned: the fallback's iterations run only where the gate is set; the final
         // correction reads the warm chain's last iterate where it is clear
         launch_delta_correct(st, d, P, S, in, out, last, true, bs, learned ? (last ? 2 : 1) : 0,
-                             learned ? ws(J - 1) : nullptr);
+                             learned ? ws(J - 1) : nullptr, lanes);
         L.after(name);
     }
     L.before("vorticity");
-    launch_vorticity(st, d, P, bs);
+    launch_vorticity(st, d, P, bs, lanes);
     L.after("vorticity");
     L.before("confinement");
-    launch_confinement(st, d, P, S, bs);
+    launch_confinement(st, d, P, S, bs, lanes);
     L.after("confinement");
     L.before("xsph");
-    launch_xsph(st, d, P, bs);
+    launch_xsph(st, d, P, bs, lanes);
     L.after("xsph");
 }
 
@@ -951,6 +954,8 @@ void validate_tuning(const akua_tuning& t) {
     if (t.slabBatch != 0 && t.slabBatch != 4 && t.slabBatch != 8)
         fail(AKUA_ERR_INVALID_ARG, "tuning.slabBatch must be 0, 4 or 8");
     if (t.fuseSweep != 0 && t.fuseSweep != 1) fail(AKUA_ERR_INVALID_ARG, "tuning.fuseSweep must be 0 or 1");
+    if (t.laneGroup != 0 && t.laneGroup != 4 && t.laneGroup != 8 && t.laneGroup != 16)
+        fail(AKUA_ERR_INVALID_ARG, "tuning.laneGroup must be 0, 4, 8 or 16");
 }
 
 void apply_tuning(akua_solver* s, const akua_tuning& t) {
@@ -1030,8 +1035,8 @@ std::string tuning_key(akua_solver* s) {
 }
 
 // Cache lines: "<key> | block level hashBits layoutBits interval balanceTile slabBatch
-// searchCells fuseSweep".  The last line for a key wins; a line without the last field (written
-// before it existed) reads as fuseSweep 0.
+// searchCells fuseSweep laneGroup".  The last line for a key wins; a line without the last fields
+// (written before they existed) reads them as 0.
 bool cache_lookup(const char* path, const std::string& key, akua_tuning& t, int& cells) {
     std::ifstream f(path);
     if (!f) return false;
@@ -1047,6 +1052,7 @@ bool cache_lookup(const char* path, const std::string& key, akua_tuning& t, int&
               c.balanceTile >> c.slabBatch >> sc))
             continue;
         if (!(v >> c.fuseSweep)) c.fuseSweep = 0;
+        if (!(v >> c.laneGroup)) c.laneGroup = 0;
         try {
             validate_tuning(c);
         } catch (const AkuaError&) {
@@ -1065,7 +1071,7 @@ void cache_store(const char* path, const std::string& key, const akua_tuning& t,
     if (!f) fail(AKUA_ERR_INVALID_ARG, std::string("cannot append to the tuning cache ") + path);
     f << key << " | " << t.block << ' ' << t.layoutLevel << ' ' << t.hashDigitBits << ' '
       << t.layoutDigitBits << ' ' << t.layoutInterval << ' ' << t.balanceTile << ' ' << t.slabBatch
-      << ' ' << cells << ' ' << t.fuseSweep << '\n';
+      << ' ' << cells << ' ' << t.fuseSweep << ' ' << t.laneGroup << '\n';
 }
 
 // ---- learned warm start ----------------------------------------------------------------------
@@ -1392,6 +1398,7 @@ void autotune(akua_solver* s, float dt, const float* bmin, const float* bmax, in
     sweep(&akua_tuning::balanceTile, {0, 512, 1024, 2048});
     sweep(&akua_tuning::slabBatch, {0, 4, 8});
     sweep(&akua_tuning::fuseSweep, {0, 1});
+    sweep(&akua_tuning::laneGroup, {0, 4, 8, 16});
     // the non-parity search over h / 2 cells, only when the caller gave up bit-exactness
     int bestCells = s->sph.searchCells;
     if (!s->exact) {

@@ -30,7 +30,7 @@ Also reported, each in its own object of the same line:
 
 Usage: python bench.py [--steps K] [--warmup W] [--gpus 1] [--cpu-seconds S] [--no-cpu-baseline]
                        [--no-learned] [--no-extra]
-                       [--tuning block,level,hashBits,layoutBits,interval[,balanceTile[,slabBatch[,fuseSweep]]]]
+                       [--tuning block,level,hashBits,layoutBits,interval[,balanceTile[,slabBatch[,fuseSweep[,laneGroup]]]]]
 """
 from __future__ import annotations
 
@@ -421,7 +421,8 @@ def main() -> int:
         b_, l_, h_, d_, i_ = v[:5]
         s.set_tuning(block=b_, layoutLevel=l_, hashDigitBits=h_, layoutDigitBits=d_,
                      layoutInterval=i_, balanceTile=v[5] if len(v) > 5 else 0,
-                     slabBatch=v[6] if len(v) > 6 else 0, fuseSweep=v[7] if len(v) > 7 else 0)
+                     slabBatch=v[6] if len(v) > 6 else 0, fuseSweep=v[7] if len(v) > 7 else 0,
+                     laneGroup=v[8] if len(v) > 8 else 0)
         tuning = s.get_tuning()
     else:
         tuning = s.autotune(dt, sc.box_min, sc.box_max, trialSteps=5)

@@ -165,6 +165,12 @@ typedef struct akua_tuning {
                                  wrote, the first density/lambda sweep (or, in a learned or
                                  training step, the feature sweep); applies with balanceTile 0,
                                  slabBatch 0, the reference's search and no boundary samples */
+    int32_t laneGroup;        /* 0 (the default): one thread per particle; 4, 8 or 16: for scenes too
+                                 small to fill the chip, the neighbour sweeps of a plain step run
+                                 that many lanes per particle (each list's terms in parallel, summed
+                                 in list order) and the list build one lane per stencil cell;
+                                 applies with balanceTile 0, slabBatch 0, the reference's search and
+                                 no boundary samples, and takes precedence over fuseSweep */
 } akua_tuning;
 
 akua_status akua_solver_set_tuning(akua_solver* s, const akua_tuning* t);

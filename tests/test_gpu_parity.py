@@ -264,6 +264,56 @@ def test_fused_first_sweep_bit_exact(gpu, oracle_mod, dam_params, case):
     _assert_state_equal(s, o)
 
 
+LANE_CASES = ["unit_masses", "random_masses", "cap_1", "cap_12", "one_cell", "tile_edge_65",
+              "far_from_origin", "one_million"]
+
+
+@pytest.mark.parametrize("case", LANE_CASES)
+@pytest.mark.parametrize("lanes", [4, 8, 16])
+def test_lane_groups_bit_exact(gpu, oracle_mod, dam_params, lanes, case):
+    """The small-scene launches (tuning laneGroup: lane groups per particle in the sweeps, one
+    lane per stencil cell in the list build) equal the oracle bit for bit: unit and other masses,
+    caps of 1 (table size N, duplicates) and 12, 300 particles in one cell, 65 particles, a
+    lattice far from the origin, and (at 8 lanes) one step of the 1M workload."""
+    from akuaengine_amd import scenes
+    cfg, corr = dam_params
+    sc = scenes.dam_break_scaled(14, seed=6)
+    pos, vel, mass, steps = sc.pos, sc.vel, None, 3
+    box_min, box_max = sc.box_min, sc.box_max
+    if case == "one_million":
+        if lanes != 8:
+            pytest.skip("the 1M step runs at one group size")
+        sc = scenes.dam_break_1m()
+        pos, vel, mass, steps, box_min, box_max = sc.pos, sc.vel, sc.mass, 1, sc.box_min, sc.box_max
+    elif case == "random_masses":
+        mass = np.random.default_rng(8).uniform(0.5, 1.5, size=len(pos)).astype(np.float32)
+    elif case.startswith("cap_"):
+        cfg = dataclasses.replace(cfg, maxNeighbours=int(case.split("_")[1]))
+    elif case == "one_cell":
+        rng = np.random.default_rng(2)
+        pos = (np.float32(1.01) + rng.uniform(0, 0.08, size=(300, 3))).astype(np.float32)
+        vel = np.zeros_like(pos)
+        box_min, box_max = (0, 0, 0), (3, 3, 3)
+    elif case == "tile_edge_65":
+        pos, vel = pos[:65].copy(), vel[:65].copy()
+    elif case == "far_from_origin":
+        shift = np.array([-51.5, 51.0, -0.35], np.float32)
+        pos = (pos + shift).astype(np.float32)
+        box_min = tuple(float(a + b) for a, b in zip(box_min, shift))
+        box_max = tuple(float(a + b) for a, b in zip(box_max, shift))
+    _say(f"[parity] lane groups of {lanes}, {case}, N={len(pos)}")
+    s, o = _pair(gpu, oracle_mod, pos, vel, cfg, corr, mass)
+    s.set_tuning(block=128, layoutLevel=0, laneGroup=lanes)
+    assert s.get_tuning()["laneGroup"] == lanes
+    for _ in range(steps):
+        s.step(DT, box_min, box_max)
+        o.step(DT, box_min, box_max)
+    _assert_taps_equal(s, o)
+    _assert_state_equal(s, o)
+    if case.startswith("cap_"):
+        assert (s.debug_neighbours()[1] == cfg.maxNeighbours).any()
+
+
 def test_disabled_artificial_pressure(gpu, oracle_mod, dam_params):
     """LambdaCorrParams.enabled = False (the term dropped) equals the oracle bit for bit."""
     from akuaengine_amd import scenes

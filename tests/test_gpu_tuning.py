@@ -10,19 +10,25 @@ pytestmark = pytest.mark.gpu
 DT = 0.0083
 SETTINGS = [
     dict(block=256, layoutLevel=1, hashDigitBits=0, layoutDigitBits=0, layoutInterval=1,
-         balanceTile=0, slabBatch=0, fuseSweep=0),
+         balanceTile=0, slabBatch=0, fuseSweep=0, laneGroup=0),
     dict(block=128, layoutLevel=0, hashDigitBits=5, layoutDigitBits=5, layoutInterval=3,
-         balanceTile=1024, slabBatch=8, fuseSweep=1),
+         balanceTile=1024, slabBatch=8, fuseSweep=1, laneGroup=0),
     dict(block=512, layoutLevel=2, hashDigitBits=8, layoutDigitBits=8, layoutInterval=2,
-         balanceTile=256, slabBatch=4, fuseSweep=0),
+         balanceTile=256, slabBatch=4, fuseSweep=0, laneGroup=0),
     dict(block=256, layoutLevel=3, hashDigitBits=6, layoutDigitBits=0, layoutInterval=8,
-         balanceTile=2048, slabBatch=0, fuseSweep=0),
+         balanceTile=2048, slabBatch=0, fuseSweep=0, laneGroup=0),
     dict(block=128, layoutLevel=1, hashDigitBits=0, layoutDigitBits=0, layoutInterval=1,
-         balanceTile=768, slabBatch=8, fuseSweep=0),
+         balanceTile=768, slabBatch=8, fuseSweep=0, laneGroup=0),
     dict(block=128, layoutLevel=0, hashDigitBits=0, layoutDigitBits=0, layoutInterval=1,
-         balanceTile=0, slabBatch=0, fuseSweep=1),
+         balanceTile=0, slabBatch=0, fuseSweep=1, laneGroup=0),
     dict(block=512, layoutLevel=1, hashDigitBits=6, layoutDigitBits=8, layoutInterval=2,
-         balanceTile=0, slabBatch=0, fuseSweep=1),
+         balanceTile=0, slabBatch=0, fuseSweep=1, laneGroup=0),
+    dict(block=128, layoutLevel=1, hashDigitBits=0, layoutDigitBits=0, layoutInterval=1,
+         balanceTile=0, slabBatch=0, fuseSweep=1, laneGroup=8),
+    dict(block=256, layoutLevel=0, hashDigitBits=0, layoutDigitBits=6, layoutInterval=3,
+         balanceTile=0, slabBatch=0, fuseSweep=0, laneGroup=16),
+    dict(block=512, layoutLevel=2, hashDigitBits=5, layoutDigitBits=0, layoutInterval=1,
+         balanceTile=512, slabBatch=0, fuseSweep=0, laneGroup=4),
 ]
 
 
@@ -44,7 +50,7 @@ def _run(PBFSolver, sc, cfg, corr, steps, tuning=None):
     return s
 
 
-@pytest.mark.parametrize("tuning", SETTINGS[1:], ids=lambda t: "b{block}-L{layoutLevel}-h{hashDigitBits}-l{layoutDigitBits}-i{layoutInterval}-t{balanceTile}-s{slabBatch}-f{fuseSweep}".format(**t))
+@pytest.mark.parametrize("tuning", SETTINGS[1:], ids=lambda t: "b{block}-L{layoutLevel}-h{hashDigitBits}-l{layoutDigitBits}-i{layoutInterval}-t{balanceTile}-s{slabBatch}-f{fuseSweep}-g{laneGroup}".format(**t))
 def test_every_setting_gives_identical_trajectories(classes, dam_params, tuning):
     _, PBFSolver = classes
     from akuaengine_amd import scenes
@@ -83,7 +89,8 @@ def test_invalid_settings_are_rejected(classes, dam_params):
     for bad in (dict(block=64), dict(layoutLevel=4), dict(hashDigitBits=4), dict(layoutDigitBits=9),
                 dict(layoutInterval=0), dict(layoutInterval=9), dict(balanceTile=100),
                 dict(balanceTile=4096), dict(balanceTile=-256), dict(slabBatch=2),
-                dict(slabBatch=16), dict(fuseSweep=2), dict(fuseSweep=-1)):
+                dict(slabBatch=16), dict(fuseSweep=2), dict(fuseSweep=-1), dict(laneGroup=2),
+                dict(laneGroup=32), dict(laneGroup=1)):
         with pytest.raises(AkuaError):
             s.set_tuning(**bad)
     assert s.get_tuning() == SETTINGS[0]
@@ -121,7 +128,8 @@ def test_batch_settings_are_invariant(classes, dam_params):
     counts = [300, 4097, 2000, 9000]
     scs = [scenes.batched_scene(n, i, seed=9) for i, n in enumerate(counts)]
     out = []
-    for t in (SETTINGS[1], SETTINGS[0], SETTINGS[2], SETTINGS[3], SETTINGS[4], SETTINGS[5]):
+    for t in (SETTINGS[1], SETTINGS[0], SETTINGS[2], SETTINGS[3], SETTINGS[4], SETTINGS[5],
+              SETTINGS[7], SETTINGS[8]):
         b = BatchSolver(counts, cfg, corr)
         b.set_tuning(**t)
         b.set_state(np.concatenate([s.pos for s in scs]), np.concatenate([s.vel for s in scs]))
