@@ -60,7 +60,15 @@ struct SortBuffers {
     uint32_t* valsB;
     uint32_t* counts;     // (1 << kMaxDigitBits) * numTiles
     uint32_t* totals;     // kMaxPasses * (1 << kMaxDigitBits) * numScenes
+    uint32_t* look;       // look-back words of sorts of at most kOneSweepMaxTiles tiles:
+                          // 2 * (1 << kMaxDigitBits) * numTiles (NULL: such sorts use three
+                          // launches per pass too)
+    uint32_t* tickets;    // kMaxPasses tile tickets of those sorts
 };
+// Sorts of at most this many tiles use the single-pass look-back scatter.
+constexpr uint32_t kOneSweepMaxTiles = 64;
+// Largest slot count the look-back words can hold (30-bit counts).
+constexpr uint32_t kSortMaxSlots = (1u << 30) - 1u;
 
 // Sorts keysA with values equal to the slot index (generated in the first pass).  Returns 0 if
 // the result is in (keysA, valsA), 1 if in (keysB, valsB).  limit: slots in the key buffers.

@@ -9,6 +9,10 @@
 //   rs_scatter : per tile, stable ranks by wave64 ballots (wave w owns a contiguous quarter of
 //                the tile, in kSortItems rounds of 64 consecutive keys), a tile-local counting sort in
 //                LDS, then coalesced runs written to the global offsets.
+// Sorts of at most kOneSweepMaxTiles tiles (small scenes, where the three launches per pass cost
+// more than the work) run instead: rs_init and rs_hist_all once per sort (the digit totals of every
+// pass), then per pass one rs_scatter that finds each tile's offsets by decoupled look-back over
+// the earlier tiles' published counts.  Both give the same stable order.
 #include "radix_sort.h"
 
 #include <algorithm>
@@ -118,12 +122,109 @@ __global__ __launch_bounds__(64) void rs_scan(uint32_t* __restrict__ counts,
     if (lane == 0) totals[s * R + d] = carry;
 }
 
-template <int D, bool kIdentityVals>
+// ---- single-pass scatter with decoupled look-back (small sorts) ----
+// Look-back words: the top two bits say what the low 30 hold.
+constexpr uint32_t kLookAggregate = 1u << 30;  // the tile's own count of the digit
+constexpr uint32_t kLookPrefix = 2u << 30;     // the count in this and every earlier tile of the scene
+constexpr uint32_t kLookValue = (1u << 30) - 1u;
+// Bound on the unanswered polls of one thread's whole look-back (each followed by a short sleep,
+// together well under a second).  A consistent launch never comes near it: every awaited tile has
+// started, and publishes its count before it waits on anything.  It only guarantees that the
+// wait ends.
+constexpr uint32_t kLookSpinLimit = 1u << 20;
+// Earlier tiles' words read per round of the look-back.
+constexpr uint32_t kLookBatch = 16;
+
+// Clears the digit totals of every pass, the first pass's look-back words and the tile tickets.
+__global__ __launch_bounds__(kSortBlock) void rs_init(uint32_t* __restrict__ totals, uint32_t nTotals,
+                                                      uint32_t* __restrict__ look, uint32_t nLook,
+                                                      uint32_t* __restrict__ tickets) {
+    const uint32_t i = blockIdx.x * kSortBlock + threadIdx.x;
+    const uint32_t stride = gridDim.x * kSortBlock;
+    for (uint32_t k = i; k < nTotals; k += stride) totals[k] = 0u;
+    for (uint32_t k = i; k < nLook; k += stride) look[k] = 0u;
+    if (i < (uint32_t)kMaxPasses) tickets[i] = 0u;
+}
+
+// Per tile, the digit histogram of every pass at once (the counts do not depend on the order),
+// added to totals[pass][scene][digit].
+template <int D>
+__global__ __launch_bounds__(kSortBlock) void rs_hist_all(const uint32_t* __restrict__ keys,
+                                                          const SortTile* __restrict__ tiles,
+                                                          uint32_t passes, uint32_t numScenes,
+                                                          uint32_t* __restrict__ totals) {
+    constexpr uint32_t R = 1u << D;
+    __shared__ uint32_t hist[kMaxPasses * R];
+    const SortTile t = tiles[blockIdx.x];
+    for (uint32_t k = threadIdx.x; k < passes * R; k += kSortBlock) hist[k] = 0u;
+    __syncthreads();
+#pragma unroll
+    for (int it = 0; it < kSortItems; ++it) {
+        const uint32_t idx = t.begin + it * kSortBlock + threadIdx.x;
+        if (idx < t.end) {
+            const uint32_t key = keys[idx];
+            for (uint32_t p = 0; p < passes; ++p) atomicAdd(&hist[p * R + ((key >> (p * D)) & (R - 1))], 1u);
+        }
+    }
+    __syncthreads();
+    for (uint32_t k = threadIdx.x; k < passes * R; k += kSortBlock) {
+        const uint32_t c = hist[k];
+        if (c) atomicAdd(&totals[((size_t)(k / R) * numScenes + t.scene) * R + k % R], c);
+    }
+}
+
+__device__ __forceinline__ uint32_t look_load(const uint32_t* p) {
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ void look_store(uint32_t* p, uint32_t v) {
+    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// The count of digit d in the scene's tiles before tile ti (the scene's first tile is
+// ti - localTile), from their look-back words, kLookBatch words loaded at a time, adding counts until
+// a prefix word; then this tile's own prefix word.
+__device__ __forceinline__ uint32_t look_back(uint32_t* look, uint32_t R, uint32_t ti, uint32_t localTile,
+                                              uint32_t d, uint32_t mine) {
+    uint32_t before = 0;
+    if (localTile > 0) {
+        const uint32_t first = ti - localTile;
+        uint32_t spin = 0;
+        uint32_t j = ti;  // tiles [j, ti) are accounted for
+        bool done = false;
+        while (!done) {
+            const uint32_t m = min(kLookBatch, j - first);
+            uint32_t v[kLookBatch];
+#pragma unroll
+            for (uint32_t u = 0; u < kLookBatch; ++u)
+                v[u] = u < m ? look_load(look + (size_t)(j - 1u - u) * R + d) : kLookPrefix;
+#pragma unroll
+            for (uint32_t u = 0; u < kLookBatch; ++u) {
+                if (done || u >= m) continue;
+                while (!(v[u] & (kLookAggregate | kLookPrefix)) && spin < kLookSpinLimit) {
+                    ++spin;
+                    __builtin_amdgcn_s_sleep(1);
+                    v[u] = look_load(look + (size_t)(j - 1u - u) * R + d);
+                }
+                before += v[u] & kLookValue;
+                done = (v[u] & kLookPrefix) != 0u;
+            }
+            j -= m;
+            done = done || j == first;
+        }
+        look_store(look + (size_t)ti * R + d, kLookPrefix | (before + mine));
+    }
+    return before;
+}
+
+// kLookBack: tiles are taken in ticket order (look-back words in `look`, the next pass's cleared in
+// `lookNext`, `offsets` unused); otherwise tile blockIdx.x with its offsets from rs_scan.
+template <int D, bool kIdentityVals, bool kLookBack>
 __global__ __launch_bounds__(kSortBlock) void rs_scatter(
     const uint32_t* __restrict__ keysIn, const uint32_t* __restrict__ valsIn,
     uint32_t* __restrict__ keysOut, uint32_t* __restrict__ valsOut,
     const SortTile* __restrict__ tiles, uint32_t shift, const uint32_t* __restrict__ offsets,
-    const uint32_t* __restrict__ totals, uint32_t limit) {
+    const uint32_t* __restrict__ totals, uint32_t limit, uint32_t ntiles, uint32_t* look,
+    uint32_t* __restrict__ lookNext, uint32_t* ticket) {
     constexpr uint32_t R = 1u << D;
     constexpr int kWaves = kSortBlock / 64;
     constexpr int kRounds = kSortTile / kSortBlock;  // rounds of 64 keys per wave
@@ -133,13 +234,19 @@ __global__ __launch_bounds__(kSortBlock) void rs_scatter(
     __shared__ uint32_t tileStart[R];
     __shared__ uint32_t globalBase[R];
     __shared__ uint32_t scanTmp[4];
+    __shared__ uint32_t sTicket;
 
-    const SortTile t = tiles[blockIdx.x];
+    if (kLookBack && threadIdx.x == 0) sTicket = atomicAdd(ticket, 1u);
     const uint32_t lane = lane_id();
     const uint32_t wave = threadIdx.x >> 6;
     for (uint32_t d = threadIdx.x; d < R; d += kSortBlock)
         for (int w = 0; w < kWaves; ++w) waveCnt[w][d] = 0;
     __syncthreads();
+    const uint32_t ti = kLookBack ? sTicket : blockIdx.x;
+    if (ti >= ntiles) return;  // one ticket per workgroup of an ntiles grid: never taken
+    const SortTile t = tiles[ti];
+    if (kLookBack)
+        for (uint32_t d = threadIdx.x; d < R; d += kSortBlock) lookNext[(size_t)ti * R + d] = 0u;
 
     // Phase 1: stable ranks.  Wave w owns slots [begin + w*64*kRounds, +64*kRounds) in order.
     uint32_t key[kRounds], val[kRounds], rankDigit[kRounds];
@@ -173,6 +280,9 @@ __global__ __launch_bounds__(kSortBlock) void rs_scatter(
                 waveCnt[w][d] = tot;
                 tot += c;
             }
+            // publish this tile's count before waiting on any other tile
+            if (kLookBack)
+                look_store(look + (size_t)ti * R + d, (t.localTile == 0 ? kLookPrefix : kLookAggregate) | tot);
         }
         uint32_t start = block_exclusive_scan_256(d < R ? tot : 0u, scanTmp);
         // the scene's keys with a smaller digit
@@ -180,7 +290,9 @@ __global__ __launch_bounds__(kSortBlock) void rs_scatter(
         if (d < R) {
             tileStart[d] = start;
             const uint32_t sceneBegin = t.begin - t.localTile * (uint32_t)kSortTile;
-            globalBase[d] = sceneBegin + below + offsets[t.countBase * R + d * t.sceneTiles + t.localTile];
+            const uint32_t earlier = kLookBack ? look_back(look, R, ti, t.localTile, d, tot)
+                                               : offsets[t.countBase * R + d * t.sceneTiles + t.localTile];
+            globalBase[d] = sceneBegin + below + earlier;
         }
     }
     __syncthreads();
@@ -224,12 +336,52 @@ void launch_pass(hipStream_t st, bool first, const uint32_t* kin, const uint32_t
     if (hook) hook(ctx, "rs_scan", 1);
     if (hook) hook(ctx, "rs_scatter", 0);
     if (first)
-        rs_scatter<D, true><<<ntiles, kSortBlock, 0, st>>>(kin, vin, kout, vout, tiles, shift, counts,
-                                                           totals, limit);
+        rs_scatter<D, true, false><<<ntiles, kSortBlock, 0, st>>>(kin, vin, kout, vout, tiles, shift, counts,
+                                                                  totals, limit, ntiles, nullptr, nullptr,
+                                                                  nullptr);
     else
-        rs_scatter<D, false><<<ntiles, kSortBlock, 0, st>>>(kin, vin, kout, vout, tiles, shift, counts,
-                                                            totals, limit);
+        rs_scatter<D, false, false><<<ntiles, kSortBlock, 0, st>>>(kin, vin, kout, vout, tiles, shift, counts,
+                                                                   totals, limit, ntiles, nullptr, nullptr,
+                                                                   nullptr);
     if (hook) hook(ctx, "rs_scatter", 1);
+}
+
+// A sort of at most kOneSweepMaxTiles tiles: the totals of every pass at once, then one
+// look-back scatter per pass (look-back words alternate between the two halves of buf.look).
+template <int D>
+void launch_onesweep_sort(hipStream_t st, const SortPlan& plan, const SortBuffers& buf,
+                          const SortTile* tiles, uint32_t ntiles, uint32_t nscenes, uint32_t limit,
+                          LaunchHook hook, void* ctx) {
+    constexpr uint32_t R = 1u << D;
+    const int P = plan.passes();
+    const uint32_t nTotals = (uint32_t)P * nscenes * R, nLook = ntiles * R;
+    const uint32_t most = std::max(nTotals, nLook);
+    if (hook) hook(ctx, "rs_hist", 0);
+    rs_init<<<std::min<uint32_t>((most + kSortBlock - 1) / kSortBlock, 64u), kSortBlock, 0, st>>>(
+        buf.totals, nTotals, buf.look, nLook, buf.tickets);
+    rs_hist_all<D><<<ntiles, kSortBlock, 0, st>>>(buf.keysA, tiles, (uint32_t)P, nscenes, buf.totals);
+    if (hook) hook(ctx, "rs_hist", 1);
+    uint32_t* kin = buf.keysA;
+    uint32_t* vin = buf.valsA;
+    uint32_t* kout = buf.keysB;
+    uint32_t* vout = buf.valsB;
+    for (int p = 0; p < P; ++p) {
+        uint32_t* look = buf.look + (size_t)(p & 1) * ntiles * R;
+        uint32_t* lookNext = buf.look + (size_t)((p + 1) & 1) * ntiles * R;
+        const uint32_t* tot = buf.totals + (size_t)p * nscenes * R;
+        if (hook) hook(ctx, "rs_onesweep", 0);
+        if (p == 0)
+            rs_scatter<D, true, true><<<ntiles, kSortBlock, 0, st>>>(kin, vin, kout, vout, tiles, (uint32_t)(p * D),
+                                                                     nullptr, tot, limit, ntiles, look, lookNext,
+                                                                     buf.tickets + p);
+        else
+            rs_scatter<D, false, true><<<ntiles, kSortBlock, 0, st>>>(kin, vin, kout, vout, tiles, (uint32_t)(p * D),
+                                                                      nullptr, tot, limit, ntiles, look, lookNext,
+                                                                      buf.tickets + p);
+        if (hook) hook(ctx, "rs_onesweep", 1);
+        std::swap(kin, kout);
+        std::swap(vin, vout);
+    }
 }
 
 }  // namespace
@@ -272,6 +424,17 @@ int radix_sort_pairs(hipStream_t stream, const SortPlan& plan, const SortBuffers
                      uint32_t numScenes, uint32_t limit, LaunchHook hook, void* hookCtx) {
     const int P = plan.passes();
     const uint32_t R = 1u << plan.digitBits;
+    if (numTiles <= kOneSweepMaxTiles && buf.look && buf.tickets) {
+        (void)dScenes;
+        switch (plan.digitBits) {
+            case 4: launch_onesweep_sort<4>(stream, plan, buf, dTiles, numTiles, numScenes, limit, hook, hookCtx); break;
+            case 5: launch_onesweep_sort<5>(stream, plan, buf, dTiles, numTiles, numScenes, limit, hook, hookCtx); break;
+            case 6: launch_onesweep_sort<6>(stream, plan, buf, dTiles, numTiles, numScenes, limit, hook, hookCtx); break;
+            case 7: launch_onesweep_sort<7>(stream, plan, buf, dTiles, numTiles, numScenes, limit, hook, hookCtx); break;
+            default: launch_onesweep_sort<8>(stream, plan, buf, dTiles, numTiles, numScenes, limit, hook, hookCtx); break;
+        }
+        return (P % 2 == 0) ? 0 : 1;
+    }
     uint32_t* kin = buf.keysA;
     uint32_t* vin = buf.valsA;
     uint32_t* kout = buf.keysB;

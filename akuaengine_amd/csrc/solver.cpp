@@ -184,7 +184,7 @@ struct akua_solver {
     DevBuf<float4> X, V, VS, DP, TDP, X0, XsA, XsB, XsL, W, Vc, color;
     DevBuf<float> dens, lam, Tdens, Tlam, size;
     DevBuf<uint32_t> id, Tid, lid, phys, lidT, physT, srcL, keysLay, bslot, nbr, cnt, perm2keep;
-    DevBuf<uint32_t> s1KA, s1VA, s1KB, s1VB, s2KA, s2VA, s2KB, s2VB, counts, totals;
+    DevBuf<uint32_t> s1KA, s1VA, s1KB, s1VB, s2KA, s2VA, s2KB, s2VB, counts, totals, look, tickets;
     DevBuf<uint4> table;
     DevBuf<uint32_t> work, cntW, nbrW;  // the sweeps' balanced work order (allocated when used)
     DevBuf<SortTile> dTiles;
@@ -308,7 +308,7 @@ struct akua_solver {
             b->release();
         for (auto* b : {&id, &Tid, &lid, &phys, &lidT, &physT, &srcL, &keysLay, &bslot, &nbr, &cnt,
                         &perm2keep, &s1KA, &s1VA, &s1KB, &s1VB, &s2KA, &s2VA, &s2KB, &s2VB, &counts,
-                        &totals, &gate, &fallbacks, &work, &cntW, &nbrW})
+                        &totals, &look, &tickets, &gate, &fallbacks, &work, &cntW, &nbrW})
             b->release();
         table.release(); dTiles.release(); dSortScenes.release(); nbrSum.release();
         probeMax.release(); probeSum.release(); renderKeys.release(); renderRgb.release();
@@ -398,7 +398,7 @@ void setup(akua_solver* s, const std::vector<uint32_t>& counts) {
         uint64_t next = (uint64_t)slot + n;
         next = (next + kSlotAlign - 1) / kSlotAlign * kSlotAlign;
         if (counts.size() == 1) next = slot + n;  // a single scene needs no padding
-        if (next > 0x7FFFFFFFull) fail(AKUA_ERR_UNSUPPORTED, "too many particles");
+        if (next > kSortMaxSlots) fail(AKUA_ERR_UNSUPPORTED, "too many particles (at most 2^30 - 1 slots)");
         slot = (uint32_t)next;
     }
     s->nslots = slot;
@@ -433,6 +433,10 @@ void setup(akua_solver* s, const std::vector<uint32_t>& counts) {
     s->s2KA.alloc(ns); s->s2VA.alloc(ns); s->s2KB.alloc(ns); s->s2VB.alloc(ns);
     s->counts.alloc((size_t)s->numTiles * (1u << kMaxDigitBits));
     s->totals.alloc((size_t)kMaxPasses * (1u << kMaxDigitBits) * s->scenes.size());
+    if (s->numTiles <= kOneSweepMaxTiles) {
+        s->look.alloc((size_t)2 * s->numTiles * (1u << kMaxDigitBits));
+        s->tickets.alloc((size_t)kMaxPasses);
+    }
     s->table.alloc(s->tableTotal);
     s->bslot.alloc(ns);
     s->nbr.alloc(nsList * (size_t)list_chunk_count(s->cfg.maxNeighbours) * 4u);
@@ -540,14 +544,14 @@ void collect_timing(akua_solver* s) {
 
 // The logical order: hashes stably sorted over the previous logical order.
 void sort_hash(akua_solver* s, hipStream_t st, bool timed) {
-    SortBuffers b{s->s1KA.p, s->s1VA.p, s->s1KB.p, s->s1VB.p, s->counts.p, s->totals.p};
+    SortBuffers b{s->s1KA.p, s->s1VA.p, s->s1KB.p, s->s1VB.p, s->counts.p, s->totals.p, s->look.p, s->tickets.p};
     radix_sort_pairs(st, s->plan1, b, s->dTiles.p, s->numTiles, s->dSortScenes.p,
                      (uint32_t)s->scenes.size(), s->nslots, timed ? timing_hook : nullptr, s);
 }
 
 // The physical layout: super-cell Morton keys stably sorted over the new logical order.
 void sort_layout(akua_solver* s, hipStream_t st, bool timed) {
-    SortBuffers b{s->s2KA.p, s->s2VA.p, s->s2KB.p, s->s2VB.p, s->counts.p, s->totals.p};
+    SortBuffers b{s->s2KA.p, s->s2VA.p, s->s2KB.p, s->s2VB.p, s->counts.p, s->totals.p, s->look.p, s->tickets.p};
     radix_sort_pairs(st, s->plan2, b, s->dTiles.p, s->numTiles, s->dSortScenes.p,
                      (uint32_t)s->scenes.size(), s->nslots, timed ? timing_hook : nullptr, s);
 }

@@ -78,6 +78,7 @@ BYTES = {
     "rs_hist": lambda nb: 4,
     "rs_scan": lambda nb: 0,
     "rs_scatter": lambda nb: 16,
+    "rs_onesweep": lambda nb: 16,
     "fill_table": lambda nb: 8,
 }
 
@@ -271,8 +272,9 @@ def full_run(tuning, total: int = 500, edges=(0, 10, 100, 200, 300, 400, 500)):
 
 
 def extra_workloads(steps: int = 20, warmup: int = 3):
-    """Throughput of the 8M two-column scene and of the 256-scene batch (plain, 4 iterations), the
-    cost of the wall samples and of headless rendering."""
+    """Throughput of the 27k dam break (the reference's CPU-runnable scene, all 200 steps), of the
+    8M two-column scene and of the 256-scene batch (plain, 4 iterations), the cost of the wall
+    samples and of headless rendering."""
     import numpy as np
     from akuaengine_amd import scenes
     from akuaengine_amd.config import DAM_BREAK_DT, dam_break_params
@@ -280,6 +282,19 @@ def extra_workloads(steps: int = 20, warmup: int = 3):
     cfg, corr = dam_break_params(4)
     dt = DAM_BREAK_DT
     out = {}
+    # the 27k dam break from its initial state, all 200 steps, at the autotuner's setting (the
+    # tuner's trials start from the initial state and restore it); the CPU baseline times the same
+    sc = scenes.dam_break_27k()
+    s = PBFSolver(sc.n, cfg, corr)
+    s.set_state(sc.pos, sc.vel, sc.mass)
+    tuning = s.autotune(dt, sc.box_min, sc.box_max, trialSteps=5)
+    el = time_steps(s, dt, sc.box_min, sc.box_max, 200)
+    emax, emean = s.density_error()
+    out["dam_break_27k"] = {"particles": sc.n, "steps": 200, "ms_per_step": round(el / 200 * 1e3, 4),
+                            "value": round(sc.n * 200 / el, 1), "unit": "particle-substeps/s",
+                            "tuning": tuning, "density_error_after_200": [float(emax), float(emean)]}
+    log(f"[bench] 27k: {el / 200 * 1e3:.3f} ms/step")
+    del s
     sc = scenes.dam_break_8m()
     s = PBFSolver(sc.n, cfg, corr)
     s.set_state(sc.pos, sc.vel, sc.mass)
