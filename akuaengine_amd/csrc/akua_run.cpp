@@ -54,6 +54,9 @@ const char* kUsage =
     "  --boundary L            L layers of wall samples (0 = off, the reference's walls; 1..4)\n"
     "  --boundary-spacing S    their spacing (default: particle_spacing)\n"
     "  --autotune              pick launch and layout settings before the run (results unchanged)\n"
+    "  --tuning LIST           set them instead: block,level,hashBits,layoutBits,interval,\n"
+    "                          balanceTile,slabBatch,fuseSweep,laneGroup (akua_tuning's fields in\n"
+    "                          order; missing trailing fields are 0)\n"
     "  --network FILE.npy      learned warm start with this parameter blob (float32 .npy)\n"
     "  --learned-iterations J  (default 1)\n"
     "  --gate-ratio R          fallback when r(x*_w) > R r(x*_pred) + tolerance (default 1)\n"
@@ -78,6 +81,7 @@ struct Options {
     int64_t every = 1;
     uint32_t fields = AKUA_FIELD_POSITION | AKUA_FIELD_VELOCITY | AKUA_FIELD_DENSITY | AKUA_FIELD_ID;
     bool autotune = false;
+    std::vector<int32_t> tuning;  // --tuning fields, in akua_tuning's order
     int64_t renderEvery = 0;
     std::string renderDir = ".";
     int width = 1280, height = 720;
@@ -167,6 +171,18 @@ Options parse(int argc, char** argv) {
         else if (a == "--every") o.every = parse_int(a, val(), 1, INT32_MAX);
         else if (a == "--fields") o.fields = parse_fields(val());
         else if (a == "--autotune") o.autotune = true;
+        else if (a == "--tuning") {
+            std::string list = val();
+            size_t pos = 0;
+            while (pos <= list.size()) {
+                const size_t comma = list.find(',', pos);
+                const std::string item = list.substr(pos, comma == std::string::npos ? std::string::npos : comma - pos);
+                o.tuning.push_back((int32_t)parse_int("--tuning", item.c_str(), 0, 1 << 20));
+                if (comma == std::string::npos) break;
+                pos = comma + 1;
+            }
+            if (o.tuning.size() < 5 || o.tuning.size() > 9) usage_error("--tuning takes 5 to 9 values");
+        }
         else if (a == "--render-every") o.renderEvery = parse_int(a, val(), 1, INT32_MAX);
         else if (a == "--render-dir") o.renderDir = val();
         else if (a == "--width") o.width = (int)parse_int(a, val(), 1, 16384);
@@ -187,6 +203,7 @@ Options parse(int argc, char** argv) {
         usage_error("--frame-time and --max-steps-per-frame go with --frames");
     if (o.steps < 0 && o.frames < 0) o.steps = 0;
     if (o.spacing < 0.0f) usage_error("--spacing must be positive");
+    if (o.autotune && !o.tuning.empty()) usage_error("give --autotune or --tuning, not both");
     return o;
 }
 
@@ -413,7 +430,14 @@ int main(int argc, char** argv) {
         check(akua_solver_set_network(s, net.data(), (int32_t)net.size()), "set_network");
         check(akua_solver_set_learned(s, 1, o.learnedIters, 0, o.gateRatio, o.gateTol), "set_learned");
     }
-    akua_tuning tuning;
+    akua_tuning tuning{};
+    if (!o.tuning.empty()) {
+        int32_t* f[] = {&tuning.block, &tuning.layoutLevel, &tuning.hashDigitBits, &tuning.layoutDigitBits,
+                        &tuning.layoutInterval, &tuning.balanceTile, &tuning.slabBatch, &tuning.fuseSweep,
+                        &tuning.laneGroup};
+        for (size_t k = 0; k < o.tuning.size(); ++k) *f[k] = o.tuning[k];
+        check(akua_solver_set_tuning(s, &tuning), "set_tuning");
+    }
     if (o.autotune) check(akua_solver_autotune(s, dt, boxMin, boxMax, 3, nullptr, &tuning), "autotune");
     check(akua_solver_get_tuning(s, &tuning), "get_tuning");
 

@@ -1320,9 +1320,9 @@ __global__ __launch_bounds__(BS) void k_xsph(DeviceState d, SphParams P) {
 // ---- lane groups: L lanes per particle, for scenes too small to fill the chip ------------------
 // Thread T of a lane-group launch works on particle T / L (physical order).  The group's L lanes
 // take the list entries L at a time, lane r entry k0 + r, kLaneBatch rounds of loads in flight,
-// and compute each entry's terms; then every lane of the group adds the L terms of a round in
-// entry order (lane shuffles), so each sum is the one-lane sweep's sum, term for term and in list
-// order.  Entries past the list's end contribute +0: no sum here starts at or reaches -0 (each
+// and compute each entry's terms; each lane stores its terms in its wave's rows of LDS, and the
+// group's first lane adds the L terms of a round in entry order, so each sum is the one-lane
+// sweep's sum, term for term and in list order (only that lane's sums are used).  Entries past the list's end contribute +0: no sum here starts at or reaches -0 (each
 // starts at +0 or a positive value, and x + (-x) rounds to +0), so adding +0 changes nothing.
 // Only plain steps (no learned gate, no residual check), the physical order and no wall samples.
 constexpr int kLaneBatch = 4;
@@ -1344,10 +1344,14 @@ __device__ __forceinline__ uint32_t list_entry(const uint4* __restrict__ lp, uin
     return reinterpret_cast<const uint32_t*>(lp + (size_t)(k >> 2) * 64u)[k & 3u];
 }
 
+// sm: this wave's NA rows of 64 floats in LDS.  LDS instructions of one wave complete in issue
+// order, so a lane reads the terms the other lanes stored before the wave barrier; the second
+// barrier keeps the next round's stores after this round's reads.
 template <int L, int NA, class T, class Load, class Term>
 __device__ __forceinline__ void walk_split(const uint4* __restrict__ lp, uint32_t n,
                                            const LaneGroup<L>& g, Load load, Term term,
-                                           float (&acc)[NA]) {
+                                           float (&acc)[NA], float* sm) {
+    const uint32_t lane = __lane_id();
     for (uint32_t k0 = 0; k0 < n; k0 += (uint32_t)(L * kLaneBatch)) {
         T v[kLaneBatch];
         bool ok[kLaneBatch];
@@ -1363,18 +1367,33 @@ __device__ __forceinline__ void walk_split(const uint4* __restrict__ lp, uint32_
             float t[NA];
             term(v[b], t);
 #pragma unroll
-            for (int c = 0; c < NA; ++c) t[c] = ok[b] ? t[c] : 0.0f;
+            for (int c = 0; c < NA; ++c) sm[c * 64 + lane] = ok[b] ? t[c] : 0.0f;
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+            if (g.r == 0) {
 #pragma unroll
-            for (int s = 0; s < L; ++s)
+                for (int c = 0; c < NA; ++c)
 #pragma unroll
-                for (int c = 0; c < NA; ++c) acc[c] = acc[c] + __shfl(t[c], (int)(g.base + s), 64);
+                    for (int s = 0; s < L; ++s) acc[c] = acc[c] + sm[c * 64 + g.base + s];
+            }
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
         }
     }
+}
+
+// This wave's NA rows of 64 floats in the workgroup's LDS block.
+template <int NA>
+__device__ __forceinline__ float* wave_rows(float* block) {
+    return block + (threadIdx.x >> 6) * (NA * 64);
 }
 
 template <int BS, int L, bool kFast>
 __global__ __launch_bounds__(BS) void k_density_lambda_group(DeviceState d, SphParams P,
                                                              const float4* __restrict__ Xs) {
+    __shared__ float lds[(BS / 64) * 5 * 64];
     const LaneGroup<L> g = lane_group<L>(xcd_slot<BS>());
     if (g.i >= d.nslots || g.i >= scene_of(d, g.i).end) return;  // whole groups
     const float4 xi4 = Xs[g.i];
@@ -1400,7 +1419,7 @@ __global__ __launch_bounds__(BS) void k_density_lambda_group(DeviceState d, SphP
                 t[4] = dot3(gj, gj);
             }
         },
-        acc);
+        acc, wave_rows<5>(lds));
     if (g.r != 0) return;
     f3 gi{acc[1], acc[2], acc[3]};
     gi = gi * P.invRest;
@@ -1415,6 +1434,7 @@ template <int BS, int L, bool kFinal, bool kFast>
 __global__ __launch_bounds__(BS) void k_delta_correct_group(DeviceState d, SphParams P, StepParams S,
                                                             const float4* __restrict__ XsIn,
                                                             float4* __restrict__ XsOut) {
+    __shared__ float lds[(BS / 64) * 3 * 64];
     const LaneGroup<L> g = lane_group<L>(xcd_slot<BS>());
     if (g.i >= d.nslots) return;
     const SceneDev& sc = scene_of(d, g.i);
@@ -1445,7 +1465,7 @@ __global__ __launch_bounds__(BS) void k_delta_correct_group(DeviceState d, SphPa
             const f3 u = kFast ? coef * gr : (coef * nb.x.w) * gr;
             t[0] = u.x; t[1] = u.y; t[2] = u.z;
         },
-        acc);
+        acc, wave_rows<3>(lds));
     if (g.r != 0) return;
     f3 dp{acc[0], acc[1], acc[2]};
     const f3 q = correct_position(xi, dp, P, sc);
@@ -1458,6 +1478,7 @@ __global__ __launch_bounds__(BS) void k_delta_correct_group(DeviceState d, SphPa
 
 template <int BS, int L, bool kFast>
 __global__ __launch_bounds__(BS) void k_vorticity_group(DeviceState d, SphParams P) {
+    __shared__ float lds[(BS / 64) * 3 * 64];
     const LaneGroup<L> g = lane_group<L>(xcd_slot<BS>());
     if (g.i >= d.nslots || g.i >= scene_of(d, g.i).end) return;
     const f3 xi = make3(d.X[g.i]);
@@ -1473,7 +1494,7 @@ __global__ __launch_bounds__(BS) void k_vorticity_group(DeviceState d, SphParams
             const f3 u = (kFast ? -1.0f : -nb.x.w) * c;
             t[0] = u.x; t[1] = u.y; t[2] = u.z;
         },
-        acc);
+        acc, wave_rows<3>(lds));
     if (g.r != 0) return;
     const f3 w{acc[0], acc[1], acc[2]};
     const float wl = length3(w);
@@ -1483,6 +1504,7 @@ __global__ __launch_bounds__(BS) void k_vorticity_group(DeviceState d, SphParams
 
 template <int BS, int L, bool kFast>
 __global__ __launch_bounds__(BS) void k_confinement_group(DeviceState d, SphParams P, StepParams S) {
+    __shared__ float lds[(BS / 64) * 3 * 64];
     const LaneGroup<L> g = lane_group<L>(xcd_slot<BS>());
     if (g.i >= d.nslots || g.i >= scene_of(d, g.i).end) return;
     const float4 xi4 = d.X[g.i];
@@ -1500,13 +1522,14 @@ __global__ __launch_bounds__(BS) void k_confinement_group(DeviceState d, SphPara
             const f3 u = (kFast ? diff : nb.w * diff) * gr;
             t[0] = u.x; t[1] = u.y; t[2] = u.z;
         },
-        acc);
+        acc, wave_rows<3>(lds));
     if (g.r != 0) return;
     confine_finish(d, P, S, g.i, xi4.w, wi, f3{acc[0], acc[1], acc[2]}, invD);
 }
 
 template <int BS, int L>
 __global__ __launch_bounds__(BS) void k_xsph_group(DeviceState d, SphParams P) {
+    __shared__ float lds[(BS / 64) * 3 * 64];
     const LaneGroup<L> g = lane_group<L>(xcd_slot<BS>());
     if (g.i >= d.nslots || g.i >= scene_of(d, g.i).end) return;
     const f3 xi = make3(d.X[g.i]);
@@ -1522,7 +1545,7 @@ __global__ __launch_bounds__(BS) void k_xsph_group(DeviceState d, SphParams P) {
             const f3 u = (nb.v.w * vij) * w;
             t[0] = u.x; t[1] = u.y; t[2] = u.z;
         },
-        acc);
+        acc, wave_rows<3>(lds));
     if (g.r != 0) return;
     f3 vn = vi + P.visc * f3{acc[0], acc[1], acc[2]};
     d.V[g.i] = make_float4(vn.x, vn.y, vn.z, 0.0f);
