@@ -81,7 +81,7 @@ class AkuaTuning(ctypes.Structure):
                 ("hashDigitBits", ctypes.c_int32), ("layoutDigitBits", ctypes.c_int32),
                 ("layoutInterval", ctypes.c_int32), ("balanceTile", ctypes.c_int32),
                 ("slabBatch", ctypes.c_int32), ("fuseSweep", ctypes.c_int32),
-                ("laneGroup", ctypes.c_int32)]
+                ("laneGroup", ctypes.c_int32), ("workQueue", ctypes.c_int32)]
 
     def as_dict(self):
         return {k: int(getattr(self, k)) for k, _ in self._fields_}

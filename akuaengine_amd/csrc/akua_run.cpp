@@ -55,8 +55,8 @@ const char* kUsage =
     "  --boundary-spacing S    their spacing (default: particle_spacing)\n"
     "  --autotune              pick launch and layout settings before the run (results unchanged)\n"
     "  --tuning LIST           set them instead: block,level,hashBits,layoutBits,interval,\n"
-    "                          balanceTile,slabBatch,fuseSweep,laneGroup (akua_tuning's fields in\n"
-    "                          order; missing trailing fields are 0)\n"
+    "                          balanceTile,slabBatch,fuseSweep,laneGroup,workQueue (akua_tuning's\n"
+    "                          fields in order; missing trailing fields are 0)\n"
     "  --network FILE.npy      learned warm start with this parameter blob (float32 .npy)\n"
     "  --learned-iterations J  (default 1)\n"
     "  --gate-ratio R          fallback when r(x*_w) > R r(x*_pred) + tolerance (default 1)\n"
@@ -181,7 +181,7 @@ Options parse(int argc, char** argv) {
                 if (comma == std::string::npos) break;
                 pos = comma + 1;
             }
-            if (o.tuning.size() < 5 || o.tuning.size() > 9) usage_error("--tuning takes 5 to 9 values");
+            if (o.tuning.size() < 5 || o.tuning.size() > 10) usage_error("--tuning takes 5 to 10 values");
         }
         else if (a == "--render-every") o.renderEvery = parse_int(a, val(), 1, INT32_MAX);
         else if (a == "--render-dir") o.renderDir = val();
@@ -434,7 +434,7 @@ int main(int argc, char** argv) {
     if (!o.tuning.empty()) {
         int32_t* f[] = {&tuning.block, &tuning.layoutLevel, &tuning.hashDigitBits, &tuning.layoutDigitBits,
                         &tuning.layoutInterval, &tuning.balanceTile, &tuning.slabBatch, &tuning.fuseSweep,
-                        &tuning.laneGroup};
+                        &tuning.laneGroup, &tuning.workQueue};
         for (size_t k = 0; k < o.tuning.size(); ++k) *f[k] = o.tuning[k];
         check(akua_solver_set_tuning(s, &tuning), "set_tuning");
     }
@@ -491,12 +491,12 @@ int main(int argc, char** argv) {
     std::printf("{\"source\": %s, \"particles\": %d, \"params\": \"%s\", \"iterations\": %d, \"dt\": %.9g, "
                 "\"frames\": %lld, \"steps\": %lld, \"seconds\": %.6f, \"io_seconds\": %.6f, "
                 "\"particle_substeps_per_s\": %.6g, \"max_density_error\": %.6g, \"mean_density_error\": %.6g, "
-                "\"tuning\": [%d, %d, %d, %d, %d, %d, %d, %d, %d], \"graph\": %s, \"boundary_layers\": %d",
+                "\"tuning\": [%d, %d, %d, %d, %d, %d, %d, %d, %d, %d], \"graph\": %s, \"boundary_layers\": %d",
                 json_str(source).c_str(), n, referenceParams ? "reference" : "dam_break", o.iterations, dt,
                 (long long)frames, (long long)done, total, ioSeconds,
                 stepSeconds > 0.0 ? (double)n * (double)done / stepSeconds : 0.0, maxErr, meanErr, tuning.block,
                 tuning.layoutLevel, tuning.hashDigitBits, tuning.layoutDigitBits, tuning.layoutInterval,
-                tuning.balanceTile, tuning.slabBatch, tuning.fuseSweep, tuning.laneGroup,
+                tuning.balanceTile, tuning.slabBatch, tuning.fuseSweep, tuning.laneGroup, tuning.workQueue,
                 o.graph ? "true" : "false", o.boundary);
     if (!o.network.empty())
         std::printf(", \"learned\": {\"iterations\": %d, \"learned_steps\": %lld, \"fallbacks\": %lld}", o.learnedIters,

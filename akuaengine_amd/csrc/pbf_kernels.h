@@ -82,6 +82,14 @@ struct DeviceState {
     const uint32_t* work;
     const uint32_t* cntW;
     const uint4* nbrW;
+    // the sweeps' work queue (tuning workQueue): per XCD x, queue[x * queueT8 + k] is its k-th
+    // 256-slot tile (heaviest first) and qcount[x] how many it holds; heads holds the counters of
+    // the dynamic mode (queue_head); queueMode 1: dynamic, 2: static
+    const uint32_t* queue;
+    const uint32_t* qcount;
+    uint32_t* heads;
+    uint32_t queueT8;
+    uint32_t queueMode;
     unsigned long long* nbrSum;  // per-workgroup partial sums of neighbour counts (statistics)
     // learned warm start (null when unused)
     float* feat;               // 16 features per slot at x*_pred
@@ -181,6 +189,32 @@ void launch_vorticity(hipStream_t st, const DeviceState& d, const SphParams& P, 
 void launch_confinement(hipStream_t st, const DeviceState& d, const SphParams& P,
                         const StepParams& S, int bs, int lanes = 1);
 void launch_xsph(hipStream_t st, const DeviceState& d, const SphParams& P, int bs, int lanes = 1);
+
+// The work queue of the sweeps (DeviceState::queue): per-tile neighbour counts into cost, each
+// XCD's tiles heaviest first, and the pass counters of `passes` persistent launches cleared.
+// The queued sweeps run as many 256-thread workgroups as the device holds at once (a multiple
+// of 8, at most 8 per queued tile of an XCD); `pass` numbers the launch within the step (its
+// counters in DeviceState::heads).
+constexpr uint32_t kQueueTileSlots = 256;
+constexpr uint32_t kXcdCount = 8;  // XCDs of the MI355X; workgroup b runs on XCD b % 8
+// Dynamic mode: each XCD's queue is served through kQueueSubs counters, each on its own 128-byte
+// line (kQueueHeadStride counters apart); heads holds passes * kXcdCount * kQueueSubs of them.
+constexpr uint32_t kQueueSubs = 8;
+constexpr uint32_t kQueueHeadStride = 32;
+inline size_t queue_heads_size(uint32_t passes) {
+    return (size_t)passes * kXcdCount * kQueueSubs * kQueueHeadStride;
+}
+void launch_tile_queue(hipStream_t st, const DeviceState& d, const SphParams& P, uint32_t* cost,
+                       uint32_t* queue, uint32_t* qcount, uint32_t* heads, uint32_t passes);
+void launch_density_lambda_queue(hipStream_t st, const DeviceState& d, const SphParams& P,
+                                 const float4* Xs, uint32_t pass);
+void launch_delta_correct_queue(hipStream_t st, const DeviceState& d, const SphParams& P,
+                                const StepParams& S, const float4* XsIn, float4* XsOut, bool final,
+                                uint32_t pass);
+void launch_vorticity_queue(hipStream_t st, const DeviceState& d, const SphParams& P, uint32_t pass);
+void launch_confinement_queue(hipStream_t st, const DeviceState& d, const SphParams& P,
+                              const StepParams& S, uint32_t pass);
+void launch_xsph_queue(hipStream_t st, const DeviceState& d, const SphParams& P, uint32_t pass);
 // Exact SPH density at the positions held in XsA; rho by logical index, per-scene error max and
 // sum.
 void launch_density_probe(hipStream_t st, const DeviceState& d, const SphParams& P, float* rho,

@@ -24,6 +24,10 @@
 #include "pbf_kernels.h"
 
 #include <algorithm>
+#include <map>
+#include <mutex>
+#include <stdexcept>
+#include <string>
 #include "boundary.h"
 #include "pbf_math.h"
 
@@ -31,7 +35,7 @@ namespace akua {
 
 namespace {
 
-constexpr uint32_t kXcds = 8;
+constexpr uint32_t kXcds = kXcdCount;
 
 __device__ __forceinline__ const SceneDev& scene_of(const DeviceState& d, uint32_t i) {
     return d.scenes[d.chunkScene[i >> 8]];
@@ -1223,9 +1227,8 @@ __global__ __launch_bounds__(BS) void k_delta_correct(DeviceState d, SphParams P
 // kernel_compute_vorticities (IntegrationCUDA.cu:104-128).  W.w keeps |omega| for the next pass,
 // which also finds it beside the position in XsB (free between the last Jacobi iteration and the
 // next step's): (x, |omega|), one gather per neighbour there.
-template <int BS, bool kFast>
-__global__ __launch_bounds__(BS) void k_vorticity(DeviceState d, SphParams P) {
-    const uint32_t t = xcd_slot<BS>();
+template <bool kFast>
+__device__ __forceinline__ void vorticity_slot(const DeviceState& d, const SphParams& P, uint32_t t) {
     if (t >= d.nslots) return;
     const uint32_t i = work_slot(d, t);
     if (i >= scene_of(d, i).end) return;
@@ -1245,6 +1248,11 @@ __global__ __launch_bounds__(BS) void k_vorticity(DeviceState d, SphParams P) {
     const float wl = length3(w);
     d.W[i] = make_float4(w.x, w.y, w.z, wl);
     d.XsB[i] = make_float4(xi.x, xi.y, xi.z, wl);
+}
+
+template <int BS, bool kFast>
+__global__ __launch_bounds__(BS) void k_vorticity(DeviceState d, SphParams P) {
+    vorticity_slot<kFast>(d, P, xcd_slot<BS>());
 }
 
 // The confinement force from the summed eta (IntegrationCUDA.cu:146-163) and the write of Vc.
@@ -1267,9 +1275,9 @@ __device__ __forceinline__ void confine_finish(const DeviceState& d, const SphPa
 // kernel_apply_vorticity_confinement (IntegrationCUDA.cu:130-165), writing Vc: the confined
 // velocity, with the particle's m / rho in .w for the XSPH pass (the quotient the reference
 // forms per neighbour, IntegrationCUDA.cu:184, computed once per particle instead).
-template <int BS, bool kFast>
-__global__ __launch_bounds__(BS) void k_confinement(DeviceState d, SphParams P, StepParams S) {
-    const uint32_t t = xcd_slot<BS>();
+template <bool kFast>
+__device__ __forceinline__ void confinement_slot(const DeviceState& d, const SphParams& P,
+                                                 const StepParams& S, uint32_t t) {
     if (t >= d.nslots) return;
     const uint32_t i = work_slot(d, t);
     if (i >= scene_of(d, i).end) return;
@@ -1293,10 +1301,13 @@ __global__ __launch_bounds__(BS) void k_confinement(DeviceState d, SphParams P, 
     confine_finish(d, P, S, i, xi4w, wi, eta, invD);
 }
 
+template <int BS, bool kFast>
+__global__ __launch_bounds__(BS) void k_confinement(DeviceState d, SphParams P, StepParams S) {
+    confinement_slot<kFast>(d, P, S, xcd_slot<BS>());
+}
+
 // kernel_apply_xsph_viscosity (IntegrationCUDA.cu:167-195), Jacobi: reads Vc, writes V.
-template <int BS>
-__global__ __launch_bounds__(BS) void k_xsph(DeviceState d, SphParams P) {
-    const uint32_t t = xcd_slot<BS>();
+__device__ __forceinline__ void xsph_slot(const DeviceState& d, const SphParams& P, uint32_t t) {
     if (t >= d.nslots) return;
     const uint32_t i = work_slot(d, t);
     if (i >= scene_of(d, i).end) return;
@@ -1315,6 +1326,151 @@ __global__ __launch_bounds__(BS) void k_xsph(DeviceState d, SphParams P) {
         });
     f3 vn = vi + P.visc * dv;
     d.V[i] = make_float4(vn.x, vn.y, vn.z, 0.0f);
+}
+
+template <int BS>
+__global__ __launch_bounds__(BS) void k_xsph(DeviceState d, SphParams P) {
+    xsph_slot(d, P, xcd_slot<BS>());
+}
+
+// ---- work queue: the sweeps as persistent kernels over 256-slot tiles, heaviest first -------
+// A batch of uneven scenes (and the uneven lists of one scene) leave each sweep's last waves
+// running on a few CUs.  With the work queue, the tiles of each XCD's contiguous range (the range
+// xcd_slot gives it) are ordered by the sum of their neighbour counts, heaviest first, once per
+// step after the list build; each sweep then runs a resident grid whose workgroups take tiles of
+// their XCD's queue until it is empty: in the dynamic mode the next free position (through
+// counters), in the static mode every R-th position from their own rank (R resident workgroups
+// per XCD).  Which workgroup runs which tile decides nothing: every slot is computed as in the
+// one-launch sweep.
+constexpr uint32_t kQueueTile = kQueueTileSlots;
+constexpr uint32_t kQueueBuckets = 512;
+constexpr uint32_t kQueueThreads = 1024;
+
+// Per tile: the sum of its slots' neighbour counts (one wave per tile).
+__global__ __launch_bounds__(256) void k_tile_cost(DeviceState d, uint32_t ntiles,
+                                                   uint32_t* __restrict__ cost) {
+    const uint32_t tile = blockIdx.x * 4u + (threadIdx.x >> 6);
+    if (tile >= ntiles) return;  // whole waves
+    uint32_t c = 0;
+    for (uint32_t k = __lane_id(); k < kQueueTile; k += 64u) {
+        const uint32_t s = tile * kQueueTile + k;
+        if (s < d.nslots) c += d.cntW[s];
+    }
+    for (int off = 32; off > 0; off >>= 1) c += __shfl_xor(c, off, 64);
+    if (__lane_id() == 0) cost[tile] = c;
+}
+
+// One workgroup per XCD x: its tiles [x * T8, min((x + 1) * T8, ntiles)) by cost class, heaviest
+// first (a counting sort over kQueueBuckets classes of cost >> shift; the order within a class is
+// whatever the LDS atomics give), their number, and the XCD's pass counters cleared.
+__global__ __launch_bounds__(kQueueThreads) void k_tile_queue(const uint32_t* __restrict__ cost,
+                                                             uint32_t ntiles, uint32_t T8,
+                                                             uint32_t shift, uint32_t* __restrict__ queue,
+                                                             uint32_t* __restrict__ qcount,
+                                                             uint32_t* __restrict__ heads,
+                                                             uint32_t passes) {
+    __shared__ uint32_t start[kQueueBuckets];
+    __shared__ uint32_t wsum[kQueueThreads / 64];
+    const uint32_t x = blockIdx.x, tid = threadIdx.x, lane = __lane_id(), wave = tid >> 6;
+    const uint32_t t0 = x * T8, t1 = min(t0 + T8, ntiles);
+    for (uint32_t k = tid; k < kQueueBuckets; k += kQueueThreads) start[k] = 0u;
+    for (uint32_t k = tid; k < passes * kQueueSubs; k += kQueueThreads)
+        heads[(((size_t)(k / kQueueSubs) * kXcds + x) * kQueueSubs + k % kQueueSubs) * kQueueHeadStride] = 0u;
+    __syncthreads();
+    for (uint32_t t = t0 + tid; t < t1; t += kQueueThreads)
+        atomicAdd(&start[min(cost[t] >> shift, kQueueBuckets - 1u)], 1u);
+    __syncthreads();
+    // exclusive prefix in descending class order: thread r < kQueueBuckets holds class B - 1 - r
+    const uint32_t v = tid < kQueueBuckets ? start[kQueueBuckets - 1u - tid] : 0u;
+    uint32_t inc = v;
+    for (int off = 1; off < 64; off <<= 1) {
+        const uint32_t y = __shfl_up(inc, off, 64);
+        if ((int)lane >= off) inc += y;
+    }
+    if (lane == 63u) wsum[wave] = inc;
+    __syncthreads();
+    uint32_t base = 0;
+    for (uint32_t w = 0; w < wave; ++w) base += wsum[w];
+    __syncthreads();
+    if (tid < kQueueBuckets) start[kQueueBuckets - 1u - tid] = base + inc - v;
+    __syncthreads();
+    for (uint32_t t = t0 + tid; t < t1; t += kQueueThreads) {
+        const uint32_t pos = atomicAdd(&start[min(cost[t] >> shift, kQueueBuckets - 1u)], 1u);
+        queue[t0 + pos] = t;
+    }
+    if (tid == 0) qcount[x] = t1 > t0 ? t1 - t0 : 0u;
+}
+
+// Counter of sub-queue `sub` of XCD x in pass `pass` (the dynamic mode).
+__device__ __forceinline__ uint32_t* queue_head(const DeviceState& d, uint32_t pass, uint32_t x, uint32_t sub) {
+    return d.heads + (((size_t)pass * kXcds + x) * kQueueSubs + sub) * kQueueHeadStride;
+}
+
+// The persistent loop: workgroup b serves XCD b % 8 (where the hardware places it); f(slot) runs
+// once per thread and tile.  Static: rank r = b / 8 of R takes positions r, r + R, ...  Dynamic:
+// the positions sub + 8 m form eight sub-queues (one counter each, so that no single counter
+// takes every request); the workgroup takes the next m of sub-queue (b / 8) % 8 until it is empty,
+// then of the seven others in turn.  Every workgroup leaves once each sub-queue is empty.
+template <class F>
+__device__ __forceinline__ void queue_loop(const DeviceState& d, uint32_t pass, uint32_t& sq, F f) {
+    const uint32_t x = blockIdx.x % kXcds;
+    const uint32_t n = d.qcount[x];
+    const uint32_t* q = d.queue + (size_t)x * d.queueT8;
+    if (d.queueMode == 2) {
+        const uint32_t R = gridDim.x / kXcds;
+        for (uint32_t k = blockIdx.x / kXcds; k < n; k += R) f(q[k] * kQueueTile + threadIdx.x);
+        return;
+    }
+    const uint32_t sub0 = (blockIdx.x / kXcds) % kQueueSubs;
+    for (uint32_t u = 0; u < kQueueSubs; ++u) {
+        const uint32_t sub = (sub0 + u) % kQueueSubs;
+        const uint32_t len = n > sub ? (n - sub + kQueueSubs - 1u) / kQueueSubs : 0u;
+        uint32_t* head = queue_head(d, pass, x, sub);
+        for (;;) {
+            if (threadIdx.x == 0) sq = atomicAdd(head, 1u);
+            __syncthreads();
+            const uint32_t m = sq;
+            __syncthreads();  // every thread has read sq before thread 0 takes the next position
+            if (m >= len) break;
+            f(q[sub + kQueueSubs * m] * kQueueTile + threadIdx.x);
+        }
+    }
+}
+
+template <bool kFast, bool kBnd>
+__global__ __launch_bounds__(kQueueTile) void k_density_lambda_queue(DeviceState d, SphParams P,
+                                                                     const float4* __restrict__ Xs,
+                                                                     uint32_t pass) {
+    __shared__ uint32_t sq;
+    queue_loop(d, pass, sq, [&](uint32_t t) { density_lambda_slot<kQueueTile, kFast, kBnd>(d, P, Xs, -1, 0, t); });
+}
+
+template <bool kFinal, bool kFast, bool kBnd>
+__global__ __launch_bounds__(kQueueTile) void k_delta_correct_queue(DeviceState d, SphParams P, StepParams S,
+                                                                    const float4* __restrict__ XsIn,
+                                                                    float4* __restrict__ XsOut, uint32_t pass) {
+    __shared__ uint32_t sq;
+    queue_loop(d, pass, sq, [&](uint32_t t) {
+        delta_correct_slot<kQueueTile, kFinal, true, kFast, kBnd>(d, P, S, XsIn, XsOut, nullptr, 0, t);
+    });
+}
+
+template <bool kFast>
+__global__ __launch_bounds__(kQueueTile) void k_vorticity_queue(DeviceState d, SphParams P, uint32_t pass) {
+    __shared__ uint32_t sq;
+    queue_loop(d, pass, sq, [&](uint32_t t) { vorticity_slot<kFast>(d, P, t); });
+}
+
+template <bool kFast>
+__global__ __launch_bounds__(kQueueTile) void k_confinement_queue(DeviceState d, SphParams P, StepParams S,
+                                                                  uint32_t pass) {
+    __shared__ uint32_t sq;
+    queue_loop(d, pass, sq, [&](uint32_t t) { confinement_slot<kFast>(d, P, S, t); });
+}
+
+__global__ __launch_bounds__(kQueueTile) void k_xsph_queue(DeviceState d, SphParams P, uint32_t pass) {
+    __shared__ uint32_t sq;
+    queue_loop(d, pass, sq, [&](uint32_t t) { xsph_slot(d, P, t); });
 }
 
 // ---- lane groups: L lanes per particle, for scenes too small to fill the chip ------------------
@@ -1844,6 +2000,92 @@ void launch_xsph(hipStream_t st, const DeviceState& d, const SphParams& P, int b
         return;
     }
     AKUA_BY_BLOCK(bs, (k_xsph<BS><<<grid_xcd(d.nslots, BS), BS, 0, st>>>(d, P)));
+}
+namespace {
+// Workgroups of 256 threads that the device holds at once for `kernel`, per XCD (cached per
+// kernel and device).
+unsigned resident_per_xcd(const void* kernel) {
+    static std::mutex mu;
+    static std::map<std::pair<const void*, int>, unsigned> cache;
+    auto check = [](hipError_t e, const char* what) {
+        if (e != hipSuccess) throw std::runtime_error(std::string(what) + ": " + hipGetErrorString(e));
+    };
+    int dev = 0;
+    check(hipGetDevice(&dev), "hipGetDevice");
+    std::lock_guard<std::mutex> lock(mu);
+    const auto key = std::make_pair(kernel, dev);
+    auto it = cache.find(key);
+    if (it != cache.end()) return it->second;
+    int perCu = 0, cus = 0;
+    check(hipOccupancyMaxActiveBlocksPerMultiprocessor(&perCu, kernel, (int)kQueueTile, 0),
+          "hipOccupancyMaxActiveBlocksPerMultiprocessor");
+    check(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev), "hipDeviceGetAttribute");
+    const unsigned v = std::max(1u, (unsigned)std::max(perCu, 1) * (unsigned)std::max(cus, 1) / kXcds);
+    cache.emplace(key, v);
+    return v;
+}
+// The grid of a queued sweep: a multiple of 8 workgroups, no more per XCD than its queue holds.
+template <class K>
+unsigned queue_grid(K kernel, const DeviceState& d) {
+    return kXcds * std::max(1u, std::min(resident_per_xcd(reinterpret_cast<const void*>(kernel)), d.queueT8));
+}
+}  // namespace
+
+void launch_tile_queue(hipStream_t st, const DeviceState& d, const SphParams& P, uint32_t* cost,
+                       uint32_t* queue, uint32_t* qcount, uint32_t* heads, uint32_t passes) {
+    const uint32_t ntiles = (d.nslots + kQueueTile - 1) / kQueueTile;
+    if (ntiles) k_tile_cost<<<(ntiles + 3) / 4, 256, 0, st>>>(d, ntiles, cost);
+    // cost classes: the largest possible tile cost (256 full lists) spans kQueueBuckets classes
+    const uint64_t maxCost = (uint64_t)kQueueTile * (uint64_t)std::max(P.maxN, 1);
+    uint32_t shift = 0;
+    while ((maxCost >> shift) >= kQueueBuckets) ++shift;
+    k_tile_queue<<<kXcds, kQueueThreads, 0, st>>>(cost, ntiles, d.queueT8, shift, queue, qcount, heads, passes);
+}
+void launch_density_lambda_queue(hipStream_t st, const DeviceState& d, const SphParams& P,
+                                 const float4* Xs, uint32_t pass) {
+#define AKUA_DLQ(K, B) \
+    k_density_lambda_queue<K, B><<<queue_grid(k_density_lambda_queue<K, B>, d), kQueueTile, 0, st>>>(d, P, Xs, pass)
+    if (P.bd.on) {
+        if (P.fast) AKUA_DLQ(true, true); else AKUA_DLQ(false, true);
+    } else {
+        if (P.fast) AKUA_DLQ(true, false); else AKUA_DLQ(false, false);
+    }
+#undef AKUA_DLQ
+}
+void launch_delta_correct_queue(hipStream_t st, const DeviceState& d, const SphParams& P,
+                                const StepParams& S, const float4* XsIn, float4* XsOut, bool final,
+                                uint32_t pass) {
+#define AKUA_DCQ(F, K, B)                                                                             \
+    k_delta_correct_queue<F, K, B><<<queue_grid(k_delta_correct_queue<F, K, B>, d), kQueueTile, 0, st>>>( \
+        d, P, S, XsIn, XsOut, pass)
+    if (P.bd.on) {
+        if (final) {
+            if (P.fast) AKUA_DCQ(true, true, true); else AKUA_DCQ(true, false, true);
+        } else {
+            if (P.fast) AKUA_DCQ(false, true, true); else AKUA_DCQ(false, false, true);
+        }
+    } else if (final) {
+        if (P.fast) AKUA_DCQ(true, true, false); else AKUA_DCQ(true, false, false);
+    } else {
+        if (P.fast) AKUA_DCQ(false, true, false); else AKUA_DCQ(false, false, false);
+    }
+#undef AKUA_DCQ
+}
+void launch_vorticity_queue(hipStream_t st, const DeviceState& d, const SphParams& P, uint32_t pass) {
+    if (P.fast)
+        k_vorticity_queue<true><<<queue_grid(k_vorticity_queue<true>, d), kQueueTile, 0, st>>>(d, P, pass);
+    else
+        k_vorticity_queue<false><<<queue_grid(k_vorticity_queue<false>, d), kQueueTile, 0, st>>>(d, P, pass);
+}
+void launch_confinement_queue(hipStream_t st, const DeviceState& d, const SphParams& P,
+                              const StepParams& S, uint32_t pass) {
+    if (P.fast)
+        k_confinement_queue<true><<<queue_grid(k_confinement_queue<true>, d), kQueueTile, 0, st>>>(d, P, S, pass);
+    else
+        k_confinement_queue<false><<<queue_grid(k_confinement_queue<false>, d), kQueueTile, 0, st>>>(d, P, S, pass);
+}
+void launch_xsph_queue(hipStream_t st, const DeviceState& d, const SphParams& P, uint32_t pass) {
+    k_xsph_queue<<<queue_grid(k_xsph_queue, d), kQueueTile, 0, st>>>(d, P, pass);
 }
 void launch_density_probe(hipStream_t st, const DeviceState& d, const SphParams& P, float* rho,
                           unsigned int* errMaxBits, double* errSum) {

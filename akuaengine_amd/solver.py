@@ -121,9 +121,10 @@ class PBFSolver:
     # -- tuning -------------------------------------------------------------------------------
     def set_tuning(self, block: int = 256, layoutLevel: int = 1, hashDigitBits: int = 0,
                    layoutDigitBits: int = 0, layoutInterval: int = 1, balanceTile: int = 0,
-                   slabBatch: int = 0, fuseSweep: int = 0, laneGroup: int = 0) -> None:
+                   slabBatch: int = 0, fuseSweep: int = 0, laneGroup: int = 0,
+                   workQueue: int = 0) -> None:
         t = _lib.AkuaTuning(block, layoutLevel, hashDigitBits, layoutDigitBits, layoutInterval,
-                            balanceTile, slabBatch, fuseSweep, laneGroup)
+                            balanceTile, slabBatch, fuseSweep, laneGroup, workQueue)
         _lib.check(self._L.akua_solver_set_tuning(self._h, ctypes.byref(t)))
 
     def set_search(self, cellsPerH: int) -> None:

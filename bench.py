@@ -30,7 +30,7 @@ Also reported, each in its own object of the same line:
 
 Usage: python bench.py [--steps K] [--warmup W] [--gpus 1] [--cpu-seconds S] [--no-cpu-baseline]
                        [--no-learned] [--no-extra]
-                       [--tuning block,level,hashBits,layoutBits,interval[,balanceTile[,slabBatch[,fuseSweep[,laneGroup]]]]]
+                       [--tuning block,level,hashBits,layoutBits,interval[,balanceTile[,slabBatch[,fuseSweep[,laneGroup[,workQueue]]]]]]
 """
 from __future__ import annotations
 
@@ -80,6 +80,7 @@ BYTES = {
     "rs_scatter": lambda nb: 16,
     "rs_onesweep": lambda nb: 16,
     "fill_table": lambda nb: 8,
+    "tile_queue": lambda nb: 4,
 }
 
 
@@ -313,8 +314,9 @@ def extra_workloads(steps: int = 20, warmup: int = 3):
     b.set_state(np.concatenate([x.pos for x in scs]), np.concatenate([x.vel for x in scs]))
     for i, x in enumerate(scs):
         b.set_box(i, x.box_min, x.box_max)
-    b.autotune(dt, trialSteps=3)
     b.step_batch(dt, warmup)
+    b.autotune(dt, trialSteps=3)  # on a developed state, as the single scenes
+    b.step_batch(dt, 1)
     b.synchronize()
     t0 = time.perf_counter()
     b.step_batch(dt, steps)
@@ -437,7 +439,7 @@ def main() -> int:
         s.set_tuning(block=b_, layoutLevel=l_, hashDigitBits=h_, layoutDigitBits=d_,
                      layoutInterval=i_, balanceTile=v[5] if len(v) > 5 else 0,
                      slabBatch=v[6] if len(v) > 6 else 0, fuseSweep=v[7] if len(v) > 7 else 0,
-                     laneGroup=v[8] if len(v) > 8 else 0)
+                     laneGroup=v[8] if len(v) > 8 else 0, workQueue=v[9] if len(v) > 9 else 0)
         tuning = s.get_tuning()
     else:
         tuning = s.autotune(dt, sc.box_min, sc.box_max, trialSteps=5)

@@ -171,6 +171,14 @@ typedef struct akua_tuning {
                                  in list order) and the list build one lane per stencil cell;
                                  applies with balanceTile 0, slabBatch 0, the reference's search and
                                  no boundary samples, and takes precedence over fuseSweep */
+    int32_t workQueue;        /* 0 (the default): each neighbour sweep after the list build is one
+                                 launch with a workgroup per block of slots; 1 or 2: after the list
+                                 build, the 256-slot tiles each XCD serves are queued heaviest first
+                                 (by their neighbour counts) and the sweeps of a plain step run as
+                                 resident grids whose workgroups take tiles from their XCD's queue
+                                 until it is empty: 1 the next free position (through counters),
+                                 2 every R-th position from the workgroup's rank among the R of
+                                 its XCD; laneGroup takes precedence */
 } akua_tuning;
 
 akua_status akua_solver_set_tuning(akua_solver* s, const akua_tuning* t);
@@ -194,11 +202,12 @@ akua_status akua_solver_get_search(const akua_solver* s, int32_t* cellsPerH);
 akua_status akua_solver_set_exact(akua_solver* s, int32_t exact);
 akua_status akua_solver_get_tuning(const akua_solver* s, akua_tuning* t);
 /* Times short trials of candidate settings by coordinate descent (block, then layout level, then
- * the two digit widths, then the layout interval, the balance tile and the slab batch) and keeps
- * the fastest.  Each trial restarts from a copy of the current
+ * the two digit widths, then the layout interval, the balance tile, the slab batch and the lane
+ * group, then, without lane groups, the fused first sweep and the work queue) and keeps the
+ * fastest.  Each trial restarts from a copy of the current
  * particle state, which is restored afterwards, so the call leaves the state as it found it.
  * boxMin/boxMax: the box the trials step in (NULL: each scene's current box).  trialSteps: timed
- * steps per candidate (1..1000).  cachePath (NULL: no cache): a text file of earlier results keyed
+ * steps per candidate (1..1000), run twice; the faster run counts.  cachePath (NULL: no cache): a text file of earlier results keyed
  * by device architecture, log2 of the particle count, log2 of the scene count, neighbour cap and
  * iteration count; a hit applies the cached setting without trials, a miss appends the result.
  * best (may be NULL) receives the setting now in use. */

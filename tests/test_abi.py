@@ -101,7 +101,7 @@ def test_null_pointers_are_rejected(lib):
 
 def test_tuning_struct_and_null_arguments(lib):
     from akuaengine_amd import _lib
-    assert ctypes.sizeof(_lib.AkuaTuning) == 36
+    assert ctypes.sizeof(_lib.AkuaTuning) == 40
     t = _lib.AkuaTuning(256, 1, 0, 0, 1, 0, 0)
     assert lib.akua_solver_set_tuning(None, ctypes.byref(t)) == 1
     assert lib.akua_solver_get_tuning(None, ctypes.byref(t)) == 1

@@ -314,6 +314,55 @@ def test_lane_groups_bit_exact(gpu, oracle_mod, dam_params, lanes, case):
         assert (s.debug_neighbours()[1] == cfg.maxNeighbours).any()
 
 
+QUEUE_CASES = ["unit_masses", "random_masses", "cap_1", "tile_edge_65", "iterations_7", "balanced_order",
+               "fused_first_sweep", "boundary_samples", "one_million"]
+
+
+@pytest.mark.parametrize("case", QUEUE_CASES)
+@pytest.mark.parametrize("mode", [1, 2])
+def test_work_queue_bit_exact(gpu, oracle_mod, dam_params, mode, case):
+    """The sweeps as persistent launches over each XCD's queue of tiles (tuning workQueue 1:
+    positions taken through counters, 2: fixed positions per workgroup) equal the oracle bit for
+    bit: unit and other masses, a cap of 1, 65 particles (one partial
+    tile), 7 iterations (17 queued launches a step), with the balanced work order, with the first
+    sweep in the list build's launch, with boundary samples, and one step of the 1M workload."""
+    from akuaengine_amd import scenes
+    cfg, corr = dam_params
+    sc = scenes.dam_break_scaled(14, seed=6)
+    pos, vel, mass, steps = sc.pos, sc.vel, None, 3
+    tuning = dict(block=256, layoutLevel=1, workQueue=mode)
+    layers = 0
+    if case == "one_million":
+        sc = scenes.dam_break_1m()
+        pos, vel, mass, steps = sc.pos, sc.vel, sc.mass, 1
+    elif case == "random_masses":
+        mass = np.random.default_rng(8).uniform(0.5, 1.5, size=len(pos)).astype(np.float32)
+    elif case == "cap_1":
+        cfg = dataclasses.replace(cfg, maxNeighbours=1)
+    elif case == "tile_edge_65":
+        pos, vel = pos[:65].copy(), vel[:65].copy()
+    elif case == "iterations_7":
+        cfg = dataclasses.replace(cfg, solverIterations=7)
+    elif case == "balanced_order":
+        tuning["balanceTile"] = 512
+    elif case == "fused_first_sweep":
+        tuning["fuseSweep"] = 1
+    elif case == "boundary_samples":
+        layers = 2
+    _say(f"[parity] work queue mode {mode}, {case}, N={len(pos)}")
+    s, o = _pair(gpu, oracle_mod, pos, vel, cfg, corr, mass)
+    if layers:
+        s.set_boundary(layers, 0.0)
+        o.set_boundary(layers, 0.0)
+    s.set_tuning(**tuning)
+    assert s.get_tuning()["workQueue"] == mode
+    for _ in range(steps):
+        s.step(DT, sc.box_min, sc.box_max)
+        o.step(DT, sc.box_min, sc.box_max)
+    _assert_taps_equal(s, o)
+    _assert_state_equal(s, o)
+
+
 def test_disabled_artificial_pressure(gpu, oracle_mod, dam_params):
     """LambdaCorrParams.enabled = False (the term dropped) equals the oracle bit for bit."""
     from akuaengine_amd import scenes

@@ -1,6 +1,6 @@
 """Tuning choices (workgroup size of the neighbour kernels, physical super-cell size, radix digit
 widths, layout interval, the balanced work order of the neighbour sweeps, the slab walk, the
-first sweep fused into the list build) change only speed: every
+first sweep fused into the list build, lane groups, the sweeps' work queue) change only speed: every
 setting yields bit-identical trajectories, and the autotuner leaves the particle state as it found
 it."""
 import numpy as np
@@ -10,25 +10,35 @@ pytestmark = pytest.mark.gpu
 DT = 0.0083
 SETTINGS = [
     dict(block=256, layoutLevel=1, hashDigitBits=0, layoutDigitBits=0, layoutInterval=1,
-         balanceTile=0, slabBatch=0, fuseSweep=0, laneGroup=0),
+         balanceTile=0, slabBatch=0, fuseSweep=0, laneGroup=0, workQueue=0),
     dict(block=128, layoutLevel=0, hashDigitBits=5, layoutDigitBits=5, layoutInterval=3,
-         balanceTile=1024, slabBatch=8, fuseSweep=1, laneGroup=0),
+         balanceTile=1024, slabBatch=8, fuseSweep=1, laneGroup=0, workQueue=0),
     dict(block=512, layoutLevel=2, hashDigitBits=8, layoutDigitBits=8, layoutInterval=2,
-         balanceTile=256, slabBatch=4, fuseSweep=0, laneGroup=0),
+         balanceTile=256, slabBatch=4, fuseSweep=0, laneGroup=0, workQueue=0),
     dict(block=256, layoutLevel=3, hashDigitBits=6, layoutDigitBits=0, layoutInterval=8,
-         balanceTile=2048, slabBatch=0, fuseSweep=0, laneGroup=0),
+         balanceTile=2048, slabBatch=0, fuseSweep=0, laneGroup=0, workQueue=0),
     dict(block=128, layoutLevel=1, hashDigitBits=0, layoutDigitBits=0, layoutInterval=1,
-         balanceTile=768, slabBatch=8, fuseSweep=0, laneGroup=0),
+         balanceTile=768, slabBatch=8, fuseSweep=0, laneGroup=0, workQueue=0),
     dict(block=128, layoutLevel=0, hashDigitBits=0, layoutDigitBits=0, layoutInterval=1,
-         balanceTile=0, slabBatch=0, fuseSweep=1, laneGroup=0),
+         balanceTile=0, slabBatch=0, fuseSweep=1, laneGroup=0, workQueue=0),
     dict(block=512, layoutLevel=1, hashDigitBits=6, layoutDigitBits=8, layoutInterval=2,
-         balanceTile=0, slabBatch=0, fuseSweep=1, laneGroup=0),
+         balanceTile=0, slabBatch=0, fuseSweep=1, laneGroup=0, workQueue=0),
     dict(block=128, layoutLevel=1, hashDigitBits=0, layoutDigitBits=0, layoutInterval=1,
-         balanceTile=0, slabBatch=0, fuseSweep=1, laneGroup=8),
+         balanceTile=0, slabBatch=0, fuseSweep=1, laneGroup=8, workQueue=0),
     dict(block=256, layoutLevel=0, hashDigitBits=0, layoutDigitBits=6, layoutInterval=3,
-         balanceTile=0, slabBatch=0, fuseSweep=0, laneGroup=16),
+         balanceTile=0, slabBatch=0, fuseSweep=0, laneGroup=16, workQueue=0),
     dict(block=512, layoutLevel=2, hashDigitBits=5, layoutDigitBits=0, layoutInterval=1,
-         balanceTile=512, slabBatch=0, fuseSweep=0, laneGroup=4),
+         balanceTile=512, slabBatch=0, fuseSweep=0, laneGroup=4, workQueue=0),
+    dict(block=256, layoutLevel=1, hashDigitBits=0, layoutDigitBits=0, layoutInterval=1,
+         balanceTile=0, slabBatch=0, fuseSweep=0, laneGroup=0, workQueue=1),
+    dict(block=128, layoutLevel=0, hashDigitBits=5, layoutDigitBits=0, layoutInterval=2,
+         balanceTile=1024, slabBatch=8, fuseSweep=0, laneGroup=0, workQueue=2),
+    dict(block=512, layoutLevel=1, hashDigitBits=0, layoutDigitBits=0, layoutInterval=1,
+         balanceTile=0, slabBatch=0, fuseSweep=1, laneGroup=0, workQueue=1),
+    dict(block=256, layoutLevel=1, hashDigitBits=0, layoutDigitBits=0, layoutInterval=1,
+         balanceTile=0, slabBatch=0, fuseSweep=0, laneGroup=8, workQueue=2),
+    dict(block=128, layoutLevel=0, hashDigitBits=0, layoutDigitBits=0, layoutInterval=1,
+         balanceTile=0, slabBatch=0, fuseSweep=1, laneGroup=0, workQueue=2),
 ]
 
 
@@ -50,7 +60,7 @@ def _run(PBFSolver, sc, cfg, corr, steps, tuning=None):
     return s
 
 
-@pytest.mark.parametrize("tuning", SETTINGS[1:], ids=lambda t: "b{block}-L{layoutLevel}-h{hashDigitBits}-l{layoutDigitBits}-i{layoutInterval}-t{balanceTile}-s{slabBatch}-f{fuseSweep}-g{laneGroup}".format(**t))
+@pytest.mark.parametrize("tuning", SETTINGS[1:], ids=lambda t: "b{block}-L{layoutLevel}-h{hashDigitBits}-l{layoutDigitBits}-i{layoutInterval}-t{balanceTile}-s{slabBatch}-f{fuseSweep}-g{laneGroup}-q{workQueue}".format(**t))
 def test_every_setting_gives_identical_trajectories(classes, dam_params, tuning):
     _, PBFSolver = classes
     from akuaengine_amd import scenes
@@ -90,7 +100,7 @@ def test_invalid_settings_are_rejected(classes, dam_params):
                 dict(layoutInterval=0), dict(layoutInterval=9), dict(balanceTile=100),
                 dict(balanceTile=4096), dict(balanceTile=-256), dict(slabBatch=2),
                 dict(slabBatch=16), dict(fuseSweep=2), dict(fuseSweep=-1), dict(laneGroup=2),
-                dict(laneGroup=32), dict(laneGroup=1)):
+                dict(laneGroup=32), dict(laneGroup=1), dict(workQueue=3), dict(workQueue=-1)):
         with pytest.raises(AkuaError):
             s.set_tuning(**bad)
     assert s.get_tuning() == SETTINGS[0]
@@ -129,7 +139,7 @@ def test_batch_settings_are_invariant(classes, dam_params):
     scs = [scenes.batched_scene(n, i, seed=9) for i, n in enumerate(counts)]
     out = []
     for t in (SETTINGS[1], SETTINGS[0], SETTINGS[2], SETTINGS[3], SETTINGS[4], SETTINGS[5],
-              SETTINGS[7], SETTINGS[8]):
+              SETTINGS[7], SETTINGS[8], SETTINGS[10], SETTINGS[11], SETTINGS[12], SETTINGS[14]):
         b = BatchSolver(counts, cfg, corr)
         b.set_tuning(**t)
         b.set_state(np.concatenate([s.pos for s in scs]), np.concatenate([s.vel for s in scs]))
