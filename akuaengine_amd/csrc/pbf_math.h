@@ -49,10 +49,16 @@ __device__ __forceinline__ float poly6(float d2, const SphParams& P) {
 //   q = div_fmas(t, r, q); q = div_fixup(q, b, a).
 // For a positive divisor and a numerator of magnitude in [2^-40, 2^40] neither div_scale scales
 // (exponents far from the limits, difference below 96) nor div_fixup changes anything, and
-// div_fmas is a plain fma, so the same operations without those three instructions give the
-// identical quotient -- and the reciprocal refinement can be shared by several numerators over one
-// divisor.  Outside that range the '/' operator is used.  akua_selftest_arith checks the
-// equality on the device.
+// div_fmas is a plain fma.  Two facts shorten the rest:
+//  - the refined reciprocal r = fma(fma(-b, rcp(b), 1), rcp(b), rcp(b)) is RN(1/b), the correctly
+//    rounded reciprocal, for every significand of b (checked exhaustively on the device; the
+//    exponent only scales rcp's result in this range), and it is shared by every numerator over
+//    one divisor;
+//  - with y = RN(1/b), q = RN(a y) and the exact remainder t = a - b q (one fma), RN(q + t y) is
+//    RN(a / b) (Markstein's theorem on the final correction of a division), so one correction
+//    step instead of two gives the same quotient.
+// Outside that range the '/' operator is used.  akua_selftest_arith checks the reciprocal and the
+// quotient against '/' on the device.
 struct Recip {
     float b, r;
 };
@@ -70,9 +76,7 @@ __device__ __forceinline__ Recip recip(float b) {
 // lose (one bit insert instead of a compare and a select).
 __device__ __forceinline__ float div_rn_unchecked(float a, const Recip& R) {
     float q = a * R.r;
-    float t = fmaf(-R.b, q, a);
-    q = fmaf(t, R.r, q);
-    t = fmaf(-R.b, q, a);
+    const float t = fmaf(-R.b, q, a);
     q = fmaf(t, R.r, q);
     return __uint_as_float((__float_as_uint(q) & 0x7FFFFFFFu) | (__float_as_uint(a) & 0x80000000u));
 }
@@ -121,9 +125,11 @@ __device__ __forceinline__ f3 div3_rn(f3 v, float b) {
 
 // a / b where b and an upper bound of |a| are known to lie in the exact-division range: only
 // the lower bound of |a| (zero or >= 2^-40) is tested.
+// The fallback is marked unlikely so that the compiler keeps it behind a branch that every wave
+// skips, instead of computing the full division for every term and selecting.
 __device__ __forceinline__ float div_rn_small(float a, const Recip& R) {
     float q = div_rn_unchecked(a, R);
-    if (low2(a) < kDivLo2) q = a / R.b;
+    if (__builtin_expect(low2(a) < kDivLo2, 0)) q = a / R.b;
     return q;
 }
 

@@ -1,5 +1,5 @@
-// selftest.hip -- on-device check that the written-out division and square root (pbf_math.h)
-// return the same bits as the correctly rounded '/' and sqrtf on this GPU.
+// selftest.hip -- on-device check that the written-out division (and its shared reciprocal) and
+// square root (pbf_math.h) return the same bits as the correctly rounded '/' and sqrtf on this GPU.
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
@@ -49,6 +49,9 @@ __global__ void k_selftest(uint64_t n, uint64_t seed, unsigned long long* counts
         }
         const akua::Recip R = akua::recip(b);
         if (!same_bits(akua::div_rn(a, R), a / b)) bad[0]++;
+        // the shared reciprocal is the correctly rounded one over the divisors div_rn keeps
+        const float c = fabsf(rand_float(z2, -40, 39));
+        if (!same_bits(akua::recip(c).r, 1.0f / c)) bad[0]++;
     }
     // square root: x >= 2^-96 must match sqrtf; below, the result must stay under 2^-47
     {

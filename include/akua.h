@@ -384,7 +384,8 @@ akua_status akua_debug_probe_density(akua_solver* s, float* rho, int32_t n);
 akua_status akua_stream_copy_bandwidth(int64_t bytes, int32_t reps, double* gbps);
 
 /* Device self-test of the written-out correctly rounded division and square root the kernels use
- * (exact equality with '/' and sqrtf over n random inputs per check).  mismatches[0]: division,
+ * (exact equality with '/' and sqrtf over n random inputs per check).  mismatches[0]: division
+ * and its shared reciprocal (equal to 1.0f / b over divisors in [2^-40, 2^40)),
  * [1]: square root over [2^-96, 2^128), [2]: square roots below 2^-96 that reach 2^-47,
  * [3]: square root over squared lengths in [0, 2^-2).  All four are 0 on a conforming device. */
 akua_status akua_selftest_arith(uint64_t n, uint64_t seed, uint64_t mismatches[4]);
