@@ -101,6 +101,14 @@ struct XW {
     float w;
 };
 
+// Element j of a per-slot array through a 32-bit byte offset from its base: slot counts stay
+// below kMaxSlots (2^28), so j * sizeof(T) <= 16 j fits in 32 bits, and the gather then takes the
+// base from scalar registers and one 32-bit lane offset instead of a 64-bit address per lane.
+template <class T>
+__device__ __forceinline__ T ld(const T* __restrict__ base, uint32_t j) {
+    return *reinterpret_cast<const T*>(reinterpret_cast<const char*>(base) + (uint32_t)(j * (uint32_t)sizeof(T)));
+}
+
 // Physical slot of this thread under the XCD-contiguous workgroup order (the grid is a multiple
 // of 8 workgroups of BS threads).
 template <int BS>
@@ -350,10 +358,10 @@ __device__ __forceinline__ void neighbours_slot(const DeviceState& d, const SphP
                             const uint32_t m = min(m0 + u, b - 1u);
                             if (run) {
                                 ph[u] = (uint32_t)((int32_t)m + shift);
-                                c[u] = d.XsA[ph[u]];
+                                c[u] = ld(d.XsA, ph[u]);
                             } else {
-                                c[u] = d.XsL[m];
-                                ph[u] = d.physT[m];
+                                c[u] = ld(d.XsL, m);
+                                ph[u] = ld(d.physT, m);
                             }
                         }
                         // test the whole batch without branches, then append in order
@@ -424,10 +432,10 @@ __global__ __launch_bounds__(BS) void k_neighbours_cells(DeviceState d, SphParam
             const uint32_t m = min(m0 + u, b - 1u);
             if (run) {
                 ph[u] = (uint32_t)((int32_t)m + shift);
-                cand[u] = d.XsA[ph[u]];
+                cand[u] = ld(d.XsA, ph[u]);
             } else {
-                cand[u] = d.XsL[m];
-                ph[u] = d.physT[m];
+                cand[u] = ld(d.XsL, m);
+                ph[u] = ld(d.physT, m);
             }
         }
         uint32_t acc = 0u;
@@ -639,10 +647,10 @@ __global__ __launch_bounds__(BS) void k_neighbours_slab(DeviceState d, SphParams
                     for (int u = 0; u < B; ++u) {
                         if (rr[u]) {
                             ph[u] = mm[u];
-                            c[u] = d.XsA[ph[u]];
+                            c[u] = ld(d.XsA, ph[u]);
                         } else {
-                            c[u] = d.XsL[mm[u]];
-                            ph[u] = d.physT[mm[u]];
+                            c[u] = ld(d.XsL, mm[u]);
+                            ph[u] = ld(d.physT, mm[u]);
                         }
                     }
                     uint32_t acc = 0u;
@@ -719,10 +727,10 @@ __global__ __launch_bounds__(BS) void k_neighbours_fine(DeviceState d, SphParams
                                 const uint32_t m = min(m0 + u, b - 1u);
                                 if (run) {
                                     ph[u] = (uint32_t)((int32_t)m + shift);
-                                    c[u] = d.XsA[ph[u]];
+                                    c[u] = ld(d.XsA, ph[u]);
                                 } else {
-                                    c[u] = d.XsL[m];
-                                    ph[u] = d.physT[m];
+                                    c[u] = ld(d.XsL, m);
+                                    ph[u] = ld(d.physT, m);
                                 }
                             }
 #pragma unroll
@@ -804,21 +812,25 @@ __device__ __forceinline__ void density_lambda_slot(const DeviceState& d, const 
         float sum = 0.0f;
         const float negInv = -P.invRest;
         walk_list<float4>(
-            lp, n, [&](uint32_t j) { return Xs[j]; },
+            lp, n, [&](uint32_t j) { return ld(Xs, j); },
             [&](const float4& xj4) {
                 f3 sep = xi - make3(xj4);
                 float d2 = dot3(sep, sep);
-                f3 g = grad_spiky<kFast>(sep, P);
-                if (kFast) {  // m_j == 1: every product by m_j is the other factor itself
-                    rho = rho + poly6(d2, P);
-                    gi = gi + g;
-                    f3 gj = negInv * g;
-                    sum = sum + dot3(gj, gj);
-                } else {
-                    rho = rho + xj4.w * poly6(d2, P);
-                    gi = gi + xj4.w * g;
-                    f3 gj = (negInv * xj4.w) * g;
-                    sum = sum + dot3(gj, gj);
+                // m_j == 1 under kFast: every product by m_j is the other factor itself
+                rho = rho + (kFast ? poly6(d2, P) : xj4.w * poly6(d2, P));
+                // a zero gradient would add zeros to gi and sum, which change neither (both start
+                // at +0, and x + (-x) rounds to +0, so neither is ever -0): it is skipped
+                f3 g;
+                if (grad_spiky_nonzero<kFast>(sep, P, g)) {
+                    if (kFast) {
+                        gi = gi + g;
+                        f3 gj = negInv * g;
+                        sum = sum + dot3(gj, gj);
+                    } else {
+                        gi = gi + xj4.w * g;
+                        f3 gj = (negInv * xj4.w) * g;
+                        sum = sum + dot3(gj, gj);
+                    }
                 }
             });
         if (kBnd) {
@@ -900,7 +912,7 @@ __device__ __forceinline__ void features_slot(const DeviceState& d, const SphPar
         const float negInv = -P.invRest;
         const Recip Rdq = recip(P.wdq);
         walk_list<float4>(
-            lp, n, [&](uint32_t j) { return d.XsA[j]; },
+            lp, n, [&](uint32_t j) { return ld(d.XsA, j); },
             [&](const float4& xj4) {
                 f3 sep = xi - make3(xj4);
                 float d2 = dot3(sep, sep);
@@ -1180,10 +1192,10 @@ __device__ __forceinline__ void delta_correct_slot(const DeviceState& d, const S
             lp, n,
             [&](uint32_t j) {
                 if (kFast) {
-                    const float4 a = d.W[j];
+                    const float4 a = ld(d.W, j);
                     return XLam{a, a.w};
                 }
-                return XLam{XsIn[j], d.lam[j]};
+                return XLam{ld(XsIn, j), ld(d.lam, j)};
             },
             [&](const XLam& nb) {
                 f3 sep = xi - make3(nb.x);
@@ -1238,7 +1250,7 @@ __device__ __forceinline__ void vorticity_slot(const DeviceState& d, const SphPa
     const uint4* lp = d.nbrW + list_base(t, P.maxN);
     f3 w{0.0f, 0.0f, 0.0f};
     walk_list<XV, kBatchVort>(
-        lp, n, [&](uint32_t j) { return XV{d.X[j], d.V[j]}; },
+        lp, n, [&](uint32_t j) { return XV{ld(d.X, j), ld(d.V, j)}; },
         [&](const XV& nb) {
             f3 vij = make3(nb.v) - vi;
             f3 sep = xi - make3(nb.x);
@@ -1291,7 +1303,7 @@ __device__ __forceinline__ void confinement_slot(const DeviceState& d, const Sph
     f3 eta{0.0f, 0.0f, 0.0f};
     // XW.x: (x_j, |omega_j|); XW.w: m_j (read only without unit masses)
     walk_list<XW, kBatchConf>(
-        lp, n, [&](uint32_t j) { return XW{d.XsB[j], kFast ? 1.0f : d.X[j].w}; },
+        lp, n, [&](uint32_t j) { return XW{ld(d.XsB, j), kFast ? 1.0f : ld(d.X, j).w}; },
         [&](const XW& nb) {
             float diff = wi.w - nb.x.w;
             f3 sep = xi - make3(nb.x);
@@ -1317,7 +1329,7 @@ __device__ __forceinline__ void xsph_slot(const DeviceState& d, const SphParams&
     const uint4* lp = d.nbrW + list_base(t, P.maxN);
     f3 dv{0.0f, 0.0f, 0.0f};
     walk_list<XV, kBatchXsph>(
-        lp, n, [&](uint32_t j) { return XV{d.X[j], d.Vc[j]}; },
+        lp, n, [&](uint32_t j) { return XV{ld(d.X, j), ld(d.Vc, j)}; },
         [&](const XV& nb) {
             f3 vij = make3(nb.v) - vi;
             f3 sep = xi - make3(nb.x);
@@ -1557,7 +1569,7 @@ __global__ __launch_bounds__(BS) void k_density_lambda_group(DeviceState d, SphP
     const float negInv = -P.invRest;
     float acc[5] = {xi4.w * P.w0, 0.0f, 0.0f, 0.0f, 0.0f};  // rho, grad_i C (x, y, z), sum |grad_j C|^2
     walk_split<L, 5, float4>(
-        d.nbr4 + list_base(g.i, P.maxN), d.cnt[g.i], g, [&](uint32_t j) { return Xs[j]; },
+        d.nbr4 + list_base(g.i, P.maxN), d.cnt[g.i], g, [&](uint32_t j) { return ld(Xs, j); },
         [&](const float4& xj4, float (&t)[5]) {
             f3 sep = xi - make3(xj4);
             float d2 = dot3(sep, sep);
@@ -1604,10 +1616,10 @@ __global__ __launch_bounds__(BS) void k_delta_correct_group(DeviceState d, SphPa
         d.nbr4 + list_base(g.i, P.maxN), d.cnt[g.i], g,
         [&](uint32_t j) {
             if (kFast) {
-                const float4 a = d.W[j];
+                const float4 a = ld(d.W, j);
                 return XLam{a, a.w};
             }
-            return XLam{XsIn[j], d.lam[j]};
+            return XLam{ld(XsIn, j), ld(d.lam, j)};
         },
         [&](const XLam& nb, float (&t)[3]) {
             f3 sep = xi - make3(nb.x);
@@ -1642,7 +1654,7 @@ __global__ __launch_bounds__(BS) void k_vorticity_group(DeviceState d, SphParams
     float acc[3] = {0.0f, 0.0f, 0.0f};
     walk_split<L, 3, XV>(
         d.nbr4 + list_base(g.i, P.maxN), d.cnt[g.i], g,
-        [&](uint32_t j) { return XV{d.X[j], d.V[j]}; },
+        [&](uint32_t j) { return XV{ld(d.X, j), ld(d.V, j)}; },
         [&](const XV& nb, float (&t)[3]) {
             f3 vij = make3(nb.v) - vi;
             f3 sep = xi - make3(nb.x);
@@ -1670,7 +1682,7 @@ __global__ __launch_bounds__(BS) void k_confinement_group(DeviceState d, SphPara
     float acc[3] = {0.0f, 0.0f, 0.0f};
     walk_split<L, 3, XW>(
         d.nbr4 + list_base(g.i, P.maxN), d.cnt[g.i], g,
-        [&](uint32_t j) { return XW{d.XsB[j], kFast ? 1.0f : d.X[j].w}; },
+        [&](uint32_t j) { return XW{ld(d.XsB, j), kFast ? 1.0f : ld(d.X, j).w}; },
         [&](const XW& nb, float (&t)[3]) {
             float diff = wi.w - nb.x.w;
             f3 sep = xi - make3(nb.x);
@@ -1693,7 +1705,7 @@ __global__ __launch_bounds__(BS) void k_xsph_group(DeviceState d, SphParams P) {
     float acc[3] = {0.0f, 0.0f, 0.0f};
     walk_split<L, 3, XV>(
         d.nbr4 + list_base(g.i, P.maxN), d.cnt[g.i], g,
-        [&](uint32_t j) { return XV{d.X[j], d.Vc[j]}; },
+        [&](uint32_t j) { return XV{ld(d.X, j), ld(d.Vc, j)}; },
         [&](const XV& nb, float (&t)[3]) {
             f3 vij = make3(nb.v) - vi;
             f3 sep = xi - make3(nb.x);
@@ -1747,7 +1759,7 @@ __global__ __launch_bounds__(BS) void k_density_probe(DeviceState d, SphParams P
                 const int32_t shift = (int32_t)e.z - (int32_t)e.x;
                 for (uint32_t m = e.x; m < e.y; ++m) {
                     if (m == self) continue;
-                    float4 xk = e.w ? d.XsA[(uint32_t)((int32_t)m + shift)] : d.XsL[m];
+                    float4 xk = e.w ? ld(d.XsA, (uint32_t)((int32_t)m + shift)) : ld(d.XsL, m);
                     f3 sep = xi - make3(xk);
                     float d2 = dot3(sep, sep);
                     if (d2 < P.h2) rho = rho + xk.w * poly6(d2, P);

@@ -162,14 +162,24 @@ __device__ __forceinline__ float sqrt_rn_unscaled(float x) {
 // below 2^-96 gives |r| < 1e-5, the zero branch, whatever its last bits.  kFast: |r| lies in
 // [1e-5, h] and |cs| h^3 bounds the numerators, both inside the exact-division range
 // (SphParams::fast), so only the numerators' lower bound is tested.
+// grad_spiky_nonzero: the same gradient into g, or false (g untouched) where it is zero, for
+// callers whose sums a zero gradient leaves unchanged.
 template <bool kFast = false>
-__device__ __forceinline__ f3 grad_spiky(f3 r, const SphParams& P) {
+__device__ __forceinline__ bool grad_spiky_nonzero(f3 r, const SphParams& P, f3& g) {
     const float d2 = dot3(r, r);
     const float rm = sqrt_rn_unscaled(d2);
-    if (rm > P.h || rm < 1e-5f) return {0.0f, 0.0f, 0.0f};
+    if (rm > P.h || rm < 1e-5f) return false;
     float t = P.h - rm;
     float s = P.cs * (t * t);
-    return kFast ? div3_rn_small(s * r, rm) : div3_rn(s * r, rm);
+    g = kFast ? div3_rn_small(s * r, rm) : div3_rn(s * r, rm);
+    return true;
+}
+
+template <bool kFast = false>
+__device__ __forceinline__ f3 grad_spiky(f3 r, const SphParams& P) {
+    f3 g{0.0f, 0.0f, 0.0f};
+    grad_spiky_nonzero<kFast>(r, P, g);
+    return g;
 }
 
 // powf(x, n) of the artificial pressure (ConstraintSolverCUDA.cu:124): an integer n is the n-fold

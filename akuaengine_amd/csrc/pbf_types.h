@@ -105,6 +105,9 @@ __host__ __device__ inline uint32_t fast_mod(uint32_t x, uint32_t d, uint32_t ma
 }
 
 constexpr uint32_t kSlotAlign = 256;
+// Most slots of a solver: per-slot arrays of up to 16-byte elements are then addressed by 32-bit
+// byte offsets in the kernels.
+constexpr uint32_t kMaxSlots = 1u << 28;
 constexpr uint32_t kEmptyKey = 0xFFFFFFFFu;  // never a hash: hashes are < tableSize <= 2^32 - 1
 
 }  // namespace akua

@@ -403,7 +403,8 @@ void setup(akua_solver* s, const std::vector<uint32_t>& counts) {
         uint64_t next = (uint64_t)slot + n;
         next = (next + kSlotAlign - 1) / kSlotAlign * kSlotAlign;
         if (counts.size() == 1) next = slot + n;  // a single scene needs no padding
-        if (next > kSortMaxSlots) fail(AKUA_ERR_UNSUPPORTED, "too many particles (at most 2^30 - 1 slots)");
+        if (next > std::min<uint64_t>(kSortMaxSlots, kMaxSlots))
+            fail(AKUA_ERR_UNSUPPORTED, "too many particles (at most 2^28 slots)");
         slot = (uint32_t)next;
     }
     s->nslots = slot;
