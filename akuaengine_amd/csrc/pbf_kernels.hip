@@ -28,6 +28,7 @@
 #include <mutex>
 #include <stdexcept>
 #include <string>
+#include <type_traits>
 #include "boundary.h"
 #include "pbf_math.h"
 
@@ -1188,29 +1189,38 @@ __device__ __forceinline__ void delta_correct_slot(const DeviceState& d, const S
         const uint4* lp = d.nbrW + list_base(t, P.maxN);
         const Recip Rdq = recip(P.wdq);
         // unit masses: x*_j and lambda_j from one float4 written by the density pass (W)
-        walk_list<XLam, kBatchDelta>(
-            lp, n,
-            [&](uint32_t j) {
-                if (kFast) {
-                    const float4 a = ld(d.W, j);
-                    return XLam{a, a.w};
-                }
-                return XLam{ld(XsIn, j), ld(d.lam, j)};
-            },
-            [&](const XLam& nb) {
+        const auto load = [&](uint32_t j) {
+            if (kFast) {
+                const float4 a = ld(d.W, j);
+                return XLam{a, a.w};
+            }
+            return XLam{ld(XsIn, j), ld(d.lam, j)};
+        };
+        // the walk once with and once without the artificial pressure s_corr, chosen per launch
+        const auto walk = [&](auto corrOn) {
+            constexpr bool kCorr = decltype(corrOn)::value;
+            walk_list<XLam, kBatchDelta>(lp, n, load, [&](const XLam& nb) {
                 f3 sep = xi - make3(nb.x);
                 float d2 = dot3(sep, sep);
                 float coef = li + nb.lam;
-                if (P.corrOn) {  // uniform: the artificial pressure s_corr
+                if (kCorr) {  // w = poly6 >= +0
                     const float w = poly6(d2, P);
-                    coef = coef + P.negCorrK * corr_pow(kFast ? div_rn_small(w, Rdq) : div_rn(w, Rdq), P);
+                    coef = coef + P.negCorrK * corr_pow(kFast ? div_rn_small_nonneg(w, Rdq) : div_rn(w, Rdq), P);
                 }
-                f3 g = grad_spiky<kFast>(sep, P);
-                if (kFast)  // m_j == 1
-                    dp = dp + coef * g;
-                else
-                    dp = dp + (coef * nb.x.w) * g;
+                const float c = kFast ? coef : coef * nb.x.w;  // m_j == 1 under kFast
+                f3 g;
+                if (__builtin_expect(grad_spiky_nonzero<kFast>(sep, P, g), 1)) {
+                    dp = dp + c * g;
+                } else {  // the zero gradient's term c * 0: zeros, or NaN where c is not finite
+                    const f3 z{0.0f, 0.0f, 0.0f};
+                    dp = dp + c * z;
+                }
             });
+        };
+        if (P.corrOn)
+            walk(std::true_type{});
+        else
+            walk(std::false_type{});
         if (kBnd) {  // lambda_i times the boundary gradient sum of the density pass (same x*)
             f3 gb{0.0f, 0.0f, 0.0f};
             if (boundary_near(P.bd, xi.x, xi.y, xi.z, P.h2)) gb = make3(d.Bg[i]);

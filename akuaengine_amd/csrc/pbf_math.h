@@ -133,6 +133,16 @@ __device__ __forceinline__ float div_rn_small(float a, const Recip& R) {
     return q;
 }
 
+// div_rn_small for a numerator that is +0 or positive: the one-correction quotient keeps the
+// sign by itself there (+0 stays +0), so no sign fix is needed.
+__device__ __forceinline__ float div_rn_small_nonneg(float a, const Recip& R) {
+    float q = a * R.r;
+    const float t = fmaf(-R.b, q, a);
+    q = fmaf(t, R.r, q);
+    if (__builtin_expect(low2(a) < kDivLo2, 0)) q = a / R.b;
+    return q;
+}
+
 // (x, y, z) / b on the same terms as div_rn_small.
 __device__ __forceinline__ f3 div3_rn_small(f3 v, float b) {
     const Recip R = recip(b);
@@ -186,6 +196,7 @@ __device__ __forceinline__ f3 grad_spiky(f3 r, const SphParams& P) {
 // left-to-right product; any other n calls powf.
 __device__ __forceinline__ float corr_pow(float x, const SphParams& P) {
     int k = P.corrNInt;
+    if (k == 4) return ((x * x) * x) * x;  // the usual exponent, without the loop
     if (k < 0) return powf(x, P.corrN);
     if (k == 0) return 1.0f;
     float r = x;
