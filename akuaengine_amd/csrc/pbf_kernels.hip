@@ -318,8 +318,12 @@ __device__ __forceinline__ void neighbours_slot(const DeviceState& d, const SphP
             const f3 xi = make3(d.XsA[p]);
             int cx, cy, cz;
             discretize(xi, P.cellSearch, cx, cy, cz);
-            // entry k of this list: element k % 4 of chunk k / 4
-            uint32_t* out = reinterpret_cast<uint32_t*>(d.nbr4 + list_base(p, P.maxN));
+            // entry k of this list: element k % 4 of chunk k / 4, at byte offset
+            // (p % 64) * 16 + (k / 4) * 1024 + (k % 4) * 4 from the wave's list block, whose
+            // address is the same in every lane (the wave's 64 slots share p / 64)
+            char* const waveOut = reinterpret_cast<char*>(
+                d.nbr4 + (size_t)__builtin_amdgcn_readfirstlane(p >> 6) * list_chunks(P.maxN) * 64u);
+            const uint32_t laneOff = (p & 63u) * 16u;
             const uint32_t maxN = (uint32_t)P.maxN;
             // stencil cells c = 0..26 in (dx, dy, dz) order, probed kProbeGroup at a time
             for (int g = 0; g < 27 / kProbeGroup && count < maxN; ++g) {
@@ -382,7 +386,8 @@ __device__ __forceinline__ void neighbours_slot(const DeviceState& d, const SphP
                             for (int u = 0; u < kWalkBatch; ++u) {
                                 if (acc & (1u << u)) {
                                     const uint32_t k = count + (uint32_t)__popc(acc & ((1u << u) - 1u));
-                                    out[((size_t)(k >> 2) * 64u) * 4u + (k & 3u)] = ph[u];
+                                    *reinterpret_cast<uint32_t*>(waveOut + (laneOff + ((k & ~3u) << 8) +
+                                                                            ((k & 3u) << 2))) = ph[u];
                                 }
                             }
                             count += (uint32_t)__popc(acc);
