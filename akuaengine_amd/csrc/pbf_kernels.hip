@@ -1269,8 +1269,12 @@ __device__ __forceinline__ void vorticity_slot(const DeviceState& d, const SphPa
         [&](const XV& nb) {
             f3 vij = make3(nb.v) - vi;
             f3 sep = xi - make3(nb.x);
-            const f3 c = cross3(vij, grad_spiky<kFast>(sep, P));
-            w = w + (kFast ? -1.0f : -nb.x.w) * c;  // m_j == 1 under kFast
+            const float mj = kFast ? -1.0f : -nb.x.w;  // m_j == 1 under kFast
+            f3 g;
+            if (__builtin_expect(grad_spiky_nonzero<kFast>(sep, P, g), 1))
+                w = w + mj * cross3(vij, g);
+            else  // the zero gradient's term: zeros, or NaN where v_ij is not finite
+                w = w + mj * cross3(vij, f3{0.0f, 0.0f, 0.0f});
         });
     const float wl = length3(w);
     d.W[i] = make_float4(w.x, w.y, w.z, wl);
@@ -1322,8 +1326,12 @@ __device__ __forceinline__ void confinement_slot(const DeviceState& d, const Sph
         [&](const XW& nb) {
             float diff = wi.w - nb.x.w;
             f3 sep = xi - make3(nb.x);
-            const f3 g = grad_spiky<kFast>(sep, P);
-            eta = eta + (kFast ? diff : nb.w * diff) * g;  // m_j == 1 under kFast
+            const float c = kFast ? diff : nb.w * diff;  // m_j == 1 under kFast
+            f3 g;
+            if (__builtin_expect(grad_spiky_nonzero<kFast>(sep, P, g), 1))
+                eta = eta + c * g;
+            else  // the zero gradient's term c * 0: zeros, or NaN where c is not finite
+                eta = eta + c * f3{0.0f, 0.0f, 0.0f};
         });
     confine_finish(d, P, S, i, xi4w, wi, eta, invD);
 }
