@@ -252,22 +252,45 @@ __global__ __launch_bounds__(kBlock) void k_gather(DeviceState d, SphParams P, S
     d.physT[k] = p;
 }
 
-// Per bucket (at its first logical index k): where the bucket starts in physical order and
-// whether its particles occupy consecutive physical slots in logical order (always true for a
-// bucket of one cell, which lies inside one super-cell).  Reads physT in logical order.
+// Per bucket: where it starts in physical order and whether its particles occupy consecutive
+// physical slots in logical order (always true for a bucket of one cell, which lies inside one
+// super-cell; buckets of more than 256 particles are not marked).  One thread per logical index k,
+// all reads in logical order: the bucket's first index ANDs (first physical slot | run bit) into
+// the record's w, which the table fill left all ones, and every index whose logical successor in
+// the bucket is not in the next physical slot clears the run bit (ANDs commute, so the order of
+// the two does not matter).  Only hash-collision buckets take that second path.
+__device__ __forceinline__ uint32_t table_slot_of(const uint4* __restrict__ table, const SceneDev& sc,
+                                                  uint32_t h) {
+    const uint32_t mask = (1u << sc.tabLog2) - 1u;
+    uint32_t s = table_slot(h, sc.tabLog2);
+    for (uint32_t probe = 0; probe <= mask; ++probe) {
+        const uint32_t key = table[sc.tabOffset + s].x;
+        if (key == h) return sc.tabOffset + s;
+        if (key == kEmptyKey) return kEmptyKey;
+        s = (s + 1u) & mask;
+    }
+    return kEmptyKey;
+}
+
 __global__ __launch_bounds__(kBlock) void k_bucket_phys(DeviceState d) {
     const uint32_t k = blockIdx.x * kBlock + threadIdx.x;
     if (k >= d.nslots) return;
     const SceneDev& sc = scene_of(d, k);
     if (k >= sc.end) return;
-    if (!(k == sc.begin || d.skeys[k - 1] != d.skeys[k])) return;
-    const uint32_t slot = d.bslot[k];
-    if (slot >= d.tableTotal) return;
-    const uint32_t len = d.table[slot].z - k;
-    const uint32_t p0 = d.physT[k];
-    bool run = len <= 256u && k + len <= sc.end;
-    for (uint32_t t = 1; run && t < len; ++t) run = d.physT[k + t] == p0 + t;
-    d.table[slot].w = p0 | (run ? 0x80000000u : 0u);
+    const uint32_t h = d.skeys[k];
+    const uint32_t p = d.physT[k];
+    if (k == sc.begin || d.skeys[k - 1] != h) {
+        const uint32_t slot = d.bslot[k];
+        if (slot < d.tableTotal) {
+            const uint32_t len = d.table[slot].z - k;
+            const bool small = len <= 256u && k + len <= sc.end;
+            atomicAnd(&d.table[slot].w, p | (small ? 0x80000000u : 0u));
+        }
+    }
+    if (k + 1u < sc.end && d.skeys[k + 1u] == h && d.physT[k + 1u] != p + 1u) {
+        const uint32_t slot = table_slot_of(d.table, sc, h);
+        if (slot != kEmptyKey) atomicAnd(&d.table[slot].w, 0x7FFFFFFFu);
+    }
 }
 
 // A bucket's entry (first logical index, one past the last, first physical slot, contiguity)
