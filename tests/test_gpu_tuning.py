@@ -91,6 +91,22 @@ def test_setting_changes_between_steps(classes, dam_params):
         np.testing.assert_array_equal(got[k], want[k], err_msg=k)
 
 
+def test_slot_limit_is_refused(classes, dam_params):
+    """A solver holds at most 2^28 particle slots (per-slot arrays are addressed by 32-bit byte
+    offsets): one scene or a batch past that is refused before any allocation."""
+    import dataclasses
+    BatchSolver, PBFSolver = classes
+    from akuaengine_amd._lib import AkuaError
+    cfg, corr = dam_params
+    cfg = dataclasses.replace(cfg, maxNeighbours=1)  # keeps maxNeighbours * N within 32 bits
+    with pytest.raises(AkuaError) as e:
+        PBFSolver((1 << 28) + 1, cfg, corr)
+    assert e.value.status == 6  # AKUA_ERR_UNSUPPORTED
+    with pytest.raises(AkuaError) as e:
+        BatchSolver([1 << 27, (1 << 27) + 1], cfg, corr)
+    assert e.value.status == 6
+
+
 def test_invalid_settings_are_rejected(classes, dam_params):
     _, PBFSolver = classes
     from akuaengine_amd._lib import AkuaError
