@@ -7,7 +7,8 @@
 //   physical -- where the particle's data lives in HBM: sorted by the Morton code of its
 //               2x2x2 super-cell (ties in logical order), so the 64 particles of a wave are
 //               spatial neighbours and their neighbour gathers hit the same cache lines.
-// lid[p] is the logical index of physical slot p, phys[k] the physical slot of logical index k.
+// lid[p] is the logical index of physical slot p (kept by the step); its inverse, the physical slot
+// of each logical index, is physT during a step and is rebuilt from lid where needed outside it.
 // Every per-particle sum runs over the neighbour list in the reference's order, so the physical
 // layout changes no result, only memory locality.
 //
@@ -45,7 +46,6 @@ struct DeviceState {
     float* lam;
     uint32_t* id;
     uint32_t* lid;             // logical index of each physical slot (persistent)
-    uint32_t* phys;            // physical slot of each logical index (persistent)
     float4* VS;                // new_velocity of the last step (persistent)
     float4* DP;                // position_delta of the last iteration (persistent)
     float4* TDP;               // DP moved to the new physical slots (a step without iterations)
@@ -60,9 +60,9 @@ struct DeviceState {
     uint32_t* Tid;
     uint32_t* lidT;            // this step's logical index of each physical slot
     uint32_t* physT;           // this step's physical slot of each logical index
-    float4* XsL;               // x* (and mass) by logical index, for the bucket walks
     uint32_t* srcL;            // previous physical slot of each logical index
-    uint32_t* keysLay;         // layout keys at their previous logical index
+    uint2* layPhys;            // (layout key, physical slot) at the step start, at the previous
+                               // logical index
     uint32_t* keysL;           // hashes at their previous logical index (first sort's input)
     const uint32_t* skeys;     // hashes in logical order (first sort's output)
     const uint32_t* perm1;     // logical index -> previous logical index
@@ -131,6 +131,8 @@ void launch_gather(hipStream_t st, const DeviceState& d, const SphParams& P, con
                    bool carry, bool carryDelta);
 // Physical start and contiguity of every bucket (after the gather).
 void launch_bucket_phys(hipStream_t st, const DeviceState& d);
+// phys[lid[p]] = p over every scene's slots.
+void launch_invert_order(hipStream_t st, const DeviceState& d, uint32_t* phys);
 // bs: workgroup size of the neighbour kernels (128, 256 or 512; results do not depend on it).
 // slabBatch: 0 walks each stencil bucket in batches of eight candidates; 4 or 8 walks each dx slab
 // of nine buckets as one stream in batches of that many (identical lists).

@@ -158,8 +158,8 @@ __device__ __forceinline__ uint32_t layout_key(int cx, int cy, int cz, const Sce
 
 // kernel_predict_position (IntegrationCUDA.cu:27-36) + kernel_compute_hashes
 // (NeighbourSearchCUDA.cu:36-44) with smoothRadius as the cell size (NeighbourSearchCUDA.cu:163).
-// The hash and the layout key are stored at the particle's previous logical index, the first
-// sort's input order.
+// The hash, and the layout key with the particle's current physical slot, are stored at its
+// previous logical index, the first sort's input order.
 __global__ __launch_bounds__(kBlock) void k_predict_hash(DeviceState d, SphParams P, StepParams S) {
     uint32_t p = blockIdx.x * kBlock + threadIdx.x;
     if (p >= d.nslots) return;
@@ -171,7 +171,7 @@ __global__ __launch_bounds__(kBlock) void k_predict_hash(DeviceState d, SphParam
     discretize(xs, P.cellHash, cx, cy, cz);
     const uint32_t k = d.lid[p];
     d.keysL[k] = P.searchCells == 2 ? cell_key_fine(cx, cy, cz) : cell_hash(cx, cy, cz, sc);
-    d.keysLay[k] = layout_key(cx, cy, cz, sc);
+    d.layPhys[k] = make_uint2(layout_key(cx, cy, cz, sc), p);
 }
 
 __device__ __forceinline__ uint32_t table_insert(uint4* table, const SceneDev& sc, uint32_t h) {
@@ -200,11 +200,12 @@ __global__ __launch_bounds__(kBlock) void k_cell_keys(DeviceState d, SphParams P
     if (k >= sc.end) return;
     const uint32_t prevLogical = d.perm1[k];
     if (prevLogical >= d.nslots) return;  // a consistent sort never produces this
-    const uint32_t src = d.phys[prevLogical];
+    const uint2 lp = d.layPhys[prevLogical];  // (layout key, previous physical slot)
+    const uint32_t src = lp.y;
     if (kKeep) {
         if (src < d.nslots) d.perm2keep[src] = k;
     } else {
-        d.keysM[k] = d.keysLay[prevLogical];
+        d.keysM[k] = lp.x;
     }
     d.srcL[k] = src;
     const uint32_t h = d.skeys[k];
@@ -241,7 +242,6 @@ __global__ __launch_bounds__(kBlock) void k_gather(DeviceState d, SphParams P, S
     const float4 xs4 = make_float4(xs.x, xs.y, xs.z, x.w);
     d.X0[p] = x;
     d.XsA[p] = xs4;
-    d.XsL[k] = xs4;
     if (kCarry) {
         d.Tdens[p] = d.dens[src];
         d.Tlam[p] = d.lam[src];
@@ -250,6 +250,17 @@ __global__ __launch_bounds__(kBlock) void k_gather(DeviceState d, SphParams P, S
     d.Tid[p] = d.id[src];
     d.lidT[p] = k;
     d.physT[k] = p;
+}
+
+// The physical slot of each logical index from the step's persistent lid (outside a step, where
+// physT no longer holds it: rendering draws in logical order).
+__global__ __launch_bounds__(kBlock) void k_invert_order(DeviceState d, uint32_t* __restrict__ phys) {
+    const uint32_t p = blockIdx.x * kBlock + threadIdx.x;
+    if (p >= d.nslots) return;
+    const SceneDev& sc = scene_of(d, p);
+    if (p >= sc.end) return;
+    const uint32_t k = d.lid[p];
+    if (k < d.nslots) phys[k] = p;
 }
 
 // Per bucket: where it starts in physical order and whether its particles occupy consecutive
@@ -332,6 +343,21 @@ __device__ __forceinline__ bool table_find(const uint4* __restrict__ table, cons
 constexpr int kWalkBatch = 8;
 constexpr int kProbeGroup = 3;
 
+// Physical slots of bucket positions m0 .. m0 + N - 1 (clamped to the bucket's last, b - 1): by
+// the bucket's offset when its slots are consecutive (run), else from physT.  The physT loads sit
+// on a branch of their own, so the walks' position loads stay one batch of independent requests.
+template <int N>
+__device__ __forceinline__ void batch_slots(const DeviceState& d, bool run, uint32_t m0, uint32_t b,
+                                            int32_t shift, uint32_t (&ph)[N]) {
+    if (__builtin_expect(run, 1)) {
+#pragma unroll
+        for (int u = 0; u < N; ++u) ph[u] = (uint32_t)((int32_t)min(m0 + u, b - 1u) + shift);
+    } else {
+#pragma unroll
+        for (int u = 0; u < N; ++u) ph[u] = ld(d.physT, min(m0 + u, b - 1u));
+    }
+}
+
 __device__ __forceinline__ void neighbours_slot(const DeviceState& d, const SphParams& P,
                                                 uint32_t p) {
     uint32_t count = 0;
@@ -375,23 +401,15 @@ __device__ __forceinline__ void neighbours_slot(const DeviceState& d, const SphP
                 for (int q = 0; q < kProbeGroup; ++q) {
                     const uint32_t a = e[q].x, b = e[q].y;
                     // bucket of consecutive physical slots: read the cell-ordered positions
-                    // directly; otherwise go through the logical-order copy
+                    // directly; otherwise (hash collisions) find each slot through physT first
                     const bool run = e[q].w != 0u;
                     const int32_t shift = (int32_t)e[q].z - (int32_t)a;  // physical - logical
                     for (uint32_t m0 = a; m0 < b && count < maxN; m0 += kWalkBatch) {
                         float4 c[kWalkBatch];
                         uint32_t ph[kWalkBatch];
+                        batch_slots<kWalkBatch>(d, run, m0, b, shift, ph);
 #pragma unroll
-                        for (int u = 0; u < kWalkBatch; ++u) {
-                            const uint32_t m = min(m0 + u, b - 1u);
-                            if (run) {
-                                ph[u] = (uint32_t)((int32_t)m + shift);
-                                c[u] = ld(d.XsA, ph[u]);
-                            } else {
-                                c[u] = ld(d.XsL, m);
-                                ph[u] = ld(d.physT, m);
-                            }
-                        }
+                        for (int u = 0; u < kWalkBatch; ++u) c[u] = ld(d.XsA, ph[u]);
                         // test the whole batch without branches, then append in order
                         uint32_t acc = 0u;
 #pragma unroll
@@ -456,17 +474,9 @@ __global__ __launch_bounds__(BS) void k_neighbours_cells(DeviceState d, SphParam
     // the accepted candidates of bucket positions [m0, m0 + kWalkBatch), one bit each
     const auto batch = [&](uint32_t m0, uint32_t (&ph)[kWalkBatch]) {
         float4 cand[kWalkBatch];
+        batch_slots<kWalkBatch>(d, run, m0, b, shift, ph);
 #pragma unroll
-        for (int u = 0; u < kWalkBatch; ++u) {
-            const uint32_t m = min(m0 + u, b - 1u);
-            if (run) {
-                ph[u] = (uint32_t)((int32_t)m + shift);
-                cand[u] = ld(d.XsA, ph[u]);
-            } else {
-                cand[u] = ld(d.XsL, m);
-                ph[u] = ld(d.physT, m);
-            }
-        }
+        for (int u = 0; u < kWalkBatch; ++u) cand[u] = ld(d.XsA, ph[u]);
         uint32_t acc = 0u;
 #pragma unroll
         for (int u = 0; u < kWalkBatch; ++u) {
@@ -672,16 +682,19 @@ __global__ __launch_bounds__(BS) void k_neighbours_slab(DeviceState d, SphParams
                     }
                     float4 c[B];
                     uint32_t ph[B];
+                    bool anyLogical = false;
 #pragma unroll
                     for (int u = 0; u < B; ++u) {
-                        if (rr[u]) {
-                            ph[u] = mm[u];
-                            c[u] = ld(d.XsA, ph[u]);
-                        } else {
-                            c[u] = ld(d.XsL, mm[u]);
-                            ph[u] = ld(d.physT, mm[u]);
-                        }
+                        ph[u] = mm[u];
+                        anyLogical |= !rr[u];
                     }
+                    if (__builtin_expect(anyLogical, 0)) {  // hash collisions: slots through physT
+#pragma unroll
+                        for (int u = 0; u < B; ++u)
+                            if (!rr[u]) ph[u] = ld(d.physT, mm[u]);
+                    }
+#pragma unroll
+                    for (int u = 0; u < B; ++u) c[u] = ld(d.XsA, ph[u]);
                     uint32_t acc = 0u;
 #pragma unroll
                     for (int u = 0; u < B; ++u) {
@@ -751,17 +764,9 @@ __global__ __launch_bounds__(BS) void k_neighbours_fine(DeviceState d, SphParams
                         for (uint32_t m0 = a; m0 < b && count < maxN; m0 += 4) {
                             float4 c[4];
                             uint32_t ph[4];
+                            batch_slots<4>(d, run, m0, b, shift, ph);
 #pragma unroll
-                            for (int u = 0; u < 4; ++u) {
-                                const uint32_t m = min(m0 + u, b - 1u);
-                                if (run) {
-                                    ph[u] = (uint32_t)((int32_t)m + shift);
-                                    c[u] = ld(d.XsA, ph[u]);
-                                } else {
-                                    c[u] = ld(d.XsL, m);
-                                    ph[u] = ld(d.physT, m);
-                                }
-                            }
+                            for (int u = 0; u < 4; ++u) c[u] = ld(d.XsA, ph[u]);
 #pragma unroll
                             for (int u = 0; u < 4; ++u) {
                                 const f3 sep = xi - make3(c[u]);
@@ -1175,9 +1180,7 @@ __device__ __forceinline__ void finish_step(const DeviceState& d, const StepPara
     d.X[i] = make_float4(q.x, q.y, q.z, xi4.w);
     d.V[i] = make_float4(v.x, v.y, v.z, 0.0f);
     d.id[i] = d.Tid[i];
-    const uint32_t k = d.lidT[i];
-    d.lid[i] = k;
-    d.phys[k] = i;
+    d.lid[i] = d.lidT[i];
     if (kApplyDelta) {  // the record's position_delta (ConstraintSolverCUDA.cu:129)
         d.DP[i] = make_float4(dp.x, dp.y, dp.z, 0.0f);
     } else {  // no iteration ran: density, lambda and the position delta carry over
@@ -1805,7 +1808,7 @@ __global__ __launch_bounds__(BS) void k_density_probe(DeviceState d, SphParams P
                 const int32_t shift = (int32_t)e.z - (int32_t)e.x;
                 for (uint32_t m = e.x; m < e.y; ++m) {
                     if (m == self) continue;
-                    float4 xk = e.w ? ld(d.XsA, (uint32_t)((int32_t)m + shift)) : ld(d.XsL, m);
+                    float4 xk = ld(d.XsA, e.w ? (uint32_t)((int32_t)m + shift) : ld(d.physT, m));
                     f3 sep = xi - make3(xk);
                     float d2 = dot3(sep, sep);
                     if (d2 < P.h2) rho = rho + xk.w * poly6(d2, P);
@@ -1898,6 +1901,10 @@ void launch_gather(hipStream_t st, const DeviceState& d, const SphParams& P, con
     else
         k_gather<false, false><<<grid(d.nslots), kBlock, 0, st>>>(d, P, S);
 }
+void launch_invert_order(hipStream_t st, const DeviceState& d, uint32_t* phys) {
+    k_invert_order<<<grid(d.nslots), kBlock, 0, st>>>(d, phys);
+}
+
 void launch_bucket_phys(hipStream_t st, const DeviceState& d) {
     k_bucket_phys<<<grid(d.nslots), kBlock, 0, st>>>(d);
 }

@@ -181,9 +181,10 @@ struct akua_solver {
     // device buffers
     DevBuf<SceneDev> dScenes;
     DevBuf<uint32_t> dChunkScene;
-    DevBuf<float4> X, V, VS, DP, TDP, X0, XsA, XsB, XsL, W, Vc, color;
+    DevBuf<float4> X, V, VS, DP, TDP, X0, XsA, XsB, XsC, W, Vc, color;
     DevBuf<float> dens, lam, Tdens, Tlam, size;
-    DevBuf<uint32_t> id, Tid, lid, phys, lidT, physT, srcL, keysLay, bslot, nbr, cnt, perm2keep;
+    DevBuf<uint32_t> id, Tid, lid, phys, lidT, physT, srcL, bslot, nbr, cnt, perm2keep;
+    DevBuf<uint2> layPhys;
     DevBuf<uint32_t> s1KA, s1VA, s1KB, s1VB, s2KA, s2VA, s2KB, s2VB, counts, totals, look, tickets;
     DevBuf<uint4> table;
     DevBuf<uint32_t> work, cntW, nbrW;  // the sweeps' balanced work order (allocated when used)
@@ -262,11 +263,11 @@ struct akua_solver {
         d.chunkScene = dChunkScene.p;
         d.X = X.p; d.V = V.p; d.dens = dens.p; d.lam = lam.p; d.id = id.p;
         d.VS = VS.p; d.DP = DP.p; d.TDP = TDP.p;
-        d.lid = lid.p; d.phys = phys.p;
+        d.lid = lid.p;
         d.X0 = X0.p; d.XsA = XsA.p; d.XsB = XsB.p; d.W = W.p; d.Vc = Vc.p; d.Bg = Bg.p;
         d.Tdens = Tdens.p; d.Tlam = Tlam.p; d.Tid = Tid.p;
         d.lidT = lidT.p; d.physT = physT.p;
-        d.XsL = XsL.p; d.srcL = srcL.p; d.keysLay = keysLay.p;
+        d.srcL = srcL.p; d.layPhys = layPhys.p;
         const bool r1 = plan1.passes() % 2 == 1, r2 = plan2.passes() % 2 == 1;
         d.keysL = s1KA.p;
         d.skeys = r1 ? s1KB.p : s1KA.p;
@@ -305,17 +306,17 @@ struct akua_solver {
 
     void release_buffers() {
         dScenes.release(); dChunkScene.release();
-        for (auto* b : {&X, &V, &VS, &DP, &TDP, &X0, &XsA, &XsB, &XsL, &W, &Vc, &color, &Bg, &XsP, &XsW})
+        for (auto* b : {&X, &V, &VS, &DP, &TDP, &X0, &XsA, &XsB, &XsC, &W, &Vc, &color, &Bg, &XsP, &XsW})
             b->release();
         for (auto* b : {&dens, &lam, &Tdens, &Tlam, &size, &probeRho, &bndPsi, &net, &feat, &targ,
                         &lamHat, &adamM, &adamV, &part, &batchF, &batchT, &lossBuf})
             b->release();
-        for (auto* b : {&id, &Tid, &lid, &phys, &lidT, &physT, &srcL, &keysLay, &bslot, &nbr, &cnt,
+        for (auto* b : {&id, &Tid, &lid, &phys, &lidT, &physT, &srcL, &bslot, &nbr, &cnt,
                         &perm2keep, &s1KA, &s1VA, &s1KB, &s1VB, &s2KA, &s2VA, &s2KB, &s2VB, &counts,
                         &totals, &look, &tickets, &gate, &fallbacks, &work, &cntW, &nbrW, &qcost,
                         &queue, &qcount, &heads})
             b->release();
-        table.release(); dTiles.release(); dSortScenes.release(); nbrSum.release();
+        layPhys.release(); table.release(); dTiles.release(); dSortScenes.release(); nbrSum.release();
         probeMax.release(); probeSum.release(); renderKeys.release(); renderRgb.release();
         resid.release(); residPart.release(); residBlk.release(); aosStage.release();
     }
@@ -434,7 +435,7 @@ void setup(akua_solver* s, const std::vector<uint32_t>& counts) {
     s->dens.alloc(ns); s->lam.alloc(ns); s->Tdens.alloc(ns); s->Tlam.alloc(ns); s->size.alloc(ns);
     s->id.alloc(ns); s->Tid.alloc(ns);
     s->lid.alloc(ns); s->phys.alloc(ns); s->lidT.alloc(ns); s->physT.alloc(ns);
-    s->XsL.alloc(ns); s->srcL.alloc(ns); s->keysLay.alloc(ns); s->perm2keep.alloc(ns);
+    s->XsC.alloc(ns); s->srcL.alloc(ns); s->layPhys.alloc(ns); s->perm2keep.alloc(ns);
     s->s1KA.alloc(ns); s->s1VA.alloc(ns); s->s1KB.alloc(ns); s->s1VB.alloc(ns);
     s->s2KA.alloc(ns); s->s2VA.alloc(ns); s->s2KB.alloc(ns); s->s2VB.alloc(ns);
     s->counts.alloc((size_t)s->numTiles * (1u << kMaxDigitBits));
@@ -466,8 +467,8 @@ void setup(akua_solver* s, const std::vector<uint32_t>& counts) {
     HIP_CHECK(hipMemset(s->perm2keep.p, 0, ns * 4));
     HIP_CHECK(hipMemset(s->bslot.p, 0xFF, ns * 4));
     HIP_CHECK(hipMemset(s->table.p, 0xFF, (size_t)s->tableTotal * sizeof(uint4)));
-    HIP_CHECK(hipMemset(s->keysLay.p, 0, ns * 4));
-    HIP_CHECK(hipMemset(s->XsL.p, 0, ns * sizeof(float4)));
+    HIP_CHECK(hipMemset(s->layPhys.p, 0, ns * sizeof(uint2)));
+    HIP_CHECK(hipMemset(s->XsC.p, 0, ns * sizeof(float4)));
     HIP_CHECK(hipMemset(s->physT.p, 0, ns * 4));
     HIP_CHECK(hipMemset(s->lid.p, 0, ns * 4));
     HIP_CHECK(hipMemset(s->phys.p, 0, ns * 4));
@@ -631,10 +632,10 @@ void enqueue_step(akua_solver* s, hipStream_t st, const StepParams& S, bool time
     // record: targets are x* before the last Jt iterations and the lambda computed there
     const int Jt = std::min(std::max(s->trainIters, 1), I);
     float4* xs[2] = {s->XsA.p, s->XsB.p};
-    // the warm chain: x*_w stays in XsW (for the taps); later iterates alternate between XsL
-    // (free once the neighbour lists are built) and XsB (which the fallback may overwrite only
+    // the warm chain: x*_w stays in XsW (for the taps); later iterates alternate between XsC
+    // (scratch) and XsB (which the fallback may overwrite only
     // in scenes whose gate is set, where the warm chain's result is discarded)
-    auto ws = [&](int j) { return j == 0 ? s->XsW.p : (j & 1) ? s->XsL.p : s->XsB.p; };
+    auto ws = [&](int j) { return j == 0 ? s->XsW.p : (j & 1) ? s->XsC.p : s->XsB.p; };
     if ((learned || record) && !fusedFeatures) {
         L.before("features");
         launch_features(st, d, P, record, bs);
@@ -974,7 +975,6 @@ void upload_soa(akua_solver* s, const float* pos, const float* vel, const float*
     std::vector<uint32_t> ident(s->nslots);
     for (uint32_t i = 0; i < s->nslots; ++i) ident[i] = i;
     HIP_CHECK(hipMemcpy(s->lid.p, ident.data(), ident.size() * 4, hipMemcpyHostToDevice));
-    HIP_CHECK(hipMemcpy(s->phys.p, ident.data(), ident.size() * 4, hipMemcpyHostToDevice));
     HIP_CHECK(hipMemcpy(s->color.p, C.data(), C.size() * sizeof(float4), hipMemcpyHostToDevice));
     HIP_CHECK(hipMemcpy(s->size.p, Z.data(), Z.size() * 4, hipMemcpyHostToDevice));
     s->hasState = true;
@@ -1025,7 +1025,7 @@ void apply_tuning(akua_solver* s, const akua_tuning& t) {
 struct Snapshot {
     DevBuf<float4> X, V, W, VS, DP;
     DevBuf<float> dens, lam;
-    DevBuf<uint32_t> id, lid, phys;
+    DevBuf<uint32_t> id, lid;
 };
 
 template <class T>
@@ -1037,13 +1037,12 @@ void take_snapshot(akua_solver* s, Snapshot& q) {
     const size_t n = s->nslots;
     q.X.alloc(n); q.V.alloc(n); q.W.alloc(n); q.VS.alloc(n); q.DP.alloc(n);
     q.dens.alloc(n); q.lam.alloc(n);
-    q.id.alloc(n); q.lid.alloc(n); q.phys.alloc(n);
+    q.id.alloc(n); q.lid.alloc(n);
     hipStream_t st = s->stream;
     copy_dev(q.X.p, s->X.p, n, st); copy_dev(q.V.p, s->V.p, n, st); copy_dev(q.W.p, s->W.p, n, st);
     copy_dev(q.VS.p, s->VS.p, n, st); copy_dev(q.DP.p, s->DP.p, n, st);
     copy_dev(q.dens.p, s->dens.p, n, st); copy_dev(q.lam.p, s->lam.p, n, st);
     copy_dev(q.id.p, s->id.p, n, st); copy_dev(q.lid.p, s->lid.p, n, st);
-    copy_dev(q.phys.p, s->phys.p, n, st);
 }
 
 void restore_snapshot(akua_solver* s, const Snapshot& q) {
@@ -1054,7 +1053,6 @@ void restore_snapshot(akua_solver* s, const Snapshot& q) {
     copy_dev(s->VS.p, q.VS.p, n, st); copy_dev(s->DP.p, q.DP.p, n, st);
     copy_dev(s->dens.p, q.dens.p, n, st); copy_dev(s->lam.p, q.lam.p, n, st);
     copy_dev(s->id.p, q.id.p, n, st); copy_dev(s->lid.p, q.lid.p, n, st);
-    copy_dev(s->phys.p, q.phys.p, n, st);
 }
 
 // Device milliseconds per step of `steps` graph-launched steps from the snapshot, after one
@@ -1971,6 +1969,8 @@ akua_status akua_render(akua_solver* s, const akua_camera* cam, int32_t width, i
         HIP_CHECK(hipStreamSynchronize(s->stream));
         if (s->renderKeys.n < pixels) s->renderKeys.alloc(pixels);
         if (s->renderRgb.n < 3 * pixels) s->renderRgb.alloc(3 * pixels);
+        upload_scenes(s);  // the order inversion reads the scene ranges
+        launch_invert_order(s->stream, s->devState(), s->phys.p);
         launch_render(s->stream, s->X.p, s->color.p, s->size.p, s->id.p, s->lid.p, s->phys.p,
                       s->scenes[0].end, R, s->renderKeys.p, s->renderRgb.p);
         HIP_CHECK(hipGetLastError());
