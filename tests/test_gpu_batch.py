@@ -104,17 +104,18 @@ def test_batch_scenes_equal_the_oracle(solvers, oracle_mod, dam_params):
         off += s.n
 
 
-@pytest.mark.parametrize("mode", [1, 2])
-def test_work_queue_batch_equals_the_oracle(solvers, oracle_mod, dam_params, mode):
+@pytest.mark.parametrize("mode,level", [(1, 1), (2, 1), (2, 0)])
+def test_work_queue_batch_equals_the_oracle(solvers, oracle_mod, dam_params, mode, level):
     """With the sweeps' work queue (tuning workQueue 1 and 2), every scene of the 7-scene batch equals
-    the FP32 oracle stepping that scene alone, bit for bit."""
+    the FP32 oracle stepping that scene alone, bit for bit, with multi-cell and single-cell
+    super-cells."""
     BatchSolver, _ = solvers
     from akuaengine_amd import scenes
     cfg, corr = dam_params
     counts = [100, 4097, 2000, 513, 8000, 1, 777]
     scs = [scenes.batched_scene(n, i, seed=5) for i, n in enumerate(counts)]
     b = BatchSolver(counts, cfg, corr)
-    b.set_tuning(workQueue=mode)
+    b.set_tuning(workQueue=mode, layoutLevel=level)
     b.set_state(np.concatenate([s.pos for s in scs]), np.concatenate([s.vel for s in scs]))
     for i, s in enumerate(scs):
         b.set_box(i, s.box_min, s.box_max)

@@ -429,19 +429,25 @@ def test_non_integer_corr_exponent_within_tolerance(gpu, oracle_mod, dam_params)
         assert np.abs(g[k] - r[k]).max() <= 1e-5 * scale, k
 
 
-def test_graph_and_direct_launches_agree(gpu, dam_params):
+@pytest.mark.parametrize("level", [1, 0])
+def test_graph_and_direct_launches_agree(gpu, dam_params, level):
+    # graph-launched, directly launched and timed steps (events around every launch), with
+    # multi-cell and single-cell super-cells
     from akuaengine_amd import scenes
     cfg, corr = dam_params
     sc = scenes.dam_break_scaled(16)
     out = []
-    for graph in (True, False):
+    for graph, timed in ((True, False), (False, False), (False, True)):
         s = gpu(sc.n, cfg, corr)
+        s.set_tuning(block=256, layoutLevel=level)
         s.set_graph(graph)
+        s.set_timing(timed)
         s.set_state(sc.pos, sc.vel)
         s.step_n(DT, sc.box_min, sc.box_max, 5)
         out.append(s.get_state())
-    for k in ("pos", "vel", "ids"):
-        np.testing.assert_array_equal(out[0][k], out[1][k])
+    for o in out[1:]:
+        for k in ("pos", "vel", "ids", "density", "lam"):
+            np.testing.assert_array_equal(out[0][k], o[k])
 
 
 def test_gravity_and_box_changes_are_applied(gpu, oracle_mod, dam_params):
