@@ -27,6 +27,7 @@ EXPORTED = [
     "akua_solver_step", "akua_solver_step_n", "akua_solver_step_device",
     "akua_solver_set_gravity", "akua_solver_synchronize", "akua_solver_num_particles",
     "akua_solver_set_graph", "akua_debug_hashes", "akua_debug_permutation",
+    "akua_debug_physical_order",
     "akua_debug_cell_ranges", "akua_debug_neighbours", "akua_solver_set_timing",
     "akua_solver_reset_timing", "akua_solver_get_timing", "akua_solver_mean_neighbours",
     "akua_solver_density_error", "akua_debug_probe_density", "akua_stream_copy_bandwidth",
@@ -153,6 +154,7 @@ def load() -> ctypes.CDLL:
         "akua_solver_set_graph": [P, i32],
         "akua_debug_hashes": [P, P, i32],
         "akua_debug_permutation": [P, P, i32],
+        "akua_debug_physical_order": [P, P, i32],
         "akua_debug_cell_ranges": [P, P, P, P, i32, ctypes.POINTER(i32)],
         "akua_debug_neighbours": [P, P, P, i32],
         "akua_solver_set_timing": [P, i32],

@@ -61,8 +61,9 @@ struct DeviceState {
     uint32_t* lidT;            // this step's logical index of each physical slot
     uint32_t* physT;           // this step's physical slot of each logical index
     uint32_t* srcL;            // previous physical slot of each logical index
-    uint2* layPhys;            // (layout key, physical slot) at the step start, at the previous
-                               // logical index
+    uint2* layPhys;            // at each previous logical index: (layout key, physical slot at the
+                               // step start), or with layoutPrev (new logical index, physical
+                               // slot at the step start)
     uint32_t* keysL;           // hashes at their previous logical index (first sort's input)
     const uint32_t* skeys;     // hashes in logical order (first sort's output)
     const uint32_t* perm1;     // logical index -> previous logical index
@@ -118,17 +119,20 @@ constexpr int kBlock = 256;
 void launch_fill_u32(hipStream_t st, uint32_t* p, uint32_t n, uint32_t v);
 // Every record of the cell table empty.
 void launch_clear_table(hipStream_t st, uint4* table, uint32_t n);
+// layoutPrev (the three launches of a step alike): the layout keys are sorted in the previous
+// logical order, in the same launches as the hashes, instead of in the new logical order after
+// them.  Only for single-cell super-cells (layout level 0), where both orders agree within each key.
 void launch_predict_hash(hipStream_t st, const DeviceState& d, const SphParams& P,
-                         const StepParams& S);
+                         const StepParams& S, bool layoutPrev);
 // keepLayout: the step keeps every particle in its physical slot (no layout sort); the map
 // physical slot -> logical index is then written to perm2keep.
 void launch_cell_keys(hipStream_t st, const DeviceState& d, const SphParams& P,
-                      const StepParams& S, bool keepLayout);
+                      const StepParams& S, bool keepLayout, bool layoutPrev);
 // carry: also move the previous density and lambda (needed by a step without iterations and by
 // the learned features); carryDelta: also the position delta (a step without iterations keeps
 // it, as the reference's record does).
 void launch_gather(hipStream_t st, const DeviceState& d, const SphParams& P, const StepParams& S,
-                   bool carry, bool carryDelta);
+                   bool carry, bool carryDelta, bool layoutPrev);
 // Physical start and contiguity of every bucket (after the gather).
 void launch_bucket_phys(hipStream_t st, const DeviceState& d);
 // phys[lid[p]] = p over every scene's slots.

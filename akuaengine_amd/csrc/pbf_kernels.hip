@@ -160,7 +160,12 @@ __device__ __forceinline__ uint32_t layout_key(int cx, int cy, int cz, const Sce
 // (NeighbourSearchCUDA.cu:36-44) with smoothRadius as the cell size (NeighbourSearchCUDA.cu:163).
 // The hash, and the layout key with the particle's current physical slot, are stored at its
 // previous logical index, the first sort's input order.
-__global__ __launch_bounds__(kBlock) void k_predict_hash(DeviceState d, SphParams P, StepParams S) {
+// layoutPrev: the layout sort runs over the previous logical order too, in the hash sort's
+// launches (single-cell super-cells, where the previous and the new logical order agree within
+// every layout key): its keys go straight into its input (keysM) and the record keeps only the
+// slot, to which k_cell_keys adds the new logical index.
+__global__ __launch_bounds__(kBlock) void k_predict_hash(DeviceState d, SphParams P, StepParams S,
+                                                         bool layoutPrev) {
     uint32_t p = blockIdx.x * kBlock + threadIdx.x;
     if (p >= d.nslots) return;
     const SceneDev& sc = scene_of(d, p);
@@ -171,7 +176,12 @@ __global__ __launch_bounds__(kBlock) void k_predict_hash(DeviceState d, SphParam
     discretize(xs, P.cellHash, cx, cy, cz);
     const uint32_t k = d.lid[p];
     d.keysL[k] = P.searchCells == 2 ? cell_key_fine(cx, cy, cz) : cell_hash(cx, cy, cz, sc);
-    d.layPhys[k] = make_uint2(layout_key(cx, cy, cz, sc), p);
+    if (layoutPrev) {
+        d.keysM[k] = layout_key(cx, cy, cz, sc);
+        d.layPhys[k] = make_uint2(0u, p);
+    } else {
+        d.layPhys[k] = make_uint2(layout_key(cx, cy, cz, sc), p);
+    }
 }
 
 __device__ __forceinline__ uint32_t table_insert(uint4* table, const SceneDev& sc, uint32_t h) {
@@ -192,22 +202,29 @@ __device__ __forceinline__ uint32_t table_insert(uint4* table, const SceneDev& s
 // NeighbourSearchCUDA.cu:52-65, plus the bucket end the reference finds by scanning).
 // kKeep: the physical layout is kept this step (every particle stays in its slot), so instead of
 // the layout key, the inverse map physical slot -> new logical index is written directly.
+// layoutPrev (not with kKeep): the layout was sorted over the previous logical order, so the
+// record at the previous logical index receives the new one instead (read back by k_gather).
 template <bool kKeep>
-__global__ __launch_bounds__(kBlock) void k_cell_keys(DeviceState d, SphParams P, StepParams S) {
+__global__ __launch_bounds__(kBlock) void k_cell_keys(DeviceState d, SphParams P, StepParams S,
+                                                      bool layoutPrev) {
     uint32_t k = blockIdx.x * kBlock + threadIdx.x;
     if (k >= d.nslots) return;
     const SceneDev& sc = scene_of(d, k);
     if (k >= sc.end) return;
     const uint32_t prevLogical = d.perm1[k];
     if (prevLogical >= d.nslots) return;  // a consistent sort never produces this
-    const uint2 lp = d.layPhys[prevLogical];  // (layout key, previous physical slot)
-    const uint32_t src = lp.y;
-    if (kKeep) {
-        if (src < d.nslots) d.perm2keep[src] = k;
+    if (!kKeep && layoutPrev) {
+        reinterpret_cast<uint32_t*>(d.layPhys)[2u * prevLogical] = k;
     } else {
-        d.keysM[k] = lp.x;
+        const uint2 lp = d.layPhys[prevLogical];  // (layout key, previous physical slot)
+        const uint32_t src = lp.y;
+        if (kKeep) {
+            if (src < d.nslots) d.perm2keep[src] = k;
+        } else {
+            d.keysM[k] = lp.x;
+        }
+        d.srcL[k] = src;
     }
-    d.srcL[k] = src;
     const uint32_t h = d.skeys[k];
     const bool first = (k == sc.begin) || (d.skeys[k - 1] != h);
     const bool last = (k + 1 == sc.end) || (d.skeys[k + 1] != h);
@@ -227,16 +244,28 @@ __global__ __launch_bounds__(kBlock) void k_cell_keys(DeviceState d, SphParams P
 // kCarry: also the previous density and lambda (read by a step without iterations and by the
 // learned features; a step with iterations rewrites both before anything reads them).
 // kCarryDelta: also the position delta (a step without iterations keeps the previous one).
+// layoutPrev: the layout was sorted over the previous logical order, so perm2 gives the previous
+// logical index, whose record holds the new logical index and the previous physical slot.
 template <bool kCarry, bool kCarryDelta>
-__global__ __launch_bounds__(kBlock) void k_gather(DeviceState d, SphParams P, StepParams S) {
+__global__ __launch_bounds__(kBlock) void k_gather(DeviceState d, SphParams P, StepParams S,
+                                                   bool layoutPrev) {
     uint32_t p = blockIdx.x * kBlock + threadIdx.x;
     if (p >= d.nslots) return;
     const SceneDev& sc = scene_of(d, p);
     if (p >= sc.end) return;
-    const uint32_t k = d.perm2[p];
-    if (k >= d.nslots) return;  // consistent sorts never produce these
-    const uint32_t src = d.srcL[k];
-    if (src >= d.nslots) return;
+    uint32_t k, src;
+    if (layoutPrev) {
+        const uint32_t prev = d.perm2[p];
+        if (prev >= d.nslots) return;  // consistent sorts never produce these
+        const uint2 r = d.layPhys[prev];
+        k = r.x;
+        src = r.y;
+    } else {
+        k = d.perm2[p];
+        if (k >= d.nslots) return;
+        src = d.srcL[k];
+    }
+    if (k >= d.nslots || src >= d.nslots) return;
     float4 x = d.X[src], v = d.V[src];
     f3 xs = predict_x(make3(x), make3(v), S);
     const float4 xs4 = make_float4(xs.x, xs.y, xs.z, x.w);
@@ -1882,24 +1911,24 @@ void launch_clear_table(hipStream_t st, uint4* table, uint32_t n) {
     if (n) k_fill_u4<<<grid(n), kBlock, 0, st>>>(table, n, kEmptyKey);
 }
 void launch_predict_hash(hipStream_t st, const DeviceState& d, const SphParams& P,
-                         const StepParams& S) {
-    k_predict_hash<<<grid(d.nslots), kBlock, 0, st>>>(d, P, S);
+                         const StepParams& S, bool layoutPrev) {
+    k_predict_hash<<<grid(d.nslots), kBlock, 0, st>>>(d, P, S, layoutPrev);
 }
 void launch_cell_keys(hipStream_t st, const DeviceState& d, const SphParams& P,
-                      const StepParams& S, bool keepLayout) {
+                      const StepParams& S, bool keepLayout, bool layoutPrev) {
     if (keepLayout)
-        k_cell_keys<true><<<grid(d.nslots), kBlock, 0, st>>>(d, P, S);
+        k_cell_keys<true><<<grid(d.nslots), kBlock, 0, st>>>(d, P, S, false);
     else
-        k_cell_keys<false><<<grid(d.nslots), kBlock, 0, st>>>(d, P, S);
+        k_cell_keys<false><<<grid(d.nslots), kBlock, 0, st>>>(d, P, S, layoutPrev);
 }
 void launch_gather(hipStream_t st, const DeviceState& d, const SphParams& P, const StepParams& S,
-                   bool carry, bool carryDelta) {
+                   bool carry, bool carryDelta, bool layoutPrev) {
     if (carryDelta)
-        k_gather<true, true><<<grid(d.nslots), kBlock, 0, st>>>(d, P, S);
+        k_gather<true, true><<<grid(d.nslots), kBlock, 0, st>>>(d, P, S, layoutPrev);
     else if (carry)
-        k_gather<true, false><<<grid(d.nslots), kBlock, 0, st>>>(d, P, S);
+        k_gather<true, false><<<grid(d.nslots), kBlock, 0, st>>>(d, P, S, layoutPrev);
     else
-        k_gather<false, false><<<grid(d.nslots), kBlock, 0, st>>>(d, P, S);
+        k_gather<false, false><<<grid(d.nslots), kBlock, 0, st>>>(d, P, S, layoutPrev);
 }
 void launch_invert_order(hipStream_t st, const DeviceState& d, uint32_t* phys) {
     k_invert_order<<<grid(d.nslots), kBlock, 0, st>>>(d, phys);

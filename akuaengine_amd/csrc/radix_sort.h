@@ -77,5 +77,13 @@ using LaunchHook = void (*)(void* ctx, const char* name, int phase);
 int radix_sort_pairs(hipStream_t stream, const SortPlan& plan, const SortBuffers& buf,
                      const SortTile* dTiles, uint32_t numTiles, const SortScene* dScenes,
                      uint32_t numScenes, uint32_t limit, LaunchHook hook, void* hookCtx);
+// Two independent sorts over the same tiles and scenes (separate buffers, counts, totals and
+// look-back words) with the same digit width, every pass of both in shared launches.  Returns
+// false, launching nothing, if the digit widths differ; else sets each result as
+// radix_sort_pairs returns it.
+bool radix_sort_pairs2(hipStream_t stream, const SortPlan& planA, const SortBuffers& bufA,
+                       const SortPlan& planB, const SortBuffers& bufB, const SortTile* dTiles,
+                       uint32_t numTiles, const SortScene* dScenes, uint32_t numScenes, uint32_t limit,
+                       LaunchHook hook, void* hookCtx, int& resultA, int& resultB);
 
 }  // namespace akua

@@ -13,6 +13,9 @@
 // more than the work) run instead: rs_init and rs_hist_all once per sort (the digit totals of every
 // pass), then per pass one rs_scatter that finds each tile's offsets by decoupled look-back over
 // the earlier tiles' published counts.  Both give the same stable order.
+// Two sorts over the same tiles with the same digit width (radix_sort_pairs2) share every launch:
+// each kernel's grid holds both sorts' blocks, the second sort's after the first's, and a pass of
+// only one sort (the longer one's last) runs alone.
 #include "radix_sort.h"
 
 #include <algorithm>
@@ -22,6 +25,19 @@ namespace akua {
 namespace {
 
 __device__ __forceinline__ uint32_t lane_id() { return __lane_id(); }
+
+// One sort's buffers in one pass.
+struct SortIO {
+    const uint32_t* keysIn;
+    const uint32_t* valsIn;
+    uint32_t* keysOut;
+    uint32_t* valsOut;
+    uint32_t* counts;    // [scene][digit][tile] counts, then offsets (three-launch passes)
+    uint32_t* totals;    // this pass's [scene][digit] totals
+    uint32_t* look;      // look-back words of this pass, and of the next (cleared here)
+    uint32_t* lookNext;
+    uint32_t* ticket;    // this pass's tile ticket
+};
 
 // Lanes of this wave (among `valid`) whose digit equals mine.
 template <int D>
@@ -54,14 +70,17 @@ __device__ __forceinline__ uint32_t block_exclusive_scan_256(uint32_t v, uint32_
     return base + inc - v;
 }
 
+// Blocks [0, ntiles) count sort a's tiles, blocks [ntiles, 2 ntiles) sort b's.
 template <int D>
-__global__ __launch_bounds__(kSortBlock) void rs_hist(const uint32_t* __restrict__ keys,
-                                                      const SortTile* __restrict__ tiles,
-                                                      uint32_t shift, uint32_t* __restrict__ counts) {
+__global__ __launch_bounds__(kSortBlock) void rs_hist(SortIO a, SortIO b, const SortTile* __restrict__ tiles,
+                                                      uint32_t ntiles, uint32_t shift) {
     constexpr uint32_t R = 1u << D;
     constexpr int kWaves = kSortBlock / 64;
     __shared__ uint32_t hist[kWaves][R];  // one histogram per wave: fewer atomic collisions
-    const SortTile t = tiles[blockIdx.x];
+    const bool second = blockIdx.x >= ntiles;
+    const uint32_t* __restrict__ keys = second ? b.keysIn : a.keysIn;
+    uint32_t* __restrict__ counts = second ? b.counts : a.counts;
+    const SortTile t = tiles[blockIdx.x - (second ? ntiles : 0u)];
     const uint32_t wave = threadIdx.x >> 6;
     for (uint32_t d = threadIdx.x; d < R; d += kSortBlock)
         for (int w = 0; w < kWaves; ++w) hist[w][d] = 0;
@@ -86,13 +105,17 @@ __global__ __launch_bounds__(kSortBlock) void rs_hist(const uint32_t* __restrict
 // One wave per (scene, digit): offsets[scene][digit][tile] = the keys of this digit in earlier
 // tiles of the scene, and totals[scene][digit] = the scene's keys with this digit.  The scatter
 // adds the scene's first slot and the keys of smaller digits (no atomics anywhere).
+// Blocks [0, nscenes R) scan sort a's counts, the next nscenes R sort b's.
 template <int D>
-__global__ __launch_bounds__(64) void rs_scan(uint32_t* __restrict__ counts,
-                                              uint32_t* __restrict__ totals,
-                                              const SortScene* __restrict__ scenes) {
+__global__ __launch_bounds__(64) void rs_scan(SortIO a, SortIO b, const SortScene* __restrict__ scenes,
+                                              uint32_t nscenes) {
     constexpr uint32_t R = 1u << D;
     constexpr uint32_t kPer = 8;  // consecutive tiles per lane: one memory round trip per 512
-    const uint32_t s = blockIdx.x / R, d = blockIdx.x % R;
+    const bool second = blockIdx.x >= nscenes * R;
+    uint32_t* __restrict__ counts = second ? b.counts : a.counts;
+    uint32_t* __restrict__ totals = second ? b.totals : a.totals;
+    const uint32_t bi = blockIdx.x - (second ? nscenes * R : 0u);
+    const uint32_t s = bi / R, d = bi % R;
     const SortScene sc = scenes[s];
     const uint32_t lane = lane_id();
     uint32_t carry = 0;
@@ -135,27 +158,36 @@ constexpr uint32_t kLookSpinLimit = 1u << 20;
 // Earlier tiles' words read per round of the look-back.
 constexpr uint32_t kLookBatch = 16;
 
-// Clears the digit totals of every pass, the first pass's look-back words and the tile tickets.
-__global__ __launch_bounds__(kSortBlock) void rs_init(uint32_t* __restrict__ totals, uint32_t nTotals,
-                                                      uint32_t* __restrict__ look, uint32_t nLook,
-                                                      uint32_t* __restrict__ tickets) {
+// Clears the digit totals of every pass, the first pass's look-back words and the tile tickets
+// of sort a, and of sort b when `both`.
+__global__ __launch_bounds__(kSortBlock) void rs_init(SortIO a, uint32_t nTotalsA, SortIO b, uint32_t nTotalsB,
+                                                      uint32_t nLook, bool both) {
     const uint32_t i = blockIdx.x * kSortBlock + threadIdx.x;
     const uint32_t stride = gridDim.x * kSortBlock;
-    for (uint32_t k = i; k < nTotals; k += stride) totals[k] = 0u;
-    for (uint32_t k = i; k < nLook; k += stride) look[k] = 0u;
-    if (i < (uint32_t)kMaxPasses) tickets[i] = 0u;
+    for (int side = 0; side < (both ? 2 : 1); ++side) {
+        const SortIO& io = side ? b : a;
+        const uint32_t nTotals = side ? nTotalsB : nTotalsA;
+        for (uint32_t k = i; k < nTotals; k += stride) io.totals[k] = 0u;
+        for (uint32_t k = i; k < nLook; k += stride) io.look[k] = 0u;
+        if (i < (uint32_t)kMaxPasses) io.ticket[i] = 0u;
+    }
 }
 
 // Per tile, the digit histogram of every pass at once (the counts do not depend on the order),
 // added to totals[pass][scene][digit].
+// Blocks [0, ntiles) count sort a's tiles over passesA passes, the next ntiles sort b's over
+// passesB (totals: every pass's, from the SortIO's totals).
 template <int D>
-__global__ __launch_bounds__(kSortBlock) void rs_hist_all(const uint32_t* __restrict__ keys,
-                                                          const SortTile* __restrict__ tiles,
-                                                          uint32_t passes, uint32_t numScenes,
-                                                          uint32_t* __restrict__ totals) {
+__global__ __launch_bounds__(kSortBlock) void rs_hist_all(SortIO a, uint32_t passesA, SortIO b, uint32_t passesB,
+                                                          const SortTile* __restrict__ tiles, uint32_t ntiles,
+                                                          uint32_t numScenes) {
     constexpr uint32_t R = 1u << D;
     __shared__ uint32_t hist[kMaxPasses * R];
-    const SortTile t = tiles[blockIdx.x];
+    const bool second = blockIdx.x >= ntiles;
+    const uint32_t* __restrict__ keys = second ? b.keysIn : a.keysIn;
+    uint32_t* __restrict__ totals = second ? b.totals : a.totals;
+    const uint32_t passes = second ? passesB : passesA;
+    const SortTile t = tiles[blockIdx.x - (second ? ntiles : 0u)];
     for (uint32_t k = threadIdx.x; k < passes * R; k += kSortBlock) hist[k] = 0u;
     __syncthreads();
 #pragma unroll
@@ -217,14 +249,22 @@ __device__ __forceinline__ uint32_t look_back(uint32_t* look, uint32_t R, uint32
 }
 
 // kLookBack: tiles are taken in ticket order (look-back words in `look`, the next pass's cleared in
-// `lookNext`, `offsets` unused); otherwise tile blockIdx.x with its offsets from rs_scan.
+// `lookNext`, the counts unused); otherwise tile blockIdx.x with its offsets from rs_scan.  Blocks
+// [0, ntiles) move sort a's keys, blocks [ntiles, 2 ntiles) sort b's, each taking its own sort's
+// tickets.
 template <int D, bool kIdentityVals, bool kLookBack>
-__global__ __launch_bounds__(kSortBlock) void rs_scatter(
-    const uint32_t* __restrict__ keysIn, const uint32_t* __restrict__ valsIn,
-    uint32_t* __restrict__ keysOut, uint32_t* __restrict__ valsOut,
-    const SortTile* __restrict__ tiles, uint32_t shift, const uint32_t* __restrict__ offsets,
-    const uint32_t* __restrict__ totals, uint32_t limit, uint32_t ntiles, uint32_t* look,
-    uint32_t* __restrict__ lookNext, uint32_t* ticket) {
+__global__ __launch_bounds__(kSortBlock) void rs_scatter(SortIO a, SortIO b, const SortTile* __restrict__ tiles,
+                                                         uint32_t shift, uint32_t limit, uint32_t ntiles) {
+    const bool second = blockIdx.x >= ntiles;
+    const uint32_t* __restrict__ keysIn = second ? b.keysIn : a.keysIn;
+    const uint32_t* __restrict__ valsIn = second ? b.valsIn : a.valsIn;
+    uint32_t* __restrict__ keysOut = second ? b.keysOut : a.keysOut;
+    uint32_t* __restrict__ valsOut = second ? b.valsOut : a.valsOut;
+    const uint32_t* __restrict__ offsets = second ? b.counts : a.counts;
+    const uint32_t* __restrict__ totals = second ? b.totals : a.totals;
+    uint32_t* look = second ? b.look : a.look;
+    uint32_t* __restrict__ lookNext = second ? b.lookNext : a.lookNext;
+    uint32_t* ticket = second ? b.ticket : a.ticket;
     constexpr uint32_t R = 1u << D;
     constexpr int kWaves = kSortBlock / 64;
     constexpr int kRounds = kSortTile / kSortBlock;  // rounds of 64 keys per wave
@@ -242,7 +282,7 @@ __global__ __launch_bounds__(kSortBlock) void rs_scatter(
     for (uint32_t d = threadIdx.x; d < R; d += kSortBlock)
         for (int w = 0; w < kWaves; ++w) waveCnt[w][d] = 0;
     __syncthreads();
-    const uint32_t ti = kLookBack ? sTicket : blockIdx.x;
+    const uint32_t ti = kLookBack ? sTicket : blockIdx.x - (second ? ntiles : 0u);
     if (ti >= ntiles) return;  // one ticket per workgroup of an ntiles grid: never taken
     const SortTile t = tiles[ti];
     if (kLookBack)
@@ -323,64 +363,85 @@ __global__ __launch_bounds__(kSortBlock) void rs_scatter(
     }
 }
 
-template <int D>
-void launch_pass(hipStream_t st, bool first, const uint32_t* kin, const uint32_t* vin,
-                 uint32_t* kout, uint32_t* vout, const SortTile* tiles, uint32_t ntiles,
-                 const SortScene* scenes, uint32_t nscenes, uint32_t shift, uint32_t* counts,
-                 uint32_t* totals, uint32_t limit, LaunchHook hook, void* ctx) {
-    if (hook) hook(ctx, "rs_hist", 0);
-    rs_hist<D><<<ntiles, kSortBlock, 0, st>>>(kin, tiles, shift, counts);
-    if (hook) hook(ctx, "rs_hist", 1);
-    if (hook) hook(ctx, "rs_scan", 0);
-    rs_scan<D><<<nscenes * (1u << D), 64, 0, st>>>(counts, totals, scenes);
-    if (hook) hook(ctx, "rs_scan", 1);
-    if (hook) hook(ctx, "rs_scatter", 0);
-    if (first)
-        rs_scatter<D, true, false><<<ntiles, kSortBlock, 0, st>>>(kin, vin, kout, vout, tiles, shift, counts,
-                                                                  totals, limit, ntiles, nullptr, nullptr,
-                                                                  nullptr);
-    else
-        rs_scatter<D, false, false><<<ntiles, kSortBlock, 0, st>>>(kin, vin, kout, vout, tiles, shift, counts,
-                                                                   totals, limit, ntiles, nullptr, nullptr,
-                                                                   nullptr);
-    if (hook) hook(ctx, "rs_scatter", 1);
+// Sort `buf`'s buffers for pass p: the keys are read from (keysA, valsA) in even passes.
+SortIO pass_io(const SortBuffers& buf, int p, uint32_t R, uint32_t nscenes, uint32_t ntiles, bool oneSweep) {
+    const bool even = (p % 2) == 0;
+    SortIO io{};
+    io.keysIn = even ? buf.keysA : buf.keysB;
+    io.valsIn = even ? buf.valsA : buf.valsB;
+    io.keysOut = even ? buf.keysB : buf.keysA;
+    io.valsOut = even ? buf.valsB : buf.valsA;
+    io.counts = buf.counts;
+    io.totals = buf.totals + (size_t)p * nscenes * R;
+    if (oneSweep) {  // look-back words alternate between the two halves of buf.look
+        io.look = buf.look + (size_t)(p & 1) * ntiles * R;
+        io.lookNext = buf.look + (size_t)((p + 1) & 1) * ntiles * R;
+        io.ticket = buf.tickets + p;
+    }
+    return io;
 }
 
-// A sort of at most kOneSweepMaxTiles tiles: the totals of every pass at once, then one
-// look-back scatter per pass (look-back words alternate between the two halves of buf.look).
+// Sort A (PA passes) and, when B is given, sort B (PB passes) over the same tiles, every pass of
+// both in shared launches.  oneSweep: sorts of at most kOneSweepMaxTiles tiles, the totals of every
+// pass at once, then one look-back scatter per pass; otherwise three launches per pass.
 template <int D>
-void launch_onesweep_sort(hipStream_t st, const SortPlan& plan, const SortBuffers& buf,
-                          const SortTile* tiles, uint32_t ntiles, uint32_t nscenes, uint32_t limit,
-                          LaunchHook hook, void* ctx) {
+void run_sorts(hipStream_t st, const SortBuffers& A, int PA, const SortBuffers* B, int PB,
+               const SortTile* tiles, uint32_t ntiles, const SortScene* scenes, uint32_t nscenes,
+               uint32_t limit, bool oneSweep, LaunchHook hook, void* ctx) {
     constexpr uint32_t R = 1u << D;
-    const int P = plan.passes();
-    const uint32_t nTotals = (uint32_t)P * nscenes * R, nLook = ntiles * R;
-    const uint32_t most = std::max(nTotals, nLook);
-    if (hook) hook(ctx, "rs_hist", 0);
-    rs_init<<<std::min<uint32_t>((most + kSortBlock - 1) / kSortBlock, 64u), kSortBlock, 0, st>>>(
-        buf.totals, nTotals, buf.look, nLook, buf.tickets);
-    rs_hist_all<D><<<ntiles, kSortBlock, 0, st>>>(buf.keysA, tiles, (uint32_t)P, nscenes, buf.totals);
-    if (hook) hook(ctx, "rs_hist", 1);
-    uint32_t* kin = buf.keysA;
-    uint32_t* vin = buf.valsA;
-    uint32_t* kout = buf.keysB;
-    uint32_t* vout = buf.valsB;
+    const int P = std::max(PA, B ? PB : 0);
+    if (oneSweep) {
+        const SortIO a0 = pass_io(A, 0, R, nscenes, ntiles, true);
+        const SortIO b0 = B ? pass_io(*B, 0, R, nscenes, ntiles, true) : a0;
+        const uint32_t nTotA = (uint32_t)PA * nscenes * R, nTotB = B ? (uint32_t)PB * nscenes * R : 0u;
+        const uint32_t nLook = ntiles * R;
+        const uint32_t most = std::max(std::max(nTotA, nTotB), nLook);
+        if (hook) hook(ctx, "rs_hist", 0);
+        rs_init<<<std::min<uint32_t>((most + kSortBlock - 1) / kSortBlock, 64u), kSortBlock, 0, st>>>(
+            a0, nTotA, b0, nTotB, nLook, B != nullptr);
+        rs_hist_all<D><<<(B ? 2u : 1u) * ntiles, kSortBlock, 0, st>>>(a0, (uint32_t)PA, b0, B ? (uint32_t)PB : 0u,
+                                                                       tiles, ntiles, nscenes);
+        if (hook) hook(ctx, "rs_hist", 1);
+    }
     for (int p = 0; p < P; ++p) {
-        uint32_t* look = buf.look + (size_t)(p & 1) * ntiles * R;
-        uint32_t* lookNext = buf.look + (size_t)((p + 1) & 1) * ntiles * R;
-        const uint32_t* tot = buf.totals + (size_t)p * nscenes * R;
-        if (hook) hook(ctx, "rs_onesweep", 0);
-        if (p == 0)
-            rs_scatter<D, true, true><<<ntiles, kSortBlock, 0, st>>>(kin, vin, kout, vout, tiles, (uint32_t)(p * D),
-                                                                     nullptr, tot, limit, ntiles, look, lookNext,
-                                                                     buf.tickets + p);
-        else
-            rs_scatter<D, false, true><<<ntiles, kSortBlock, 0, st>>>(kin, vin, kout, vout, tiles, (uint32_t)(p * D),
-                                                                      nullptr, tot, limit, ntiles, look, lookNext,
-                                                                      buf.tickets + p);
-        if (hook) hook(ctx, "rs_onesweep", 1);
-        std::swap(kin, kout);
-        std::swap(vin, vout);
+        const bool inA = p < PA, inB = B && p < PB;
+        const SortIO x = inA ? pass_io(A, p, R, nscenes, ntiles, oneSweep) : pass_io(*B, p, R, nscenes, ntiles, oneSweep);
+        const SortIO y = inA && inB ? pass_io(*B, p, R, nscenes, ntiles, oneSweep) : x;
+        const uint32_t sorts = inA && inB ? 2u : 1u;
+        const uint32_t shift = (uint32_t)(p * D);
+        if (oneSweep) {
+            if (hook) hook(ctx, "rs_onesweep", 0);
+            if (p == 0)
+                rs_scatter<D, true, true><<<sorts * ntiles, kSortBlock, 0, st>>>(x, y, tiles, shift, limit, ntiles);
+            else
+                rs_scatter<D, false, true><<<sorts * ntiles, kSortBlock, 0, st>>>(x, y, tiles, shift, limit, ntiles);
+            if (hook) hook(ctx, "rs_onesweep", 1);
+        } else {
+            if (hook) hook(ctx, "rs_hist", 0);
+            rs_hist<D><<<sorts * ntiles, kSortBlock, 0, st>>>(x, y, tiles, ntiles, shift);
+            if (hook) hook(ctx, "rs_hist", 1);
+            if (hook) hook(ctx, "rs_scan", 0);
+            rs_scan<D><<<sorts * nscenes * R, 64, 0, st>>>(x, y, scenes, nscenes);
+            if (hook) hook(ctx, "rs_scan", 1);
+            if (hook) hook(ctx, "rs_scatter", 0);
+            if (p == 0)
+                rs_scatter<D, true, false><<<sorts * ntiles, kSortBlock, 0, st>>>(x, y, tiles, shift, limit, ntiles);
+            else
+                rs_scatter<D, false, false><<<sorts * ntiles, kSortBlock, 0, st>>>(x, y, tiles, shift, limit, ntiles);
+            if (hook) hook(ctx, "rs_scatter", 1);
+        }
+    }
+}
+
+void dispatch_sorts(int digitBits, hipStream_t st, const SortBuffers& A, int PA, const SortBuffers* B, int PB,
+                    const SortTile* tiles, uint32_t ntiles, const SortScene* scenes, uint32_t nscenes,
+                    uint32_t limit, bool oneSweep, LaunchHook hook, void* ctx) {
+    switch (digitBits) {
+        case 4: run_sorts<4>(st, A, PA, B, PB, tiles, ntiles, scenes, nscenes, limit, oneSweep, hook, ctx); break;
+        case 5: run_sorts<5>(st, A, PA, B, PB, tiles, ntiles, scenes, nscenes, limit, oneSweep, hook, ctx); break;
+        case 6: run_sorts<6>(st, A, PA, B, PB, tiles, ntiles, scenes, nscenes, limit, oneSweep, hook, ctx); break;
+        case 7: run_sorts<7>(st, A, PA, B, PB, tiles, ntiles, scenes, nscenes, limit, oneSweep, hook, ctx); break;
+        default: run_sorts<8>(st, A, PA, B, PB, tiles, ntiles, scenes, nscenes, limit, oneSweep, hook, ctx); break;
     }
 }
 
@@ -423,40 +484,24 @@ int radix_sort_pairs(hipStream_t stream, const SortPlan& plan, const SortBuffers
                      const SortTile* dTiles, uint32_t numTiles, const SortScene* dScenes,
                      uint32_t numScenes, uint32_t limit, LaunchHook hook, void* hookCtx) {
     const int P = plan.passes();
-    const uint32_t R = 1u << plan.digitBits;
-    if (numTiles <= kOneSweepMaxTiles && buf.look && buf.tickets) {
-        (void)dScenes;
-        switch (plan.digitBits) {
-            case 4: launch_onesweep_sort<4>(stream, plan, buf, dTiles, numTiles, numScenes, limit, hook, hookCtx); break;
-            case 5: launch_onesweep_sort<5>(stream, plan, buf, dTiles, numTiles, numScenes, limit, hook, hookCtx); break;
-            case 6: launch_onesweep_sort<6>(stream, plan, buf, dTiles, numTiles, numScenes, limit, hook, hookCtx); break;
-            case 7: launch_onesweep_sort<7>(stream, plan, buf, dTiles, numTiles, numScenes, limit, hook, hookCtx); break;
-            default: launch_onesweep_sort<8>(stream, plan, buf, dTiles, numTiles, numScenes, limit, hook, hookCtx); break;
-        }
-        return (P % 2 == 0) ? 0 : 1;
-    }
-    uint32_t* kin = buf.keysA;
-    uint32_t* vin = buf.valsA;
-    uint32_t* kout = buf.keysB;
-    uint32_t* vout = buf.valsB;
-    for (int p = 0; p < P; ++p) {
-        const uint32_t shift = (uint32_t)(p * plan.digitBits);
-        uint32_t* tot = buf.totals + (size_t)p * R * numScenes;
-#define AKUA_PASS(DB)                                                                        \
-    launch_pass<DB>(stream, p == 0, kin, vin, kout, vout, dTiles, numTiles, dScenes, numScenes, \
-                    shift, buf.counts, tot, limit, hook, hookCtx)
-        switch (plan.digitBits) {
-            case 4: AKUA_PASS(4); break;
-            case 5: AKUA_PASS(5); break;
-            case 6: AKUA_PASS(6); break;
-            case 7: AKUA_PASS(7); break;
-            default: AKUA_PASS(8); break;
-        }
-#undef AKUA_PASS
-        std::swap(kin, kout);
-        std::swap(vin, vout);
-    }
+    const bool oneSweep = numTiles <= kOneSweepMaxTiles && buf.look && buf.tickets;
+    dispatch_sorts(plan.digitBits, stream, buf, P, nullptr, 0, dTiles, numTiles, dScenes, numScenes, limit,
+                   oneSweep, hook, hookCtx);
     return (P % 2 == 0) ? 0 : 1;
+}
+
+bool radix_sort_pairs2(hipStream_t stream, const SortPlan& planA, const SortBuffers& bufA,
+                       const SortPlan& planB, const SortBuffers& bufB, const SortTile* dTiles,
+                       uint32_t numTiles, const SortScene* dScenes, uint32_t numScenes, uint32_t limit,
+                       LaunchHook hook, void* hookCtx, int& resultA, int& resultB) {
+    if (planA.digitBits != planB.digitBits) return false;
+    const int PA = planA.passes(), PB = planB.passes();
+    const bool oneSweep = numTiles <= kOneSweepMaxTiles && bufA.look && bufA.tickets && bufB.look && bufB.tickets;
+    dispatch_sorts(planA.digitBits, stream, bufA, PA, &bufB, PB, dTiles, numTiles, dScenes, numScenes, limit,
+                   oneSweep, hook, hookCtx);
+    resultA = (PA % 2 == 0) ? 0 : 1;
+    resultB = (PB % 2 == 0) ? 0 : 1;
+    return true;
 }
 
 }  // namespace akua

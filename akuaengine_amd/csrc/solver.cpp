@@ -186,6 +186,7 @@ struct akua_solver {
     DevBuf<uint32_t> id, Tid, lid, phys, lidT, physT, srcL, bslot, nbr, cnt, perm2keep;
     DevBuf<uint2> layPhys;
     DevBuf<uint32_t> s1KA, s1VA, s1KB, s1VB, s2KA, s2VA, s2KB, s2VB, counts, totals, look, tickets;
+    DevBuf<uint32_t> counts2, totals2, look2, tickets2;  // the layout sort's, beside the hash sort
     DevBuf<uint4> table;
     DevBuf<uint32_t> work, cntW, nbrW;  // the sweeps' balanced work order (allocated when used)
     DevBuf<uint32_t> qcost, queue, qcount, heads;  // the sweeps' work queue (allocated when used)
@@ -313,7 +314,7 @@ struct akua_solver {
             b->release();
         for (auto* b : {&id, &Tid, &lid, &phys, &lidT, &physT, &srcL, &bslot, &nbr, &cnt,
                         &perm2keep, &s1KA, &s1VA, &s1KB, &s1VB, &s2KA, &s2VA, &s2KB, &s2VB, &counts,
-                        &totals, &look, &tickets, &gate, &fallbacks, &work, &cntW, &nbrW, &qcost,
+                        &totals, &look, &tickets, &counts2, &totals2, &look2, &tickets2, &gate, &fallbacks, &work, &cntW, &nbrW, &qcost,
                         &queue, &qcount, &heads})
             b->release();
         layPhys.release(); table.release(); dTiles.release(); dSortScenes.release(); nbrSum.release();
@@ -438,11 +439,12 @@ void setup(akua_solver* s, const std::vector<uint32_t>& counts) {
     s->XsC.alloc(ns); s->srcL.alloc(ns); s->layPhys.alloc(ns); s->perm2keep.alloc(ns);
     s->s1KA.alloc(ns); s->s1VA.alloc(ns); s->s1KB.alloc(ns); s->s1VB.alloc(ns);
     s->s2KA.alloc(ns); s->s2VA.alloc(ns); s->s2KB.alloc(ns); s->s2VB.alloc(ns);
-    s->counts.alloc((size_t)s->numTiles * (1u << kMaxDigitBits));
-    s->totals.alloc((size_t)kMaxPasses * (1u << kMaxDigitBits) * s->scenes.size());
+    for (auto* c : {&s->counts, &s->counts2}) c->alloc((size_t)s->numTiles * (1u << kMaxDigitBits));
+    for (auto* t : {&s->totals, &s->totals2})
+        t->alloc((size_t)kMaxPasses * (1u << kMaxDigitBits) * s->scenes.size());
     if (s->numTiles <= kOneSweepMaxTiles) {
-        s->look.alloc((size_t)2 * s->numTiles * (1u << kMaxDigitBits));
-        s->tickets.alloc((size_t)kMaxPasses);
+        for (auto* l : {&s->look, &s->look2}) l->alloc((size_t)2 * s->numTiles * (1u << kMaxDigitBits));
+        for (auto* t : {&s->tickets, &s->tickets2}) t->alloc((size_t)kMaxPasses);
     }
     s->table.alloc(s->tableTotal);
     s->bslot.alloc(ns);
@@ -556,6 +558,16 @@ void sort_hash(akua_solver* s, hipStream_t st, bool timed) {
                      (uint32_t)s->scenes.size(), s->nslots, timed ? timing_hook : nullptr, s);
 }
 
+// Both sorts in shared launches, the layout keys in the previous logical order (single-cell
+// super-cells with equal digit widths); false if the widths differ (nothing launched).
+bool sort_both(akua_solver* s, hipStream_t st, bool timed) {
+    SortBuffers h{s->s1KA.p, s->s1VA.p, s->s1KB.p, s->s1VB.p, s->counts.p, s->totals.p, s->look.p, s->tickets.p};
+    SortBuffers l{s->s2KA.p, s->s2VA.p, s->s2KB.p, s->s2VB.p, s->counts2.p, s->totals2.p, s->look2.p, s->tickets2.p};
+    int r1 = 0, r2 = 0;
+    return radix_sort_pairs2(st, s->plan1, h, s->plan2, l, s->dTiles.p, s->numTiles, s->dSortScenes.p,
+                             (uint32_t)s->scenes.size(), s->nslots, timed ? timing_hook : nullptr, s, r1, r2);
+}
+
 // The physical layout: super-cell Morton keys stably sorted over the new logical order.
 void sort_layout(akua_solver* s, hipStream_t st, bool timed) {
     SortBuffers b{s->s2KA.p, s->s2VA.p, s->s2KB.p, s->s2VB.p, s->counts.p, s->totals.p, s->look.p, s->tickets.p};
@@ -579,18 +591,28 @@ void enqueue_step(akua_solver* s, hipStream_t st, const StepParams& S, bool time
     L.before("fill_table");
     launch_clear_table(st, s->table.p, s->tableTotal);
     L.after("fill_table");
+    // single-cell super-cells: the layout keys are sorted in the previous logical order, which
+    // agrees with the new one within each key, so both sorts share their launches
+    const bool together = !keepLayout && s->tuning.layoutLevel == 0 && s->plan1.digitBits == s->plan2.digitBits;
     L.before("predict_hash");
-    launch_predict_hash(st, d, P, S);
+    launch_predict_hash(st, d, P, S, together);
     L.after("predict_hash");
-    sort_hash(s, st, timed);
-    L.before("cell_keys");
-    launch_cell_keys(st, d, P, S, keepLayout);
-    L.after("cell_keys");
-    if (!keepLayout) sort_layout(s, st, timed);
+    if (together) {
+        sort_both(s, st, timed);
+        L.before("cell_keys");
+        launch_cell_keys(st, d, P, S, false, true);
+        L.after("cell_keys");
+    } else {
+        sort_hash(s, st, timed);
+        L.before("cell_keys");
+        launch_cell_keys(st, d, P, S, keepLayout, false);
+        L.after("cell_keys");
+        if (!keepLayout) sort_layout(s, st, timed);
+    }
     L.before("gather");
     launch_gather(st, d, P, S,
                   s->cfg.solverIterations == 0 || record || (s->mode == 1 && s->haveNet),
-                  s->cfg.solverIterations == 0);
+                  s->cfg.solverIterations == 0, together);
     L.after("gather");
     L.before("bucket_phys");
     launch_bucket_phys(st, d);
@@ -721,34 +743,48 @@ void upload_scenes(akua_solver* s) {
     s->scenesDirty = false;
 }
 
-// Super-cell layout of a scene from its box: Morton coordinates start one super-cell below the
-// box and cover it with one super-cell to spare; bits per axis (at most 10) cover the longest
-// axis.  Particles outside are clamped to the edge of the key range, which only costs locality.
 SortPlan make_plan(int bits, int digitBits) {
     SortPlan p = SortPlan::balanced(bits);
     if (digitBits >= 4 && digitBits <= kMaxDigitBits) p.digitBits = digitBits;
     return p;
 }
 
+// Super-cell layout of the scenes from their boxes: bits per axis (at most 10, the same for every
+// scene) cover the longest axis with kLayoutMarginCells hash cells (at least one super-cell) to
+// spare beyond each wall, and each axis's key range is centred on the box, so predicted positions
+// that overshoot a wall keep keys of their own.  Positions beyond the range are clamped to its edge,
+// which puts several cells under one key: that costs locality only, and with the layout sorted in
+// the previous logical order it also splits their buckets (a key's cells then interleave).
+constexpr int kLayoutMarginCells = 4;
+
 void update_layout(akua_solver* s) {
     int bits = 1;
     const int L = s->tuning.layoutLevel;
     const float span = (float)(1 << L);
+    const int margin = std::max(1, (kLayoutMarginCells + (1 << L) - 1) >> L);  // in super-cells
     for (SceneDev& sc : s->scenes) {
         int ext = 1;
         for (int a = 0; a < 3; ++a) {
-            const int lo = (int)std::floor(std::floor(sc.bmin[a] / s->sph.cellHash) / span) - 1;
-            const int hi = (int)std::floor(std::floor(sc.bmax[a] / s->sph.cellHash) / span) + 1;
-            sc.layoutOrigin[a] = lo;
-            ext = std::max(ext, hi - lo + 1);
+            const int lo = (int)std::floor(std::floor(sc.bmin[a] / s->sph.cellHash) / span);
+            const int hi = (int)std::floor(std::floor(sc.bmax[a] / s->sph.cellHash) / span);
+            ext = std::max(ext, hi - lo + 1 + 2 * margin);
+        }
+        bits = std::max(bits, (int)std::min<uint32_t>(10u, std::max<uint32_t>(1u, ceil_log2((uint64_t)ext))));
+    }
+    for (SceneDev& sc : s->scenes) {
+        for (int a = 0; a < 3; ++a) {
+            const int lo = (int)std::floor(std::floor(sc.bmin[a] / s->sph.cellHash) / span);
+            const int hi = (int)std::floor(std::floor(sc.bmax[a] / s->sph.cellHash) / span);
+            sc.layoutOrigin[a] = lo - ((1 << bits) - (hi - lo + 1)) / 2;
         }
         sc.layoutOrigin[3] = L;
-        sc.layoutBits = std::min<uint32_t>(10u, std::max<uint32_t>(1u, ceil_log2((uint64_t)ext)));
-        bits = std::max(bits, (int)sc.layoutBits);
     }
     for (SceneDev& sc : s->scenes) sc.layoutBits = (uint32_t)bits;
-    s->plan2 = make_plan(3 * bits, s->tuning.layoutDigitBits);
     s->plan1 = make_plan(s->plan1.bits, s->tuning.hashDigitBits);
+    // single-cell super-cells with the layout digit width left to the solver: the hash sort's
+    // width, so that the two sorts share their launches
+    const int layoutDigits = L == 0 && s->tuning.layoutDigitBits == 0 ? s->plan1.digitBits : s->tuning.layoutDigitBits;
+    s->plan2 = make_plan(3 * bits, layoutDigits);
     s->scenesDirty = true;
 }
 
@@ -905,11 +941,11 @@ void run_density_probe(akua_solver* s) {
     HIP_CHECK(hipMemsetAsync(s->probeMax.p, 0, s->scenes.size() * sizeof(unsigned int), st));
     HIP_CHECK(hipMemsetAsync(s->probeSum.p, 0, s->scenes.size() * sizeof(double), st));
     launch_clear_table(st, s->table.p, s->tableTotal);
-    launch_predict_hash(st, d, P0, S0);
+    launch_predict_hash(st, d, P0, S0, false);
     sort_hash(s, st, false);
-    launch_cell_keys(st, d, P0, S0, false);
+    launch_cell_keys(st, d, P0, S0, false, false);
     sort_layout(s, st, false);
-    launch_gather(st, d, P0, S0, false, false);
+    launch_gather(st, d, P0, S0, false, false, false);
     launch_bucket_phys(st, d);
     launch_density_probe(st, d, P0, s->probeRho.p, s->probeMax.p, s->probeSum.p);
     HIP_CHECK(hipGetLastError());
@@ -2139,6 +2175,17 @@ akua_status akua_debug_permutation(akua_solver* s, uint32_t* out, int32_t n) {
         size_t k = 0;
         for (const SceneDev& sc : s->scenes)
             for (uint32_t slot = sc.begin; slot < sc.end; ++slot) out[k++] = h[slot] - sc.begin;
+    });
+}
+
+akua_status akua_debug_physical_order(akua_solver* s, uint32_t* out, int32_t n) {
+    return guarded([&] {
+        if (!s || !out) fail(AKUA_ERR_INVALID_ARG, "NULL argument");
+        check_n(s, n);
+        auto L = download(s, s->lid.p, s->nslots);
+        size_t k = 0;
+        for (const SceneDev& sc : s->scenes)
+            for (uint32_t slot = sc.begin; slot < sc.end; ++slot) out[k++] = L[slot] - sc.begin;
     });
 }
 

@@ -251,6 +251,12 @@ class PBFSolver:
         _lib.check(self._L.akua_debug_permutation(self._h, _p(out), self.numParticles))
         return out
 
+    def debug_physical_order(self) -> np.ndarray:
+        """Logical index of the particle in each physical slot (the library's memory layout)."""
+        out = np.zeros(self.numParticles, np.uint32)
+        _lib.check(self._L.akua_debug_physical_order(self._h, _p(out), self.numParticles))
+        return out
+
     def debug_cell_ranges(self):
         n = ctypes.c_int32(0)
         st = self._L.akua_debug_cell_ranges(self._h, None, None, None, 0, ctypes.byref(n))

@@ -351,6 +351,11 @@ akua_status akua_batch_density_error(akua_solver* s, float* maxError, double* me
 akua_status akua_debug_hashes(akua_solver* s, uint32_t* sorted_hash, int32_t n);
 /* perm[k] = slot, in the previous order, of the particle now in slot k. */
 akua_status akua_debug_permutation(akua_solver* s, uint32_t* perm, int32_t n);
+/* order[p] = logical index (the reference's array position) of the particle stored in physical
+ * slot p, per scene (p and the index counted from the scene's first slot).  The physical order is
+ * the library's own memory layout; no result depends on it.  No reference counterpart: a check
+ * of the layout, for tests. */
+akua_status akua_debug_physical_order(akua_solver* s, uint32_t* order, int32_t n);
 /* Occupied buckets in sorted order, looked up through the device cell table: for each bucket c
  * (c < *ncells), its hash, first slot and one-past-last slot.  With cap too small, *ncells is
  * still set and AKUA_ERR_INVALID_ARG returned. */
