@@ -1883,10 +1883,13 @@ inline unsigned grid_xcd(uint32_t n, int bs) {
         }                               \
     } while (0)
 
-// Lane-group launches: L in {4, 8, 16}, nslots * L threads.
+// Lane-group launches: L in {2, 4, 8, 16}, nslots * L threads.
 #define AKUA_BY_LANES(lanes, CALL)      \
     do {                                \
-        if ((lanes) == 4) {             \
+        if ((lanes) == 2) {             \
+            constexpr int L = 2;        \
+            CALL;                       \
+        } else if ((lanes) == 4) {      \
             constexpr int L = 4;        \
             CALL;                       \
         } else if ((lanes) == 8) {      \

@@ -623,15 +623,21 @@ void enqueue_step(akua_solver* s, hipStream_t st, const StepParams& S, bool time
     const bool plainOrder = !d.work && s->tuning.slabBatch == 0 && P.searchCells == 1 && !P.bd.on;
     // small scenes: lane groups in the sweeps of a plain step, one lane per stencil cell in the
     // list build
-    const int lanes = s->tuning.laneGroup > 1 && plainOrder && !learned && !record ? s->tuning.laneGroup : 1;
+    const int lanesSet = s->tuning.laneGroup > 1 && plainOrder && !learned && !record ? s->tuning.laneGroup : 1;
+    // with the work queue too, the queue runs the list build, the Jacobi sweeps and confinement, and
+    // the lane groups run vorticity and XSPH (two gathered records per neighbour, whose latency
+    // more lanes per particle hide better)
+    const bool queueSet = s->tuning.workQueue > 0 && !learned && !record && s->queue.p &&
+                          s->heads.n >= queue_heads_size(queue_passes(s));
+    const int lanes = queueSet ? 1 : lanesSet;
+    const int postLanes = lanesSet;
     // the first plain density/lambda sweep, or the learned step's feature sweep, runs inside
     // the list build's launch
     const bool fusable = s->tuning.fuseSweep == 1 && plainOrder && lanes == 1;
     const bool fusedDensity = fusable && I > 0 && !learned && !record;
     const bool fusedFeatures = fusable && (learned || record);
     // the sweeps of a plain step as persistent launches over each XCD's queue of tiles
-    const bool queued = s->tuning.workQueue > 0 && lanes == 1 && !learned && !record && s->queue.p &&
-                        s->heads.n >= queue_heads_size(queue_passes(s));
+    const bool queued = queueSet;
     uint32_t pass = 0;
     const char* nbrName = fusedDensity ? "neighbours_density" : fusedFeatures ? "neighbours_features"
                                                                               : "neighbours";
@@ -716,10 +722,10 @@ void enqueue_step(akua_solver* s, hipStream_t st, const StepParams& S, bool time
         L.after(name);
     }
     L.before("vorticity");
-    if (queued)
+    if (queued && postLanes == 1)
         launch_vorticity_queue(st, d, P, pass++);
     else
-        launch_vorticity(st, d, P, bs, lanes);
+        launch_vorticity(st, d, P, bs, postLanes);
     L.after("vorticity");
     L.before("confinement");
     if (queued)
@@ -728,10 +734,10 @@ void enqueue_step(akua_solver* s, hipStream_t st, const StepParams& S, bool time
         launch_confinement(st, d, P, S, bs, lanes);
     L.after("confinement");
     L.before("xsph");
-    if (queued)
+    if (queued && postLanes == 1)
         launch_xsph_queue(st, d, P, pass++);
     else
-        launch_xsph(st, d, P, bs, lanes);
+        launch_xsph(st, d, P, bs, postLanes);
     L.after("xsph");
 }
 
@@ -1045,8 +1051,8 @@ void validate_tuning(const akua_tuning& t) {
     if (t.slabBatch != 0 && t.slabBatch != 4 && t.slabBatch != 8)
         fail(AKUA_ERR_INVALID_ARG, "tuning.slabBatch must be 0, 4 or 8");
     if (t.fuseSweep != 0 && t.fuseSweep != 1) fail(AKUA_ERR_INVALID_ARG, "tuning.fuseSweep must be 0 or 1");
-    if (t.laneGroup != 0 && t.laneGroup != 4 && t.laneGroup != 8 && t.laneGroup != 16)
-        fail(AKUA_ERR_INVALID_ARG, "tuning.laneGroup must be 0, 4, 8 or 16");
+    if (t.laneGroup != 0 && t.laneGroup != 2 && t.laneGroup != 4 && t.laneGroup != 8 && t.laneGroup != 16)
+        fail(AKUA_ERR_INVALID_ARG, "tuning.laneGroup must be 0, 2, 4, 8 or 16");
     if (t.workQueue < 0 || t.workQueue > 2) fail(AKUA_ERR_INVALID_ARG, "tuning.workQueue must be 0, 1 or 2");
 }
 
@@ -1495,11 +1501,26 @@ void autotune(akua_solver* s, float dt, const float* bmin, const float* bmax, in
     sweep(&akua_tuning::layoutInterval, {1, 2, 4});
     sweep(&akua_tuning::balanceTile, {0, 512, 1024, 2048});
     sweep(&akua_tuning::slabBatch, {0, 4, 8});
-    sweep(&akua_tuning::laneGroup, {0, 4, 8, 16});
-    // lane groups take precedence over the fused first sweep and the work queue
-    if (best.laneGroup == 0) {
-        sweep(&akua_tuning::fuseSweep, {0, 1});
-        sweep(&akua_tuning::workQueue, {0, 1, 2});
+    // the sweeps' launch forms interact (lane groups beside the work queue run only vorticity and
+    // XSPH, without it every sweep and then no fused first sweep), so they are timed together
+    {
+        const akua_tuning base = best;
+        for (int f : {0, 1})
+            for (int wq : {0, 1, 2})
+                for (int l : {0, 2, 4, 8, 16}) {
+                    if (l > 1 && wq == 0 && f == 1) continue;  // the same launches as f == 0
+                    if (f == base.fuseSweep && wq == base.workQueue && l == base.laneGroup) continue;
+                    akua_tuning t = base;
+                    t.fuseSweep = f;
+                    t.workQueue = wq;
+                    t.laneGroup = l;
+                    apply_tuning(s, t);
+                    const double ms = time_trial(s, q, dt, trialSteps);
+                    if (ms < bestMs) {
+                        bestMs = ms;
+                        best = t;
+                    }
+                }
     }
     // the non-parity search over h / 2 cells, only when the caller gave up bit-exactness
     int bestCells = s->sph.searchCells;

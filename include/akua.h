@@ -165,12 +165,14 @@ typedef struct akua_tuning {
                                  wrote, the first density/lambda sweep (or, in a learned or
                                  training step, the feature sweep); applies with balanceTile 0,
                                  slabBatch 0, the reference's search and no boundary samples */
-    int32_t laneGroup;        /* 0 (the default): one thread per particle; 4, 8 or 16: for scenes too
-                                 small to fill the chip, the neighbour sweeps of a plain step run
-                                 that many lanes per particle (each list's terms in parallel, summed
-                                 in list order) and the list build one lane per stencil cell;
+    int32_t laneGroup;        /* 0 (the default): one thread per particle; 2, 4, 8 or 16: for scenes
+                                 too small to fill the chip, the neighbour sweeps of a plain step
+                                 run that many lanes per particle (each list's terms in parallel,
+                                 summed in list order) and the list build one lane per stencil cell;
                                  applies with balanceTile 0, slabBatch 0, the reference's search and
-                                 no boundary samples, and takes precedence over fuseSweep */
+                                 no boundary samples, and takes precedence over fuseSweep.  With
+                                 workQueue set too, only the vorticity and XSPH sweeps run in lane
+                                 groups; the queue runs the others */
     int32_t workQueue;        /* 0 (the default): each neighbour sweep after the list build is one
                                  launch with a workgroup per block of slots; 1 or 2: after the list
                                  build, the 256-slot tiles each XCD serves are queued heaviest first
@@ -178,7 +180,8 @@ typedef struct akua_tuning {
                                  resident grids whose workgroups take tiles from their XCD's queue
                                  until it is empty: 1 the next free position (through counters),
                                  2 every R-th position from the workgroup's rank among the R of
-                                 its XCD; laneGroup takes precedence */
+                                 its XCD; with laneGroup set too, vorticity and XSPH run in lane
+                                 groups instead */
 } akua_tuning;
 
 akua_status akua_solver_set_tuning(akua_solver* s, const akua_tuning* t);

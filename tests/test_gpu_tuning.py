@@ -39,6 +39,10 @@ SETTINGS = [
          balanceTile=0, slabBatch=0, fuseSweep=0, laneGroup=8, workQueue=2),
     dict(block=128, layoutLevel=0, hashDigitBits=0, layoutDigitBits=0, layoutInterval=1,
          balanceTile=0, slabBatch=0, fuseSweep=1, laneGroup=0, workQueue=2),
+    dict(block=128, layoutLevel=0, hashDigitBits=0, layoutDigitBits=0, layoutInterval=1,
+         balanceTile=0, slabBatch=0, fuseSweep=1, laneGroup=4, workQueue=1),
+    dict(block=256, layoutLevel=1, hashDigitBits=0, layoutDigitBits=0, layoutInterval=1,
+         balanceTile=0, slabBatch=0, fuseSweep=0, laneGroup=2, workQueue=0),
 ]
 
 
@@ -142,7 +146,7 @@ def test_invalid_settings_are_rejected(classes, dam_params):
     for bad in (dict(block=64), dict(layoutLevel=4), dict(hashDigitBits=4), dict(layoutDigitBits=9),
                 dict(layoutInterval=0), dict(layoutInterval=9), dict(balanceTile=100),
                 dict(balanceTile=4096), dict(balanceTile=-256), dict(slabBatch=2),
-                dict(slabBatch=16), dict(fuseSweep=2), dict(fuseSweep=-1), dict(laneGroup=2),
+                dict(slabBatch=16), dict(fuseSweep=2), dict(fuseSweep=-1), dict(laneGroup=3),
                 dict(laneGroup=32), dict(laneGroup=1), dict(workQueue=3), dict(workQueue=-1)):
         with pytest.raises(AkuaError):
             s.set_tuning(**bad)
