@@ -598,7 +598,7 @@ void enqueue_step(akua_solver* s, hipStream_t st, const StepParams& S, bool time
     launch_predict_hash(st, d, P, S, together);
     L.after("predict_hash");
     if (together) {
-        sort_both(s, st, timed);
+        if (!sort_both(s, st, timed)) fail(AKUA_ERR_STATE, "paired sorts need equal digit widths");
         L.before("cell_keys");
         launch_cell_keys(st, d, P, S, false, true);
         L.after("cell_keys");
