@@ -164,8 +164,15 @@ __device__ __forceinline__ uint32_t layout_key(int cx, int cy, int cz, const Sce
 // launches (single-cell super-cells, where the previous and the new logical order agree within
 // every layout key): its keys go straight into its input (keysM) and the record keeps only the
 // slot, to which k_cell_keys adds the new logical index.
+// Blocks from predictBlocks on clear the cell table instead (every record empty), which the
+// prediction does not touch: one launch for both.
 __global__ __launch_bounds__(kBlock) void k_predict_hash(DeviceState d, SphParams P, StepParams S,
-                                                         bool layoutPrev) {
+                                                         bool layoutPrev, uint32_t predictBlocks) {
+    if (blockIdx.x >= predictBlocks) {
+        const uint32_t i = (blockIdx.x - predictBlocks) * kBlock + threadIdx.x;
+        if (i < d.tableTotal) d.table[i] = make_uint4(kEmptyKey, kEmptyKey, kEmptyKey, kEmptyKey);
+        return;
+    }
     uint32_t p = blockIdx.x * kBlock + threadIdx.x;
     if (p >= d.nslots) return;
     const SceneDev& sc = scene_of(d, p);
@@ -1914,8 +1921,10 @@ void launch_clear_table(hipStream_t st, uint4* table, uint32_t n) {
     if (n) k_fill_u4<<<grid(n), kBlock, 0, st>>>(table, n, kEmptyKey);
 }
 void launch_predict_hash(hipStream_t st, const DeviceState& d, const SphParams& P,
-                         const StepParams& S, bool layoutPrev) {
-    k_predict_hash<<<grid(d.nslots), kBlock, 0, st>>>(d, P, S, layoutPrev);
+                         const StepParams& S, bool layoutPrev, bool clearTable) {
+    const uint32_t predict = grid(d.nslots);
+    const uint32_t clear = clearTable && d.tableTotal ? grid(d.tableTotal) : 0u;
+    k_predict_hash<<<predict + clear, kBlock, 0, st>>>(d, P, S, layoutPrev, predict);
 }
 void launch_cell_keys(hipStream_t st, const DeviceState& d, const SphParams& P,
                       const StepParams& S, bool keepLayout, bool layoutPrev) {

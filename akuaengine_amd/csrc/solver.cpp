@@ -588,14 +588,11 @@ void enqueue_step(akua_solver* s, hipStream_t st, const StepParams& S, bool time
     Launcher L{s, st, timed};
     DeviceState d = s->devState(keepLayout);
     const SphParams& P = s->sph;
-    L.before("fill_table");
-    launch_clear_table(st, s->table.p, s->tableTotal);
-    L.after("fill_table");
     // single-cell super-cells: the layout keys are sorted in the previous logical order, which
     // agrees with the new one within each key, so both sorts share their launches
     const bool together = !keepLayout && s->tuning.layoutLevel == 0 && s->plan1.digitBits == s->plan2.digitBits;
     L.before("predict_hash");
-    launch_predict_hash(st, d, P, S, together);
+    launch_predict_hash(st, d, P, S, together, true);  // with the cell table's clear
     L.after("predict_hash");
     if (together) {
         if (!sort_both(s, st, timed)) fail(AKUA_ERR_STATE, "paired sorts need equal digit widths");
@@ -947,7 +944,7 @@ void run_density_probe(akua_solver* s) {
     HIP_CHECK(hipMemsetAsync(s->probeMax.p, 0, s->scenes.size() * sizeof(unsigned int), st));
     HIP_CHECK(hipMemsetAsync(s->probeSum.p, 0, s->scenes.size() * sizeof(double), st));
     launch_clear_table(st, s->table.p, s->tableTotal);
-    launch_predict_hash(st, d, P0, S0, false);
+    launch_predict_hash(st, d, P0, S0, false, false);
     sort_hash(s, st, false);
     launch_cell_keys(st, d, P0, S0, false, false);
     sort_layout(s, st, false);
