@@ -33,6 +33,7 @@
 #include <hip/hip_runtime.h>
 
 #include "pbf_types.h"
+#include "radix_sort.h"
 
 namespace akua {
 
@@ -122,9 +123,11 @@ void launch_clear_table(hipStream_t st, uint4* table, uint32_t n);
 // layoutPrev (the three launches of a step alike): the layout keys are sorted in the previous
 // logical order, in the same launches as the hashes, instead of in the new logical order after
 // them.  Only for single-cell super-cells (layout level 0), where both orders agree within each key.
-// clearTable: the same launch also empties every record of the cell table (launch_clear_table).
+// clearTable: the same launch also empties every record of the cell table (launch_clear_table);
+// zero (may be NULL): and zeroes those spans of the sorts' scratch.
 void launch_predict_hash(hipStream_t st, const DeviceState& d, const SphParams& P,
-                         const StepParams& S, bool layoutPrev, bool clearTable);
+                         const StepParams& S, bool layoutPrev, bool clearTable,
+                         const SortZeroSpans* zero = nullptr);
 // keepLayout: the step keeps every particle in its physical slot (no layout sort); the map
 // physical slot -> logical index is then written to perm2keep.
 void launch_cell_keys(hipStream_t st, const DeviceState& d, const SphParams& P,

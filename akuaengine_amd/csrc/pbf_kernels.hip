@@ -164,10 +164,23 @@ __device__ __forceinline__ uint32_t layout_key(int cx, int cy, int cz, const Sce
 // launches (single-cell super-cells, where the previous and the new logical order agree within
 // every layout key): its keys go straight into its input (keysM) and the record keeps only the
 // slot, to which k_cell_keys adds the new logical index.
-// Blocks from predictBlocks on clear the cell table instead (every record empty), which the
-// prediction does not touch: one launch for both.
+// Blocks from predictBlocks on clear the cell table instead (every record empty), and the last
+// kZeroBlocks zero the sorts' scratch spans: neither is touched by the prediction, so one launch
+// serves all three.
+constexpr uint32_t kZeroBlocks = 8;
+
 __global__ __launch_bounds__(kBlock) void k_predict_hash(DeviceState d, SphParams P, StepParams S,
-                                                         bool layoutPrev, uint32_t predictBlocks) {
+                                                         bool layoutPrev, uint32_t predictBlocks,
+                                                         uint32_t clearBlocks, SortZeroSpans zero) {
+    if (blockIdx.x >= predictBlocks + clearBlocks) {
+        const uint32_t i = (blockIdx.x - predictBlocks - clearBlocks) * kBlock + threadIdx.x;
+#pragma unroll
+        for (uint32_t s = 0; s < 6; ++s) {
+            if (s >= zero.count) break;
+            for (uint32_t k = i; k < zero.n[s]; k += kZeroBlocks * kBlock) zero.p[s][k] = 0u;
+        }
+        return;
+    }
     if (blockIdx.x >= predictBlocks) {
         const uint32_t i = (blockIdx.x - predictBlocks) * kBlock + threadIdx.x;
         if (i < d.tableTotal) d.table[i] = make_uint4(kEmptyKey, kEmptyKey, kEmptyKey, kEmptyKey);
@@ -1921,10 +1934,13 @@ void launch_clear_table(hipStream_t st, uint4* table, uint32_t n) {
     if (n) k_fill_u4<<<grid(n), kBlock, 0, st>>>(table, n, kEmptyKey);
 }
 void launch_predict_hash(hipStream_t st, const DeviceState& d, const SphParams& P,
-                         const StepParams& S, bool layoutPrev, bool clearTable) {
+                         const StepParams& S, bool layoutPrev, bool clearTable, const SortZeroSpans* zero) {
     const uint32_t predict = grid(d.nslots);
     const uint32_t clear = clearTable && d.tableTotal ? grid(d.tableTotal) : 0u;
-    k_predict_hash<<<predict + clear, kBlock, 0, st>>>(d, P, S, layoutPrev, predict);
+    SortZeroSpans z{};
+    if (zero) z = *zero;
+    const uint32_t zeroBlocks = z.count ? kZeroBlocks : 0u;
+    k_predict_hash<<<predict + clear + zeroBlocks, kBlock, 0, st>>>(d, P, S, layoutPrev, predict, clear, z);
 }
 void launch_cell_keys(hipStream_t st, const DeviceState& d, const SphParams& P,
                       const StepParams& S, bool keepLayout, bool layoutPrev) {

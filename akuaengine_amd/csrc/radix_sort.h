@@ -81,9 +81,23 @@ int radix_sort_pairs(hipStream_t stream, const SortPlan& plan, const SortBuffers
 // look-back words) with the same digit width, every pass of both in shared launches.  Returns
 // false, launching nothing, if the digit widths differ; else sets each result as
 // radix_sort_pairs returns it.
+// scratchZeroed: the spans radix_sort_zero_spans names were zeroed since the last sort (by a
+// launch of the caller's), so the sort skips its own clearing launch.
 bool radix_sort_pairs2(hipStream_t stream, const SortPlan& planA, const SortBuffers& bufA,
                        const SortPlan& planB, const SortBuffers& bufB, const SortTile* dTiles,
                        uint32_t numTiles, const SortScene* dScenes, uint32_t numScenes, uint32_t limit,
-                       LaunchHook hook, void* hookCtx, int& resultA, int& resultB);
+                       LaunchHook hook, void* hookCtx, int& resultA, int& resultB,
+                       bool scratchZeroed = false);
+
+// Scratch that two sorts of at most kOneSweepMaxTiles tiles need zeroed before they start: per sort
+// the totals of every pass, the first pass's look-back words and the tile tickets.  count is 0
+// when the sorts take three launches per pass (nothing to zero).
+struct SortZeroSpans {
+    uint32_t* p[6];
+    uint32_t n[6];
+    uint32_t count;
+};
+SortZeroSpans radix_sort_zero_spans(const SortPlan& planA, const SortBuffers& bufA, const SortPlan& planB,
+                                    const SortBuffers& bufB, uint32_t numTiles, uint32_t numScenes);
 
 }  // namespace akua

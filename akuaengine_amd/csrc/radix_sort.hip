@@ -387,7 +387,7 @@ SortIO pass_io(const SortBuffers& buf, int p, uint32_t R, uint32_t nscenes, uint
 template <int D>
 void run_sorts(hipStream_t st, const SortBuffers& A, int PA, const SortBuffers* B, int PB,
                const SortTile* tiles, uint32_t ntiles, const SortScene* scenes, uint32_t nscenes,
-               uint32_t limit, bool oneSweep, LaunchHook hook, void* ctx) {
+               uint32_t limit, bool oneSweep, bool zeroed, LaunchHook hook, void* ctx) {
     constexpr uint32_t R = 1u << D;
     const int P = std::max(PA, B ? PB : 0);
     if (oneSweep) {
@@ -397,8 +397,9 @@ void run_sorts(hipStream_t st, const SortBuffers& A, int PA, const SortBuffers* 
         const uint32_t nLook = ntiles * R;
         const uint32_t most = std::max(std::max(nTotA, nTotB), nLook);
         if (hook) hook(ctx, "rs_hist", 0);
-        rs_init<<<std::min<uint32_t>((most + kSortBlock - 1) / kSortBlock, 64u), kSortBlock, 0, st>>>(
-            a0, nTotA, b0, nTotB, nLook, B != nullptr);
+        if (!zeroed)
+            rs_init<<<std::min<uint32_t>((most + kSortBlock - 1) / kSortBlock, 64u), kSortBlock, 0, st>>>(
+                a0, nTotA, b0, nTotB, nLook, B != nullptr);
         rs_hist_all<D><<<(B ? 2u : 1u) * ntiles, kSortBlock, 0, st>>>(a0, (uint32_t)PA, b0, B ? (uint32_t)PB : 0u,
                                                                        tiles, ntiles, nscenes);
         if (hook) hook(ctx, "rs_hist", 1);
@@ -435,13 +436,13 @@ void run_sorts(hipStream_t st, const SortBuffers& A, int PA, const SortBuffers* 
 
 void dispatch_sorts(int digitBits, hipStream_t st, const SortBuffers& A, int PA, const SortBuffers* B, int PB,
                     const SortTile* tiles, uint32_t ntiles, const SortScene* scenes, uint32_t nscenes,
-                    uint32_t limit, bool oneSweep, LaunchHook hook, void* ctx) {
+                    uint32_t limit, bool oneSweep, bool zeroed, LaunchHook hook, void* ctx) {
     switch (digitBits) {
-        case 4: run_sorts<4>(st, A, PA, B, PB, tiles, ntiles, scenes, nscenes, limit, oneSweep, hook, ctx); break;
-        case 5: run_sorts<5>(st, A, PA, B, PB, tiles, ntiles, scenes, nscenes, limit, oneSweep, hook, ctx); break;
-        case 6: run_sorts<6>(st, A, PA, B, PB, tiles, ntiles, scenes, nscenes, limit, oneSweep, hook, ctx); break;
-        case 7: run_sorts<7>(st, A, PA, B, PB, tiles, ntiles, scenes, nscenes, limit, oneSweep, hook, ctx); break;
-        default: run_sorts<8>(st, A, PA, B, PB, tiles, ntiles, scenes, nscenes, limit, oneSweep, hook, ctx); break;
+        case 4: run_sorts<4>(st, A, PA, B, PB, tiles, ntiles, scenes, nscenes, limit, oneSweep, zeroed, hook, ctx); break;
+        case 5: run_sorts<5>(st, A, PA, B, PB, tiles, ntiles, scenes, nscenes, limit, oneSweep, zeroed, hook, ctx); break;
+        case 6: run_sorts<6>(st, A, PA, B, PB, tiles, ntiles, scenes, nscenes, limit, oneSweep, zeroed, hook, ctx); break;
+        case 7: run_sorts<7>(st, A, PA, B, PB, tiles, ntiles, scenes, nscenes, limit, oneSweep, zeroed, hook, ctx); break;
+        default: run_sorts<8>(st, A, PA, B, PB, tiles, ntiles, scenes, nscenes, limit, oneSweep, zeroed, hook, ctx); break;
     }
 }
 
@@ -486,22 +487,41 @@ int radix_sort_pairs(hipStream_t stream, const SortPlan& plan, const SortBuffers
     const int P = plan.passes();
     const bool oneSweep = numTiles <= kOneSweepMaxTiles && buf.look && buf.tickets;
     dispatch_sorts(plan.digitBits, stream, buf, P, nullptr, 0, dTiles, numTiles, dScenes, numScenes, limit,
-                   oneSweep, hook, hookCtx);
+                   oneSweep, false, hook, hookCtx);
     return (P % 2 == 0) ? 0 : 1;
 }
 
 bool radix_sort_pairs2(hipStream_t stream, const SortPlan& planA, const SortBuffers& bufA,
                        const SortPlan& planB, const SortBuffers& bufB, const SortTile* dTiles,
                        uint32_t numTiles, const SortScene* dScenes, uint32_t numScenes, uint32_t limit,
-                       LaunchHook hook, void* hookCtx, int& resultA, int& resultB) {
+                       LaunchHook hook, void* hookCtx, int& resultA, int& resultB, bool scratchZeroed) {
     if (planA.digitBits != planB.digitBits) return false;
     const int PA = planA.passes(), PB = planB.passes();
     const bool oneSweep = numTiles <= kOneSweepMaxTiles && bufA.look && bufA.tickets && bufB.look && bufB.tickets;
     dispatch_sorts(planA.digitBits, stream, bufA, PA, &bufB, PB, dTiles, numTiles, dScenes, numScenes, limit,
-                   oneSweep, hook, hookCtx);
+                   oneSweep, scratchZeroed, hook, hookCtx);
     resultA = (PA % 2 == 0) ? 0 : 1;
     resultB = (PB % 2 == 0) ? 0 : 1;
     return true;
+}
+
+SortZeroSpans radix_sort_zero_spans(const SortPlan& planA, const SortBuffers& bufA, const SortPlan& planB,
+                                    const SortBuffers& bufB, uint32_t numTiles, uint32_t numScenes) {
+    SortZeroSpans z{};
+    const bool oneSweep = numTiles <= kOneSweepMaxTiles && bufA.look && bufA.tickets && bufB.look && bufB.tickets;
+    if (!oneSweep || planA.digitBits != planB.digitBits) return z;
+    const uint32_t R = 1u << planA.digitBits;
+    const SortBuffers* bufs[2] = {&bufA, &bufB};
+    const int passes[2] = {planA.passes(), planB.passes()};
+    for (int i = 0; i < 2; ++i) {
+        z.p[z.count] = bufs[i]->totals;
+        z.n[z.count++] = (uint32_t)passes[i] * numScenes * R;
+        z.p[z.count] = bufs[i]->look;
+        z.n[z.count++] = numTiles * R;
+        z.p[z.count] = bufs[i]->tickets;
+        z.n[z.count++] = (uint32_t)kMaxPasses;
+    }
+    return z;
 }
 
 }  // namespace akua

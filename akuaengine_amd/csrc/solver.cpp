@@ -558,14 +558,25 @@ void sort_hash(akua_solver* s, hipStream_t st, bool timed) {
                      (uint32_t)s->scenes.size(), s->nslots, timed ? timing_hook : nullptr, s);
 }
 
+// The two sorts' buffers and scratch.
+SortBuffers hash_sort_buffers(akua_solver* s) {
+    return SortBuffers{s->s1KA.p, s->s1VA.p, s->s1KB.p, s->s1VB.p, s->counts.p, s->totals.p, s->look.p, s->tickets.p};
+}
+SortBuffers layout_sort_buffers(akua_solver* s) {
+    return SortBuffers{s->s2KA.p, s->s2VA.p, s->s2KB.p, s->s2VB.p, s->counts2.p, s->totals2.p, s->look2.p, s->tickets2.p};
+}
+SortZeroSpans sort_both_zero_spans(akua_solver* s) {
+    return radix_sort_zero_spans(s->plan1, hash_sort_buffers(s), s->plan2, layout_sort_buffers(s), s->numTiles,
+                                 (uint32_t)s->scenes.size());
+}
 // Both sorts in shared launches, the layout keys in the previous logical order (single-cell
 // super-cells with equal digit widths); false if the widths differ (nothing launched).
-bool sort_both(akua_solver* s, hipStream_t st, bool timed) {
-    SortBuffers h{s->s1KA.p, s->s1VA.p, s->s1KB.p, s->s1VB.p, s->counts.p, s->totals.p, s->look.p, s->tickets.p};
-    SortBuffers l{s->s2KA.p, s->s2VA.p, s->s2KB.p, s->s2VB.p, s->counts2.p, s->totals2.p, s->look2.p, s->tickets2.p};
+// zeroed: the spans of sort_both_zero_spans were zeroed since the last sort.
+bool sort_both(akua_solver* s, hipStream_t st, bool timed, bool zeroed) {
     int r1 = 0, r2 = 0;
-    return radix_sort_pairs2(st, s->plan1, h, s->plan2, l, s->dTiles.p, s->numTiles, s->dSortScenes.p,
-                             (uint32_t)s->scenes.size(), s->nslots, timed ? timing_hook : nullptr, s, r1, r2);
+    return radix_sort_pairs2(st, s->plan1, hash_sort_buffers(s), s->plan2, layout_sort_buffers(s), s->dTiles.p,
+                             s->numTiles, s->dSortScenes.p, (uint32_t)s->scenes.size(), s->nslots,
+                             timed ? timing_hook : nullptr, s, r1, r2, zeroed);
 }
 
 // The physical layout: super-cell Morton keys stably sorted over the new logical order.
@@ -591,11 +602,14 @@ void enqueue_step(akua_solver* s, hipStream_t st, const StepParams& S, bool time
     // single-cell super-cells: the layout keys are sorted in the previous logical order, which
     // agrees with the new one within each key, so both sorts share their launches
     const bool together = !keepLayout && s->tuning.layoutLevel == 0 && s->plan1.digitBits == s->plan2.digitBits;
+    // the prediction's launch also clears the cell table and, for paired look-back sorts, their
+    // scratch
+    const SortZeroSpans zero = together ? sort_both_zero_spans(s) : SortZeroSpans{};
     L.before("predict_hash");
-    launch_predict_hash(st, d, P, S, together, true);  // with the cell table's clear
+    launch_predict_hash(st, d, P, S, together, true, &zero);
     L.after("predict_hash");
     if (together) {
-        if (!sort_both(s, st, timed)) fail(AKUA_ERR_STATE, "paired sorts need equal digit widths");
+        if (!sort_both(s, st, timed, zero.count > 0)) fail(AKUA_ERR_STATE, "paired sorts need equal digit widths");
         L.before("cell_keys");
         launch_cell_keys(st, d, P, S, false, true);
         L.after("cell_keys");
