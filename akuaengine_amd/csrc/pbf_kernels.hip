@@ -342,8 +342,9 @@ __global__ __launch_bounds__(kBlock) void k_bucket_phys(DeviceState d) {
     if (k == sc.begin || d.skeys[k - 1] != h) {
         const uint32_t slot = d.bslot[k];
         if (slot < d.tableTotal) {
-            const uint32_t len = d.table[slot].z - k;
-            const bool small = len <= 256u && k + len <= sc.end;
+            // at most 256 indices: the one 256 ahead is past the scene or in another bucket (the
+            // sorted hashes, read nearby, instead of the record's end through the table)
+            const bool small = k + 256u >= sc.end || d.skeys[k + 256u] != h;
             atomicAnd(&d.table[slot].w, p | (small ? 0x80000000u : 0u));
         }
     }
