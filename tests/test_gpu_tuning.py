@@ -197,3 +197,25 @@ def test_batch_settings_are_invariant(classes, dam_params):
     for o in out[1:]:
         for k in ("pos", "vel", "ids"):
             np.testing.assert_array_equal(o[k], out[0][k], err_msg=k)
+
+
+def test_batch_paired_sorts_with_three_launches(classes, dam_params):
+    """A batch of more than 64 sort tiles (150,778 particles in five scenes) at layout level 0:
+    the hash and layout sorts share three launches per pass, scene-segmented.  Bit-identical to
+    multi-cell super-cells (sequential sorts)."""
+    BatchSolver, _ = classes
+    from akuaengine_amd import scenes
+    cfg, corr = dam_params
+    counts = [60000, 50000, 40000, 1, 777]
+    scs = [scenes.batched_scene(n, i, seed=3) for i, n in enumerate(counts)]
+    out = []
+    for level in (1, 0):
+        b = BatchSolver(counts, cfg, corr)
+        b.set_tuning(block=256, layoutLevel=level)
+        b.set_state(np.concatenate([s.pos for s in scs]), np.concatenate([s.vel for s in scs]))
+        for i, s in enumerate(scs):
+            b.set_box(i, s.box_min, s.box_max)
+        b.step_batch(DT, 3)
+        out.append(b.get_state())
+    for k in ("pos", "vel", "ids", "density", "lam"):
+        np.testing.assert_array_equal(out[1][k], out[0][k], err_msg=k)
