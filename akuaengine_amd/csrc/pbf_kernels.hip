@@ -1557,7 +1557,8 @@ __global__ __launch_bounds__(kQueueTile) void k_density_lambda_queue(DeviceState
 }
 
 template <bool kFinal, bool kFast, bool kBnd>
-__global__ __launch_bounds__(kQueueTile) void k_delta_correct_queue(DeviceState d, SphParams P, StepParams S,
+__global__ __launch_bounds__(kQueueTile) __attribute__((amdgpu_waves_per_eu(8)))
+void k_delta_correct_queue(DeviceState d, SphParams P, StepParams S,
                                                                     const float4* __restrict__ XsIn,
                                                                     float4* __restrict__ XsOut, uint32_t pass) {
     __shared__ uint32_t sq;
