@@ -1319,8 +1319,11 @@ __device__ __forceinline__ void delta_correct_slot(const DeviceState& d, const S
     finish_step<kApplyDelta>(d, S, sc, i, xi4, q, dp);
 }
 
+// Unit masses: held to 8 waves per SIMD (its scalar registers, over the 8-wave budget, spill to
+// vector lanes); the general kernels would spill vector registers there.
 template <int BS, bool kFinal, bool kApplyDelta, bool kFast, bool kBnd>
-__global__ __launch_bounds__(BS) void k_delta_correct(DeviceState d, SphParams P, StepParams S,
+__global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(kFast && !kBnd ? 8 : 1)))
+void k_delta_correct(DeviceState d, SphParams P, StepParams S,
                                                           const float4* __restrict__ XsSet,
                                                           float4* __restrict__ XsOut,
                                                           const float4* __restrict__ XsAlt,
